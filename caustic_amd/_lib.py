@@ -1,0 +1,106 @@
+"""ctypes binding of libcaustic_ot.so (include/caustic_ot.h).
+
+There is no CPU fallback: if the HIP library is missing this raises, so a caller can never
+silently run something other than the GPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcaustic_ot.so")
+
+COT_OK, COT_EINVAL, COT_ENOMEM, COT_EHIP, COT_ENOTCONV, COT_ECOMPLEX, COT_EUNVERIFIED = range(7)
+COT_PREC_FP64, COT_PREC_MIXED = 0, 1
+COT_STOP_REFERENCE, COT_STOP_MAXREL = 0, 1
+
+STATUS_NAMES = {
+    COT_OK: "ok", COT_EINVAL: "invalid argument", COT_ENOMEM: "out of device memory",
+    COT_EHIP: "HIP error", COT_ENOTCONV: "not converged", COT_ECOMPLEX: "cell too complex",
+    COT_EUNVERIFIED: "conformity check failed",
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class cot_problem(ctypes.Structure):
+    _fields_ = [("n_sites", ctypes.c_int), ("sites", _dp), ("nu", _dp), ("rho", _dp),
+                ("pm", ctypes.c_int), ("pn", ctypes.c_int), ("proj_domain", ctypes.c_double * 4)]
+
+
+class cot_options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("sites_per_bucket", ctypes.c_double),
+                ("ring_init", ctypes.c_int), ("ring_max", ctypes.c_int),
+                ("verify_tol", ctypes.c_double), ("max_repairs", ctypes.c_int)]
+
+
+class cot_solve_opts(ctypes.Structure):
+    _fields_ = [("history", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
+                ("initial_step", ctypes.c_double), ("stop_rule", ctypes.c_int),
+                ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class cot_report(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("evaluations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("status", ctypes.c_int), ("f", ctypes.c_double),
+                ("grad_norm", ctypes.c_double), ("max_rel_err", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every symbol include/caustic_ot.h declares
+SIGNATURES = {
+    "cot_default_options": (None, [ctypes.POINTER(cot_options)]),
+    "cot_default_solve_opts": (None, [ctypes.POINTER(cot_solve_opts)]),
+    "cot_create": (ctypes.c_int, [ctypes.POINTER(cot_problem), ctypes.c_int,
+                                  ctypes.POINTER(cot_options), ctypes.POINTER(ctypes.c_void_p)]),
+    "cot_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cot_last_error": (ctypes.c_char_p, []),
+    "cot_total_sites": (ctypes.c_int, [ctypes.c_void_p]),
+    "cot_evaluate": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _up]),
+    "cot_solve": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(cot_solve_opts),
+                                 ctypes.POINTER(cot_report)]),
+    "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
+    "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+
+class cot_stats(ctypes.Structure):
+    _fields_ = [("cell_seconds", ctypes.c_double), ("cell_launches", ctypes.c_longlong),
+                ("verify_seconds", ctypes.c_double), ("repair_rounds", ctypes.c_longlong),
+                ("evaluations", ctypes.c_longlong), ("sites_evaluated", ctypes.c_longlong),
+                ("pixels_covered", ctypes.c_longlong)]
+
+_lib = None
+
+
+class CausticLibError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libcaustic_ot.so; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m caustic_amd.build` "
+                          "(the OT solver has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, allow=()):
+    if code != COT_OK and code not in allow:
+        raise CausticLibError(code, load().cot_last_error().decode())
+    return code

@@ -1,0 +1,705 @@
+// HIP kernels of the semi-discrete OT dual evaluation and the L-BFGS vector work, for gfx950.
+//
+// cell_kernel   one site per lane.  Builds the restricted power cell of the site by clipping the
+//               target rectangle with the half-planes of candidate sites taken ring by ring from
+//               the bucket grid (polygon held per lane in LDS), then integrates the
+//               piecewise-constant target over the cell exactly with Green's theorem: each edge
+//               is walked across the pixel lattice and every piece adds the closed-form integral
+//               of the row prefix of the density.  Mass, first and second moments are summed in
+//               FP64 (mass with a compensated sum).
+//               Reference semantics: polygons_in_VD + compute_inte + compute_f + compute_center
+//               (power_diagram.cpp:8-134, basic_function.cuh:743-1191).
+// verify_kernel checks that every edge of every cell is matched by the reciprocal edge of the
+//               neighbouring cell; a cell built from too few candidates fails this and is rebuilt
+//               with a larger ring.
+// reduce/accept/direction kernels: FP64 per-problem reductions and the L-BFGS vector updates.
+#include "cot_internal.h"
+
+namespace cot {
+
+// ---------------------------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum of v over the block (blockDim.x a multiple of 64, at most 1024); result valid in thread 0.
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  __syncthreads();
+  if (ln == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) s += red[k];
+  }
+  return s;
+}
+__device__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  __syncthreads();
+  if (ln == 0) red[wv] = v;
+  __syncthreads();
+  double s = -INFINITY;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) s = fmax(s, red[k]);
+  }
+  return s;
+}
+
+// Neumaier compensated accumulation.
+struct KSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+template <typename T>
+struct PolyLDS {
+  T x[kMaxVerts][kCellBlock];
+  T y[kMaxVerts][kCellBlock];
+  int lab[kMaxVerts][kCellBlock];
+};
+
+// Keep the part of the lane's convex polygon with u.d <= c; the new edge gets `label`.
+// Returns the new vertex count (0 when empty).  Clipping is in place: at step k the writes land
+// at positions <= k + 1, and vertex k + 1 has already been read into registers.
+template <typename T>
+__device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy, T c, int label,
+                              bool& overflow, bool& changed) {
+  int nout = 0, trans = 0;
+  {
+    T sd_first = P.x[0][ln] * dx + P.y[0][ln] * dy - c;
+    bool prev = sd_first > T(0);
+    nout += prev;
+    bool first = prev;
+    for (int k = 1; k < n; ++k) {
+      T sd = P.x[k][ln] * dx + P.y[k][ln] * dy - c;
+      bool o = sd > T(0);
+      nout += o;
+      trans += (o != prev);
+      prev = o;
+    }
+    trans += (prev != first);
+  }
+  changed = false;
+  if (nout == 0) return n;
+  if (nout == n) {
+    changed = true;
+    return 0;
+  }
+  if (trans != 2) return n;  // inconsistent signs from rounding on a degenerate polygon: keep
+  if (n + 1 > kMaxVerts) {
+    overflow = true;
+    return n;
+  }
+  changed = true;
+  T sx = P.x[0][ln], sy = P.y[0][ln];
+  int sl = P.lab[0][ln];
+  const T fx = sx, fy = sy;
+  const int fl = sl;
+  T ssd = sx * dx + sy * dy - c;
+  int out = 0;
+  for (int k = 0; k < n; ++k) {
+    T ex, ey;
+    int el;
+    if (k + 1 < n) {
+      ex = P.x[k + 1][ln];
+      ey = P.y[k + 1][ln];
+      el = P.lab[k + 1][ln];
+    } else {
+      ex = fx;
+      ey = fy;
+      el = fl;
+    }
+    T esd = ex * dx + ey * dy - c;
+    bool sin_ = ssd <= T(0), ein = esd <= T(0);
+    if (sin_) {
+      P.x[out][ln] = sx;
+      P.y[out][ln] = sy;
+      P.lab[out][ln] = sl;
+      ++out;
+    }
+    if (sin_ != ein) {
+      T tt = ssd / (ssd - esd);
+      P.x[out][ln] = sx + tt * (ex - sx);
+      P.y[out][ln] = sy + tt * (ey - sy);
+      P.lab[out][ln] = sin_ ? label : sl;
+      ++out;
+    }
+    sx = ex;
+    sy = ey;
+    sl = el;
+    ssd = esd;
+  }
+  return out >= 3 ? out : 0;
+}
+
+// Row-prefix cache for the Green walk: sums over full pixels k = 0 .. col-1 (local columns) of
+// rho_k * (1, k + 1/2, k^2 + k + 1/3), i.e. the integrals of 1, U, U^2 over whole pixels.
+template <typename TI>
+struct RowCache {
+  const TI* base;  // image + koy * pm + kox
+  int pm;
+  int row = -1, col = 0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+
+  __device__ __forceinline__ void add_pixel(const TI* r, int k, double sgn) {
+    double v = sgn * (double)r[k];
+    double kk = (double)k;
+    a0 += v;
+    a1 += v * (kk + 0.5);
+    a2 += v * (kk * kk + kk + 1.0 / 3.0);
+  }
+  __device__ __forceinline__ void seek(int iy, int ix) {
+    const TI* r = base + (size_t)iy * pm;
+    if (iy != row) {
+      row = iy;
+      col = 0;
+      a0 = a1 = a2 = 0.0;
+    }
+    while (col < ix) add_pixel(r, col++, 1.0);
+    while (col > ix) add_pixel(r, --col, -1.0);
+  }
+};
+
+struct Moments {
+  KSum m0;
+  double m1u = 0.0, m1v = 0.0, m2u = 0.0, m2v = 0.0;
+};
+
+// Exact contribution of one edge piece inside pixel (ix, iy) (local pixel units) to
+// oint F dV for F = int rho, int rho*U, int rho*U^2 along U, and V*F0, V^2*F0.
+template <typename TI>
+__device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix, int iy,
+                                      RowCache<TI>& rc, Moments& M) {
+  double dv = V1 - V0;
+  if (dv == 0.0) return;
+  rc.seek(iy, ix);
+  const double rc_ = (double)rc.base[(size_t)iy * rc.pm + ix];
+  const double fx = (double)ix;
+  const double Um = 0.5 * (U0 + U1), Vm = 0.5 * (V0 + V1);
+  // mass: int (A0 + rho (U - ix)) dV
+  M.m0.add((rc.a0 + rc_ * (Um - fx)) * dv);
+  // int rho U: int (A1 + rho (U^2 - ix^2)/2) dV
+  const double u2avg = (U0 * U0 + U0 * U1 + U1 * U1) * (1.0 / 3.0);
+  M.m1u += (rc.a1 + 0.5 * rc_ * (u2avg - fx * fx)) * dv;
+  // int rho V: int V (A0 + rho (U - ix)) dV
+  const double vu = (V0 * U0 + 4.0 * Vm * Um + V1 * U1) * (1.0 / 6.0) * dv;
+  M.m1v += (rc.a0 - rc_ * fx) * 0.5 * (V1 * V1 - V0 * V0) + rc_ * vu;
+  // int rho U^2: int (A2 + rho (U^3 - ix^3)/3) dV
+  const double u3 = (U0 * U0 * U0 + 4.0 * Um * Um * Um + U1 * U1 * U1) * (1.0 / 6.0) * dv;
+  M.m2u += (rc.a2 - rc_ * fx * fx * fx * (1.0 / 3.0)) * dv + rc_ * (1.0 / 3.0) * u3;
+  // int rho V^2: int V^2 (A0 + rho (U - ix)) dV
+  const double v2u = (V0 * V0 * U0 + 4.0 * Vm * Vm * Um + V1 * V1 * U1) * (1.0 / 6.0) * dv;
+  M.m2v += (rc.a0 - rc_ * fx) * (V1 * V1 * V1 - V0 * V0 * V0) * (1.0 / 3.0) + rc_ * v2u;
+}
+
+// Walk the edge (Ua,Va)->(Ub,Vb) across the pixel lattice (local pixel units, lattice limits
+// lim_u x lim_v) and add each piece.
+template <typename T, typename TI>
+__device__ void walk_edge(T Ua, T Va, T Ub, T Vb, int lim_u, int lim_v, RowCache<TI>& rc,
+                          Moments& M) {
+  const double ua = Ua, va = Va, ub = Ub, vb = Vb;
+  const double du = ub - ua, dvv = vb - va;
+  if (dvv == 0.0) return;
+  int iu = (int)floor(ua), iv = (int)floor(va);
+  const int iu_end = (int)floor(ub), iv_end = (int)floor(vb);
+  const int su = du > 0.0 ? 1 : -1, sv = dvv > 0.0 ? 1 : -1;
+  // next lattice lines crossed
+  int nu_line = du > 0.0 ? iu + 1 : iu;
+  int nv_line = dvv > 0.0 ? iv + 1 : iv;
+  double tu = du != 0.0 ? ((double)nu_line - ua) / du : 2.0;
+  double tv = ((double)nv_line - va) / dvv;
+  const int maxsteps = abs(iu_end - iu) + abs(iv_end - iv) + 2;
+  double t = 0.0, U0 = ua, V0 = va;
+  for (int s = 0; s <= maxsteps; ++s) {
+    double tn = fmin(fmin(tu, tv), 1.0);
+    if (tn < t) tn = t;
+    double U1, V1;
+    if (tn >= 1.0) {
+      U1 = ub;
+      V1 = vb;
+    } else {
+      U1 = ua + tn * du;
+      V1 = va + tn * dvv;
+    }
+    int ix = (int)floor(0.5 * (U0 + U1));
+    int iy = (int)floor(0.5 * (V0 + V1));
+    ix = min(max(ix, 0), lim_u - 1);
+    iy = min(max(iy, 0), lim_v - 1);
+    piece<TI>(U0, V0, U1, V1, ix, iy, rc, M);
+    if (tn >= 1.0) break;
+    if (tu <= tv) {
+      nu_line += su;
+      tu = ((double)nu_line - ua) / du;
+    } else {
+      nv_line += sv;
+      tv = ((double)nv_line - va) / dvv;
+    }
+    t = tn;
+    U0 = U1;
+    V0 = V1;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// cell kernel
+// ---------------------------------------------------------------------------------------------
+
+template <typename T, typename TI>
+__global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
+  __shared__ PolyLDS<T> P;
+  const BlockDesc bd = a.blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  if (!a.active[bd.prob]) return;
+  const int i = bd.first + ln;
+  if (a.redo && !a.redo[i]) return;
+  const ProbDesc pd = a.probs[bd.prob];
+  const double tstep = a.dir ? a.step[bd.prob] : 0.0;
+
+  const double2 s = a.xy[i];
+  const double wi = a.dir ? a.w[i] + tstep * a.dir[i] : a.w[i];
+
+  // target rectangle relative to the site, counter-clockwise; side labels
+  P.x[0][ln] = (T)(pd.x0 - s.x); P.y[0][ln] = (T)(pd.y0 - s.y); P.lab[0][ln] = kSideBottom;
+  P.x[1][ln] = (T)(pd.x1 - s.x); P.y[1][ln] = (T)(pd.y0 - s.y); P.lab[1][ln] = kSideRight;
+  P.x[2][ln] = (T)(pd.x1 - s.x); P.y[2][ln] = (T)(pd.y1 - s.y); P.lab[2][ln] = kSideTop;
+  P.x[3][ln] = (T)(pd.x0 - s.x); P.y[3][ln] = (T)(pd.y1 - s.y); P.lab[3][ln] = kSideLeft;
+  int nv = 4;
+
+  // bounding circle of the polygon (centre cx, cy relative to the site; radius^2 r2)
+  auto bound = [&](double& cx, double& cy, double& r2) {
+    double mnx = P.x[0][ln], mxx = mnx, mny = P.y[0][ln], mxy = mny;
+    for (int k = 1; k < nv; ++k) {
+      double xx = P.x[k][ln], yy = P.y[k][ln];
+      mnx = fmin(mnx, xx); mxx = fmax(mxx, xx);
+      mny = fmin(mny, yy); mxy = fmax(mxy, yy);
+    }
+    cx = 0.5 * (mnx + mxx);
+    cy = 0.5 * (mny + mxy);
+    r2 = 0.0;
+    for (int k = 0; k < nv; ++k) {
+      double ex = (double)P.x[k][ln] - cx, ey = (double)P.y[k][ln] - cy;
+      r2 = fmax(r2, ex * ex + ey * ey);
+    }
+  };
+  double bcx, bcy, br2;
+  bound(bcx, bcy, br2);
+
+  const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
+  const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
+  const int R = a.ring[i];
+  int used = 0;
+  bool overflow = false;
+  const int2* bk = a.buckets + pd.bkt_off;
+
+  for (int r = 0; r <= R && nv > 0; ++r) {
+    const int ylo = by - r, yhi = by + r;
+    for (int cy = max(ylo, 0); cy <= min(yhi, pd.gy - 1) && nv > 0; ++cy) {
+      const bool full = (cy == ylo) || (cy == yhi);
+      const int stepx = full ? 1 : 2 * r;
+      for (int cx = bx - r; cx <= bx + r && nv > 0; cx += stepx) {
+        if (cx < 0 || cx >= pd.gx) continue;
+        const int2 b = bk[cy * pd.gx + cx];
+        for (int jj = 0; jj < b.y && nv > 0; ++jj) {
+          const int j = pd.site_off + b.x + jj;
+          if (j == i) continue;
+          const double2 sj = a.xy[j];
+          const double wj = a.dir ? a.w[j] + tstep * a.dir[j] : a.w[j];
+          const double ddx = sj.x - s.x, ddy = sj.y - s.y;
+          const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj);
+          // the half-plane u.d <= c contains the bounding circle: nothing to clip
+          const double slack = c - (bcx * ddx + bcy * ddy);
+          if (slack >= 0.0 && slack * slack >= br2 * (ddx * ddx + ddy * ddy)) continue;
+          bool changed;
+          nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed);
+          if (changed) {
+            used = r;
+            if (nv > 0) bound(bcx, bcy, br2);
+          }
+        }
+      }
+    }
+  }
+  if (overflow) atomicAdd(&a.overflow[bd.prob], 1);
+  a.ring_used[i] = (uint8_t)used;
+  a.nedge[i] = (uint8_t)nv;
+
+  double m0 = 0.0, fi = -wi * a.nu[i] , ar = 0.0;
+  double2 cen = make_double2(0.0, 0.0), geo = make_double2(0.0, 0.0);
+  if (nv >= 3) {
+    // edge records for the conformity check, geometric area/centroid, pixel frame
+    double mnu = INFINITY, mnv = INFINITY;
+    double A2 = 0.0, gx = 0.0, gy = 0.0;
+    for (int k = 0; k < nv; ++k) {
+      const int k1 = (k + 1 == nv) ? 0 : k + 1;
+      const double xa = P.x[k][ln], ya = P.y[k][ln], xb = P.x[k1][ln], yb = P.y[k1][ln];
+      const double cr = xa * yb - xb * ya;
+      A2 += cr;
+      gx += (xa + xb) * cr;
+      gy += (ya + yb) * cr;
+      mnu = fmin(mnu, xa);
+      mnv = fmin(mnv, ya);
+      const int lab = P.lab[k][ln];
+      float2 span = make_float2(0.f, 0.f);
+      if (lab >= 0) {
+        const int lo = min(i, lab), hi = max(i, lab);
+        const double2 pl = a.xy[lo], ph = a.xy[hi];
+        const double Dx = ph.x - pl.x, Dy = ph.y - pl.y;
+        const double inv = 1.0 / sqrt(Dx * Dx + Dy * Dy);
+        const double ex = -Dy * inv, ey = Dx * inv;
+        const double mx = 0.5 * (pl.x + ph.x), my = 0.5 * (pl.y + ph.y);
+        const double t0 = (s.x + xa - mx) * ex + (s.y + ya - my) * ey;
+        const double t1 = (s.x + xb - mx) * ex + (s.y + yb - my) * ey;
+        span = make_float2((float)fmin(t0, t1), (float)fmax(t0, t1));
+      }
+      a.elab[(size_t)i * kMaxVerts + k] = lab;
+      a.espan[(size_t)i * kMaxVerts + k] = span;
+    }
+    ar = 0.5 * A2;
+    if (A2 != 0.0) {
+      geo = make_double2(s.x + gx / (3.0 * A2), s.y + gy / (3.0 * A2));
+    } else {
+      geo = s;
+    }
+
+    // Green integration in local pixel units, integer origin (kox, koy) at the cell's corner
+    const double idx_ = 1.0 / pd.dx, idy_ = 1.0 / pd.dy;
+    int kox = (int)floor((s.x + mnu - pd.x0) * idx_);
+    int koy = (int)floor((s.y + mnv - pd.y0) * idy_);
+    kox = min(max(kox, 0), pd.pm - 1);
+    koy = min(max(koy, 0), pd.pn - 1);
+    const double ou = (s.x - pd.x0) * idx_ - (double)kox;  // site in local pixel units
+    const double ov = (s.y - pd.y0) * idy_ - (double)koy;
+    const TI* img = reinterpret_cast<const TI*>(a.img) + pd.img_off;
+    RowCache<TI> rc;
+    rc.base = img + (size_t)koy * pd.pm + kox;
+    rc.pm = pd.pm;
+    Moments M;
+    const int lim_u = pd.pm - kox, lim_v = pd.pn - koy;
+    T pu = (T)(ou + (double)P.x[0][ln] * idx_), pv = (T)(ov + (double)P.y[0][ln] * idy_);
+    const T fu = pu, fv = pv;
+    for (int k = 0; k < nv; ++k) {
+      T qu, qv;
+      if (k + 1 < nv) {
+        qu = (T)(ou + (double)P.x[k + 1][ln] * idx_);
+        qv = (T)(ov + (double)P.y[k + 1][ln] * idy_);
+      } else {
+        qu = fu;
+        qv = fv;
+      }
+      walk_edge<T, TI>(pu, pv, qu, qv, lim_u, lim_v, rc, M);
+      pu = qu;
+      pv = qv;
+    }
+    m0 = M.m0.value();
+    // second moment about the site, physical units
+    const double su2 = M.m2u - 2.0 * ou * M.m1u + ou * ou * m0;
+    const double sv2 = M.m2v - 2.0 * ov * M.m1v + ov * ov * m0;
+    const double m2 = pd.dx * pd.dx * su2 + pd.dy * pd.dy * sv2;
+    fi += wi * m0 - m2;
+    if (m0 != 0.0) {
+      cen = make_double2(pd.x0 + pd.dx * ((double)kox + M.m1u / m0),
+                         pd.y0 + pd.dy * ((double)koy + M.m1v / m0));
+    } else {
+      cen = geo;
+    }
+  }
+  a.mass[i] = m0;
+  a.fcell[i] = fi;
+  a.area[i] = ar;
+  a.cen[i] = cen;
+  a.geo[i] = geo;
+  a.exist[i] = nv >= 3 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// conformity check
+// ---------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
+  const BlockDesc bd = a.blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  if (!a.active[bd.prob]) return;
+  const int i = bd.first + ln;
+  const int ne = a.nedge[i];
+  const double2 si = a.xy[i];
+  bool bad = false;
+  for (int k = 0; k < ne; ++k) {
+    const int j = a.elab[(size_t)i * kMaxVerts + k];
+    if (j < 0) continue;
+    const float2 sp = a.espan[(size_t)i * kMaxVerts + k];
+    const double2 sj = a.xy[j];
+    const double ddx = sj.x - si.x, ddy = sj.y - si.y;
+    const double tol = a.tol_rel * sqrt(ddx * ddx + ddy * ddy);
+    const double len = (double)sp.y - (double)sp.x;
+    const int nj = a.nedge[j];
+    bool found = false, match = false;
+    for (int kk = 0; kk < nj; ++kk) {
+      if (a.elab[(size_t)j * kMaxVerts + kk] == i) {
+        const float2 sq = a.espan[(size_t)j * kMaxVerts + kk];
+        found = true;
+        match = fabs((double)sp.x - (double)sq.x) + fabs((double)sp.y - (double)sq.y) <= tol;
+        break;
+      }
+    }
+    if (found ? !match : (len > tol)) {
+      bad = true;
+      a.redo[j] = 1;
+    }
+  }
+  if (bad) {
+    a.redo[i] = 1;
+    atomicAdd(&a.fails[bd.prob], 1);
+  }
+}
+
+// Raise the ring of every cell marked for a rebuild.
+__global__ void raise_ring_kernel(const BlockDesc* blocks, const uint8_t* redo, uint8_t* ring,
+                                  int ring_max) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  const int i = bd.first + ln;
+  if (redo[i]) ring[i] = (uint8_t)min((int)ring[i] + 1, ring_max);
+}
+
+// Next evaluation's ring: the outermost ring that shaped the cell plus one, at least ring_min.
+__global__ void adapt_ring_kernel(const BlockDesc* blocks, const uint8_t* used, uint8_t* ring,
+                                  int ring_min, int ring_max) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  const int i = bd.first + ln;
+  ring[i] = (uint8_t)min(max((int)used[i] + 1, ring_min), ring_max);
+}
+
+// ---------------------------------------------------------------------------------------------
+// reductions and L-BFGS vector kernels (one block per cell block; per-block partials)
+// ---------------------------------------------------------------------------------------------
+
+// Partials per block: [0] sum f_i, [1] sum mass, [2] sum area, [3] max |mass-nu|/nu,
+// [4] count(mass == 0), [5] sum (mass - nu)^2, [6] sum (mass - nu) * dir (directional
+// derivative at the trial point, dir may be null), [7] count(empty cells).
+constexpr int kEvalPartials = 8;
+
+__global__ __launch_bounds__(kCellBlock) void reduce_eval_kernel(
+    const BlockDesc* blocks, const double* mass, const double* fcell, const double* area,
+    const double* nu, const uint8_t* exist, const double* dir, double* part) {
+  __shared__ double red[kCellBlock / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  double f = 0, m = 0, ar = 0, mr = 0, z = 0, g2 = 0, gd = 0, em = 0;
+  if (ln < bd.count) {
+    const int i = bd.first + ln;
+    const double g = mass[i] - nu[i];
+    f = fcell[i];
+    m = mass[i];
+    ar = area[i];
+    mr = fabs(g) / nu[i];
+    z = mass[i] == 0.0 ? 1.0 : 0.0;
+    g2 = g * g;
+    gd = dir ? g * dir[i] : 0.0;
+    em = exist[i] ? 0.0 : 1.0;
+  }
+  double* out = part + (size_t)blockIdx.x * kEvalPartials;
+  double v;
+  v = block_sum(f, red);   if (ln == 0) out[0] = v;
+  v = block_sum(m, red);   if (ln == 0) out[1] = v;
+  v = block_sum(ar, red);  if (ln == 0) out[2] = v;
+  v = block_max(mr, red);  if (ln == 0) out[3] = v;
+  v = block_sum(z, red);   if (ln == 0) out[4] = v;
+  v = block_sum(g2, red);  if (ln == 0) out[5] = v;
+  v = block_sum(gd, red);  if (ln == 0) out[6] = v;
+  v = block_sum(em, red);  if (ln == 0) out[7] = v;
+}
+
+// g = mass - nu
+__global__ void gradient_kernel(const BlockDesc* blocks, const double* mass, const double* nu,
+                                double* g) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  const int i = bd.first + ln;
+  g[i] = mass[i] - nu[i];
+}
+
+// Direction d = -(cg * g + sum_h cy[h] * Y[h] + sum_h cs[h] * S[h]) for active problems, with
+// per-block partial g.d.  coef layout per problem: [cg, cy[kMaxHist], cs[kMaxHist]].
+__global__ __launch_bounds__(kCellBlock) void direction_kernel(
+    const BlockDesc* blocks, const uint8_t* active, const double* coef, const int* hcount,
+    const double* S, const double* Y, size_t ld, const double* g, double* d, double* part) {
+  __shared__ double red[kCellBlock / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (!active[bd.prob]) return;  // uniform per block
+  const int ln = threadIdx.x;
+  const double* cf = coef + (size_t)bd.prob * (1 + 2 * kMaxHist);
+  const int H = hcount[bd.prob];
+  double gd = 0.0;
+  if (ln < bd.count) {
+    const int i = bd.first + ln;
+    double acc = cf[0] * g[i];
+    for (int h = 0; h < H; ++h) acc += cf[1 + h] * Y[h * ld + i] + cf[1 + kMaxHist + h] * S[h * ld + i];
+    const double di = -acc;
+    d[i] = di;
+    gd = g[i] * di;
+  }
+  const double v = block_sum(gd, red);
+  if (ln == 0) part[blockIdx.x] = v;
+}
+
+// Accept a line-search trial for active problems: with g_new = mass - nu,
+//   s = t * d, y = g_new - g, stored in history slot `slot`; w += s; g = g_new.
+// Per-block partials (layout per block, width 5 * kMaxHist):
+//   [h]               s_slot . y_h
+//   [kMaxHist + h]    s_h . y_slot
+//   [2kMaxHist + h]   y_slot . y_h
+//   [3kMaxHist + h]   g_new . s_h
+//   [4kMaxHist + h]   g_new . y_h
+// for h < hcount (the count after the insertion).
+constexpr int kAcceptPartials = 5 * kMaxHist;
+
+__global__ __launch_bounds__(kCellBlock) void accept_kernel(
+    const BlockDesc* blocks, const uint8_t* active, const double* step, const int* slot,
+    const int* hcount, const double* mass, const double* nu, const double* d, double* w,
+    double* g, double* S, double* Y, size_t ld, double* part) {
+  __shared__ double red[kCellBlock / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (!active[bd.prob]) return;
+  const int ln = threadIdx.x;
+  const int sl = slot[bd.prob], H = hcount[bd.prob];
+  double sn = 0.0, yn = 0.0, gn = 0.0;
+  const bool live = ln < bd.count;
+  const int i = bd.first + ln;
+  if (live) {
+    const double t = step[bd.prob];
+    gn = mass[i] - nu[i];
+    sn = t * d[i];
+    yn = gn - g[i];
+    S[sl * ld + i] = sn;
+    Y[sl * ld + i] = yn;
+    w[i] = w[i] + t * d[i];
+    g[i] = gn;
+  }
+  double* out = part + (size_t)blockIdx.x * kAcceptPartials;
+  for (int h = 0; h < H; ++h) {
+    double sh = 0.0, yh = 0.0;
+    if (live) {
+      sh = S[h * ld + i];
+      yh = Y[h * ld + i];
+    }
+    double v;
+    v = block_sum(sn * yh, red); if (ln == 0) out[h] = v;
+    v = block_sum(sh * yn, red); if (ln == 0) out[kMaxHist + h] = v;
+    v = block_sum(yn * yh, red); if (ln == 0) out[2 * kMaxHist + h] = v;
+    v = block_sum(gn * sh, red); if (ln == 0) out[3 * kMaxHist + h] = v;
+    v = block_sum(gn * yh, red); if (ln == 0) out[4 * kMaxHist + h] = v;
+  }
+}
+
+// Fill a byte array (graph-safe replacement for memset).
+__global__ void fill_u8_kernel(uint8_t* p, size_t n, uint8_t v) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+__global__ void fill_i32_kernel(int* p, size_t n, int v) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+
+hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32_image, hipStream_t st) {
+  if (fp32_image)
+    hipLaunchKernelGGL((cell_kernel<double, float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((cell_kernel<double, double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(verify_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
+                             int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(raise_ring_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, redo, ring,
+                     rmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
+                             int rmax, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(adapt_ring_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, used, ring,
+                     rmin, rmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_eval(const BlockDesc* b, const double* mass, const double* fcell,
+                              const double* area, const double* nu, const uint8_t* exist,
+                              const double* dir, double* part, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_eval_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, mass, fcell,
+                     area, nu, exist, dir, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient(const BlockDesc* b, const double* mass, const double* nu, double* g,
+                           int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(gradient_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, mass, nu, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_direction(const BlockDesc* b, const uint8_t* active, const double* coef,
+                            const int* hcount, const double* S, const double* Y, size_t ld,
+                            const double* g, double* d, double* part, int nblocks,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(direction_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, active, coef,
+                     hcount, S, Y, ld, g, d, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_accept(const BlockDesc* b, const uint8_t* active, const double* step,
+                         const int* slot, const int* hcount, const double* mass, const double* nu,
+                         const double* d, double* w, double* g, double* S, double* Y, size_t ld,
+                         double* part, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(accept_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, active, step, slot,
+                     hcount, mass, nu, d, w, g, S, Y, ld, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n,
+                     v);
+  return hipGetLastError();
+}
+
+}  // namespace cot

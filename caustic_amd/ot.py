@@ -1,0 +1,175 @@
+"""Python host side of the MI355X OT solver: a batch of problems resident on the GPU.
+
+    batch = OTBatch([problem], precision="fp64")
+    w, reports = batch.solve(history=10, tol=1e-5)
+
+Each problem is given in the reference's conventions (sites = ``proj_xy``, nu = carry energy,
+rho = normalised picture in the layout rho[iy, ix], proj_domain = (x0, x1, y0, y1)); see
+``caustic_amd.workloads.Problem``.  ``direct_ot`` mirrors the reference operator of the same
+name (Optimal_Transport_rl/ot_op.cuh:66-70) for the parallel-light, t = 0 path.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class OTBatch:
+    """A context of one or more OT problems on the current GPU."""
+
+    def __init__(self, problems: Sequence, precision: str = "fp64", sites_per_bucket: float = 1.0,
+                 ring_init: int = 2, ring_max: int = 32, verify_tol: float = 1e-4,
+                 max_repairs: int = 12):
+        lib = L.load()
+        self._keep = []
+        arr = (L.cot_problem * len(problems))()
+        self.sizes = []
+        for k, p in enumerate(problems):
+            sites = np.ascontiguousarray(p.sites, dtype=np.float64).reshape(-1, 2)
+            nu = np.ascontiguousarray(p.nu, dtype=np.float64)
+            rho = np.ascontiguousarray(p.rho, dtype=np.float64)
+            if nu.shape[0] != sites.shape[0]:
+                raise ValueError("nu and sites differ in length")
+            self._keep += [sites, nu, rho]
+            arr[k].n_sites = sites.shape[0]
+            arr[k].sites = _ptr(sites)
+            arr[k].nu = _ptr(nu)
+            arr[k].rho = _ptr(rho)
+            arr[k].pn, arr[k].pm = rho.shape
+            for q in range(4):
+                arr[k].proj_domain[q] = float(p.proj_domain[q])
+            self.sizes.append(sites.shape[0])
+        opt = L.cot_options()
+        lib.cot_default_options(ctypes.byref(opt))
+        opt.precision = {"fp64": L.COT_PREC_FP64, "mixed": L.COT_PREC_MIXED}[precision]
+        opt.sites_per_bucket = sites_per_bucket
+        opt.ring_init = ring_init
+        opt.ring_max = ring_max
+        opt.verify_tol = verify_tol
+        opt.max_repairs = max_repairs
+        ctx = ctypes.c_void_p()
+        L.check(lib.cot_create(arr, len(problems), ctypes.byref(opt), ctypes.byref(ctx)))
+        self._keep = []  # the library copied everything
+        self._ctx = ctx
+        self._lib = lib
+        self.n_total = int(sum(self.sizes))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n_problems = len(problems)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.cot_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _w(self, w):
+        if w is None:
+            return np.zeros(self.n_total)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        if w.shape[0] != self.n_total:
+            raise ValueError(f"expected {self.n_total} weights, got {w.shape[0]}")
+        return w
+
+    def split(self, v):
+        return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n_problems)]
+
+    def evaluate(self, w=None):
+        """Masses, per-problem objective, rho-weighted centroids and existence flags at w."""
+        w = self._w(w)
+        mass = np.zeros(self.n_total)
+        f = np.zeros(self.n_problems)
+        cen = np.zeros((self.n_total, 2))
+        ex = np.zeros(self.n_total, dtype=np.uint8)
+        L.check(self._lib.cot_evaluate(self._ctx, _ptr(w), _ptr(mass), _ptr(f), _ptr(cen),
+                                       ex.ctypes.data_as(_up)))
+        return {"mass": mass, "f": f, "centroid": cen, "exist": ex.astype(bool)}
+
+    def centroids(self, w=None, weighted: bool = True):
+        w = self._w(w)
+        cen = np.zeros((self.n_total, 2))
+        ex = np.zeros(self.n_total, dtype=np.uint8)
+        L.check(self._lib.cot_centroids(self._ctx, _ptr(w), int(weighted), _ptr(cen),
+                                        ex.ctypes.data_as(_up)))
+        return cen, ex.astype(bool)
+
+    def cell_areas(self, w=None):
+        w = self._w(w)
+        a = np.zeros(self.n_total)
+        L.check(self._lib.cot_cell_areas(self._ctx, _ptr(w), _ptr(a)))
+        return a
+
+    def stats(self):
+        """Kernel time and counts accumulated since the last reset_stats()."""
+        s = L.cot_stats()
+        L.check(self._lib.cot_get_stats(self._ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.cot_stats._fields_}
+
+    def reset_stats(self):
+        L.check(self._lib.cot_reset_stats(self._ctx))
+
+    def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
+              initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
+              verbose: int = 0, raise_on_fail: bool = True):
+        """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports)."""
+        w = self._w(w0).copy()
+        o = L.cot_solve_opts()
+        self._lib.cot_default_solve_opts(ctypes.byref(o))
+        o.history = history
+        o.max_iter = max_iter
+        o.tol = tol
+        o.initial_step = initial_step
+        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL}[stop]
+        o.h0_threshold = h0_threshold
+        o.verbose = verbose
+        reps = (L.cot_report * self.n_problems)()
+        rc = self._lib.cot_solve(self._ctx, _ptr(w), ctypes.byref(o), reps)
+        if raise_on_fail:
+            L.check(rc, allow=(L.COT_ENOTCONV,))
+        out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
+        return w, out
+
+
+def direct_ot(proj_xy, weights, picture, picture_size, total_iter, convergence_tolerance,
+              initial_step, history_size, proj_domain, carry_energy, precision="fp64",
+              stop="reference", h0_threshold=1e-5):
+    """Mirror of direct_ot (ot_op.cuh:66-70) for one problem: solves in place on `weights`
+    and returns (output_points, if_exist) -- the rho-weighted centroids of the final cells.
+
+    picture is the normalised target in the reference's flat layout (picture[ix + iy * m],
+    picture_size = (m, n)); carry_energy the source masses."""
+    from .workloads import Problem
+
+    m, n = int(picture_size[0]), int(picture_size[1])
+    rho = np.asarray(picture, dtype=np.float64).reshape(n, m)
+    prob = Problem(sites=np.asarray(proj_xy, dtype=np.float64).reshape(-1, 2),
+                   nu=np.asarray(carry_energy, dtype=np.float64), rho=rho,
+                   proj_domain=tuple(proj_domain))
+    with OTBatch([prob], precision=precision) as b:
+        w, rep = b.solve(np.asarray(weights, dtype=np.float64), history=history_size,
+                         max_iter=total_iter, tol=convergence_tolerance,
+                         initial_step=initial_step, stop=stop, h0_threshold=h0_threshold)
+        cen, ex = b.centroids(w, weighted=True)
+    weights[...] = w
+    return cen.ravel(), ex

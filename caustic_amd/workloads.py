@@ -1,0 +1,149 @@
+"""Seeded synthetic OT problems: sites on jittered grids, piecewise-constant target images.
+
+Every problem lives on the unit square: proj_domain = sample_domain = [0, 1] x [0, 1], so the
+reference's area_scalar is 1 and the target is normalised to total mass 1 (sum of pixel values).
+The image layout is the reference's: ``rho[iy, ix]`` flattened row-major, i.e. element
+``ix + iy * m`` with x the fast index (Optimal_Transport_rl/basic_function.cuh:195-201).
+
+Random streams are numpy PCG64 seeded with ``base_seed + problem_index``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+UNIT_DOMAIN = (0.0, 1.0, 0.0, 1.0)
+
+
+@dataclasses.dataclass
+class Problem:
+    """One semi-discrete OT problem in the reference's conventions.
+
+    sites:  (N, 2) float64 site positions (the reference's ``proj_xy``)
+    nu:     (N,) float64 source masses (``carry_energy``), summing to ``area_scalar``
+    rho:    (H, W) float64 target, mass per pixel, summing to ``area_scalar``
+            (``picture / gray_multiple``, ot_op.cuh:81-88,154); W pixels along x
+    proj_domain: (x0, x1, y0, y1) of the target plane
+    tol:    max relative cell-mass error at which the solve stops
+    """
+
+    sites: np.ndarray
+    nu: np.ndarray
+    rho: np.ndarray
+    proj_domain: tuple = UNIT_DOMAIN
+    tol: float = 1e-5
+    name: str = ""
+    seed: int = 0
+
+    @property
+    def n(self) -> int:
+        return int(self.sites.shape[0])
+
+    @property
+    def pic_size(self) -> tuple:
+        """(m, n) = (pixels along x, pixels along y), the reference's pic_size."""
+        return int(self.rho.shape[1]), int(self.rho.shape[0])
+
+
+def rng_for(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def jittered_grid(n_side: int, rng: np.random.Generator, jitter: float = 0.25) -> np.ndarray:
+    """n_side^2 sites ((i + 1/2 + u) / n, (j + 1/2 + v) / n), u, v ~ U[-jitter, jitter]."""
+    j, i = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")
+    u = rng.uniform(-jitter, jitter, size=(n_side, n_side))
+    v = rng.uniform(-jitter, jitter, size=(n_side, n_side))
+    x = (i + 0.5 + u) / n_side
+    y = (j + 0.5 + v) / n_side
+    return np.stack([x.ravel(), y.ravel()], axis=1).astype(np.float64)
+
+
+def pixel_centres(res: int):
+    c = (np.arange(res) + 0.5) / res
+    return np.meshgrid(c, c, indexing="xy")  # X[iy, ix], Y[iy, ix]
+
+
+def gaussian_mixture(res: int, rng: np.random.Generator, k: int, sigma_range, amp_range,
+                     centre_range, floor: float) -> np.ndarray:
+    X, Y = pixel_centres(res)
+    img = np.full((res, res), floor, dtype=np.float64)
+    for _ in range(k):
+        cx, cy = rng.uniform(*centre_range, size=2)
+        s = rng.uniform(*sigma_range)
+        a = rng.uniform(*amp_range)
+        img += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * s * s))
+    return img / img.sum()
+
+
+def disks_and_rects(res: int, rng: np.random.Generator, count: int = 40,
+                    size_range=(0.01, 0.1), high: float = 1.0, floor: float = 0.01) -> np.ndarray:
+    """`count` shapes, each a disk (radius) or axis-aligned rectangle (two sides) drawn from
+    size_range, centres U[0,1]^2, value `high` on a `floor` background (pixel-centre inclusion)."""
+    X, Y = pixel_centres(res)
+    img = np.full((res, res), floor, dtype=np.float64)
+    for _ in range(count):
+        cx, cy = rng.uniform(0.0, 1.0, size=2)
+        if rng.uniform() < 0.5:
+            r = rng.uniform(*size_range)
+            img[(X - cx) ** 2 + (Y - cy) ** 2 <= r * r] = high
+        else:
+            wx, wy = rng.uniform(*size_range, size=2)
+            img[(np.abs(X - cx) <= 0.5 * wx) & (np.abs(Y - cy) <= 0.5 * wy)] = high
+    return img / img.sum()
+
+
+def make_problem(sites: np.ndarray, rho: np.ndarray, tol: float, name: str, seed: int) -> Problem:
+    n = sites.shape[0]
+    return Problem(sites=sites, nu=np.full(n, 1.0 / n), rho=rho, tol=tol, name=name, seed=seed)
+
+
+def plain_fp64(seed: int = 1000, n_side: int = 64, res: int = 256) -> Problem:
+    """N = 4,096 sites on a 64x64 jittered grid; 256^2 target of 3 Gaussians
+    (centres U[0.2,0.8]^2, sigma U[0.05,0.15], amplitude U[0.5,1]) on a 0.05 floor; tol 1e-4."""
+    rng = rng_for(seed)
+    sites = jittered_grid(n_side, rng)
+    rho = gaussian_mixture(res, rng, 3, (0.05, 0.15), (0.5, 1.0), (0.2, 0.8), 0.05)
+    return make_problem(sites, rho, 1e-4, f"gauss3_n{n_side * n_side}_{res}px", seed)
+
+
+def large_shapes(seed: int = 2000, n_side: int = 256, res: int = 1024) -> Problem:
+    """N = 65,536 sites on a 256x256 jittered grid; 1024^2 target of 40 disks / rectangles at 1.0
+    on a 0.01 floor (about 100:1 contrast); tol 1e-5."""
+    rng = rng_for(seed)
+    sites = jittered_grid(n_side, rng)
+    rho = disks_and_rects(res, rng)
+    return make_problem(sites, rho, 1e-5, f"shapes40_n{n_side * n_side}_{res}px", seed)
+
+
+def batched_uneven(seed: int = 3000, count: int = 64, res: int = 512,
+                   n_range=(2048, 32768)) -> list:
+    """`count` problems, N_k = round(sqrt(exp(U[ln 2048, ln 32768])))^2 sites each, 512^2 targets
+    of K ~ U{1..8} Gaussians (sigma U[0.02,0.2], amplitude U[0.5,1], centres U[0.1,0.9]^2) on a
+    0.02 floor; tol 1e-5 each."""
+    out = []
+    for k in range(count):
+        rng = rng_for(seed + k)
+        n_side = int(round(math.sqrt(math.exp(rng.uniform(math.log(n_range[0]),
+                                                          math.log(n_range[1]))))))
+        sites = jittered_grid(n_side, rng)
+        kk = int(rng.integers(1, 9))
+        rho = gaussian_mixture(res, rng, kk, (0.02, 0.2), (0.5, 1.0), (0.1, 0.9), 0.02)
+        out.append(make_problem(sites, rho, 1e-5, f"batch{k}_n{n_side * n_side}", seed + k))
+    return out
+
+
+def warm_start_sample(seed: int, n_side: int = 128, res: int = 512) -> Problem:
+    """A shapes-style problem on a 128x128 jittered grid (N = 16,384), for warm-start data."""
+    rng = rng_for(seed)
+    sites = jittered_grid(n_side, rng)
+    rho = disks_and_rects(res, rng)
+    return make_problem(sites, rho, 1e-5, f"warm_n{n_side * n_side}_{res}px_s{seed}", seed)
+
+
+def box_downsample(rho: np.ndarray, out: int) -> np.ndarray:
+    """Box-filter a square image to out x out (sum-preserving)."""
+    f = rho.shape[0] // out
+    return rho.reshape(out, f, out, f).sum(axis=(1, 3))

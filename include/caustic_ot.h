@@ -1,0 +1,132 @@
+/* caustic_ot.h -- C ABI of the MI355X semi-discrete optimal-transport dual solver.
+ *
+ * One context holds a batch of OT problems (one or many, uneven sizes) resident on one GPU:
+ * sites, source masses nu, a piecewise-constant target image rho per problem, and the solver's
+ * device state.  Every pointer argument is a caller-owned HOST buffer in the caller's site order
+ * (the library keeps its own bucket-sorted order internally).  Calls return COT_OK or a status
+ * code and never abort; cot_last_error() describes the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (rmk523/caustic, Optimal_Transport_rl/):
+ *   cot_create / cot_destroy   the device buffers direct_ot allocates and frees per call
+ *                              (ot_op.cuh:102-152, 548-588)
+ *   cot_evaluate               polygons_in_VD + compute_inte + compute_subtract + compute_f
+ *                              (power_diagram.cpp:8-134; basic_function.cuh:743-952, 988-995)
+ *   cot_solve                  the L-BFGS loop of direct_ot (ot_op.cuh:66-537)
+ *   cot_centroids              compute_center, weighted or geometric, with if_exist
+ *                              (basic_function.cuh:1056-1191; ot_op.cuh:539-546)
+ *   cot_cell_areas             compute_clip_area, the source-mass step (basic_function.cuh:610-633)
+ */
+#ifndef CAUSTIC_OT_H_
+#define CAUSTIC_OT_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  COT_OK = 0,
+  COT_EINVAL = 1,      /* bad argument */
+  COT_ENOMEM = 2,      /* device allocation failed */
+  COT_EHIP = 3,        /* HIP runtime error */
+  COT_ENOTCONV = 4,    /* iteration limit reached before the stopping rule held */
+  COT_ECOMPLEX = 5,    /* a cell exceeded the vertex limit at every trial step */
+  COT_EUNVERIFIED = 6  /* cells still failed the conformity check at the largest ring */
+};
+
+enum { COT_PREC_FP64 = 0, COT_PREC_MIXED = 1 };
+enum { COT_STOP_REFERENCE = 0, COT_STOP_MAXREL = 1 };
+
+typedef struct cot_context cot_context;
+
+/* One problem.  rho is mass per pixel in the reference layout rho[ix + iy * pm], pm pixels
+ * along x and pn along y over proj_domain = {x0, x1, y0, y1} (basic_function.cuh:145-204,
+ * 758-760).  sites are n (x, y) pairs, nu n source masses. */
+typedef struct {
+  int n_sites;
+  const double* sites;
+  const double* nu;
+  const double* rho;
+  int pm, pn;
+  double proj_domain[4];
+} cot_problem;
+
+typedef struct {
+  int precision;            /* COT_PREC_FP64: FP64 image; COT_PREC_MIXED: FP32 image */
+  double sites_per_bucket;  /* bucket grid density (default 1.0) */
+  int ring_init;            /* first candidate ring radius (default 2) */
+  int ring_max;             /* largest ring radius (default 32) */
+  double verify_tol;        /* edge-match tolerance relative to the site spacing (default 1e-4) */
+  int max_repairs;          /* conformity repair rounds per evaluation (default 12) */
+} cot_options;
+
+typedef struct {
+  int history;              /* L-BFGS memory m (history_size) */
+  int max_iter;             /* total_iter */
+  double tol;               /* convergence_tolerance */
+  double initial_step;      /* initial_step */
+  int stop_rule;            /* COT_STOP_REFERENCE: |g|_2 < tol with no empty cell (ot_op.cuh:510);
+                               COT_STOP_MAXREL: max_i |mass_i - nu_i| / nu_i < tol */
+  double h0_threshold;      /* H0 = y.s / y.y when y.y > h0_threshold, else 1 (1e-5 reference) */
+  int verbose;              /* print a line every `verbose` iterations (0: silent) */
+} cot_solve_opts;
+
+typedef struct {
+  int iterations;
+  int evaluations;
+  int converged;
+  int status;
+  double f;                 /* objective at the returned weights */
+  double grad_norm;         /* |mass - nu|_2 */
+  double max_rel_err;       /* max_i |mass_i - nu_i| / nu_i */
+  double seconds;           /* wall time of the solve */
+} cot_report;
+
+void cot_default_options(cot_options* opt);
+void cot_default_solve_opts(cot_solve_opts* opt);
+
+int cot_create(const cot_problem* problems, int n_problems, const cot_options* opt,
+               cot_context** out);
+int cot_destroy(cot_context* ctx);
+const char* cot_last_error(void);
+
+/* Total sites over the batch. */
+int cot_total_sites(const cot_context* ctx);
+
+/* Evaluate the batch at weights w (all problems' sites concatenated in caller order).  Outputs
+ * may be NULL: mass (per site), f (per problem objective), centroid (2 per site, rho-weighted,
+ * geometric when the cell holds no mass), exist (1 byte per site). */
+int cot_evaluate(cot_context* ctx, const double* w, double* mass, double* f, double* centroid,
+                 unsigned char* exist);
+
+/* Solve every problem from the weights in w (in-out).  reports: one per problem (may be NULL).
+ * Returns COT_OK when every problem met its stopping rule, COT_ENOTCONV otherwise. */
+int cot_solve(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
+
+/* Centroids of the cells at weights w: weighted != 0 -> rho-weighted (geometric fallback),
+ * weighted == 0 -> geometric centroid of the domain-clipped cell; exist per site. */
+int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centroid,
+                  unsigned char* exist);
+
+/* Area of each cell (inside proj_domain) at weights w. */
+int cot_cell_areas(cot_context* ctx, const double* w, double* area);
+
+/* Kernel time accumulated since the last reset, measured with HIP events on the library's
+ * stream around each full-batch launch of the cell kernel (construction + integration) and of
+ * the conformity check.  Repair launches (only the cells that failed) are counted separately. */
+typedef struct {
+  double cell_seconds;        /* full-batch cell-kernel time */
+  long long cell_launches;    /* full-batch cell-kernel launches */
+  double verify_seconds;      /* conformity-check time */
+  long long repair_rounds;    /* repair launches */
+  long long evaluations;      /* evaluations (trials included) */
+  long long sites_evaluated;  /* sites covered by the full-batch launches */
+  long long pixels_covered;   /* target pixels of the problems covered by those launches */
+} cot_stats;
+
+int cot_get_stats(const cot_context* ctx, cot_stats* out);
+int cot_reset_stats(cot_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAUSTIC_OT_H_ */

@@ -1,0 +1,365 @@
+"""ORACLE -- test infrastructure only.
+
+FP64 CPU restatement of the reference's semi-discrete OT dual solve (rmk523/caustic,
+Optimal_Transport_rl/), used as the checker for the HIP product path.  Only tests/,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import it.
+
+Pieces and what they follow:
+  * ``regular_neighbours`` -- the regular (weighted Delaunay) triangulation by the textbook lifting
+    map: sites lifted to (x, y, x^2 + y^2 - w); the lower convex hull (scipy's qhull) gives the
+    triangulation.  Stands in for CGAL's Regular_triangulation_2 (power_diagram.cpp:51-52),
+    which is not available here.
+  * ``power_cells`` -- restricted power cells; the C function ``orc_power_cells`` clips the box by
+    each neighbour's half-plane (semantics of power_diagram.cpp:8-134 inside proj_domain).
+  * ``evaluate`` -- compute_inte / compute_f / compute_center (basic_function.cuh:743-1191).
+  * ``lbfgs`` -- the L-BFGS driver of direct_ot (ot_op.cuh:289-537) with the reference's rules;
+    ``stop="maxrel"`` swaps only the stopping test for max_i |mass_i - nu_i| / nu_i < tol.
+  * ``read_pic_txt`` / ``format_weight`` -- text formats (basic_function.cuh:145-204,
+    main.cu:295-299).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_power_cells.restype = ctypes.c_longlong
+        L.orc_power_cells.argtypes = [ctypes.c_int, _dp, _dp, _dp, _ip, _ip, ctypes.c_longlong,
+                                      _ip, _ip, _dp]
+        L.orc_cell_integrals.restype = None
+        L.orc_cell_integrals.argtypes = [ctypes.c_int, _dp, _dp, _dp, _ip, _ip, _dp, _dp,
+                                         ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _up,
+                                         _dp, _dp]
+        L.orc_clip_area.restype = None
+        L.orc_clip_area.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.orc_render.restype = None
+        L.orc_render.argtypes = [ctypes.c_int, _dp, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int,
+                                 _dp, _dp]
+        L.orc_fresnel_rate.restype = None
+        L.orc_fresnel_rate.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_edge_green_moment.restype = ctypes.c_double
+        L.orc_edge_green_moment.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_fan_moment.restype = ctypes.c_double
+        L.orc_fan_moment.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------------------------
+# Power diagram
+# --------------------------------------------------------------------------------------------
+
+def regular_neighbours(xy: np.ndarray, w: np.ndarray):
+    """CSR neighbour lists of the regular triangulation of weighted sites (lifting map)."""
+    from scipy.spatial import ConvexHull
+
+    xy = _f64(xy)
+    w = _f64(w)
+    n = xy.shape[0]
+    # Centre and scale for qhull's conditioning; the lower hull is invariant under this
+    # similarity (weights scale with the square of the length scale, plus a gauge shift).
+    c = xy.mean(axis=0)
+    s = max(float(np.ptp(xy[:, 0])), float(np.ptp(xy[:, 1])), 1e-300)
+    p = (xy - c) / s
+    ww = (w - w.mean()) / (s * s)
+    lifted = np.column_stack([p, (p * p).sum(axis=1) - ww])
+    hull = ConvexHull(lifted, qhull_options="Qt Qbb")
+    lower = hull.equations[:, 2] < -1e-12
+    tri = hull.simplices[lower]
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    e = np.concatenate([e, e[:, ::-1]])
+    e = np.unique(e, axis=0)
+    counts = np.bincount(e[:, 0], minlength=n)
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(counts, out=ptr[1:])
+    return ptr, np.ascontiguousarray(e[:, 1], dtype=np.int32)
+
+
+def power_cells(xy, w, box, method: str = "qhull"):
+    """Restricted power cells of sites `xy` (N,2) with weights `w` in box (x0,x1,y0,y1).
+
+    Returns (start, length, verts) in the reference's flattened CCW layout."""
+    xy = _f64(xy)
+    w = _f64(w)
+    box = _f64(box)
+    n = xy.shape[0]
+    start = np.zeros(n, dtype=np.int32)
+    length = np.zeros(n, dtype=np.int32)
+    cap = 64 * n + 64
+    verts = np.zeros(2 * cap, dtype=np.float64)
+    if method == "qhull" and n >= 4:
+        ptr, idx = regular_neighbours(xy, w)
+        tot = lib().orc_power_cells(n, _d(xy), _d(w), _d(box), _i(ptr), _i(idx), cap, _i(start),
+                                    _i(length), _d(verts))
+    else:
+        tot = lib().orc_power_cells(n, _d(xy), _d(w), _d(box), None, None, cap, _i(start),
+                                    _i(length), _d(verts))
+    if tot < 0:
+        raise RuntimeError("oracle polygon buffer overflow")
+    return start, length, verts[: 2 * tot].copy()
+
+
+def cell_polygon(start, length, verts, i):
+    return verts[2 * start[i]: 2 * (start[i] + length[i])].reshape(-1, 2)
+
+
+# --------------------------------------------------------------------------------------------
+# Per-cell integrals
+# --------------------------------------------------------------------------------------------
+
+class Evaluation:
+    __slots__ = ("mass", "f_cell", "f", "centroid", "exist", "area", "geo", "max_len", "cells")
+
+
+def evaluate(xy, w, nu, rho, proj, method: str = "qhull", box=None) -> Evaluation:
+    """Masses, objective, centroids of the power diagram of (xy, w) over image rho.
+
+    rho: (H, W) mass per pixel (reference layout, x fast); proj = (x0, x1, y0, y1).
+    box: bounding box of the cells before the domain clip (default: proj)."""
+    xy = _f64(xy)
+    w = _f64(w)
+    nu = _f64(nu)
+    rho = _f64(rho)
+    proj = _f64(proj)
+    n = xy.shape[0]
+    H, W = rho.shape
+    start, length, verts = power_cells(xy, w, proj if box is None else box, method)
+    ev = Evaluation()
+    ev.mass = np.zeros(n)
+    ev.f_cell = np.zeros(n)
+    ev.centroid = np.zeros((n, 2))
+    ev.exist = np.zeros(n, dtype=np.uint8)
+    ev.area = np.zeros(n)
+    ev.geo = np.zeros((n, 2))
+    lib().orc_cell_integrals(n, _d(xy), _d(w), _d(nu), _i(start), _i(length), _d(verts),
+                             _d(rho), W, H, _d(proj), _d(ev.mass), _d(ev.f_cell),
+                             _d(ev.centroid), ev.exist.ctypes.data_as(_up), _d(ev.area),
+                             _d(ev.geo))
+    ev.f = float(np.sum(ev.f_cell))
+    ev.max_len = int(length.max()) if n else 0
+    ev.cells = (start, length, verts)
+    return ev
+
+
+def clip_area(cells, dom):
+    start, length, verts = cells
+    out = np.zeros(start.shape[0])
+    lib().orc_clip_area(start.shape[0], _i(start), _i(length), _d(verts), _d(_f64(dom)),
+                        _d(out))
+    return out
+
+
+def render(cells, nu, rho_shape, proj):
+    start, length, verts = cells
+    H, W = rho_shape
+    img = np.zeros(H * W)
+    lib().orc_render(start.shape[0], _d(_f64(nu)), _i(start), _i(length), _d(verts), W, H,
+                     _d(_f64(proj)), _d(img))
+    return img.reshape(H, W)
+
+
+def source_masses(points, sample_domain):
+    """nu_i = |Vor_i n sample_domain| / sum * area_scalar for a parallel light
+    (ot_op.cuh:90-92,158,199-206): the unweighted Voronoi cells are built in the sample domain
+    widened by 1/40 of its size, then clipped to the sample domain."""
+    sd = _f64(sample_domain)
+    xs, ys = sd[1] - sd[0], sd[3] - sd[2]
+    corner = np.array([sd[0] - xs / 40, sd[1] + xs / 40, sd[2] - ys / 40, sd[3] + ys / 40])
+    cells = power_cells(points, np.zeros(points.shape[0]), corner)
+    a = clip_area(cells, sd)
+    return a / a.sum() * (xs * ys)
+
+
+def fresnel_rate(points, in_dir, ins, height, targets, z_len, z_recv, n_idx):
+    n = points.shape[0]
+    out = np.zeros(n)
+    lib().orc_fresnel_rate(n, _d(_f64(points)), _d(_f64(in_dir)), _d(_f64(ins)),
+                           _d(_f64(height)), _d(_f64(targets)), z_len, z_recv, n_idx, _d(out))
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# L-BFGS driver (ot_op.cuh:289-537)
+# --------------------------------------------------------------------------------------------
+
+class LbfgsResult:
+    def __init__(self):
+        self.w = None
+        self.iterations = 0
+        self.evaluations = 0
+        self.converged = False
+        self.history = []  # per-iteration (f, |g|_2, maxrel, step)
+
+
+def max_rel_error(mass, nu):
+    return float(np.max(np.abs(mass - nu) / nu))
+
+
+def lbfgs(evaluate_fn, w0, nu, m=10, total_iter=50001, tol=1e-2, initial_step=1.0,
+          stop="reference", h0_threshold=1e-5, max_vertices=200, callback=None) -> LbfgsResult:
+    """evaluate_fn(w) -> (f, mass, max_len).  Rules follow ot_op.cuh:289-537:
+
+    * d = -g, two-loop recursion over <= m pairs, skipped when `restart` is set (:298-337)
+    * H0 = (y.s)/(y.y) if y.y > h0_threshold else 1; rho = 1/(y.s) unchecked (:455-463)
+    * g.d >= 0 -> d = -g, history cleared (:339-351)
+    * trials halve the step on max_len > max_vertices or f >= f_prev (:360-411)
+    * step <= 1e-4*step0 -> revert w, g, f; restart; clear history; skip the rest (:413-431)
+    * stop (reference): |g_start|_2 < tol and no zero mass at the start of the iteration (:510)
+    * stop (maxrel):    max |g|/nu < tol on the current iterate (checked after each update).
+    """
+    w = _f64(w0).copy()
+    nu = _f64(nu)
+    f, mass, _ = evaluate_fn(w)
+    g = mass - nu
+    res = LbfgsResult()
+    res.evaluations = 1
+    S = [None] * m
+    Y = [None] * m
+    rho = [0.0] * m
+    alpha = [0.0] * m
+    H0 = 1.0
+    restart = False
+    loop_iter = 0
+    if stop == "maxrel" and max_rel_error(mass, nu) < tol:
+        res.w, res.converged = w, True
+        return res
+    for it in range(total_iter):
+        res.iterations = it + 1
+        max_idx = min(loop_iter, m)
+        d = -g
+        mass_start = mass.copy()
+        g_start = g.copy()
+        if not restart:
+            for i in range(1, max_idx + 1):
+                idx = (loop_iter - i) % m
+                alpha[idx] = float(np.dot(S[idx], d)) * rho[idx]
+                d = d - alpha[idx] * Y[idx]
+            d = d * H0
+            for i in range(max_idx, 0, -1):
+                idx = (loop_iter - i) % m
+                beta = rho[idx] * float(np.dot(Y[idx], d))
+                d = d + (alpha[idx] - beta) * S[idx]
+        if float(np.dot(g, d)) >= 0:
+            d = -g
+            loop_iter = 0
+            H0 = 1.0
+        g_prev, w_prev, f_prev = g, w, f
+        step = initial_step
+        while True:
+            w_try = w_prev + step * d
+            f_try, mass_try, max_len = evaluate_fn(w_try)
+            res.evaluations += 1
+            if max_len > max_vertices:
+                step *= 0.5
+            elif f_try < f_prev:
+                w, f, mass = w_try, f_try, mass_try
+                g = mass - nu
+                restart = False
+                break
+            else:
+                step *= 0.5
+            if step <= 1e-4 * initial_step:
+                restart = True
+                g = g_prev
+                loop_iter = 0
+                H0 = 1.0
+                w = w_prev
+                f = f_prev
+                break
+        if step <= 1e-4 * initial_step:
+            continue
+        hist = loop_iter % m
+        S[hist] = step * d
+        Y[hist] = g - g_prev
+        ys = float(np.dot(Y[hist], S[hist]))
+        yy = float(np.dot(Y[hist], Y[hist]))
+        rho[hist] = 1.0 / ys
+        H0 = ys / yy if yy > h0_threshold else 1.0
+        loop_iter += 1
+        mr = max_rel_error(mass, nu)
+        res.history.append((f, float(np.linalg.norm(g_start)), mr, step))
+        if callback:
+            callback(it, w, f, mass)
+        if stop == "reference":
+            if np.linalg.norm(g_start) < tol and not np.any(mass_start == 0):
+                res.converged = True
+                break
+        else:
+            if mr < tol:
+                res.converged = True
+                break
+    res.w = w
+    return res
+
+
+def solve(problem, w0=None, m=10, stop="maxrel", tol=None, total_iter=5000, method="qhull",
+          h0_threshold=0.0):
+    """Solve a workloads.Problem on the CPU oracle (unit-square conventions)."""
+    proj = np.asarray(problem.proj_domain, dtype=np.float64)
+    nu = problem.nu
+
+    def ev(w):
+        e = evaluate(problem.sites, w, nu, problem.rho, proj, method=method)
+        return e.f, e.mass, e.max_len
+
+    w0 = np.zeros(problem.n) if w0 is None else w0
+    return lbfgs(ev, w0, nu, m=m, total_iter=total_iter,
+                 tol=problem.tol if tol is None else tol, stop=stop, h0_threshold=h0_threshold)
+
+
+# --------------------------------------------------------------------------------------------
+# Text formats
+# --------------------------------------------------------------------------------------------
+
+def read_pic_txt(path):
+    """Whitespace matrix of m rows and n columns -> (m, n, data) with data[iy*m + ix] =
+    text[m-1-ix][iy] (basic_function.cuh:145-204).  Returned as a (n, m) array A[iy, ix]."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            vals = line.split()
+            if vals:
+                rows.append([float(v) for v in vals])
+    mat = np.array(rows, dtype=np.float64)
+    m, n = mat.shape
+    # element (ix, iy) = mat[m-1-ix, iy]; A[iy, ix]
+    return mat[::-1, :].T.copy()
+
+
+def format_weight(x: float) -> str:
+    """C++ default ostream formatting of a double (precision 6, %g style)."""
+    return "%g" % x

@@ -1,0 +1,95 @@
+"""GPU parity of the HIP OT path against the FP64 CPU oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from caustic_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(problems, **kw):
+    from caustic_amd.ot import OTBatch
+
+    return OTBatch(problems, **kw)
+
+
+def _compare(ev_gpu, ev_orc, nu, mass_tol, cen_tol):
+    dm = np.abs(ev_gpu["mass"] - ev_orc.mass) / nu
+    assert dm.max() <= mass_tol, f"mass error {dm.max():.3e} (relative to nu)"
+    ex = ev_gpu["exist"]
+    assert np.array_equal(ex, ev_orc.exist.astype(bool))
+    dc = np.abs(ev_gpu["centroid"][ex] - ev_orc.centroid[ex]).max() if ex.any() else 0.0
+    assert dc <= cen_tol, f"centroid error {dc:.3e}"
+    return dm.max()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return W.plain_fp64()
+
+
+@pytest.mark.parametrize("precision,mass_tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_eval_zero_weights(gpu_available, c1, precision, mass_tol):
+    print(f"[case] eval c1 w=0 {precision}", flush=True)
+    w = np.zeros(c1.n)
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    with _batch([c1], precision=precision) as b:
+        ev = b.evaluate(w)
+    _compare(ev, orc, c1.nu, mass_tol, 1e-9 if precision == "fp64" else 1e-6)
+    assert abs(ev["f"][0] - orc.f) <= 1e-12 * abs(orc.f) + 1e-18
+    assert abs(ev["mass"].sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("precision,mass_tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_eval_random_weights_hidden_cells(gpu_available, c1, precision, mass_tol):
+    print(f"[case] eval c1 random w {precision}", flush=True)
+    rng = np.random.default_rng(7)
+    w = rng.normal(0.0, 2e-4, c1.n)
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    assert (~orc.exist.astype(bool)).sum() > 10  # the case exercises hidden cells
+    with _batch([c1], precision=precision) as b:
+        ev = b.evaluate(w)
+    _compare(ev, orc, c1.nu, mass_tol, 1e-9 if precision == "fp64" else 1e-6)
+    rel_f = abs(ev["f"][0] - orc.f) / abs(orc.f)
+    assert rel_f <= (1e-12 if precision == "fp64" else 1e-6)
+
+
+def test_solve_c1_maxrel(gpu_available, c1):
+    print("[case] solve c1 fp64 maxrel 1e-4", flush=True)
+    with _batch([c1], precision="fp64") as b:
+        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000)
+    assert rep[0]["converged"] == 1, rep
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    mr = O.max_rel_error(orc.mass, c1.nu)
+    assert mr < 1.0001e-4, mr
+    assert abs(w.mean()) < 1e-12  # sum(g) = 0 keeps the gauge sum(w) = 0
+
+
+def test_batch_matches_single(gpu_available, c1):
+    print("[case] batch of 3 vs single", flush=True)
+    small = W.plain_fp64(seed=11, n_side=24, res=96)
+    other = W.plain_fp64(seed=12, n_side=40, res=128)
+    rng = np.random.default_rng(3)
+    ws = [rng.normal(0, 1e-4, p.n) for p in (small, c1, other)]
+    with _batch([small, c1, other]) as b:
+        ev = b.evaluate(np.concatenate(ws))
+    for k, p in enumerate((small, c1, other)):
+        with _batch([p]) as s:
+            ev1 = s.evaluate(ws[k])
+        sl = slice(int(b.offsets[k]), int(b.offsets[k + 1]))
+        np.testing.assert_allclose(ev["mass"][sl], ev1["mass"], rtol=0, atol=1e-15)
+        assert abs(ev["f"][k] - ev1["f"][0]) <= 1e-14 * abs(ev1["f"][0])
+
+
+def test_eval_large_shapes_full_size(gpu_available):
+    """N = 65,536 sites, 1024^2 target of disks/rectangles at 100:1 contrast."""
+    print("[case] eval c2 full size", flush=True)
+    p = W.large_shapes()
+    rng = np.random.default_rng(5)
+    w = rng.normal(0.0, 2e-5, p.n)
+    orc = O.evaluate(p.sites, w, p.nu, p.rho, p.proj_domain)
+    for prec, tol in (("fp64", 1e-9), ("mixed", 1e-6)):
+        with _batch([p], precision=prec) as b:
+            ev = b.evaluate(w)
+        _compare(ev, orc, p.nu, tol, 1e-8 if prec == "fp64" else 1e-6)
