@@ -39,7 +39,9 @@ class cot_options(ctypes.Structure):
 class cot_solve_opts(ctypes.Structure):
     _fields_ = [("history", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
                 ("initial_step", ctypes.c_double), ("stop_rule", ctypes.c_int),
-                ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int)]
+                ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int),
+                ("device_loop", ctypes.c_int), ("accept_rule", ctypes.c_int),
+                ("repair_rounds", ctypes.c_int)]
 
 
 class cot_report(ctypes.Structure):

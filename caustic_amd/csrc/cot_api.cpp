@@ -23,10 +23,10 @@
 #include "cot_internal.h"
 
 namespace cot {
-hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32_image, hipStream_t st);
+hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             int nblocks, hipStream_t st);
+                             float* bhalf, const uint8_t* boxhit, int nblocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
                              int rmax, int nblocks, hipStream_t st);
 hipError_t launch_reduce_eval(const BlockDesc* b, const double* mass, const double* fcell,
@@ -44,6 +44,29 @@ hipError_t launch_accept(const BlockDesc* b, const uint8_t* active, const double
                          double* part, int nblocks, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
 hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t st);
+hipError_t launch_dev_count(const DevState* st, int np, int* out, hipStream_t stream);
+hipError_t launch_dev_direction(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
+                                const SolveParams& sp, const double* S, const double* Y,
+                                size_t ld, const double* g, double* d, double* part,
+                                size_t pstride, int* counter, uint8_t* active, int nblk,
+                                hipStream_t stream);
+hipError_t launch_dev_decide(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
+                             const SolveParams& sp, const double* mass, const double* fcell,
+                             const double* nu, const double* g, const double* d, const int* fails,
+                             int* overflow, double* part, size_t pstride, int* counter,
+                             uint8_t* active, int* fail_counters, int n_fail_counters, int np,
+                             int nblk, hipStream_t stream);
+hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
+                             const SolveParams& sp, const double* mass, const double* nu,
+                             const double* d, double* w, double* g, double* S, double* Y,
+                             size_t ld, double* part, size_t pstride, int* counter,
+                             uint8_t* active, int nblk, hipStream_t stream);
+hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
+                                 uint8_t* redo, uint8_t* ring, int ring_max, float* bhalf,
+                                 const uint8_t* boxhit, int nblk, hipStream_t stream);
+hipError_t launch_dev_adapt_ring(const BlockDesc* blocks, const DevState* st,
+                                 const int* final_fails, const uint8_t* used, uint8_t* ring,
+                                 int ring_min, int ring_max, int nblk, hipStream_t stream);
 constexpr int kEvalPartials = 8;
 constexpr int kAcceptPartials = 5 * kMaxHist;
 constexpr int kCoefWidth = 1 + 2 * kMaxHist;
@@ -99,7 +122,7 @@ struct cot_context {
   hipStream_t st = nullptr;
   cot_options opt{};
   int np = 0, ntot = 0, nblk = 0;
-  bool fp32_img = false;
+  bool fp32 = false;
   std::vector<ProbDesc> probs;
   std::vector<BlockDesc> blocks;
   std::vector<int> perm;  // sorted global index -> caller global index
@@ -116,6 +139,9 @@ struct cot_context {
   double2 *d_cen = nullptr, *d_geo = nullptr;
   uint8_t *d_exist = nullptr, *d_nedge = nullptr, *d_ring = nullptr, *d_used = nullptr;
   uint8_t *d_redo = nullptr, *d_active = nullptr;
+  double2* d_org = nullptr;      // FP32 geometry: clipping frame origins
+  float* d_bhalf = nullptr;      // FP32 geometry: artificial box half-sizes
+  uint8_t* d_boxhit = nullptr;   // FP32 geometry: box reached the final polygon
   int* d_elab = nullptr;
   float2* d_espan = nullptr;
   int *d_overflow = nullptr, *d_fails = nullptr;
@@ -124,6 +150,16 @@ struct cot_context {
   int hist_alloc = 0;
   double* d_coef = nullptr;
   int *d_slot = nullptr, *d_hcount = nullptr;
+  // device-resident solver
+  DevState* d_state = nullptr;
+  int* d_cnt = nullptr;        // last-block counters: direction, decide, accept (np each)
+  double* d_pdir = nullptr;    // direction partials  [1][nblk]
+  double* d_peval = nullptr;   // decide partials     [5][nblk]
+  double* d_pacc = nullptr;    // accept partials     [5K][nblk]
+  int* d_rfails = nullptr;     // per-round conformity failures [(R+1)][np]
+  int rfail_rounds = 0;
+  int* d_ndone = nullptr;
+  int* h_ndone = nullptr;      // pinned, 2 slots
   // pinned staging
   double* h_part = nullptr;
   int* h_counts = nullptr;  // fails[np], overflow[np]
@@ -145,6 +181,7 @@ struct cot_context {
     if (h_coef) hipHostFree(h_coef);
     if (h_ismall) hipHostFree(h_ismall);
     if (h_active) hipHostFree(h_active);
+    if (h_ndone) hipHostFree(h_ndone);
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     if (st) hipStreamDestroy(st);
@@ -257,7 +294,11 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.buckets = c->d_buckets;
   ca.img = c->d_img;
   ca.ring = c->d_ring;
-  ca.redo = nullptr;
+  ca.redo = c->d_redo;
+  ca.redo_mode = 0;
+  ca.st = nullptr;
+  ca.g = c->d_g;
+  ca.gate = nullptr;
   ca.active = c->d_active;
   ca.mass = c->d_mass;
   ca.fcell = c->d_fcell;
@@ -270,6 +311,9 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.espan = c->d_espan;
   ca.ring_used = c->d_used;
   ca.overflow = c->d_overflow;
+  ca.org = c->d_org;
+  ca.bhalf = c->d_bhalf;
+  ca.boxhit = c->d_boxhit;
 
   VerifyArgs va{};
   va.probs = c->d_probs;
@@ -279,12 +323,14 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   va.elab = c->d_elab;
   va.espan = c->d_espan;
   va.active = c->d_active;
+  va.gate = nullptr;
   va.redo = c->d_redo;
   va.fails = c->d_fails;
   va.tol_rel = c->opt.verify_tol;
+  va.boxhit = c->fp32 ? c->d_boxhit : nullptr;
 
   HIPCHK(hipEventRecord(c->ev[0], st));
-  HIPCHK(launch_cell(ca, c->nblk, c->fp32_img, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
   HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_verify(va, c->nblk, st));
   HIPCHK(hipEventRecord(c->ev[2], st));
@@ -314,9 +360,10 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
     for (int p = 0; p < c->np; ++p) tot += active[p] ? c->h_counts[p] : 0;
     if (tot == 0 || rounds >= c->opt.max_repairs) break;
     ++rounds;
-    HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max, c->nblk, st));
-    ca.redo = c->d_redo;
-    HIPCHK(launch_cell(ca, c->nblk, c->fp32_img, st));
+    HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max,
+                             c->fp32 ? c->d_bhalf : nullptr, c->d_boxhit, c->nblk, st));
+    ca.redo_mode = 1;
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
     HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
     HIPCHK(launch_verify(va, c->nblk, st));
@@ -409,6 +456,228 @@ int two_loop(LState& L, double* coef) {
   return L.restart ? 0 : maxidx;
 }
 
+int ensure_device_solver(cot_context* c, int repair_rounds) {
+  if (!c->d_state) {
+    CHK(dalloc(c, &c->d_state, c->np));
+    CHK(dalloc(c, &c->d_cnt, 3 * (size_t)c->np));
+    CHK(dalloc(c, &c->d_pdir, c->nblk));
+    CHK(dalloc(c, &c->d_peval, 5 * (size_t)c->nblk));
+    CHK(dalloc(c, &c->d_pacc, (size_t)kAcceptPartials * c->nblk));
+    CHK(dalloc(c, &c->d_ndone, 2));
+    if (hipHostMalloc((void**)&c->h_ndone, 2 * sizeof(int)) != hipSuccess)
+      return fail(COT_ENOMEM, "hipHostMalloc failed");
+    HIPCHK(hipMemset(c->d_cnt, 0, 3 * sizeof(int) * c->np));
+  }
+  if (c->rfail_rounds < repair_rounds + 1) {
+    CHK(dalloc(c, &c->d_rfails, (size_t)(repair_rounds + 1) * c->np));
+    c->rfail_rounds = repair_rounds + 1;
+  }
+  HIPCHK(hipMemset(c->d_rfails, 0, sizeof(int) * c->rfail_rounds * c->np));
+  return COT_OK;
+}
+
+// One device round (see cot_solver.hip).  Captured once per solve into a hipGraph.
+int launch_round(cot_context* c, const SolveParams& sp, int R, hipEvent_t t0, hipEvent_t t1) {
+  hipStream_t st = c->st;
+  const size_t ld = c->ntot, ps = c->nblk;
+  const int np = c->np;
+  HIPCHK(launch_dev_direction(c->d_probs, c->d_blocks, c->d_state, sp, c->d_S, c->d_Y, ld, c->d_g,
+                              c->d_dir, c->d_pdir, ps, c->d_cnt, c->d_active, c->nblk, st));
+  CellArgs ca{};
+  ca.probs = c->d_probs;
+  ca.blocks = c->d_blocks;
+  ca.xy = c->d_xy;
+  ca.w = c->d_w;
+  ca.dir = c->d_dir;
+  ca.step = nullptr;
+  ca.nu = c->d_nu;
+  ca.buckets = c->d_buckets;
+  ca.img = c->d_img;
+  ca.ring = c->d_ring;
+  ca.redo = c->d_redo;
+  ca.redo_mode = 0;
+  ca.active = c->d_active;
+  ca.st = c->d_state;
+  ca.g = c->d_g;
+  ca.gate = nullptr;
+  ca.mass = c->d_mass;
+  ca.fcell = c->d_fcell;
+  ca.area = c->d_area;
+  ca.cen = c->d_cen;
+  ca.geo = c->d_geo;
+  ca.exist = c->d_exist;
+  ca.nedge = c->d_nedge;
+  ca.elab = c->d_elab;
+  ca.espan = c->d_espan;
+  ca.ring_used = c->d_used;
+  ca.overflow = c->d_overflow;
+  ca.org = c->d_org;
+  ca.bhalf = c->d_bhalf;
+  ca.boxhit = c->d_boxhit;
+  VerifyArgs va{};
+  va.probs = c->d_probs;
+  va.blocks = c->d_blocks;
+  va.xy = c->d_xy;
+  va.nedge = c->d_nedge;
+  va.elab = c->d_elab;
+  va.espan = c->d_espan;
+  va.active = c->d_active;
+  va.gate = nullptr;
+  va.boxhit = c->fp32 ? c->d_boxhit : nullptr;
+  va.redo = c->d_redo;
+  va.fails = c->d_rfails;
+  va.tol_rel = c->opt.verify_tol;
+  if (t0) HIPCHK(hipEventRecord(t0, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+  if (t1) HIPCHK(hipEventRecord(t1, st));
+  HIPCHK(launch_verify(va, c->nblk, st));
+  for (int r = 1; r <= R; ++r) {
+    const int* gate = c->d_rfails + (size_t)(r - 1) * np;
+    HIPCHK(launch_dev_raise_ring(c->d_blocks, c->d_state, gate, c->d_redo, c->d_ring,
+                                 c->opt.ring_max, c->fp32 ? c->d_bhalf : nullptr, c->d_boxhit,
+                                 c->nblk, st));
+    ca.redo_mode = 1;
+    ca.gate = gate;
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+    va.gate = gate;
+    va.fails = c->d_rfails + (size_t)r * np;
+    HIPCHK(launch_verify(va, c->nblk, st));
+  }
+  const int* final_fails = c->d_rfails + (size_t)R * np;
+  HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
+                               c->opt.ring_max, c->nblk, st));
+  HIPCHK(launch_dev_decide(c->d_probs, c->d_blocks, c->d_state, sp, c->d_mass, c->d_fcell,
+                           c->d_nu, c->d_g, c->d_dir, final_fails, c->d_overflow, c->d_peval, ps,
+                           c->d_cnt + np, c->d_active, c->d_rfails, R + 1, np, c->nblk, st));
+  HIPCHK(launch_dev_accept(c->d_probs, c->d_blocks, c->d_state, sp, c->d_mass, c->d_nu,
+                           c->d_dir, c->d_w, c->d_g, c->d_S, c->d_Y, ld, c->d_pacc, ps,
+                           c->d_cnt + 2 * np, c->d_active, c->nblk, st));
+  return COT_OK;
+}
+
+// Device-resident solve: the host only replays the round graph and polls a finished count.
+int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report* reports) {
+  const auto t_start = std::chrono::steady_clock::now();
+  const int np = c->np;
+  const int R = o.repair_rounds;
+  CHK(ensure_history(c, o.history));
+  CHK(ensure_device_solver(c, R));
+  CHK(upload_sorted(c, w, c->d_w));
+
+  // initial evaluation (host path, with its own conformity repairs)
+  std::vector<uint8_t> act(np, 1);
+  std::vector<EvalSum> cur;
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, c->nblk, c->st));
+  std::vector<DevState> hs(np);
+  for (int p = 0; p < np; ++p) {
+    DevState& s = hs[p];
+    std::memset(&s, 0, sizeof(DevState));
+    s.H0 = 1.0;
+    s.t = o.initial_step;
+    s.f_cur = cur[p].f;
+    s.g2_cur = cur[p].g2;
+    s.zeros_cur = cur[p].zeros;
+    s.maxrel_cur = cur[p].maxrel;
+    s.evals = 1;
+    s.coef[0] = 1.0;  // first direction: -g
+    s.phase = kPhaseDir;
+    if (cur[p].fails) s.status = COT_EUNVERIFIED;
+    if (o.stop_rule == COT_STOP_MAXREL && cur[p].maxrel < o.tol) {
+      s.conv = 1;
+      s.phase = kPhaseDone;
+    }
+    c->h_active[p] = 0;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_state, hs.data(), sizeof(DevState) * np, hipMemcpyHostToDevice,
+                        c->st));
+  HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, np, hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_fill_i32(c->d_fails, 2 * (size_t)np, 0, c->st));
+
+  SolveParams sp{};
+  sp.m = o.history;
+  sp.max_iter = o.max_iter;
+  sp.stop_rule = o.stop_rule == COT_STOP_MAXREL ? 1 : 0;
+  sp.mixed_ls = o.accept_rule;
+  sp.ring_max = c->opt.ring_max;
+  sp.max_unverified = 4;
+  sp.tol = o.tol;
+  sp.t0 = o.initial_step;
+  sp.h0_thr = o.h0_threshold;
+
+  // capture one round
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int rc = launch_round(c, sp, R, c->ev[0], c->ev[1]);
+  hipError_t ce = hipStreamEndCapture(c->st, &graph);
+  if (rc != COT_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIPCHK(ce);
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+
+  // replay in batches of rounds, then read the finished count
+  const int kBatch = 8;
+  const long long max_rounds = (long long)(o.max_iter + 1) * 20 + 64;
+  long long rounds = 0;
+  for (;;) {
+    for (int k = 0; k < kBatch; ++k) HIPCHK(hipGraphLaunch(exec, c->st));
+    rounds += kBatch;
+    HIPCHK(launch_dev_count(c->d_state, np, c->d_ndone, c->st));
+    HIPCHK(hipMemcpyAsync(c->h_ndone, c->d_ndone, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    const int left = c->h_ndone[0];
+    // live kernel timing from the event nodes around the full cell launch of the batch's last
+    // round (only while that round still had work)
+    float ms = 0.f;
+    if (left > 0 && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+      c->stats.cell_seconds += ms * 1e-3;
+      c->stats.cell_launches += 1;
+    }
+    if (left == 0 || rounds > max_rounds) break;
+  }
+  HIPCHK(hipMemcpy(hs.data(), c->d_state, sizeof(DevState) * np, hipMemcpyDeviceToHost));
+  CHK(download_sorted(c, c->d_w, w));
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  int result = COT_OK;
+  for (int p = 0; p < np; ++p) {
+    const DevState& s = hs[p];
+    int status = s.status;
+    if (!s.conv && status == COT_OK) status = COT_ENOTCONV;
+    if (status != COT_OK && result == COT_OK) result = status;
+    c->stats.evaluations += s.evals;
+    if (reports) {
+      cot_report& r = reports[p];
+      r.iterations = s.iters;
+      r.evaluations = s.evals;
+      r.converged = s.conv;
+      r.status = status;
+      r.f = s.f_cur;
+      r.grad_norm = std::sqrt(s.g2_cur);
+      r.max_rel_err = s.maxrel_cur;
+      r.seconds = secs;
+    }
+    if (o.verbose > 0)
+      std::printf("[cot] prob %d: %s after %d iterations (%d evaluations), maxrel %.3e\n", p,
+                  s.conv ? "converged" : "stopped", s.iters, s.evals, s.maxrel_cur);
+  }
+  if (result == COT_ENOTCONV) g_err = "iteration limit reached";
+  if (result == COT_EUNVERIFIED) g_err = "cells failed the conformity check at the largest ring";
+  return result;
+}
+
 }  // namespace
 
 extern "C" {
@@ -430,6 +699,9 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->stop_rule = COT_STOP_MAXREL;
   o->h0_threshold = 1e-5;
   o->verbose = 0;
+  o->device_loop = 1;
+  o->accept_rule = 0;
+  o->repair_rounds = 2;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -462,7 +734,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!(c->opt.sites_per_bucket > 0)) c->opt.sites_per_bucket = 1.0;
   if (!(c->opt.verify_tol > 0)) c->opt.verify_tol = 1e-4;
   if (c->opt.max_repairs < 0) c->opt.max_repairs = 0;
-  c->fp32_img = c->opt.precision == COT_PREC_MIXED;
+  c->fp32 = c->opt.precision == COT_PREC_MIXED;
   c->np = n_problems;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(COT_EHIP, "hipStreamCreate failed"));
@@ -533,11 +805,13 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_overflow, c->np)) || (rc = dalloc(c, &c->d_fails, 2 * c->np)) ||
       (rc = dalloc(c, &c->d_part, (size_t)c->nblk * std::max(kEvalPartials, kAcceptPartials))) ||
       (rc = dalloc(c, &c->d_coef, (size_t)c->np * kCoefWidth)) ||
-      (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)))
+      (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
+      (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_bhalf, N)) ||
+      (rc = dalloc(c, &c->d_boxhit, N)))
     return bail(rc);
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
-  if (c->fp32_img) {
+  if (c->fp32) {
     float* im = nullptr;
     if ((rc = dalloc(c, &im, img_total))) return bail(rc);
     c->d_img = im;
@@ -568,12 +842,22 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   for (int p = 0; p < n_problems && ok; ++p) {
     const cot_problem& pr = problems[p];
     const size_t P = (size_t)pr.pm * pr.pn;
-    if (c->fp32_img) {
+    if (c->fp32) {
       std::vector<float> tmp(pr.rho, pr.rho + P);
       ok = up((float*)c->d_img + c->probs[p].img_off, tmp.data(), sizeof(float) * P);
     } else {
       ok = up((double*)c->d_img + c->probs[p].img_off, pr.rho, sizeof(double) * P);
     }
+  }
+  {
+    // FP32 frames start at the sites with a box of two buckets
+    std::vector<float> bh(N);
+    for (int p = 0; p < n_problems; ++p)
+      for (int r = 0; r < c->probs[p].n; ++r)
+        bh[c->probs[p].site_off + r] = (float)(2.0 * c->probs[p].bsize);
+    ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N) &&
+         up(c->d_bhalf, bh.data(), sizeof(float) * N) &&
+         hipMemset(c->d_boxhit, 0, N) == hipSuccess;
   }
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
@@ -650,6 +934,9 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
   }
   if (o.history < 1 || o.history > kMaxHist)
     return fail(COT_EINVAL, "history must be in [1, " + std::to_string(kMaxHist) + "]");
+  if (o.repair_rounds < 1) o.repair_rounds = 1;
+  if (o.repair_rounds > 8) o.repair_rounds = 8;
+  if (o.device_loop) return solve_device(c, w, o, reports);
   const auto t_start = std::chrono::steady_clock::now();
   const int np = c->np;
   const size_t ld = c->ntot;
@@ -769,7 +1056,7 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
       if (trial[p].fails) status[p] = COT_EUNVERIFIED;
       if (trial[p].overflow > 0) {
         t[p] *= 0.5;
-      } else if (trial[p].f < f_prev[p]) {
+      } else if (trial[p].f < f_prev[p] || (o.accept_rule && trial[p].gd <= 0.0)) {
         ok = true;
       } else {
         t[p] *= 0.5;

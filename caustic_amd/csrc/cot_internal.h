@@ -14,8 +14,9 @@ constexpr int kCellBlock = 128;
 constexpr int kVecBlock = 256;
 // Largest L-BFGS history.
 constexpr int kMaxHist = 16;
-// Edge labels below zero: the four sides of the target rectangle.
-constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4;
+// Edge labels below zero: the four sides of the target rectangle, and the artificial box that
+// bounds an FP32 cell before its constraints are applied.
+constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4, kSideBox = -5;
 
 // One OT problem of a batch.  All per-site arrays are indexed by global sorted site index.
 struct ProbDesc {
@@ -48,8 +49,12 @@ struct CellArgs {
   const int2* buckets;      // (start, count) per bucket
   const void* img;          // float or double image, per-problem offsets
   uint8_t* ring;            // per-site candidate ring radius (read, raised on repair)
-  const uint8_t* redo;      // repair pass: only sites with redo[i] != 0 run (nullptr: all)
+  uint8_t* redo;            // repair flags; every processed site's flag is cleared
+  int redo_mode;            // 0: all sites of active problems; 1: only sites with redo[i] != 0
   const uint8_t* active;    // per-problem: evaluate only problems with active[p] != 0
+  const struct DevState* st;  // device-driven solve: step, direction sign, activity per problem
+  const double* g;          // gradient (steepest-descent trials)
+  const int* gate;          // repair pass: skip problems whose counter is zero (nullptr: none)
   // outputs
   double* mass;
   double* fcell;
@@ -62,6 +67,39 @@ struct CellArgs {
   float2* espan;            // [site * kMaxVerts + k]: edge span along the canonical bisector axis
   uint8_t* ring_used;       // outermost ring that cut the cell
   int* overflow;            // per-problem count of cells that exceeded kMaxVerts
+  // FP32 geometry only: clipping frame origin (last geometric centroid) and half-size of the
+  // artificial box around it; boxhit marks cells whose final polygon still touches the box.
+  double2* org;
+  float* bhalf;
+  uint8_t* boxhit;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Device-resident L-BFGS (one state per problem, updated by the last block of each kernel)
+// ---------------------------------------------------------------------------------------------
+enum : int { kPhaseDir = 0, kPhaseTrial = 1, kPhaseDone = 2 };
+
+struct SolveParams {
+  int m;             // history size
+  int max_iter;      // iteration limit
+  int stop_rule;     // 0: reference (lagged |g|_2 < tol, no empty cell), 1: max relative error
+  int mixed_ls;      // accept a trial on f decrease OR non-positive directional derivative
+  int ring_max;
+  int max_unverified;  // consecutive trials left unverified before giving up
+  double tol;
+  double t0;         // initial step
+  double h0_thr;     // H0 = y.s / y.y only when y.y > h0_thr
+};
+
+struct DevState {
+  int phase, loop_iter, restart, iters, evals, conv, status, steep, accept, hcount;
+  int unverified, pad;
+  double t, f_cur, f_prev, g2_cur, zeros_cur, maxrel_cur, snap_g, snap_zero, H0;
+  double f_try, g2_try, zeros_try, maxrel_try, gd_try;
+  double SY[kMaxHist][kMaxHist];
+  double YY[kMaxHist][kMaxHist];
+  double Sg[kMaxHist], Yg[kMaxHist], rho[kMaxHist], alpha[kMaxHist];
+  double coef[1 + 2 * kMaxHist];
 };
 
 struct VerifyArgs {
@@ -72,6 +110,8 @@ struct VerifyArgs {
   const int* elab;
   const float2* espan;
   const uint8_t* active;    // per-problem mask
+  const int* gate;          // skip problems whose counter is zero (nullptr: none)
+  const uint8_t* boxhit;    // FP32 geometry: cells bounded by their artificial box (nullptr: none)
   uint8_t* redo;            // set for both cells of a mismatched edge
   int* fails;               // per-problem mismatch count
   double tol_rel;           // tolerance relative to the site spacing of the pair

@@ -271,22 +271,52 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   if (!a.active[bd.prob]) return;
+  if (a.gate && a.gate[bd.prob] == 0) return;
   const int i = bd.first + ln;
-  if (a.redo && !a.redo[i]) return;
+  if (a.redo_mode && !a.redo[i]) return;
+  if (a.redo) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
-  const double tstep = a.dir ? a.step[bd.prob] : 0.0;
+  // trial weights w + t * d, with d = -g for a steepest-descent trial
+  double tstep = 0.0;
+  bool steep = false;
+  if (a.st) {
+    tstep = a.st[bd.prob].t;
+    steep = a.st[bd.prob].steep != 0;
+  } else if (a.dir) {
+    tstep = a.step[bd.prob];
+  }
+  const bool trial = a.dir != nullptr;
+  auto wtrial = [&](int j) {
+    if (!trial) return a.w[j];
+    return a.w[j] + tstep * (steep ? -a.g[j] : a.dir[j]);
+  };
 
   const double2 s = a.xy[i];
-  const double wi = a.dir ? a.w[i] + tstep * a.dir[i] : a.w[i];
-
-  // target rectangle relative to the site, counter-clockwise; side labels
-  P.x[0][ln] = (T)(pd.x0 - s.x); P.y[0][ln] = (T)(pd.y0 - s.y); P.lab[0][ln] = kSideBottom;
-  P.x[1][ln] = (T)(pd.x1 - s.x); P.y[1][ln] = (T)(pd.y0 - s.y); P.lab[1][ln] = kSideRight;
-  P.x[2][ln] = (T)(pd.x1 - s.x); P.y[2][ln] = (T)(pd.y1 - s.y); P.lab[2][ln] = kSideTop;
-  P.x[3][ln] = (T)(pd.x0 - s.x); P.y[3][ln] = (T)(pd.y1 - s.y); P.lab[3][ln] = kSideLeft;
+  const double wi = wtrial(i);
+  constexpr bool kF32 = sizeof(T) == 4;
+  // Clipping frame.  FP64: the site, starting from the target rectangle.  FP32: the cell's last
+  // centroid, starting from a box of half-size L around it, with the rectangle's sides applied
+  // as ordinary half-planes -- coordinates then stay at the scale of the cell, whatever the
+  // distance between the cell and its site.
+  const double2 o = kF32 ? a.org[i] : s;
+  const double osx = o.x - s.x, osy = o.y - s.y;  // frame origin relative to the site
+  if (kF32) {
+    const T L = (T)a.bhalf[i];
+    P.x[0][ln] = -L; P.y[0][ln] = -L; P.lab[0][ln] = kSideBox;
+    P.x[1][ln] = L;  P.y[1][ln] = -L; P.lab[1][ln] = kSideBox;
+    P.x[2][ln] = L;  P.y[2][ln] = L;  P.lab[2][ln] = kSideBox;
+    P.x[3][ln] = -L; P.y[3][ln] = L;  P.lab[3][ln] = kSideBox;
+  } else {
+    // target rectangle relative to the site, counter-clockwise; side labels
+    P.x[0][ln] = (T)(pd.x0 - s.x); P.y[0][ln] = (T)(pd.y0 - s.y); P.lab[0][ln] = kSideBottom;
+    P.x[1][ln] = (T)(pd.x1 - s.x); P.y[1][ln] = (T)(pd.y0 - s.y); P.lab[1][ln] = kSideRight;
+    P.x[2][ln] = (T)(pd.x1 - s.x); P.y[2][ln] = (T)(pd.y1 - s.y); P.lab[2][ln] = kSideTop;
+    P.x[3][ln] = (T)(pd.x0 - s.x); P.y[3][ln] = (T)(pd.y1 - s.y); P.lab[3][ln] = kSideLeft;
+  }
   int nv = 4;
+  bool overflow = false;
 
-  // bounding circle of the polygon (centre cx, cy relative to the site; radius^2 r2)
+  // bounding circle of the polygon (centre cx, cy relative to the frame; radius^2 r2)
   auto bound = [&](double& cx, double& cy, double& r2) {
     double mnx = P.x[0][ln], mxx = mnx, mny = P.y[0][ln], mxy = mny;
     for (int k = 1; k < nv; ++k) {
@@ -304,12 +334,28 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   };
   double bcx, bcy, br2;
   bound(bcx, bcy, br2);
+  if (kF32) {
+    // the four sides of the target rectangle: -y <= o.y - y0, x <= x1 - o.x, y <= y1 - o.y,
+    // -x <= o.x - x0
+    const double sd[4][3] = {{0.0, -1.0, o.y - pd.y0},
+                             {1.0, 0.0, pd.x1 - o.x},
+                             {0.0, 1.0, pd.y1 - o.y},
+                             {-1.0, 0.0, o.x - pd.x0}};
+    const int lab[4] = {kSideBottom, kSideRight, kSideTop, kSideLeft};
+    for (int q = 0; q < 4 && nv > 0; ++q) {
+      const double slack = sd[q][2] - (bcx * sd[q][0] + bcy * sd[q][1]);
+      if (slack >= 0.0 && slack * slack >= br2) continue;
+      bool changed;
+      nv = clip_halfplane<T>(P, ln, nv, (T)sd[q][0], (T)sd[q][1], (T)sd[q][2], lab[q], overflow,
+                             changed);
+      if (changed && nv > 0) bound(bcx, bcy, br2);
+    }
+  }
 
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
   const int R = a.ring[i];
   int used = 0;
-  bool overflow = false;
   const int2* bk = a.buckets + pd.bkt_off;
 
   for (int r = 0; r <= R && nv > 0; ++r) {
@@ -324,9 +370,10 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
           const int j = pd.site_off + b.x + jj;
           if (j == i) continue;
           const double2 sj = a.xy[j];
-          const double wj = a.dir ? a.w[j] + tstep * a.dir[j] : a.w[j];
+          const double wj = wtrial(j);
           const double ddx = sj.x - s.x, ddy = sj.y - s.y;
-          const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj);
+          // half-plane u.d <= c in the clipping frame (u = x - o)
+          const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj) - (osx * ddx + osy * ddy);
           // the half-plane u.d <= c contains the bounding circle: nothing to clip
           const double slack = c - (bcx * ddx + bcy * ddy);
           if (slack >= 0.0 && slack * slack >= br2 * (ddx * ddx + ddy * ddy)) continue;
@@ -346,11 +393,13 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
 
   double m0 = 0.0, fi = -wi * a.nu[i] , ar = 0.0;
   double2 cen = make_double2(0.0, 0.0), geo = make_double2(0.0, 0.0);
+  bool boxhit = false;
   if (nv >= 3) {
     // edge records for the conformity check, geometric area/centroid, pixel frame
     double mnu = INFINITY, mnv = INFINITY;
     double A2 = 0.0, gx = 0.0, gy = 0.0;
     for (int k = 0; k < nv; ++k) {
+      boxhit |= P.lab[k][ln] == kSideBox;
       const int k1 = (k + 1 == nv) ? 0 : k + 1;
       const double xa = P.x[k][ln], ya = P.y[k][ln], xb = P.x[k1][ln], yb = P.y[k1][ln];
       const double cr = xa * yb - xb * ya;
@@ -368,8 +417,8 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
         const double inv = 1.0 / sqrt(Dx * Dx + Dy * Dy);
         const double ex = -Dy * inv, ey = Dx * inv;
         const double mx = 0.5 * (pl.x + ph.x), my = 0.5 * (pl.y + ph.y);
-        const double t0 = (s.x + xa - mx) * ex + (s.y + ya - my) * ey;
-        const double t1 = (s.x + xb - mx) * ex + (s.y + yb - my) * ey;
+        const double t0 = (o.x + xa - mx) * ex + (o.y + ya - my) * ey;
+        const double t1 = (o.x + xb - mx) * ex + (o.y + yb - my) * ey;
         span = make_float2((float)fmin(t0, t1), (float)fmax(t0, t1));
       }
       a.elab[(size_t)i * kMaxVerts + k] = lab;
@@ -377,18 +426,20 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
     }
     ar = 0.5 * A2;
     if (A2 != 0.0) {
-      geo = make_double2(s.x + gx / (3.0 * A2), s.y + gy / (3.0 * A2));
+      geo = make_double2(o.x + gx / (3.0 * A2), o.y + gy / (3.0 * A2));
     } else {
-      geo = s;
+      geo = o;
     }
 
     // Green integration in local pixel units, integer origin (kox, koy) at the cell's corner
     const double idx_ = 1.0 / pd.dx, idy_ = 1.0 / pd.dy;
-    int kox = (int)floor((s.x + mnu - pd.x0) * idx_);
-    int koy = (int)floor((s.y + mnv - pd.y0) * idy_);
+    int kox = (int)floor((o.x + mnu - pd.x0) * idx_);
+    int koy = (int)floor((o.y + mnv - pd.y0) * idy_);
     kox = min(max(kox, 0), pd.pm - 1);
     koy = min(max(koy, 0), pd.pn - 1);
-    const double ou = (s.x - pd.x0) * idx_ - (double)kox;  // site in local pixel units
+    const double fu0 = (o.x - pd.x0) * idx_ - (double)kox;  // frame origin, local pixel units
+    const double fv0 = (o.y - pd.y0) * idy_ - (double)koy;
+    const double ou = (s.x - pd.x0) * idx_ - (double)kox;  // site, local pixel units
     const double ov = (s.y - pd.y0) * idy_ - (double)koy;
     const TI* img = reinterpret_cast<const TI*>(a.img) + pd.img_off;
     RowCache<TI> rc;
@@ -396,13 +447,13 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
     rc.pm = pd.pm;
     Moments M;
     const int lim_u = pd.pm - kox, lim_v = pd.pn - koy;
-    T pu = (T)(ou + (double)P.x[0][ln] * idx_), pv = (T)(ov + (double)P.y[0][ln] * idy_);
+    T pu = (T)(fu0 + (double)P.x[0][ln] * idx_), pv = (T)(fv0 + (double)P.y[0][ln] * idy_);
     const T fu = pu, fv = pv;
     for (int k = 0; k < nv; ++k) {
       T qu, qv;
       if (k + 1 < nv) {
-        qu = (T)(ou + (double)P.x[k + 1][ln] * idx_);
-        qv = (T)(ov + (double)P.y[k + 1][ln] * idy_);
+        qu = (T)(fu0 + (double)P.x[k + 1][ln] * idx_);
+        qv = (T)(fv0 + (double)P.y[k + 1][ln] * idy_);
       } else {
         qu = fu;
         qv = fv;
@@ -423,7 +474,18 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
     } else {
       cen = geo;
     }
+    if (kF32) {
+      // next frame: this centroid, box twice the cell's radius about it (at least one bucket)
+      double r2 = 0.0;
+      for (int k = 0; k < nv; ++k) {
+        const double ex = o.x + (double)P.x[k][ln] - geo.x, ey = o.y + (double)P.y[k][ln] - geo.y;
+        r2 = fmax(r2, ex * ex + ey * ey);
+      }
+      a.org[i] = geo;
+      a.bhalf[i] = boxhit ? 2.0f * a.bhalf[i] : (float)fmax(2.0 * sqrt(r2), pd.bsize);
+    }
   }
+  if (kF32) a.boxhit[i] = boxhit ? 1 : 0;
   a.mass[i] = m0;
   a.fcell[i] = fi;
   a.area[i] = ar;
@@ -441,6 +503,7 @@ __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   if (!a.active[bd.prob]) return;
+  if (a.gate && a.gate[bd.prob] == 0) return;
   const int i = bd.first + ln;
   const int ne = a.nedge[i];
   const double2 si = a.xy[i];
@@ -468,20 +531,24 @@ __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
       a.redo[j] = 1;
     }
   }
+  if (a.boxhit && a.boxhit[i]) bad = true;
   if (bad) {
     a.redo[i] = 1;
     atomicAdd(&a.fails[bd.prob], 1);
   }
 }
 
-// Raise the ring of every cell marked for a rebuild.
+// Raise the ring of every cell marked for a rebuild; with FP32 geometry also double its box
+// (unless the cell kernel already did because the box bounded the cell).
 __global__ void raise_ring_kernel(const BlockDesc* blocks, const uint8_t* redo, uint8_t* ring,
-                                  int ring_max) {
+                                  int ring_max, float* bhalf, const uint8_t* boxhit) {
   const BlockDesc bd = blocks[blockIdx.x];
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   const int i = bd.first + ln;
-  if (redo[i]) ring[i] = (uint8_t)min((int)ring[i] + 1, ring_max);
+  if (!redo[i]) return;
+  ring[i] = (uint8_t)min((int)ring[i] + 1, ring_max);
+  if (bhalf && !boxhit[i]) bhalf[i] *= 2.0f;
 }
 
 // Next evaluation's ring: the outermost ring that shaped the cell plus one, at least ring_min.
@@ -631,9 +698,9 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 // launchers
 // ---------------------------------------------------------------------------------------------
 
-hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32_image, hipStream_t st) {
-  if (fp32_image)
-    hipLaunchKernelGGL((cell_kernel<double, float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st) {
+  if (fp32)
+    hipLaunchKernelGGL((cell_kernel<float, float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   else
     hipLaunchKernelGGL((cell_kernel<double, double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   return hipGetLastError();
@@ -645,9 +712,9 @@ hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
 }
 
 hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             int nblocks, hipStream_t st) {
+                             float* bhalf, const uint8_t* boxhit, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(raise_ring_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, redo, ring,
-                     rmax);
+                     rmax, bhalf, boxhit);
   return hipGetLastError();
 }
 

@@ -131,8 +131,12 @@ class OTBatch:
 
     def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
-              verbose: int = 0, raise_on_fail: bool = True):
-        """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports)."""
+              verbose: int = 0, raise_on_fail: bool = True, device_loop: bool = True,
+              accept_rule: str = "decrease", repair_rounds: int = 2):
+        """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
+
+        accept_rule: "decrease" (f_trial < f, the reference rule) or "derivative" (also accept
+        a trial whose directional derivative is <= 0, for FP32 geometry)."""
         w = self._w(w0).copy()
         o = L.cot_solve_opts()
         self._lib.cot_default_solve_opts(ctypes.byref(o))
@@ -143,6 +147,9 @@ class OTBatch:
         o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL}[stop]
         o.h0_threshold = h0_threshold
         o.verbose = verbose
+        o.device_loop = int(bool(device_loop))
+        o.accept_rule = {"decrease": 0, "derivative": 1}[accept_rule]
+        o.repair_rounds = repair_rounds
         reps = (L.cot_report * self.n_problems)()
         rc = self._lib.cot_solve(self._ctx, _ptr(w), ctypes.byref(o), reps)
         if raise_on_fail:

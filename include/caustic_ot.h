@@ -68,6 +68,13 @@ typedef struct {
                                COT_STOP_MAXREL: max_i |mass_i - nu_i| / nu_i < tol */
   double h0_threshold;      /* H0 = y.s / y.y when y.y > h0_threshold, else 1 (1e-5 reference) */
   int verbose;              /* print a line every `verbose` iterations (0: silent) */
+  int device_loop;          /* 1: device-resident rounds replayed from a hipGraph (default);
+                               0: host-driven rounds (one host sync per phase) */
+  int accept_rule;          /* 0: accept a trial on f decrease (ot_op.cuh:397);
+                               1: also accept when the directional derivative at the trial is
+                                  <= 0 (the convex objective decreased) -- for FP32 geometry,
+                                  whose f is too coarse to resolve the last decades */
+  int repair_rounds;        /* conformity repairs inside one device round (default 2) */
 } cot_solve_opts;
 
 typedef struct {

@@ -63,6 +63,8 @@ def lib():
         L.orc_edge_green_moment.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_fan_moment.restype = ctypes.c_double
         L.orc_fan_moment.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_set_centroid_min_piece.restype = None
+        L.orc_set_centroid_min_piece.argtypes = [ctypes.c_double]
         _lib = L
     return _lib
 
@@ -145,11 +147,15 @@ class Evaluation:
     __slots__ = ("mass", "f_cell", "f", "centroid", "exist", "area", "geo", "max_len", "cells")
 
 
-def evaluate(xy, w, nu, rho, proj, method: str = "qhull", box=None) -> Evaluation:
+def evaluate(xy, w, nu, rho, proj, method: str = "qhull", box=None,
+             centroid_min_piece: float = 1e-10) -> Evaluation:
     """Masses, objective, centroids of the power diagram of (xy, w) over image rho.
 
     rho: (H, W) mass per pixel (reference layout, x fast); proj = (x0, x1, y0, y1).
-    box: bounding box of the cells before the domain clip (default: proj)."""
+    box: bounding box of the cells before the domain clip (default: proj).
+    centroid_min_piece: pixel pieces of at most this area are left out of the rho-weighted
+    centroid (the reference uses 1e-10, basic_function.cuh:1153; 0 keeps every piece)."""
+    lib().orc_set_centroid_min_piece(float(centroid_min_piece))
     xy = _f64(xy)
     w = _f64(w)
     nu = _f64(nu)

@@ -189,9 +189,15 @@ struct PixGrid {
   }
 };
 
+// Pieces at or below this area are left out of the rho-weighted centroid
+// (basic_function.cuh:1153 uses 1e-10); 0 keeps every piece of positive area.
+double g_centroid_min_piece = 1e-10;
+
 }  // namespace
 
 extern "C" {
+
+void orc_set_centroid_min_piece(double a) { g_centroid_min_piece = a; }
 
 // Restricted power cells.  box = {x0, x1, y0, y1}.  If nbr_ptr is NULL every other site is a
 // candidate (sorted by distance, stopped by the exact weight-aware bound); otherwise the
@@ -309,7 +315,7 @@ void orc_cell_integrals(int n, const double* xy, const double* w, const double* 
                 double sq = second_moment(piece.data(), np, xy[2 * i], xy[2 * i + 1]);
                 f_acc -= rv / g.area * (sq - wi * a);
               }
-              if (a > 1e-10) {
+              if (a > g_centroid_min_piece) {
                 double lcx, lcy;
                 fan_centroid(piece.data(), np, lcx, lcy);
                 double contrib = rv * a;

@@ -14,6 +14,12 @@ def _batch(problems, **kw):
     return OTBatch(problems, **kw)
 
 
+def _exact(p, w):
+    """Oracle evaluation keeping every pixel piece in the weighted centroid (the GPU integrates
+    the centroid exactly; the reference's 1e-10 piece rule is checked separately)."""
+    return O.evaluate(p.sites, w, p.nu, p.rho, p.proj_domain, centroid_min_piece=0.0)
+
+
 def _compare(ev_gpu, ev_orc, nu, mass_tol, cen_tol):
     dm = np.abs(ev_gpu["mass"] - ev_orc.mass) / nu
     assert dm.max() <= mass_tol, f"mass error {dm.max():.3e} (relative to nu)"
@@ -33,10 +39,10 @@ def c1():
 def test_eval_zero_weights(gpu_available, c1, precision, mass_tol):
     print(f"[case] eval c1 w=0 {precision}", flush=True)
     w = np.zeros(c1.n)
-    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    orc = _exact(c1, w)
     with _batch([c1], precision=precision) as b:
         ev = b.evaluate(w)
-    _compare(ev, orc, c1.nu, mass_tol, 1e-9 if precision == "fp64" else 1e-6)
+    _compare(ev, orc, c1.nu, mass_tol, 1e-11 if precision == "fp64" else 1e-6)
     assert abs(ev["f"][0] - orc.f) <= 1e-12 * abs(orc.f) + 1e-18
     assert abs(ev["mass"].sum() - 1.0) < 1e-12
 
@@ -46,24 +52,67 @@ def test_eval_random_weights_hidden_cells(gpu_available, c1, precision, mass_tol
     print(f"[case] eval c1 random w {precision}", flush=True)
     rng = np.random.default_rng(7)
     w = rng.normal(0.0, 2e-4, c1.n)
-    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    orc = _exact(c1, w)
     assert (~orc.exist.astype(bool)).sum() > 10  # the case exercises hidden cells
     with _batch([c1], precision=precision) as b:
         ev = b.evaluate(w)
-    _compare(ev, orc, c1.nu, mass_tol, 1e-9 if precision == "fp64" else 1e-6)
+    _compare(ev, orc, c1.nu, mass_tol, 1e-11 if precision == "fp64" else 1e-6)
     rel_f = abs(ev["f"][0] - orc.f) / abs(orc.f)
     assert rel_f <= (1e-12 if precision == "fp64" else 1e-6)
 
 
-def test_solve_c1_maxrel(gpu_available, c1):
-    print("[case] solve c1 fp64 maxrel 1e-4", flush=True)
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_solve_c1_maxrel(gpu_available, c1, device_loop):
+    print(f"[case] solve c1 fp64 maxrel 1e-4 device_loop={device_loop}", flush=True)
     with _batch([c1], precision="fp64") as b:
-        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000)
+        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000,
+                         device_loop=device_loop)
     assert rep[0]["converged"] == 1, rep
     orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
     mr = O.max_rel_error(orc.mass, c1.nu)
     assert mr < 1.0001e-4, mr
     assert abs(w.mean()) < 1e-12  # sum(g) = 0 keeps the gauge sum(w) = 0
+
+
+def test_device_and_host_loops_follow_the_same_iterates(gpu_available, c1):
+    """Both drivers apply the same rules to the same FP64 evaluations; only the summation order
+    of the dot products differs, so the iterates agree to well below the weights' scale."""
+    print("[case] device vs host loop, 40 iterations", flush=True)
+    out = []
+    for dl in (False, True):
+        with _batch([c1], precision="fp64") as b:
+            w, rep = b.solve(history=10, tol=1e-12, stop="maxrel", h0_threshold=0.0,
+                             max_iter=40, device_loop=dl, raise_on_fail=False)
+        out.append((w, rep[0]))
+    assert out[0][1]["iterations"] == out[1][1]["iterations"] == 40
+    assert abs(out[0][1]["evaluations"] - out[1][1]["evaluations"]) <= 2
+    scale = np.abs(out[0][0]).max()
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-6 * scale
+
+
+def test_solve_c1_mixed_precision(gpu_available, c1):
+    print("[case] solve c1 mixed maxrel 1e-4", flush=True)
+    with _batch([c1], precision="mixed") as b:
+        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000,
+                         accept_rule="derivative")
+    assert rep[0]["converged"] == 1, rep
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    # FP32 geometry: the oracle's FP64 masses at the returned weights stay within 1e-6 of nu
+    # of the solver's own stopping value
+    assert O.max_rel_error(orc.mass, c1.nu) < 1.0001e-4 + 1e-6
+
+
+def test_solve_batch_of_uneven_problems(gpu_available):
+    print("[case] batched solve of 3 uneven problems", flush=True)
+    probs = [W.plain_fp64(seed=21, n_side=24, res=96), W.plain_fp64(seed=22, n_side=48, res=192),
+             W.plain_fp64(seed=23, n_side=32, res=128)]
+    with _batch(probs) as b:
+        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000)
+    assert all(r["converged"] for r in rep), rep
+    for k, p in enumerate(probs):
+        wk = w[int(b.offsets[k]):int(b.offsets[k + 1])]
+        orc = O.evaluate(p.sites, wk, p.nu, p.rho, p.proj_domain)
+        assert O.max_rel_error(orc.mass, p.nu) < 1.0001e-4
 
 
 def test_batch_matches_single(gpu_available, c1):
@@ -88,8 +137,9 @@ def test_eval_large_shapes_full_size(gpu_available):
     p = W.large_shapes()
     rng = np.random.default_rng(5)
     w = rng.normal(0.0, 2e-5, p.n)
-    orc = O.evaluate(p.sites, w, p.nu, p.rho, p.proj_domain)
+    orc = _exact(p, w)
     for prec, tol in (("fp64", 1e-9), ("mixed", 1e-6)):
         with _batch([p], precision=prec) as b:
             ev = b.evaluate(w)
         _compare(ev, orc, p.nu, tol, 1e-8 if prec == "fp64" else 1e-6)
+

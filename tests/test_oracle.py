@@ -190,3 +190,13 @@ def test_source_masses_of_a_lattice():
     assert nu.sum() == pytest.approx(1600.0, rel=1e-14)
     interior = (pts[:, 0] > 0) & (pts[:, 0] < 40) & (pts[:, 1] > 0) & (pts[:, 1] < 40)
     np.testing.assert_allclose(nu[interior], 16.0, rtol=1e-12)
+
+
+def test_reference_centroid_piece_rule_is_below_1e8():
+    """The reference leaves pixel pieces of area <= 1e-10 out of the weighted centroid
+    (basic_function.cuh:1153); on this workload that moves centroids by < 1e-8."""
+    c1 = W.plain_fp64()
+    a = O.evaluate(c1.sites, np.zeros(c1.n), c1.nu, c1.rho, c1.proj_domain)
+    b = O.evaluate(c1.sites, np.zeros(c1.n), c1.nu, c1.rho, c1.proj_domain,
+                   centroid_min_piece=0.0)
+    assert np.abs(a.centroid - b.centroid).max() < 1e-8
