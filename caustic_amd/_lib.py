@@ -51,6 +51,14 @@ class cot_report(ctypes.Structure):
                 ("seconds", ctypes.c_double)]
 
 
+class cot_reflection(ctypes.Structure):
+    _fields_ = [("points", _dp), ("inside_dir", _dp), ("height", _dp), ("clip_area", _dp),
+                ("z_of_len", ctypes.c_double), ("z_of_receiver", ctypes.c_double),
+                ("refraction_index", ctypes.c_double), ("area_scalar", ctypes.c_double),
+                ("aa_history_size", ctypes.c_int), ("aa_fixed_para", ctypes.c_double),
+                ("if_aa_op_initial", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); every symbol include/caustic_ot.h declares
 SIGNATURES = {
     "cot_default_options": (None, [ctypes.POINTER(cot_options)]),
@@ -63,6 +71,9 @@ SIGNATURES = {
     "cot_evaluate": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _up]),
     "cot_solve": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(cot_solve_opts),
                                  ctypes.POINTER(cot_report)]),
+    "cot_solve_reflection": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(cot_solve_opts),
+                                            ctypes.POINTER(cot_reflection),
+                                            ctypes.POINTER(cot_report)]),
     "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
     "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

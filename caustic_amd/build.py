@@ -38,7 +38,7 @@ def _run(cmd):
 
 def build_library():
     objs = []
-    for src in ("cot_kernels.hip", "cot_solver.hip", "cot_api.cpp"):
+    for src in ("cot_kernels.hip", "cot_solver.hip", "cot_reflect.hip", "cot_api.cpp"):
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         extra = ["-x", "hip"] if src.endswith(".cpp") else []
         _run(["hipcc", *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])

@@ -267,6 +267,9 @@ int main(int argc, char** argv) {
   double tol = 0, initial_step = 1.0;
   std::vector<double> sample_domain, proj_domain;
   bool full_precision = false;
+  int aa_history_size = 7;
+  double aa_fixed_para = -2.0, z_of_len = 0.0, z_of_receiver = 0.0, refraction_index = 1.5;
+  bool if_aa_op_initial = true;
   try {
     JVal J = JParser(jss.str()).parse();
     (void)J.at("test_num").as_str();
@@ -275,15 +278,15 @@ int main(int argc, char** argv) {
     total_iter = (int)sp.at("total_iter").as_num();
     tol = sp.at("convergence_tolerance").as_num();
     history_size = (int)sp.at("history_size").as_num();
-    (void)sp.at("aa_history_size").as_num();
-    (void)sp.at("aa_fixed_para").as_num();
-    (void)sp.at("if_aa_op_initial").as_bool();
+    aa_history_size = (int)sp.at("aa_history_size").as_num();
+    aa_fixed_para = sp.at("aa_fixed_para").as_num();
+    if_aa_op_initial = sp.at("if_aa_op_initial").as_bool();
     initial_step = sp.at("initial_step").as_num();
     sample_domain = sp.at("sample_domain").as_nums();
     const JVal& lp = J.at("len_para");
-    (void)lp.at("z_of_len").as_num();
-    (void)lp.at("z_of_receiver").as_num();
-    (void)lp.at("refraction_index").as_num();
+    z_of_len = lp.at("z_of_len").as_num();
+    z_of_receiver = lp.at("z_of_receiver").as_num();
+    refraction_index = lp.at("refraction_index").as_num();
     proj_domain = lp.at("proj_domain").as_nums();
     const JVal& li = J.at("light_para");
     light_type = li.at("light_type").as_str();
@@ -318,8 +321,24 @@ int main(int argc, char** argv) {
   if (N == 0 || proj_xy.size() != points.size()) return die("points/proj_xy size mismatch");
   if (light_type != "parallel_light")
     return die("light_type '" + light_type + "' is not supported by this build", 3);
-  if (ot_time != "0")
-    return die("ot_time >= 1 (reflection-aware Anderson path) is not supported by this build", 3);
+  if (inside.size() < 3 * (size_t)(points.size() / 2) || height.size() < points.size() / 2)
+    return die("inside_light_direct.txt / height.txt hold fewer rows than points.txt");
+  std::vector<double> w_start(points.size() / 2, 0.0);
+  if (ot_time != "0") {
+    // warm start from the previous run (main.cu:193-198)
+    int prev = 0;
+    try {
+      prev = std::stoi(ot_time) - 1;
+    } catch (const std::exception&) {
+      return die("ot_time must be an integer", 2);
+    }
+    const std::string wpath =
+        "../../data/weights/" + xiushi + "/" + std::to_string(prev) + "/final_proj_point_weights.txt";
+    std::vector<double> wprev;
+    if (!read_doubles(wpath, wprev) || wprev.size() != points.size() / 2)
+      return die("cannot read " + wpath);
+    w_start = wprev;
+  }
 
   const double xs = sample_domain[1] - sample_domain[0], ys = sample_domain[3] - sample_domain[2];
   const double area_scalar = xs * ys;
@@ -353,7 +372,7 @@ int main(int argc, char** argv) {
   const double gray_multiple = gray / area_scalar;
   for (double& v : picture) v /= gray_multiple;
 
-  std::vector<double> w(N, 0.0);
+  std::vector<double> w = w_start;
   cot_problem pr{};
   pr.n_sites = N;
   pr.sites = proj_xy.data();
@@ -375,7 +394,25 @@ int main(int argc, char** argv) {
   so.h0_threshold = 1e-5;
   so.verbose = 20;
   cot_report rep{};
-  int rc = cot_solve(g.c, w.data(), &so, &rep);
+  int rc;
+  if (ot_time == "0") {
+    rc = cot_solve(g.c, w.data(), &so, &rep);
+  } else {
+    // reflection-aware path: Fresnel-weighted source masses, L-BFGS then Anderson
+    cot_reflection rf{};
+    rf.points = points.data();
+    rf.inside_dir = inside.data();
+    rf.height = height.data();
+    rf.clip_area = area.data();
+    rf.z_of_len = z_of_len;
+    rf.z_of_receiver = z_of_receiver;
+    rf.refraction_index = refraction_index;
+    rf.area_scalar = area_scalar;
+    rf.aa_history_size = aa_history_size;
+    rf.aa_fixed_para = aa_fixed_para;
+    rf.if_aa_op_initial = if_aa_op_initial ? 1 : 0;
+    rc = cot_solve_reflection(g.c, w.data(), &so, &rf, &rep);
+  }
   if (rc != COT_OK && rc != COT_ENOTCONV) return die(std::string("solve: ") + cot_last_error(), 5);
   std::printf("caustic_gpu: %s after %d iterations (%d evaluations), |g| %.6e, %.2f s\n",
               rep.converged ? "converged" : "stopped", rep.iterations, rep.evaluations,

@@ -21,12 +21,13 @@
 
 #include "../../include/caustic_ot.h"
 #include "cot_internal.h"
+#include "cot_layout.h"
 
 namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             float* bhalf, const uint8_t* boxhit, int nblocks, hipStream_t st);
+                             int nblocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
                              int rmax, int nblocks, hipStream_t st);
 hipError_t launch_reduce_eval(const BlockDesc* b, const double* mass, const double* fcell,
@@ -62,8 +63,23 @@ hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, Dev
                              size_t ld, double* part, size_t pstride, int* counter,
                              uint8_t* active, int nblk, hipStream_t stream);
 hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
-                                 uint8_t* redo, uint8_t* ring, int ring_max, float* bhalf,
-                                 const uint8_t* boxhit, int nblk, hipStream_t stream);
+                                 uint8_t* redo, uint8_t* ring, int ring_max, int nblk,
+                                 hipStream_t stream);
+hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points, const double* ins,
+                                 const double* height, const double* clip_area,
+                                 const double2* centroid, double z_len, double z_recv,
+                                 double n_idx, double* weight, double* part, int nblk,
+                                 hipStream_t st);
+hipError_t launch_scale(const BlockDesc* blocks, const double* weight, double scale, double* nu,
+                        int nblk, hipStream_t st);
+hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const double* g, double beta,
+                           int slot, int H, double* G, double* R, size_t ld, double* part,
+                           int nblk, hipStream_t st);
+hipError_t launch_aa_direction(const BlockDesc* blocks, const double* alpha, int H,
+                               const double* G, size_t ld, const double* w, double* d, int nblk,
+                               hipStream_t st);
+hipError_t launch_axpy(const BlockDesc* blocks, double t, const double* d, double* w, int nblk,
+                       hipStream_t st);
 hipError_t launch_dev_adapt_ring(const BlockDesc* blocks, const DevState* st,
                                  const int* final_fails, const uint8_t* used, uint8_t* ring,
                                  int ring_min, int ring_max, int nblk, hipStream_t stream);
@@ -140,8 +156,6 @@ struct cot_context {
   uint8_t *d_exist = nullptr, *d_nedge = nullptr, *d_ring = nullptr, *d_used = nullptr;
   uint8_t *d_redo = nullptr, *d_active = nullptr;
   double2* d_org = nullptr;      // FP32 geometry: clipping frame origins
-  float* d_bhalf = nullptr;      // FP32 geometry: artificial box half-sizes
-  uint8_t* d_boxhit = nullptr;   // FP32 geometry: box reached the final polygon
   int* d_elab = nullptr;
   float2* d_espan = nullptr;
   int *d_overflow = nullptr, *d_fails = nullptr;
@@ -200,56 +214,6 @@ int dalloc(cot_context* c, T** p, size_t count) {
   return COT_OK;
 }
 
-// Bucket grid and tile-ordered permutation of one problem's sites.
-void bucket_sort(const cot_problem& pr, double spb, ProbDesc& pd, std::vector<int>& order,
-                 std::vector<int2>& buckets) {
-  const int n = pr.n_sites;
-  double xmin = pr.proj_domain[0], xmax = pr.proj_domain[1];
-  double ymin = pr.proj_domain[2], ymax = pr.proj_domain[3];
-  for (int i = 0; i < n; ++i) {
-    xmin = std::min(xmin, pr.sites[2 * i]);
-    xmax = std::max(xmax, pr.sites[2 * i]);
-    ymin = std::min(ymin, pr.sites[2 * i + 1]);
-    ymax = std::max(ymax, pr.sites[2 * i + 1]);
-  }
-  const double wdt = std::max(xmax - xmin, 1e-300), hgt = std::max(ymax - ymin, 1e-300);
-  double bs = std::sqrt(wdt * hgt * spb / std::max(n, 1));
-  int gx = std::max(1, (int)std::ceil(wdt / bs));
-  int gy = std::max(1, (int)std::ceil(hgt / bs));
-  // keep the grid bounded for degenerate inputs
-  while ((long long)gx * gy > 4LL * n + 64) {
-    bs *= 1.25;
-    gx = std::max(1, (int)std::ceil(wdt / bs));
-    gy = std::max(1, (int)std::ceil(hgt / bs));
-  }
-  pd.gx = gx;
-  pd.gy = gy;
-  pd.bx0 = xmin;
-  pd.by0 = ymin;
-  pd.bsize = bs;
-  pd.binv = 1.0 / bs;
-  const int tiles_x = (gx + 7) / 8;
-  std::vector<long long> key(n);
-  std::vector<int> cell(n);
-  for (int i = 0; i < n; ++i) {
-    int cx = std::min(std::max((int)std::floor((pr.sites[2 * i] - xmin) * pd.binv), 0), gx - 1);
-    int cy =
-        std::min(std::max((int)std::floor((pr.sites[2 * i + 1] - ymin) * pd.binv), 0), gy - 1);
-    cell[i] = cy * gx + cx;
-    long long tile = (long long)(cy / 8) * tiles_x + (cx / 8);
-    key[i] = (tile * 64 + (cy % 8) * 8 + (cx % 8));
-  }
-  order.resize(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
-  buckets.assign((size_t)gx * gy, make_int2(0, 0));
-  for (int r = 0; r < n; ++r) {
-    int b = cell[order[r]];
-    if (buckets[b].y == 0) buckets[b].x = r;
-    buckets[b].y += 1;
-  }
-}
-
 // Sum the per-block partials of the last reduce into per-problem summaries.
 void sum_partials(cot_context* c, std::vector<EvalSum>& out) {
   out.assign(c->np, EvalSum());
@@ -276,7 +240,9 @@ void sum_partials(cot_context* c, std::vector<EvalSum>& out) {
 // One evaluation of every active problem at w (+ step * dir when dir is set), with the
 // conformity check and repairs.  Leaves mass/fcell/... on the device and summaries on the host.
 int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
-                    const std::vector<uint8_t>& active, std::vector<EvalSum>& sum) {
+                    const std::vector<uint8_t>& active, std::vector<EvalSum>& sum,
+                    int max_rounds = -1) {
+  if (max_rounds < 0) max_rounds = c->opt.max_repairs;
   hipStream_t st = c->st;
   std::memcpy(c->h_active, active.data(), c->np);
   HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, st));
@@ -312,8 +278,6 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.ring_used = c->d_used;
   ca.overflow = c->d_overflow;
   ca.org = c->d_org;
-  ca.bhalf = c->d_bhalf;
-  ca.boxhit = c->d_boxhit;
 
   VerifyArgs va{};
   va.probs = c->d_probs;
@@ -327,7 +291,6 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   va.redo = c->d_redo;
   va.fails = c->d_fails;
   va.tol_rel = c->opt.verify_tol;
-  va.boxhit = c->fp32 ? c->d_boxhit : nullptr;
 
   HIPCHK(hipEventRecord(c->ev[0], st));
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
@@ -358,10 +321,9 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   for (;;) {
     int tot = 0;
     for (int p = 0; p < c->np; ++p) tot += active[p] ? c->h_counts[p] : 0;
-    if (tot == 0 || rounds >= c->opt.max_repairs) break;
+    if (tot == 0 || rounds >= max_rounds) break;
     ++rounds;
-    HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max,
-                             c->fp32 ? c->d_bhalf : nullptr, c->d_boxhit, c->nblk, st));
+    HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max, c->nblk, st));
     ca.redo_mode = 1;
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
     HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
@@ -376,8 +338,13 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
     HIPCHK(hipStreamSynchronize(st));
   }
   c->stats.repair_rounds += rounds;
-  HIPCHK(launch_adapt_ring(c->d_blocks, c->d_used, c->d_ring, 1, c->opt.ring_max, c->nblk, st));
   sum_partials(c, sum);
+  // after a clean check the next evaluation starts from the outermost used ring plus one; after
+  // an unresolved one the raised rings stay
+  bool clean = true;
+  for (int p = 0; p < c->np; ++p) clean = clean && (!active[p] || sum[p].fails == 0);
+  if (clean)
+    HIPCHK(launch_adapt_ring(c->d_blocks, c->d_used, c->d_ring, 1, c->opt.ring_max, c->nblk, st));
   return COT_OK;
 }
 
@@ -512,8 +479,6 @@ int launch_round(cot_context* c, const SolveParams& sp, int R, hipEvent_t t0, hi
   ca.ring_used = c->d_used;
   ca.overflow = c->d_overflow;
   ca.org = c->d_org;
-  ca.bhalf = c->d_bhalf;
-  ca.boxhit = c->d_boxhit;
   VerifyArgs va{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
@@ -523,7 +488,6 @@ int launch_round(cot_context* c, const SolveParams& sp, int R, hipEvent_t t0, hi
   va.espan = c->d_espan;
   va.active = c->d_active;
   va.gate = nullptr;
-  va.boxhit = c->fp32 ? c->d_boxhit : nullptr;
   va.redo = c->d_redo;
   va.fails = c->d_rfails;
   va.tol_rel = c->opt.verify_tol;
@@ -534,8 +498,7 @@ int launch_round(cot_context* c, const SolveParams& sp, int R, hipEvent_t t0, hi
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
     HIPCHK(launch_dev_raise_ring(c->d_blocks, c->d_state, gate, c->d_redo, c->d_ring,
-                                 c->opt.ring_max, c->fp32 ? c->d_bhalf : nullptr, c->d_boxhit,
-                                 c->nblk, st));
+                                 c->opt.ring_max, c->nblk, st));
     ca.redo_mode = 1;
     ca.gate = gate;
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
@@ -806,8 +769,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_part, (size_t)c->nblk * std::max(kEvalPartials, kAcceptPartials))) ||
       (rc = dalloc(c, &c->d_coef, (size_t)c->np * kCoefWidth)) ||
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
-      (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_bhalf, N)) ||
-      (rc = dalloc(c, &c->d_boxhit, N)))
+      (rc = dalloc(c, &c->d_org, N)))
     return bail(rc);
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
@@ -850,14 +812,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     }
   }
   {
-    // FP32 frames start at the sites with a box of two buckets
-    std::vector<float> bh(N);
-    for (int p = 0; p < n_problems; ++p)
-      for (int r = 0; r < c->probs[p].n; ++r)
-        bh[c->probs[p].site_off + r] = (float)(2.0 * c->probs[p].bsize);
-    ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N) &&
-         up(c->d_bhalf, bh.data(), sizeof(float) * N) &&
-         hipMemset(c->d_boxhit, 0, N) == hipSuccess;
+    // FP32 clipping frames start at the sites
+    ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N);
   }
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
@@ -909,6 +865,225 @@ int cot_cell_areas(cot_context* c, const double* w, double* area) {
   std::vector<EvalSum> sum;
   CHK(evaluate_device(c, c->d_w, nullptr, act, sum));
   CHK(download_sorted(c, c->d_area, area));
+  return COT_OK;
+}
+
+namespace {
+
+// Solve (A + 1e-10 I) a = 1 for a small symmetric positive semi-definite A (Cholesky), then
+// normalise a to sum 1 (ot_op.cuh:1160-1163).  Returns false when the factorisation breaks down.
+bool anderson_weights(const std::vector<double>& A, int H, std::vector<double>& a) {
+  std::vector<double> L(H * H, 0.0);
+  for (int i = 0; i < H; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = A[i * H + j] + (i == j ? 1e-10 : 0.0);
+      for (int k = 0; k < j; ++k) s -= L[i * H + k] * L[j * H + k];
+      if (i == j) {
+        if (!(s > 0.0)) return false;
+        L[i * H + i] = std::sqrt(s);
+      } else {
+        L[i * H + j] = s / L[j * H + j];
+      }
+    }
+  }
+  std::vector<double> y(H);
+  for (int i = 0; i < H; ++i) {
+    double s = 1.0;
+    for (int k = 0; k < i; ++k) s -= L[i * H + k] * y[k];
+    y[i] = s / L[i * H + i];
+  }
+  a.assign(H, 0.0);
+  for (int i = H - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < H; ++k) s -= L[k * H + i] * a[k];
+    a[i] = s / L[i * H + i];
+  }
+  double tot = 0.0;
+  for (double v : a) tot += v;
+  if (!(std::fabs(tot) > 0.0)) return false;
+  for (double& v : a) v /= tot;
+  return true;
+}
+
+}  // namespace
+
+int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
+                         const cot_reflection* r, cot_report* rep) {
+  if (!c || !w || !r || !r->points || !r->inside_dir || !r->height || !r->clip_area)
+    return fail(COT_EINVAL, "null argument");
+  if (c->np != 1) return fail(COT_EINVAL, "the reflection solve takes one-problem contexts");
+  if (r->aa_history_size < 1 || r->aa_history_size > kMaxHist)
+    return fail(COT_EINVAL, "aa_history_size must be in [1, " + std::to_string(kMaxHist) + "]");
+  cot_solve_opts o;
+  if (so) {
+    o = *so;
+  } else {
+    cot_default_solve_opts(&o);
+  }
+  o.stop_rule = COT_STOP_REFERENCE;
+  const auto t_start = std::chrono::steady_clock::now();
+  const int N = c->ntot, nb = c->nblk;
+  const size_t ld = N;
+  hipStream_t st = c->st;
+  const int mA = r->aa_history_size;
+
+  // per-site lens data in bucket order
+  double2* d_pts = nullptr;
+  double *d_ins = nullptr, *d_h = nullptr, *d_area = nullptr, *d_wt = nullptr, *d_G = nullptr;
+  double *d_R = nullptr, *d_alpha = nullptr, *d_pp = nullptr;
+  CHK(dalloc(c, &d_pts, N));
+  CHK(dalloc(c, &d_ins, 3 * (size_t)N));
+  CHK(dalloc(c, &d_h, N));
+  CHK(dalloc(c, &d_area, N));
+  CHK(dalloc(c, &d_wt, N));
+  CHK(dalloc(c, &d_G, (size_t)mA * N));
+  CHK(dalloc(c, &d_R, (size_t)mA * N));
+  CHK(dalloc(c, &d_alpha, mA));
+  CHK(dalloc(c, &d_pp, (size_t)std::max(mA, 1) * nb));
+  {
+    std::vector<double2> pts(N);
+    std::vector<double> ins(3 * (size_t)N), h(N), ar(N);
+    for (int q = 0; q < N; ++q) {
+      const int o2 = c->perm[q];
+      pts[q] = make_double2(r->points[2 * o2], r->points[2 * o2 + 1]);
+      for (int k = 0; k < 3; ++k) ins[3 * q + k] = r->inside_dir[3 * o2 + k];
+      h[q] = r->height[o2];
+      ar[q] = r->clip_area[o2];
+    }
+    HIPCHK(hipMemcpy(d_pts, pts.data(), sizeof(double2) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ins, ins.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_h, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_area, ar.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+  std::vector<double> hp(std::max(mA, 1) * (size_t)nb);
+  // nu from the Fresnel weights at the centroids the last evaluation left in d_geo
+  auto update_nu = [&]() -> int {
+    HIPCHK(launch_fresnel_weight(c->d_blocks, d_pts, d_ins, d_h, d_area, c->d_geo, r->z_of_len,
+                                 r->z_of_receiver, r->refraction_index, d_wt, d_pp, nb, st));
+    HIPCHK(hipMemcpyAsync(hp.data(), d_pp, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double tot = 0.0;
+    for (int b = 0; b < nb; ++b) tot += hp[b];
+    if (!(tot > 0.0)) return fail(COT_EINVAL, "Fresnel-weighted source mass is zero");
+    HIPCHK(launch_scale(c->d_blocks, d_wt, r->area_scalar / tot, c->d_nu, nb, st));
+    return COT_OK;
+  };
+  // |mass - nu|^2 and the empty-cell count with the current nu
+  auto residual = [&](EvalSum& e) -> int {
+    HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu,
+                              c->d_exist, nullptr, c->d_part, nb, st));
+    HIPCHK(hipMemcpyAsync(c->h_part, c->d_part, sizeof(double) * kEvalPartials * nb,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<EvalSum> s;
+    sum_partials(c, s);
+    e.g2 = s[0].g2;
+    e.zeros = s[0].zeros;
+    e.maxrel = s[0].maxrel;
+    return COT_OK;
+  };
+
+  std::vector<uint8_t> act(1, 1);
+  std::vector<EvalSum> cur;
+  CHK(upload_sorted(c, w, c->d_w));
+  // masses at the warm start's geometric centroids (ot_op.cuh:779-795)
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  CHK(update_nu());
+  int iters_a = 0, evals_a = 0;
+  if (r->if_aa_op_initial) {
+    // stage A: L-BFGS with frozen masses (ot_op.cuh:823-1074)
+    std::vector<double> wh(N);
+    CHK(download_sorted(c, c->d_w, wh.data()));
+    cot_report ra{};
+    const int rc = cot_solve(c, wh.data(), &o, &ra);
+    if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
+    iters_a = ra.iterations;
+    evals_a = ra.evaluations;
+    CHK(upload_sorted(c, wh.data(), c->d_w));
+  }
+  // re-estimate the masses at the stage-A result (ot_op.cuh:1076-1110)
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  CHK(update_nu());
+  HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+  EvalSum now;
+  CHK(residual(now));
+
+  // stage B: Anderson acceleration of G(w) = w + beta g(w) (ot_op.cuh:1112-1323)
+  std::vector<double> A((size_t)mA * mA, 0.0), alpha;
+  int aa_loop_iter = 0, iters_b = 0, evals_b = 0;
+  bool conv = false;
+  int status = COT_OK;
+  for (int iter = 0; iter < o.max_iter; ++iter) {
+    const int H = std::min(aa_loop_iter + 1, mA);
+    const int slot = aa_loop_iter % mA;
+    HIPCHK(launch_aa_store(c->d_blocks, c->d_w, c->d_g, r->aa_fixed_para, slot, H, d_G, d_R, ld,
+                           d_pp, nb, st));
+    HIPCHK(hipMemcpyAsync(hp.data(), d_pp, sizeof(double) * H * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int h = 0; h < H; ++h) {
+      double s = 0.0;
+      for (int b = 0; b < nb; ++b) s += hp[(size_t)h * nb + b];
+      A[slot * mA + h] = A[h * mA + slot] = s;
+    }
+    std::vector<double> Ah((size_t)H * H);
+    for (int i2 = 0; i2 < H; ++i2)
+      for (int j2 = 0; j2 < H; ++j2) Ah[i2 * H + j2] = A[i2 * mA + j2];
+    if (!anderson_weights(Ah, H, alpha)) {
+      alpha.assign(H, 0.0);
+      alpha[slot] = 1.0;
+    }
+    HIPCHK(hipMemcpyAsync(d_alpha, alpha.data(), sizeof(double) * H, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_aa_direction(c->d_blocks, d_alpha, H, d_G, ld, c->d_w, c->d_dir, nb, st));
+    const double zeros_start = now.zeros;
+    double step = 1.0;
+    for (;;) {
+      c->h_small[0] = step;
+      HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double), hipMemcpyHostToDevice, st));
+      std::vector<EvalSum> tr;
+      CHK(evaluate_device(c, c->d_w, c->d_dir, act, tr));
+      ++evals_b;
+      if (tr[0].fails) status = COT_EUNVERIFIED;
+      if (tr[0].overflow > 0) {
+        step *= 0.5;
+      } else {
+        // every trial recomputes the masses from its own cells (ot_op.cuh:1212-1240)
+        CHK(update_nu());
+        HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+        HIPCHK(launch_axpy(c->d_blocks, step, c->d_dir, c->d_w, nb, st));
+        ++aa_loop_iter;
+        CHK(residual(now));
+        break;
+      }
+      if (step <= 1e-4 * o.initial_step) {
+        aa_loop_iter = 0;
+        break;
+      }
+    }
+    ++iters_b;
+    // the empty-cell count is evaluated on every 10th iteration only (ot_op.cuh:1257-1268)
+    const double degenerate = (iter % 10 == 0) ? zeros_start : 0.0;
+    if (o.verbose > 0 && iter % o.verbose == 0)
+      std::printf("[cot] anderson iter %d |g| %.6e maxrel %.3e step %g\n", iter,
+                  std::sqrt(now.g2), now.maxrel, step);
+    if (std::sqrt(now.g2) < o.tol && degenerate == 0.0) {
+      conv = true;
+      break;
+    }
+  }
+  CHK(download_sorted(c, c->d_w, w));
+  if (rep) {
+    rep->iterations = iters_a + iters_b;
+    rep->evaluations = evals_a + evals_b;
+    rep->converged = conv ? 1 : 0;
+    rep->status = status != COT_OK ? status : (conv ? COT_OK : COT_ENOTCONV);
+    rep->f = 0.0;
+    rep->grad_norm = std::sqrt(now.g2);
+    rep->max_rel_err = now.maxrel;
+    rep->seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  if (status != COT_OK) return fail(status, "cells failed the conformity check");
+  if (!conv) return fail(COT_ENOTCONV, "iteration limit reached");
   return COT_OK;
 }
 
@@ -1044,7 +1219,7 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
     if (!any_trial) break;
     HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice,
                           c->st));
-    CHK(evaluate_device(c, c->d_w, c->d_dir, act, trial));
+    CHK(evaluate_device(c, c->d_w, c->d_dir, act, trial, std::min(c->opt.max_repairs, 3)));
 
     // ---- decisions (ot_op.cuh:360-431)
     std::vector<uint8_t> acc(np, 0);
@@ -1053,8 +1228,8 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
       if (!act[p]) continue;
       ++evals[p];
       bool ok = false;
-      if (trial[p].fails) status[p] = COT_EUNVERIFIED;
-      if (trial[p].overflow > 0) {
+      if (trial[p].overflow > 0 || trial[p].fails > 0) {
+        // over the vertex limit, or unverified after the repair rounds: reject the trial
         t[p] *= 0.5;
       } else if (trial[p].f < f_prev[p] || (o.accept_rule && trial[p].gd <= 0.0)) {
         ok = true;

@@ -14,9 +14,8 @@ constexpr int kCellBlock = 128;
 constexpr int kVecBlock = 256;
 // Largest L-BFGS history.
 constexpr int kMaxHist = 16;
-// Edge labels below zero: the four sides of the target rectangle, and the artificial box that
-// bounds an FP32 cell before its constraints are applied.
-constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4, kSideBox = -5;
+// Edge labels below zero: the four sides of the target rectangle.
+constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4;
 
 // One OT problem of a batch.  All per-site arrays are indexed by global sorted site index.
 struct ProbDesc {
@@ -67,11 +66,8 @@ struct CellArgs {
   float2* espan;            // [site * kMaxVerts + k]: edge span along the canonical bisector axis
   uint8_t* ring_used;       // outermost ring that cut the cell
   int* overflow;            // per-problem count of cells that exceeded kMaxVerts
-  // FP32 geometry only: clipping frame origin (last geometric centroid) and half-size of the
-  // artificial box around it; boxhit marks cells whose final polygon still touches the box.
+  // FP32 geometry only: clipping frame origin (the cell's last geometric centroid)
   double2* org;
-  float* bhalf;
-  uint8_t* boxhit;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -111,7 +107,6 @@ struct VerifyArgs {
   const float2* espan;
   const uint8_t* active;    // per-problem mask
   const int* gate;          // skip problems whose counter is zero (nullptr: none)
-  const uint8_t* boxhit;    // FP32 geometry: cells bounded by their artificial box (nullptr: none)
   uint8_t* redo;            // set for both cells of a mismatched edge
   int* fails;               // per-problem mismatch count
   double tol_rel;           // tolerance relative to the site spacing of the pair

@@ -63,12 +63,12 @@ __device__ double block_max(double v, double* red) {
 // Neumaier compensated accumulation.
 struct KSum {
   double s = 0.0, c = 0.0;
-  __device__ __forceinline__ void add(double x) {
+  __host__ __device__ __forceinline__ void add(double x) {
     double t = s + x;
     c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
     s = t;
   }
-  __device__ __forceinline__ double value() const { return s + c; }
+  __host__ __device__ __forceinline__ double value() const { return s + c; }
 };
 
 template <typename T>
@@ -82,7 +82,7 @@ struct PolyLDS {
 // Returns the new vertex count (0 when empty).  Clipping is in place: at step k the writes land
 // at positions <= k + 1, and vertex k + 1 has already been read into registers.
 template <typename T>
-__device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy, T c, int label,
+__host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy, T c, int label,
                               bool& overflow, bool& changed) {
   int nout = 0, trans = 0;
   {
@@ -161,14 +161,14 @@ struct RowCache {
   int row = -1, col = 0;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 
-  __device__ __forceinline__ void add_pixel(const TI* r, int k, double sgn) {
+  __host__ __device__ __forceinline__ void add_pixel(const TI* r, int k, double sgn) {
     double v = sgn * (double)r[k];
     double kk = (double)k;
     a0 += v;
     a1 += v * (kk + 0.5);
     a2 += v * (kk * kk + kk + 1.0 / 3.0);
   }
-  __device__ __forceinline__ void seek(int iy, int ix) {
+  __host__ __device__ __forceinline__ void seek(int iy, int ix) {
     const TI* r = base + (size_t)iy * pm;
     if (iy != row) {
       row = iy;
@@ -187,37 +187,63 @@ struct Moments {
 
 // Exact contribution of one edge piece inside pixel (ix, iy) (local pixel units) to
 // oint F dV for F = int rho, int rho*U, int rho*U^2 along U, and V*F0, V^2*F0.
-template <typename TI>
-__device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix, int iy,
+// TP = double: everything in FP64.  TP = float: the polynomial terms of the piece are evaluated
+// in FP32 in coordinates local to the pixel (u, v in [0, 1]); the row prefixes A0..A2, the
+// integer offsets and every accumulation stay FP64.
+template <typename TP, typename TI>
+__host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix, int iy,
                                       RowCache<TI>& rc, Moments& M) {
-  double dv = V1 - V0;
+  const double dv = V1 - V0;
   if (dv == 0.0) return;
   rc.seek(iy, ix);
-  const double rc_ = (double)rc.base[(size_t)iy * rc.pm + ix];
-  const double fx = (double)ix;
-  const double Um = 0.5 * (U0 + U1), Vm = 0.5 * (V0 + V1);
-  // mass: int (A0 + rho (U - ix)) dV
-  M.m0.add((rc.a0 + rc_ * (Um - fx)) * dv);
-  // int rho U: int (A1 + rho (U^2 - ix^2)/2) dV
-  const double u2avg = (U0 * U0 + U0 * U1 + U1 * U1) * (1.0 / 3.0);
-  M.m1u += (rc.a1 + 0.5 * rc_ * (u2avg - fx * fx)) * dv;
-  // int rho V: int V (A0 + rho (U - ix)) dV
-  const double vu = (V0 * U0 + 4.0 * Vm * Um + V1 * U1) * (1.0 / 6.0) * dv;
-  M.m1v += (rc.a0 - rc_ * fx) * 0.5 * (V1 * V1 - V0 * V0) + rc_ * vu;
-  // int rho U^2: int (A2 + rho (U^3 - ix^3)/3) dV
-  const double u3 = (U0 * U0 * U0 + 4.0 * Um * Um * Um + U1 * U1 * U1) * (1.0 / 6.0) * dv;
-  M.m2u += (rc.a2 - rc_ * fx * fx * fx * (1.0 / 3.0)) * dv + rc_ * (1.0 / 3.0) * u3;
-  // int rho V^2: int V^2 (A0 + rho (U - ix)) dV
-  const double v2u = (V0 * V0 * U0 + 4.0 * Vm * Vm * Um + V1 * V1 * U1) * (1.0 / 6.0) * dv;
-  M.m2v += (rc.a0 - rc_ * fx) * (V1 * V1 * V1 - V0 * V0 * V0) * (1.0 / 3.0) + rc_ * v2u;
+  const double fx = (double)ix, fy = (double)iy;
+  if constexpr (sizeof(TP) == 4) {
+    const float r = (float)rc.base[(size_t)iy * rc.pm + ix];
+    const float u0 = (float)(U0 - fx), u1 = (float)(U1 - fx);
+    const float v0 = (float)(V0 - fy), v1 = (float)(V1 - fy);
+    const float dvf = (float)dv;
+    const float um = 0.5f * (u0 + u1), vm = 0.5f * (v0 + v1);
+    const float Iu = um * dvf;                                                   // int u dV
+    const float Iuu = (u0 * u0 + u0 * u1 + u1 * u1) * (1.0f / 3.0f) * dvf;        // int u^2
+    const float Iuuu = (u0 * u0 * u0 + 4.0f * um * um * um + u1 * u1 * u1) * (1.0f / 6.0f) * dvf;
+    const float Iv = vm * dvf;                                                   // int v
+    const float Ivv = (v0 * v0 + v0 * v1 + v1 * v1) * (1.0f / 3.0f) * dvf;        // int v^2
+    const float Ivu = (v0 * u0 + 4.0f * vm * um + v1 * u1) * (1.0f / 6.0f) * dvf;
+    const float Ivvu = (v0 * v0 * u0 + 4.0f * vm * vm * um + v1 * v1 * u1) * (1.0f / 6.0f) * dvf;
+    const double pm = rc.a0 * dv + (double)(r * Iu);  // int F0 dV over the piece
+    M.m0.add(pm);
+    M.m1u += rc.a1 * dv + fx * (double)(r * Iu) + 0.5 * (double)(r * Iuu);
+    const double vF0 = rc.a0 * (double)Iv + (double)(r * Ivu);  // int v F0 dV
+    M.m1v += fy * pm + vF0;
+    M.m2u += rc.a2 * dv + fx * fx * (double)(r * Iu) + fx * (double)(r * Iuu) +
+             (1.0 / 3.0) * (double)(r * Iuuu);
+    const double v2F0 = rc.a0 * (double)Ivv + (double)(r * Ivvu);  // int v^2 F0 dV
+    M.m2v += fy * fy * pm + 2.0 * fy * vF0 + v2F0;
+  } else {
+    const double rc_ = (double)rc.base[(size_t)iy * rc.pm + ix];
+    const double Um = 0.5 * (U0 + U1), Vm = 0.5 * (V0 + V1);
+    // mass: int (A0 + rho (U - ix)) dV
+    M.m0.add((rc.a0 + rc_ * (Um - fx)) * dv);
+    // int rho U: int (A1 + rho (U^2 - ix^2)/2) dV
+    const double u2avg = (U0 * U0 + U0 * U1 + U1 * U1) * (1.0 / 3.0);
+    M.m1u += (rc.a1 + 0.5 * rc_ * (u2avg - fx * fx)) * dv;
+    // int rho V: int V (A0 + rho (U - ix)) dV
+    const double vu = (V0 * U0 + 4.0 * Vm * Um + V1 * U1) * (1.0 / 6.0) * dv;
+    M.m1v += (rc.a0 - rc_ * fx) * 0.5 * (V1 * V1 - V0 * V0) + rc_ * vu;
+    // int rho U^2: int (A2 + rho (U^3 - ix^3)/3) dV
+    const double u3 = (U0 * U0 * U0 + 4.0 * Um * Um * Um + U1 * U1 * U1) * (1.0 / 6.0) * dv;
+    M.m2u += (rc.a2 - rc_ * fx * fx * fx * (1.0 / 3.0)) * dv + rc_ * (1.0 / 3.0) * u3;
+    // int rho V^2: int V^2 (A0 + rho (U - ix)) dV
+    const double v2u = (V0 * V0 * U0 + 4.0 * Vm * Vm * Um + V1 * V1 * U1) * (1.0 / 6.0) * dv;
+    M.m2v += (rc.a0 - rc_ * fx) * (V1 * V1 * V1 - V0 * V0 * V0) * (1.0 / 3.0) + rc_ * v2u;
+  }
 }
 
 // Walk the edge (Ua,Va)->(Ub,Vb) across the pixel lattice (local pixel units, lattice limits
 // lim_u x lim_v) and add each piece.
 template <typename T, typename TI>
-__device__ void walk_edge(T Ua, T Va, T Ub, T Vb, int lim_u, int lim_v, RowCache<TI>& rc,
-                          Moments& M) {
-  const double ua = Ua, va = Va, ub = Ub, vb = Vb;
+__host__ __device__ void walk_edge(double ua, double va, double ub, double vb, int lim_u, int lim_v,
+                          RowCache<TI>& rc, Moments& M) {
   const double du = ub - ua, dvv = vb - va;
   if (dvv == 0.0) return;
   int iu = (int)floor(ua), iv = (int)floor(va);
@@ -245,7 +271,7 @@ __device__ void walk_edge(T Ua, T Va, T Ub, T Vb, int lim_u, int lim_v, RowCache
     int iy = (int)floor(0.5 * (V0 + V1));
     ix = min(max(ix, 0), lim_u - 1);
     iy = min(max(iy, 0), lim_v - 1);
-    piece<TI>(U0, V0, U1, V1, ix, iy, rc, M);
+    piece<T, TI>(U0, V0, U1, V1, ix, iy, rc, M);
     if (tn >= 1.0) break;
     if (tu <= tv) {
       nu_line += su;
@@ -264,16 +290,16 @@ __device__ void walk_edge(T Ua, T Va, T Ub, T Vb, int lim_u, int lim_v, RowCache
 // cell kernel
 // ---------------------------------------------------------------------------------------------
 
+// One site of the cell kernel (lane `ln` of block `blk`); returns true when the cell exceeded
+// kMaxVerts.  Shared by the kernel and the host build used to test this code on the CPU.
 template <typename T, typename TI>
-__global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
-  __shared__ PolyLDS<T> P;
-  const BlockDesc bd = a.blocks[blockIdx.x];
-  const int ln = threadIdx.x;
-  if (ln >= bd.count) return;
-  if (!a.active[bd.prob]) return;
-  if (a.gate && a.gate[bd.prob] == 0) return;
+__host__ __device__ bool cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>& P) {
+  const BlockDesc bd = a.blocks[blk];
+  if (ln >= bd.count) return false;
+  if (!a.active[bd.prob]) return false;
+  if (a.gate && a.gate[bd.prob] == 0) return false;
   const int i = bd.first + ln;
-  if (a.redo_mode && !a.redo[i]) return;
+  if (a.redo_mode && !a.redo[i]) return false;
   if (a.redo) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   // trial weights w + t * d, with d = -g for a steepest-descent trial
@@ -294,25 +320,15 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   const double2 s = a.xy[i];
   const double wi = wtrial(i);
   constexpr bool kF32 = sizeof(T) == 4;
-  // Clipping frame.  FP64: the site, starting from the target rectangle.  FP32: the cell's last
-  // centroid, starting from a box of half-size L around it, with the rectangle's sides applied
-  // as ordinary half-planes -- coordinates then stay at the scale of the cell, whatever the
-  // distance between the cell and its site.
+  // Clipping frame: the site (FP64) or the cell's last centroid (FP32, so that the final
+  // vertices are small numbers).  Both start from the target rectangle, a true constraint, so the
+  // clipped polygon always contains the cell.
   const double2 o = kF32 ? a.org[i] : s;
   const double osx = o.x - s.x, osy = o.y - s.y;  // frame origin relative to the site
-  if (kF32) {
-    const T L = (T)a.bhalf[i];
-    P.x[0][ln] = -L; P.y[0][ln] = -L; P.lab[0][ln] = kSideBox;
-    P.x[1][ln] = L;  P.y[1][ln] = -L; P.lab[1][ln] = kSideBox;
-    P.x[2][ln] = L;  P.y[2][ln] = L;  P.lab[2][ln] = kSideBox;
-    P.x[3][ln] = -L; P.y[3][ln] = L;  P.lab[3][ln] = kSideBox;
-  } else {
-    // target rectangle relative to the site, counter-clockwise; side labels
-    P.x[0][ln] = (T)(pd.x0 - s.x); P.y[0][ln] = (T)(pd.y0 - s.y); P.lab[0][ln] = kSideBottom;
-    P.x[1][ln] = (T)(pd.x1 - s.x); P.y[1][ln] = (T)(pd.y0 - s.y); P.lab[1][ln] = kSideRight;
-    P.x[2][ln] = (T)(pd.x1 - s.x); P.y[2][ln] = (T)(pd.y1 - s.y); P.lab[2][ln] = kSideTop;
-    P.x[3][ln] = (T)(pd.x0 - s.x); P.y[3][ln] = (T)(pd.y1 - s.y); P.lab[3][ln] = kSideLeft;
-  }
+  P.x[0][ln] = (T)(pd.x0 - o.x); P.y[0][ln] = (T)(pd.y0 - o.y); P.lab[0][ln] = kSideBottom;
+  P.x[1][ln] = (T)(pd.x1 - o.x); P.y[1][ln] = (T)(pd.y0 - o.y); P.lab[1][ln] = kSideRight;
+  P.x[2][ln] = (T)(pd.x1 - o.x); P.y[2][ln] = (T)(pd.y1 - o.y); P.lab[2][ln] = kSideTop;
+  P.x[3][ln] = (T)(pd.x0 - o.x); P.y[3][ln] = (T)(pd.y1 - o.y); P.lab[3][ln] = kSideLeft;
   int nv = 4;
   bool overflow = false;
 
@@ -334,24 +350,6 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   };
   double bcx, bcy, br2;
   bound(bcx, bcy, br2);
-  if (kF32) {
-    // the four sides of the target rectangle: -y <= o.y - y0, x <= x1 - o.x, y <= y1 - o.y,
-    // -x <= o.x - x0
-    const double sd[4][3] = {{0.0, -1.0, o.y - pd.y0},
-                             {1.0, 0.0, pd.x1 - o.x},
-                             {0.0, 1.0, pd.y1 - o.y},
-                             {-1.0, 0.0, o.x - pd.x0}};
-    const int lab[4] = {kSideBottom, kSideRight, kSideTop, kSideLeft};
-    for (int q = 0; q < 4 && nv > 0; ++q) {
-      const double slack = sd[q][2] - (bcx * sd[q][0] + bcy * sd[q][1]);
-      if (slack >= 0.0 && slack * slack >= br2) continue;
-      bool changed;
-      nv = clip_halfplane<T>(P, ln, nv, (T)sd[q][0], (T)sd[q][1], (T)sd[q][2], lab[q], overflow,
-                             changed);
-      if (changed && nv > 0) bound(bcx, bcy, br2);
-    }
-  }
-
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
   const int R = a.ring[i];
@@ -387,27 +385,91 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
       }
     }
   }
-  if (overflow) atomicAdd(&a.overflow[bd.prob], 1);
   a.ring_used[i] = (uint8_t)used;
   a.nedge[i] = (uint8_t)nv;
 
   double m0 = 0.0, fi = -wi * a.nu[i] , ar = 0.0;
   double2 cen = make_double2(0.0, 0.0), geo = make_double2(0.0, 0.0);
-  bool boxhit = false;
   if (nv >= 3) {
-    // edge records for the conformity check, geometric area/centroid, pixel frame
+    // The final polygon in FP64, edge by edge.  FP64 geometry: the vertices as clipped.  FP32
+    // geometry: the clipping only decides which constraint bounds each edge; every vertex is
+    // then recomputed in FP64 as the intersection of its two constraint lines (frame o).
+    auto line_of = [&](int lab, double& nx, double& ny, double& c) {
+      if (lab >= 0) {
+        const double2 sj = a.xy[lab];
+        nx = sj.x - s.x;
+        ny = sj.y - s.y;
+        c = 0.5 * (nx * nx + ny * ny + wi - wtrial(lab)) - (osx * nx + osy * ny);
+      } else if (lab == kSideBottom) {
+        nx = 0.0; ny = -1.0; c = o.y - pd.y0;
+      } else if (lab == kSideRight) {
+        nx = 1.0; ny = 0.0; c = pd.x1 - o.x;
+      } else if (lab == kSideTop) {
+        nx = 0.0; ny = 1.0; c = pd.y1 - o.y;
+      } else if (lab == kSideLeft) {
+        nx = -1.0; ny = 0.0; c = o.x - pd.x0;
+      } else {
+        nx = ny = c = 0.0;
+      }
+    };
+    // FP32: extent of the clipped polygon, for the refinement guard below
+    double ext2 = 0.0;
+    if (kF32) {
+      double mnx = P.x[0][ln], mxx = mnx, mny = P.y[0][ln], mxy = mny;
+      for (int k = 1; k < nv; ++k) {
+        mnx = fmin(mnx, (double)P.x[k][ln]); mxx = fmax(mxx, (double)P.x[k][ln]);
+        mny = fmin(mny, (double)P.y[k][ln]); mxy = fmax(mxy, (double)P.y[k][ln]);
+      }
+      ext2 = (mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny);
+    }
+    auto vertex = [&](int k, double ax, double ay, double ac, double bx, double by,
+                      double bc) -> double2 {
+      const double fx = (double)P.x[k][ln], fy = (double)P.y[k][ln];
+      if (!kF32) return make_double2(fx, fy);
+      const double det = ax * by - ay * bx;
+      const double sc = sqrt((ax * ax + ay * ay) * (bx * bx + by * by));
+      if (!(fabs(det) > 1e-7 * sc)) return make_double2(fx, fy);
+      const double rx = (ac * by - bc * ay) / det, ry = (ax * bc - bx * ac) / det;
+      // a near-parallel pair that FP32 rounding made adjacent can meet far away: keep the
+      // clipped vertex when the refinement moves it by more than 1e-3 of the cell's extent
+      const double ex = rx - fx, ey = ry - fy;
+      if (ex * ex + ey * ey > 1e-6 * ext2) return make_double2(fx, fy);
+      return make_double2(rx, ry);
+    };
+    // calls f(k, va, vb) for every edge k from vertex k to vertex k+1 (frame coordinates)
+    auto for_each_edge = [&](auto&& f) {
+      double px, py, pc, kx, ky, kc;
+      line_of(P.lab[nv - 1][ln], px, py, pc);
+      line_of(P.lab[0][ln], kx, ky, kc);
+      const double2 first = vertex(0, px, py, pc, kx, ky, kc);
+      double2 vk = first;
+      for (int k = 0; k < nv; ++k) {
+        double2 vn;
+        if (k + 1 < nv) {
+          double nx_, ny_, nc_;
+          line_of(P.lab[k + 1][ln], nx_, ny_, nc_);
+          vn = vertex(k + 1, kx, ky, kc, nx_, ny_, nc_);
+          kx = nx_;
+          ky = ny_;
+          kc = nc_;
+        } else {
+          vn = first;
+        }
+        f(k, vk, vn);
+        vk = vn;
+      }
+    };
+
+    // edge records for the conformity check, geometric area/centroid, bounding corner
     double mnu = INFINITY, mnv = INFINITY;
     double A2 = 0.0, gx = 0.0, gy = 0.0;
-    for (int k = 0; k < nv; ++k) {
-      boxhit |= P.lab[k][ln] == kSideBox;
-      const int k1 = (k + 1 == nv) ? 0 : k + 1;
-      const double xa = P.x[k][ln], ya = P.y[k][ln], xb = P.x[k1][ln], yb = P.y[k1][ln];
-      const double cr = xa * yb - xb * ya;
+    for_each_edge([&](int k, double2 va, double2 vb) {
+      const double cr = va.x * vb.y - vb.x * va.y;
       A2 += cr;
-      gx += (xa + xb) * cr;
-      gy += (ya + yb) * cr;
-      mnu = fmin(mnu, xa);
-      mnv = fmin(mnv, ya);
+      gx += (va.x + vb.x) * cr;
+      gy += (va.y + vb.y) * cr;
+      mnu = fmin(mnu, va.x);
+      mnv = fmin(mnv, va.y);
       const int lab = P.lab[k][ln];
       float2 span = make_float2(0.f, 0.f);
       if (lab >= 0) {
@@ -417,13 +479,13 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
         const double inv = 1.0 / sqrt(Dx * Dx + Dy * Dy);
         const double ex = -Dy * inv, ey = Dx * inv;
         const double mx = 0.5 * (pl.x + ph.x), my = 0.5 * (pl.y + ph.y);
-        const double t0 = (o.x + xa - mx) * ex + (o.y + ya - my) * ey;
-        const double t1 = (o.x + xb - mx) * ex + (o.y + yb - my) * ey;
+        const double t0 = (o.x + va.x - mx) * ex + (o.y + va.y - my) * ey;
+        const double t1 = (o.x + vb.x - mx) * ex + (o.y + vb.y - my) * ey;
         span = make_float2((float)fmin(t0, t1), (float)fmax(t0, t1));
       }
       a.elab[(size_t)i * kMaxVerts + k] = lab;
       a.espan[(size_t)i * kMaxVerts + k] = span;
-    }
+    });
     ar = 0.5 * A2;
     if (A2 != 0.0) {
       geo = make_double2(o.x + gx / (3.0 * A2), o.y + gy / (3.0 * A2));
@@ -447,21 +509,10 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
     rc.pm = pd.pm;
     Moments M;
     const int lim_u = pd.pm - kox, lim_v = pd.pn - koy;
-    T pu = (T)(fu0 + (double)P.x[0][ln] * idx_), pv = (T)(fv0 + (double)P.y[0][ln] * idy_);
-    const T fu = pu, fv = pv;
-    for (int k = 0; k < nv; ++k) {
-      T qu, qv;
-      if (k + 1 < nv) {
-        qu = (T)(fu0 + (double)P.x[k + 1][ln] * idx_);
-        qv = (T)(fv0 + (double)P.y[k + 1][ln] * idy_);
-      } else {
-        qu = fu;
-        qv = fv;
-      }
-      walk_edge<T, TI>(pu, pv, qu, qv, lim_u, lim_v, rc, M);
-      pu = qu;
-      pv = qv;
-    }
+    for_each_edge([&](int, double2 va, double2 vb) {
+      walk_edge<T, TI>(fu0 + va.x * idx_, fv0 + va.y * idy_, fu0 + vb.x * idx_,
+                       fv0 + vb.y * idy_, lim_u, lim_v, rc, M);
+    });
     m0 = M.m0.value();
     // second moment about the site, physical units
     const double su2 = M.m2u - 2.0 * ou * M.m1u + ou * ou * m0;
@@ -474,36 +525,34 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
     } else {
       cen = geo;
     }
-    if (kF32) {
-      // next frame: this centroid, box twice the cell's radius about it (at least one bucket)
-      double r2 = 0.0;
-      for (int k = 0; k < nv; ++k) {
-        const double ex = o.x + (double)P.x[k][ln] - geo.x, ey = o.y + (double)P.y[k][ln] - geo.y;
-        r2 = fmax(r2, ex * ex + ey * ey);
-      }
-      a.org[i] = geo;
-      a.bhalf[i] = boxhit ? 2.0f * a.bhalf[i] : (float)fmax(2.0 * sqrt(r2), pd.bsize);
-    }
+    if (kF32) a.org[i] = geo;  // next frame: this centroid
   }
-  if (kF32) a.boxhit[i] = boxhit ? 1 : 0;
   a.mass[i] = m0;
   a.fcell[i] = fi;
   a.area[i] = ar;
   a.cen[i] = cen;
   a.geo[i] = geo;
   a.exist[i] = nv >= 3 ? 1 : 0;
+  return overflow;
+}
+
+template <typename T, typename TI>
+__global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
+  __shared__ PolyLDS<T> P;
+  if (cell_site<T, TI>(a, blockIdx.x, threadIdx.x, P))
+    atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
 }
 
 // ---------------------------------------------------------------------------------------------
 // conformity check
 // ---------------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
-  const BlockDesc bd = a.blocks[blockIdx.x];
-  const int ln = threadIdx.x;
-  if (ln >= bd.count) return;
-  if (!a.active[bd.prob]) return;
-  if (a.gate && a.gate[bd.prob] == 0) return;
+// One site of the conformity check; returns true when one of its edges is unmatched.
+__host__ __device__ bool verify_site(const VerifyArgs& a, int blk, int ln) {
+  const BlockDesc bd = a.blocks[blk];
+  if (ln >= bd.count) return false;
+  if (!a.active[bd.prob]) return false;
+  if (a.gate && a.gate[bd.prob] == 0) return false;
   const int i = bd.first + ln;
   const int ne = a.nedge[i];
   const double2 si = a.xy[i];
@@ -531,24 +580,23 @@ __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
       a.redo[j] = 1;
     }
   }
-  if (a.boxhit && a.boxhit[i]) bad = true;
-  if (bad) {
-    a.redo[i] = 1;
-    atomicAdd(&a.fails[bd.prob], 1);
-  }
+  if (bad) a.redo[i] = 1;
+  return bad;
 }
 
-// Raise the ring of every cell marked for a rebuild; with FP32 geometry also double its box
-// (unless the cell kernel already did because the box bounded the cell).
+__global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
+  if (verify_site(a, blockIdx.x, threadIdx.x)) atomicAdd(&a.fails[a.blocks[blockIdx.x].prob], 1);
+}
+
+// Grow the ring of every cell marked for a rebuild (r -> 2r + 1).
 __global__ void raise_ring_kernel(const BlockDesc* blocks, const uint8_t* redo, uint8_t* ring,
-                                  int ring_max, float* bhalf, const uint8_t* boxhit) {
+                                  int ring_max) {
   const BlockDesc bd = blocks[blockIdx.x];
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   const int i = bd.first + ln;
   if (!redo[i]) return;
-  ring[i] = (uint8_t)min((int)ring[i] + 1, ring_max);
-  if (bhalf && !boxhit[i]) bhalf[i] *= 2.0f;
+  ring[i] = (uint8_t)min(2 * (int)ring[i] + 1, ring_max);
 }
 
 // Next evaluation's ring: the outermost ring that shaped the cell plus one, at least ring_min.
@@ -712,9 +760,9 @@ hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
 }
 
 hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             float* bhalf, const uint8_t* boxhit, int nblocks, hipStream_t st) {
+                             int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(raise_ring_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, redo, ring,
-                     rmax, bhalf, boxhit);
+                     rmax);
   return hipGetLastError();
 }
 

@@ -1,0 +1,63 @@
+// Host-side layout of a problem: bucket grid, tile-ordered site permutation, block table.
+// Shared by the library (cot_api.cpp) and the host build of the kernels used in tests.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/caustic_ot.h"
+#include "cot_internal.h"
+
+namespace cot {
+
+// Bucket grid and tile-ordered permutation of one problem's sites.
+inline void bucket_sort(const cot_problem& pr, double spb, ProbDesc& pd, std::vector<int>& order,
+                 std::vector<int2>& buckets) {
+  const int n = pr.n_sites;
+  double xmin = pr.proj_domain[0], xmax = pr.proj_domain[1];
+  double ymin = pr.proj_domain[2], ymax = pr.proj_domain[3];
+  for (int i = 0; i < n; ++i) {
+    xmin = std::min(xmin, pr.sites[2 * i]);
+    xmax = std::max(xmax, pr.sites[2 * i]);
+    ymin = std::min(ymin, pr.sites[2 * i + 1]);
+    ymax = std::max(ymax, pr.sites[2 * i + 1]);
+  }
+  const double wdt = std::max(xmax - xmin, 1e-300), hgt = std::max(ymax - ymin, 1e-300);
+  double bs = std::sqrt(wdt * hgt * spb / std::max(n, 1));
+  int gx = std::max(1, (int)std::ceil(wdt / bs));
+  int gy = std::max(1, (int)std::ceil(hgt / bs));
+  // keep the grid bounded for degenerate inputs
+  while ((long long)gx * gy > 4LL * n + 64) {
+    bs *= 1.25;
+    gx = std::max(1, (int)std::ceil(wdt / bs));
+    gy = std::max(1, (int)std::ceil(hgt / bs));
+  }
+  pd.gx = gx;
+  pd.gy = gy;
+  pd.bx0 = xmin;
+  pd.by0 = ymin;
+  pd.bsize = bs;
+  pd.binv = 1.0 / bs;
+  const int tiles_x = (gx + 7) / 8;
+  std::vector<long long> key(n);
+  std::vector<int> cell(n);
+  for (int i = 0; i < n; ++i) {
+    int cx = std::min(std::max((int)std::floor((pr.sites[2 * i] - xmin) * pd.binv), 0), gx - 1);
+    int cy =
+        std::min(std::max((int)std::floor((pr.sites[2 * i + 1] - ymin) * pd.binv), 0), gy - 1);
+    cell[i] = cy * gx + cx;
+    long long tile = (long long)(cy / 8) * tiles_x + (cx / 8);
+    key[i] = (tile * 64 + (cy % 8) * 8 + (cx % 8));
+  }
+  order.resize(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  buckets.assign((size_t)gx * gy, make_int2(0, 0));
+  for (int r = 0; r < n; ++r) {
+    int b = cell[order[r]];
+    if (buckets[b].y == 0) buckets[b].x = r;
+    buckets[b].y += 1;
+  }
+}
+
+}  // namespace cot

@@ -237,18 +237,11 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   s.zeros_try = tot[2];
   s.g2_try = tot[3];
   s.gd_try = tot[4];
-  if (unresolved > 0) {
-    // cells still failed the conformity check: repeat the trial with the raised rings
-    if (++s.unverified > sp.max_unverified) {
-      s.status = 6;  // COT_EUNVERIFIED
-      s.phase = kPhaseDone;
-      active[p] = 0;
-    }
-    return;
-  }
-  s.unverified = 0;
   bool ok = false;
-  if (ovf > 0) {
+  if (ovf > 0 || unresolved > 0) {
+    // a cell over the vertex limit, or cells still unverified after the repair rounds: reject
+    // the trial like the reference's "polygon too complex" rule (ot_op.cuh:376-382); the raised
+    // rings stay for the next trial
     s.t *= 0.5;
   } else if (s.f_try < s.f_prev || (sp.mixed_ls && s.gd_try <= 0.0)) {
     ok = true;
@@ -387,16 +380,15 @@ __global__ __launch_bounds__(kCellBlock) void dev_accept_kernel(
 
 // Repair-round helpers driven by the per-problem failure counter of the previous check.
 __global__ void dev_raise_ring_kernel(const BlockDesc* blocks, const DevState* st,
-                                      const int* gate, uint8_t* redo, uint8_t* ring, int ring_max,
-                                      float* bhalf, const uint8_t* boxhit) {
+                                      const int* gate, uint8_t* redo, uint8_t* ring,
+                                      int ring_max) {
   const BlockDesc bd = blocks[blockIdx.x];
   if (st[bd.prob].phase != kPhaseTrial || gate[bd.prob] == 0) return;
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   const int i = bd.first + ln;
   if (!redo[i]) return;
-  ring[i] = (uint8_t)min((int)ring[i] + 1, ring_max);
-  if (bhalf && !boxhit[i]) bhalf[i] *= 2.0f;
+  ring[i] = (uint8_t)min(2 * (int)ring[i] + 1, ring_max);
 }
 
 __global__ void dev_adapt_ring_kernel(const BlockDesc* blocks, const DevState* st,
@@ -462,10 +454,10 @@ hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, Dev
 }
 
 hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
-                                 uint8_t* redo, uint8_t* ring, int ring_max, float* bhalf,
-                                 const uint8_t* boxhit, int nblk, hipStream_t stream) {
+                                 uint8_t* redo, uint8_t* ring, int ring_max, int nblk,
+                                 hipStream_t stream) {
   hipLaunchKernelGGL(dev_raise_ring_kernel, dim3(nblk), dim3(kCellBlock), 0, stream, blocks, st,
-                     gate, redo, ring, ring_max, bhalf, boxhit);
+                     gate, redo, ring, ring_max);
   return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ class OTBatch:
         self._keep = []  # the library copied everything
         self._ctx = ctx
         self._lib = lib
+        self.precision = precision
         self.n_total = int(sum(self.sizes))
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.n_problems = len(problems)
@@ -132,11 +133,14 @@ class OTBatch:
     def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop: bool = True,
-              accept_rule: str = "decrease", repair_rounds: int = 2):
+              accept_rule: str = "auto", repair_rounds: int = 2):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
-        accept_rule: "decrease" (f_trial < f, the reference rule) or "derivative" (also accept
-        a trial whose directional derivative is <= 0, for FP32 geometry)."""
+        accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
+        trial whose directional derivative is <= 0 -- for FP32 geometry, whose f is too coarse
+        for the last decades), or "auto" (derivative for mixed precision, decrease for fp64)."""
+        if accept_rule == "auto":
+            accept_rule = "derivative" if self.precision == "mixed" else "decrease"
         w = self._w(w0).copy()
         o = L.cot_solve_opts()
         self._lib.cot_default_solve_opts(ctypes.byref(o))

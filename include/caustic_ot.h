@@ -109,6 +109,28 @@ int cot_evaluate(cot_context* ctx, const double* w, double* mass, double* f, dou
  * Returns COT_OK when every problem met its stopping rule, COT_ENOTCONV otherwise. */
 int cot_solve(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
 
+/* Reflection-aware solve for ot_time >= 1 (aarl_direct_ot, ot_op.cuh:591-1385), one-problem
+ * contexts only.  Source masses follow the lens's Fresnel transmittance at the current cells'
+ * geometric centroids: nu_i = area_i T1_i T2_i / sum * area_scalar (ot_op.cuh:779-795).
+ * Stage A (if_aa_op_initial): L-BFGS with those masses frozen (the rules of cot_solve, reference
+ * stopping test).  Then the masses are re-estimated and stage B runs Anderson acceleration of
+ * G(w) = w + aa_fixed_para * (mass - nu) with aa_history_size pairs, recomputing nu at every
+ * trial, until |mass - nu|_2 < tol (ot_op.cuh:1112-1323).  w is in-out (caller order). */
+typedef struct {
+  const double* points;      /* n source points (x, y) in the lens plane (points.txt) */
+  const double* inside_dir;  /* 3n directions inside the lens (inside_light_direct.txt) */
+  const double* height;      /* n back-surface heights (<t>/height.txt) */
+  const double* clip_area;   /* n source-cell areas inside the sample domain */
+  double z_of_len, z_of_receiver, refraction_index;
+  double area_scalar;        /* area of the sample domain */
+  int aa_history_size;
+  double aa_fixed_para;
+  int if_aa_op_initial;
+} cot_reflection;
+
+int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
+                         const cot_reflection* refl, cot_report* report);
+
 /* Centroids of the cells at weights w: weighted != 0 -> rho-weighted (geometric fallback),
  * weighted == 0 -> geometric centroid of the domain-clipped cell; exist per site. */
 int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centroid,

@@ -236,7 +236,8 @@ def max_rel_error(mass, nu):
 
 
 def lbfgs(evaluate_fn, w0, nu, m=10, total_iter=50001, tol=1e-2, initial_step=1.0,
-          stop="reference", h0_threshold=1e-5, max_vertices=200, callback=None) -> LbfgsResult:
+          stop="reference", h0_threshold=1e-5, max_vertices=200, callback=None,
+          accept="decrease") -> LbfgsResult:
     """evaluate_fn(w) -> (f, mass, max_len).  Rules follow ot_op.cuh:289-537:
 
     * d = -g, two-loop recursion over <= m pairs, skipped when `restart` is set (:298-337)
@@ -246,6 +247,8 @@ def lbfgs(evaluate_fn, w0, nu, m=10, total_iter=50001, tol=1e-2, initial_step=1.
     * step <= 1e-4*step0 -> revert w, g, f; restart; clear history; skip the rest (:413-431)
     * stop (reference): |g_start|_2 < tol and no zero mass at the start of the iteration (:510)
     * stop (maxrel):    max |g|/nu < tol on the current iterate (checked after each update).
+    * accept="derivative" (not in the reference): a trial is also accepted when the directional
+      derivative (mass_trial - nu).d is <= 0, which for the convex objective certifies a decrease.
     """
     w = _f64(w0).copy()
     nu = _f64(nu)
@@ -291,7 +294,8 @@ def lbfgs(evaluate_fn, w0, nu, m=10, total_iter=50001, tol=1e-2, initial_step=1.
             res.evaluations += 1
             if max_len > max_vertices:
                 step *= 0.5
-            elif f_try < f_prev:
+            elif f_try < f_prev or (accept == "derivative" and
+                                    float(np.dot(mass_try - nu, d)) <= 0.0):
                 w, f, mass = w_try, f_try, mass_try
                 g = mass - nu
                 restart = False

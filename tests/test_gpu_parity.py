@@ -21,11 +21,15 @@ def _exact(p, w):
 
 
 def _compare(ev_gpu, ev_orc, nu, mass_tol, cen_tol):
-    dm = np.abs(ev_gpu["mass"] - ev_orc.mass) / nu
-    assert dm.max() <= mass_tol, f"mass error {dm.max():.3e} (relative to nu)"
+    """Masses within mass_tol relative to max(mass, nu) -- the cell's own mass, floored at its
+    source mass so that slivers are judged against the mass scale of the problem; centroids of
+    cells holding at least 1e-3 nu within cen_tol."""
+    dm = np.abs(ev_gpu["mass"] - ev_orc.mass) / np.maximum(ev_orc.mass, nu)
+    assert dm.max() <= mass_tol, f"mass error {dm.max():.3e} (relative to max(mass, nu))"
     ex = ev_gpu["exist"]
     assert np.array_equal(ex, ev_orc.exist.astype(bool))
-    dc = np.abs(ev_gpu["centroid"][ex] - ev_orc.centroid[ex]).max() if ex.any() else 0.0
+    big = ex & (ev_orc.mass >= 1e-3 * nu)
+    dc = np.abs(ev_gpu["centroid"][big] - ev_orc.centroid[big]).max() if big.any() else 0.0
     assert dc <= cen_tol, f"centroid error {dc:.3e}"
     return dm.max()
 
@@ -43,8 +47,9 @@ def test_eval_zero_weights(gpu_available, c1, precision, mass_tol):
     with _batch([c1], precision=precision) as b:
         ev = b.evaluate(w)
     _compare(ev, orc, c1.nu, mass_tol, 1e-11 if precision == "fp64" else 1e-6)
-    assert abs(ev["f"][0] - orc.f) <= 1e-12 * abs(orc.f) + 1e-18
-    assert abs(ev["mass"].sum() - 1.0) < 1e-12
+    f_tol = 1e-12 if precision == "fp64" else 1e-7
+    assert abs(ev["f"][0] - orc.f) <= f_tol * abs(orc.f)
+    assert abs(ev["mass"].sum() - 1.0) < (1e-12 if precision == "fp64" else 1e-7)
 
 
 @pytest.mark.parametrize("precision,mass_tol", [("fp64", 1e-10), ("mixed", 1e-6)])
@@ -58,7 +63,7 @@ def test_eval_random_weights_hidden_cells(gpu_available, c1, precision, mass_tol
         ev = b.evaluate(w)
     _compare(ev, orc, c1.nu, mass_tol, 1e-11 if precision == "fp64" else 1e-6)
     rel_f = abs(ev["f"][0] - orc.f) / abs(orc.f)
-    assert rel_f <= (1e-12 if precision == "fp64" else 1e-6)
+    assert rel_f <= (1e-12 if precision == "fp64" else 1e-7)
 
 
 @pytest.mark.parametrize("device_loop", [False, True])
@@ -90,16 +95,17 @@ def test_device_and_host_loops_follow_the_same_iterates(gpu_available, c1):
     assert np.abs(out[0][0] - out[1][0]).max() <= 1e-6 * scale
 
 
-def test_solve_c1_mixed_precision(gpu_available, c1):
-    print("[case] solve c1 mixed maxrel 1e-4", flush=True)
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_solve_c1_mixed_precision(gpu_available, c1, device_loop):
+    print(f"[case] solve c1 mixed maxrel 1e-5 device_loop={device_loop}", flush=True)
     with _batch([c1], precision="mixed") as b:
-        w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0, max_iter=5000,
-                         accept_rule="derivative")
+        w, rep = b.solve(history=10, tol=1e-5, stop="maxrel", h0_threshold=0.0, max_iter=5000,
+                         device_loop=device_loop)
     assert rep[0]["converged"] == 1, rep
     orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
-    # FP32 geometry: the oracle's FP64 masses at the returned weights stay within 1e-6 of nu
-    # of the solver's own stopping value
-    assert O.max_rel_error(orc.mass, c1.nu) < 1.0001e-4 + 1e-6
+    # FP32 geometry and image: the FP64 oracle's masses at the returned weights meet the stopping
+    # value to within 1e-6 relative
+    assert O.max_rel_error(orc.mass, c1.nu) < 1e-5 + 1e-6
 
 
 def test_solve_batch_of_uneven_problems(gpu_available):
