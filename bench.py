@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
@@ -76,7 +76,9 @@ def main():
     prob = W.large_shapes()
     tol = 1e-5
     batch = OTBatch([prob], precision=args.precision)
-    solve_kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000)
+    # verbose: the device loop prints a progress line every 2 s
+    solve_kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
+                    verbose=1)
     for _ in range(args.warmup):
         w, rep = batch.solve(**solve_kw)
         print(f"[warmup] iters {rep[0]['iterations']} evals {rep[0]['evaluations']} "
@@ -127,6 +129,12 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "avg_launch_us": cell_avg * 1e6, "launches": st["cell_launches"],
             "alg_bytes_per_launch": alg_bytes,
+        },
+        # in-kernel timers of the timed solves: total seconds and launches that did work
+        "kernels": {
+            "cell": [st["cell_seconds"], st["cell_launches"]],
+            "cell_repair": [st["repair_seconds"], st["repair_launches"]],
+            "verify": [st["verify_seconds"], st["verify_launches"]],
         },
     }
     if not args.no_cpu_baseline:

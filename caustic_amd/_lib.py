@@ -83,9 +83,9 @@ SIGNATURES = {
 
 class cot_stats(ctypes.Structure):
     _fields_ = [("cell_seconds", ctypes.c_double), ("cell_launches", ctypes.c_longlong),
-                ("verify_seconds", ctypes.c_double), ("repair_rounds", ctypes.c_longlong),
-                ("evaluations", ctypes.c_longlong), ("sites_evaluated", ctypes.c_longlong),
-                ("pixels_covered", ctypes.c_longlong)]
+                ("repair_seconds", ctypes.c_double), ("repair_launches", ctypes.c_longlong),
+                ("verify_seconds", ctypes.c_double), ("verify_launches", ctypes.c_longlong),
+                ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong)]
 
 _lib = None
 

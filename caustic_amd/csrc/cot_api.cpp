@@ -26,6 +26,9 @@
 namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
+                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
+                                 hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
                              int nblocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
@@ -149,7 +152,8 @@ struct cot_context {
   double2* d_xy = nullptr;
   double* d_nu = nullptr;
   int2* d_buckets = nullptr;
-  void* d_img = nullptr;
+  double4* d_pre = nullptr;       // row prefixes of the targets
+  double4* d_tot = nullptr;       // chunk totals
   double *d_w = nullptr, *d_dir = nullptr, *d_step = nullptr, *d_mass = nullptr;
   double *d_fcell = nullptr, *d_area = nullptr, *d_g = nullptr;
   double2 *d_cen = nullptr, *d_geo = nullptr;
@@ -158,6 +162,13 @@ struct cot_context {
   double2* d_org = nullptr;      // FP32 geometry: clipping frame origins
   int* d_elab = nullptr;
   float2* d_espan = nullptr;
+  int* d_adj = nullptr;          // previous evaluation's edge labels (candidates)
+  uint8_t* d_adjn = nullptr;     // previous evaluation's edge counts (0: none yet)
+  bool have_adj = false;
+  int* d_xcnt = nullptr;         // candidates queued by the conformity check
+  int* d_xcand = nullptr;
+  unsigned long long* d_timer = nullptr;  // kTimers records of kTimerWords
+  double clock_hz = 1e8;                  // wall-clock rate of the in-kernel timers
   int *d_overflow = nullptr, *d_fails = nullptr;
   double* d_part = nullptr;
   double *d_S = nullptr, *d_Y = nullptr;
@@ -182,7 +193,6 @@ struct cot_context {
   double* h_coef = nullptr;   // per-problem direction coefficients
   int* h_ismall = nullptr;
   uint8_t* h_active = nullptr;
-  hipEvent_t ev[4] = {};
   cot_stats stats{};
 
   ~cot_context() {
@@ -196,8 +206,6 @@ struct cot_context {
     if (h_ismall) hipHostFree(h_ismall);
     if (h_active) hipHostFree(h_active);
     if (h_ndone) hipHostFree(h_ndone);
-    for (auto& e : ev)
-      if (e) hipEventDestroy(e);
     if (st) hipStreamDestroy(st);
   }
 };
@@ -237,35 +245,21 @@ void sum_partials(cot_context* c, std::vector<EvalSum>& out) {
   }
 }
 
-// One evaluation of every active problem at w (+ step * dir when dir is set), with the
-// conformity check and repairs.  Leaves mass/fcell/... on the device and summaries on the host.
-int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
-                    const std::vector<uint8_t>& active, std::vector<EvalSum>& sum,
-                    int max_rounds = -1) {
-  if (max_rounds < 0) max_rounds = c->opt.max_repairs;
-  hipStream_t st = c->st;
-  std::memcpy(c->h_active, active.data(), c->np);
-  HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, st));
-  HIPCHK(launch_fill_i32(c->d_fails, 2 * (size_t)c->np, 0, st));
-  HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
-
-  CellArgs ca{};
+// Kernel arguments shared by every cell / conformity-check launch of the context.
+void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
+  ca = CellArgs{};
   ca.probs = c->d_probs;
   ca.blocks = c->d_blocks;
   ca.xy = c->d_xy;
-  ca.w = d_w;
-  ca.dir = d_dir;
-  ca.step = c->d_step;
+  ca.w = c->d_w;
   ca.nu = c->d_nu;
   ca.buckets = c->d_buckets;
-  ca.img = c->d_img;
+  ca.pre = c->d_pre;
+  ca.tot = c->d_tot;
   ca.ring = c->d_ring;
   ca.redo = c->d_redo;
-  ca.redo_mode = 0;
-  ca.st = nullptr;
-  ca.g = c->d_g;
-  ca.gate = nullptr;
   ca.active = c->d_active;
+  ca.g = c->d_g;
   ca.mass = c->d_mass;
   ca.fcell = c->d_fcell;
   ca.area = c->d_area;
@@ -278,8 +272,12 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.ring_used = c->d_used;
   ca.overflow = c->d_overflow;
   ca.org = c->d_org;
-
-  VerifyArgs va{};
+  ca.ring_max = c->opt.ring_max;
+  ca.adj = c->d_adj;
+  ca.adjn = c->d_adjn;
+  ca.xcnt = c->d_xcnt;
+  ca.xcand = c->d_xcand;
+  va = VerifyArgs{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
   va.xy = c->d_xy;
@@ -287,16 +285,66 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   va.elab = c->d_elab;
   va.espan = c->d_espan;
   va.active = c->d_active;
-  va.gate = nullptr;
   va.redo = c->d_redo;
   va.fails = c->d_fails;
   va.tol_rel = c->opt.verify_tol;
+  va.xcnt = c->d_xcnt;
+  va.xcand = c->d_xcand;
+  va.timer = c->d_timer + kTimerVerify * kTimerWords;
+}
 
-  HIPCHK(hipEventRecord(c->ev[0], st));
+// Candidates: the full pass takes the previous evaluation's cells' two-ring (an edge flip makes
+// a neighbour's neighbour adjacent, and the edge it creates starts too short for the conformity
+// check to see).  The first two repair rounds take the two-ring of the cells just built (larger
+// than the true cells, so their neighbour lists hold more) and the sites the check queued; later
+// rounds take the bucket rings, grown 2r + 1 before each round from the ring that held the
+// cell's last neighbours.  Without a previous evaluation the full pass uses the rings.
+constexpr int kAdjRepairRounds = 2;
+inline int repair_cand(int r) { return r <= kAdjRepairRounds ? kCandAdj2 : kCandBucket; }
+
+// Fold the in-kernel timers into the context's statistics and clear their accumulators.
+int pull_timers(cot_context* c) {
+  unsigned long long tm[kTimers * kTimerWords];
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(tm, c->d_timer, sizeof(tm), hipMemcpyDeviceToHost));
+  const double inv = 1.0 / c->clock_hz;
+  c->stats.cell_seconds += tm[kTimerCell * kTimerWords + 4] * inv;
+  c->stats.cell_launches += (long long)tm[kTimerCell * kTimerWords + 5];
+  c->stats.repair_seconds += tm[kTimerRepair * kTimerWords + 4] * inv;
+  c->stats.repair_launches += (long long)tm[kTimerRepair * kTimerWords + 5];
+  c->stats.verify_seconds += tm[kTimerVerify * kTimerWords + 4] * inv;
+  c->stats.verify_launches += (long long)tm[kTimerVerify * kTimerWords + 5];
+  for (int k = 0; k < kTimers; ++k) {
+    tm[k * kTimerWords + 4] = 0;
+    tm[k * kTimerWords + 5] = 0;
+  }
+  HIPCHK(hipMemcpy(c->d_timer, tm, sizeof(tm), hipMemcpyHostToDevice));
+  return COT_OK;
+}
+
+// One evaluation of every active problem at w (+ step * dir when dir is set), with the
+// conformity check and repairs.  Leaves mass/fcell/... on the device and summaries on the host.
+int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
+                    const std::vector<uint8_t>& active, std::vector<EvalSum>& sum,
+                    int max_rounds = -1) {
+  if (max_rounds < 0) max_rounds = c->opt.max_repairs;
+  hipStream_t st = c->st;
+  std::memcpy(c->h_active, active.data(), c->np);
+  HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_fill_i32(c->d_fails, 2 * (size_t)c->np, 0, st));
+  HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
+
+  CellArgs ca;
+  VerifyArgs va;
+  base_args(c, ca, va);
+  ca.w = d_w;
+  ca.dir = d_dir;
+  ca.step = c->d_step;
+  ca.cand = c->have_adj ? kCandAdj2 : kCandBucket;
+  ca.timer = c->d_timer + kTimerCell * kTimerWords;
+
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
-  HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_verify(va, c->nblk, st));
-  HIPCHK(hipEventRecord(c->ev[2], st));
   HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu, c->d_exist,
                             d_dir, c->d_part, c->nblk, st));
   HIPCHK(hipMemcpyAsync(c->h_part, c->d_part, sizeof(double) * kEvalPartials * c->nblk,
@@ -304,27 +352,22 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   HIPCHK(hipMemcpyAsync(c->h_counts, c->d_fails, sizeof(int) * 2 * c->np, hipMemcpyDeviceToHost,
                         st));
   HIPCHK(hipStreamSynchronize(st));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  c->stats.cell_seconds += ms * 1e-3;
-  HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
-  c->stats.verify_seconds += ms * 1e-3;
-  c->stats.cell_launches += 1;
   c->stats.evaluations += 1;
-  for (int p = 0; p < c->np; ++p) {
-    if (!active[p]) continue;
-    c->stats.sites_evaluated += c->probs[p].n;
-    c->stats.pixels_covered += (long long)c->probs[p].pm * c->probs[p].pn;
-  }
 
+  ca.redo_mode = 1;
+  ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   int rounds = 0;
   for (;;) {
     int tot = 0;
     for (int p = 0; p < c->np; ++p) tot += active[p] ? c->h_counts[p] : 0;
     if (tot == 0 || rounds >= max_rounds) break;
     ++rounds;
-    HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max, c->nblk, st));
-    ca.redo_mode = 1;
+    ca.cand = repair_cand(rounds);
+    if (ca.cand == kCandBucket)
+      HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max, c->nblk, st));
+    else
+      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
+                                   c->d_adj, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
     HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
@@ -339,8 +382,12 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   }
   c->stats.repair_rounds += rounds;
   sum_partials(c, sum);
-  // after a clean check the next evaluation starts from the outermost used ring plus one; after
-  // an unresolved one the raised rings stay
+  // the cells just built are the next evaluation's candidates
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+                               c->nblk, st));
+  c->have_adj = true;
+  // after a clean check the next bucket search starts from the outermost used ring plus one;
+  // after an unresolved one the raised rings stay
   bool clean = true;
   for (int p = 0; p < c->np; ++p) clean = clean && (!active[p] || sum[p].fails == 0);
   if (clean)
@@ -443,69 +490,43 @@ int ensure_device_solver(cot_context* c, int repair_rounds) {
   return COT_OK;
 }
 
-// One device round (see cot_solver.hip).  Captured once per solve into a hipGraph.
-int launch_round(cot_context* c, const SolveParams& sp, int R, hipEvent_t t0, hipEvent_t t1) {
+// One device round (see cot_solver.hip).  Captured once per solve into a hipGraph; the
+// previous evaluation's adjacency must exist (the solve starts with a host-driven evaluation).
+int launch_round(cot_context* c, const SolveParams& sp, int R) {
   hipStream_t st = c->st;
   const size_t ld = c->ntot, ps = c->nblk;
   const int np = c->np;
   HIPCHK(launch_dev_direction(c->d_probs, c->d_blocks, c->d_state, sp, c->d_S, c->d_Y, ld, c->d_g,
                               c->d_dir, c->d_pdir, ps, c->d_cnt, c->d_active, c->nblk, st));
-  CellArgs ca{};
-  ca.probs = c->d_probs;
-  ca.blocks = c->d_blocks;
-  ca.xy = c->d_xy;
-  ca.w = c->d_w;
+  CellArgs ca;
+  VerifyArgs va;
+  base_args(c, ca, va);
   ca.dir = c->d_dir;
-  ca.step = nullptr;
-  ca.nu = c->d_nu;
-  ca.buckets = c->d_buckets;
-  ca.img = c->d_img;
-  ca.ring = c->d_ring;
-  ca.redo = c->d_redo;
-  ca.redo_mode = 0;
-  ca.active = c->d_active;
   ca.st = c->d_state;
-  ca.g = c->d_g;
-  ca.gate = nullptr;
-  ca.mass = c->d_mass;
-  ca.fcell = c->d_fcell;
-  ca.area = c->d_area;
-  ca.cen = c->d_cen;
-  ca.geo = c->d_geo;
-  ca.exist = c->d_exist;
-  ca.nedge = c->d_nedge;
-  ca.elab = c->d_elab;
-  ca.espan = c->d_espan;
-  ca.ring_used = c->d_used;
-  ca.overflow = c->d_overflow;
-  ca.org = c->d_org;
-  VerifyArgs va{};
-  va.probs = c->d_probs;
-  va.blocks = c->d_blocks;
-  va.xy = c->d_xy;
-  va.nedge = c->d_nedge;
-  va.elab = c->d_elab;
-  va.espan = c->d_espan;
-  va.active = c->d_active;
-  va.gate = nullptr;
-  va.redo = c->d_redo;
+  ca.cand = kCandAdj2;
+  ca.timer = c->d_timer + kTimerCell * kTimerWords;
   va.fails = c->d_rfails;
-  va.tol_rel = c->opt.verify_tol;
-  if (t0) HIPCHK(hipEventRecord(t0, st));
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
-  if (t1) HIPCHK(hipEventRecord(t1, st));
   HIPCHK(launch_verify(va, c->nblk, st));
+  ca.redo_mode = 1;
+  ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
-    HIPCHK(launch_dev_raise_ring(c->d_blocks, c->d_state, gate, c->d_redo, c->d_ring,
-                                 c->opt.ring_max, c->nblk, st));
-    ca.redo_mode = 1;
+    ca.cand = repair_cand(r);
+    if (ca.cand == kCandBucket)
+      HIPCHK(launch_dev_raise_ring(c->d_blocks, c->d_state, gate, c->d_redo, c->d_ring,
+                                   c->opt.ring_max, c->nblk, st));
+    else
+      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+                                   c->nblk, st));
     ca.gate = gate;
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
     HIPCHK(launch_verify(va, c->nblk, st));
   }
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+                               c->nblk, st));
   const int* final_fails = c->d_rfails + (size_t)R * np;
   HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
                                c->opt.ring_max, c->nblk, st));
@@ -573,7 +594,7 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  int rc = launch_round(c, sp, R, c->ev[0], c->ev[1]);
+  int rc = launch_round(c, sp, R);
   hipError_t ce = hipStreamEndCapture(c->st, &graph);
   if (rc != COT_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -594,6 +615,7 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
   const int kBatch = 8;
   const long long max_rounds = (long long)(o.max_iter + 1) * 20 + 64;
   long long rounds = 0;
+  auto last_print = std::chrono::steady_clock::now();
   for (;;) {
     for (int k = 0; k < kBatch; ++k) HIPCHK(hipGraphLaunch(exec, c->st));
     rounds += kBatch;
@@ -601,14 +623,23 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
     HIPCHK(hipMemcpyAsync(c->h_ndone, c->d_ndone, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     const int left = c->h_ndone[0];
-    // live kernel timing from the event nodes around the full cell launch of the batch's last
-    // round (only while that round still had work)
-    float ms = 0.f;
-    if (left > 0 && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
-      c->stats.cell_seconds += ms * 1e-3;
-      c->stats.cell_launches += 1;
-    }
     if (left == 0 || rounds > max_rounds) break;
+    if (o.verbose > 0) {
+      // progress line at most every two seconds: the first unfinished problem's state
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last_print).count() >= 2.0) {
+        last_print = now;
+        HIPCHK(hipMemcpy(hs.data(), c->d_state, sizeof(DevState) * np, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; ++p) {
+          if (hs[p].phase == kPhaseDone) continue;
+          std::printf("[cot] %d left; prob %d iter %d evals %d maxrel %.3e step %g (%.1fs)\n",
+                      left, p, hs[p].iters, hs[p].evals, hs[p].maxrel_cur, hs[p].t,
+                      std::chrono::duration<double>(now - t_start).count());
+          std::fflush(stdout);
+          break;
+        }
+      }
+    }
   }
   HIPCHK(hipMemcpy(hs.data(), c->d_state, sizeof(DevState) * np, hipMemcpyDeviceToHost));
   CHK(download_sorted(c, c->d_w, w));
@@ -620,7 +651,7 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
     int status = s.status;
     if (!s.conv && status == COT_OK) status = COT_ENOTCONV;
     if (status != COT_OK && result == COT_OK) result = status;
-    c->stats.evaluations += s.evals;
+    c->stats.evaluations += s.evals - 1;  // the initial evaluation is already counted
     if (reports) {
       cot_report& r = reports[p];
       r.iterations = s.iters;
@@ -664,7 +695,7 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->verbose = 0;
   o->device_loop = 1;
   o->accept_rule = 0;
-  o->repair_rounds = 2;
+  o->repair_rounds = 3;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -701,14 +732,20 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   c->np = n_problems;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(COT_EHIP, "hipStreamCreate failed"));
-  for (auto& e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) return bail(fail(COT_EHIP, "hipEventCreate failed"));
+  {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0)
+      return bail(fail(COT_EHIP, "cannot read the device wall-clock rate"));
+    c->clock_hz = 1e3 * khz;
+  }
 
   // layout
   std::vector<int2> all_buckets;
   std::vector<double2> xy;
   std::vector<double> nu;
-  size_t img_total = 0;
+  size_t img_total = 0, tot_total = 0;
   int site_off = 0;
   for (int p = 0; p < n_problems; ++p) {
     const cot_problem& pr = problems[p];
@@ -717,7 +754,9 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     pd.n = pr.n_sites;
     pd.pm = pr.pm;
     pd.pn = pr.pn;
-    pd.img_off = (int)img_total;
+    pd.pre_off = (int)img_total;
+    pd.nchunk = (pr.pm + kChunk - 1) / kChunk;
+    pd.tot_off = (int)tot_total;
     pd.x0 = pr.proj_domain[0];
     pd.x1 = pr.proj_domain[1];
     pd.y0 = pr.proj_domain[2];
@@ -745,6 +784,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       nu.push_back(pr.nu[o]);
     }
     img_total += (size_t)pr.pm * pr.pn;
+    tot_total += (size_t)pd.nchunk * pr.pn;
     if (img_total > (size_t)INT32_MAX) return bail(fail(COT_EINVAL, "images too large"));
     c->probs.push_back(pd);
     site_off += pr.n_sites;
@@ -769,19 +809,15 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_part, (size_t)c->nblk * std::max(kEvalPartials, kAcceptPartials))) ||
       (rc = dalloc(c, &c->d_coef, (size_t)c->np * kCoefWidth)) ||
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
-      (rc = dalloc(c, &c->d_org, N)))
+      (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
+      (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
+      (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
+      (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
     return bail(rc);
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
-  if (c->fp32) {
-    float* im = nullptr;
-    if ((rc = dalloc(c, &im, img_total))) return bail(rc);
-    c->d_img = im;
-  } else {
-    double* im = nullptr;
-    if ((rc = dalloc(c, &im, img_total))) return bail(rc);
-    c->d_img = im;
-  }
+  if ((rc = dalloc(c, &c->d_pre, img_total)) || (rc = dalloc(c, &c->d_tot, tot_total)))
+    return bail(rc);
   if (hipHostMalloc((void**)&c->h_part,
                     sizeof(double) * c->nblk * std::max(kEvalPartials, kAcceptPartials)) ||
       hipHostMalloc((void**)&c->h_counts, sizeof(int) * 2 * c->np) ||
@@ -803,20 +839,24 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
             up(c->d_buckets, all_buckets.data(), sizeof(int2) * all_buckets.size());
   for (int p = 0; p < n_problems && ok; ++p) {
     const cot_problem& pr = problems[p];
-    const size_t P = (size_t)pr.pm * pr.pn;
-    if (c->fp32) {
-      std::vector<float> tmp(pr.rho, pr.rho + P);
-      ok = up((float*)c->d_img + c->probs[p].img_off, tmp.data(), sizeof(float) * P);
-    } else {
-      ok = up((double*)c->d_img + c->probs[p].img_off, pr.rho, sizeof(double) * P);
-    }
+    const ProbDesc& pd = c->probs[p];
+    std::vector<double4> pre((size_t)pr.pm * pr.pn), tot((size_t)pd.nchunk * pr.pn);
+    build_prefix(pr.rho, pr.pm, pr.pn, c->fp32, pre.data(), tot.data());
+    ok = up(c->d_pre + pd.pre_off, pre.data(), sizeof(double4) * pre.size()) &&
+         up(c->d_tot + pd.tot_off, tot.data(), sizeof(double4) * tot.size());
   }
   {
     // FP32 clipping frames start at the sites
     ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N);
+    // timers: no launch in flight (earliest start = max), accumulators zero
+    unsigned long long tm[kTimers * kTimerWords] = {};
+    for (int k = 0; k < kTimers; ++k) tm[k * kTimerWords] = ~0ull;
+    ok = ok && up(c->d_timer, tm, sizeof(tm));
   }
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
+      launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
+      launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
       hipMemset(c->d_fails, 0, sizeof(int) * 2 * c->np) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return bail(fail(COT_EHIP, "initialisation failed"));
@@ -1087,14 +1127,16 @@ int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
   return COT_OK;
 }
 
-int cot_get_stats(const cot_context* c, cot_stats* out) {
+int cot_get_stats(cot_context* c, cot_stats* out) {
   if (!c || !out) return fail(COT_EINVAL, "null argument");
+  CHK(pull_timers(c));
   *out = c->stats;
   return COT_OK;
 }
 
 int cot_reset_stats(cot_context* c) {
   if (!c) return fail(COT_EINVAL, "null context");
+  CHK(pull_timers(c));
   c->stats = cot_stats{};
   return COT_OK;
 }
@@ -1219,7 +1261,8 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
     if (!any_trial) break;
     HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice,
                           c->st));
-    CHK(evaluate_device(c, c->d_w, c->d_dir, act, trial, std::min(c->opt.max_repairs, 3)));
+    CHK(evaluate_device(c, c->d_w, c->d_dir, act, trial,
+                        std::min(c->opt.max_repairs, o.repair_rounds)));
 
     // ---- decisions (ot_op.cuh:360-431)
     std::vector<uint8_t> acc(np, 0);

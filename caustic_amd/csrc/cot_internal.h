@@ -14,16 +14,38 @@ constexpr int kCellBlock = 128;
 constexpr int kVecBlock = 256;
 // Largest L-BFGS history.
 constexpr int kMaxHist = 16;
+// Row prefixes of the target: columns in chunks of kChunk.  Entry (iy, ix) holds the sums over
+// the pixels k of the chunk of ix that lie left of ix, of rho_k * (1, k' + 1/2, k'^2 + k' + 1/3)
+// with k' = k - (chunk start) -- the integrals of 1, U, U^2 over whole pixels in chunk-local
+// coordinates -- and rho at (iy, ix) itself; the chunk totals hold the same three sums over
+// whole chunks.  Sums between two columns then need no subtraction of large prefixes.
+constexpr int kChunk = 32;
+
 // Edge labels below zero: the four sides of the target rectangle.
 constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4;
+// Candidate sources of the cell kernel.  kCandBucket: the bucket rings 0..ring[i] around the
+// site.  kCandAdj1 / kCandAdj2: the neighbours of the site in the previous evaluation's cells
+// (adj), and for kCandAdj2 also their neighbours.  A site with no previous neighbours (a hidden
+// cell, or no previous evaluation) uses its bucket rings, widened to 2 ring + 1 for kCandAdj2.
+// After an adjacency pass ring_used[i] is the bucket ring holding all of the cell's neighbours.
+enum : int { kCandBucket = 0, kCandAdj1 = 1, kCandAdj2 = 2 };
+// Candidates the conformity check hands to a repair: when site i has an edge with site j that
+// j's cell lacks, i is queued for j (at most kMaxExtra per site and round).
+constexpr int kMaxExtra = 4;
+// Kernel timer (in-kernel wall clock, one record per kernel kind): [0] earliest block start of
+// the current launch, [1] latest block end, [2] blocks arrived, [3] blocks that did work,
+// [4] accumulated ticks of launches that did work, [5] such launches.
+constexpr int kTimerWords = 6;
+enum : int { kTimerCell = 0, kTimerRepair = 1, kTimerVerify = 2, kTimers = 3 };
 
 // One OT problem of a batch.  All per-site arrays are indexed by global sorted site index.
 struct ProbDesc {
   int site_off, n;        // sites [site_off, site_off + n) in bucket order
-  int img_off, pm, pn;    // target image: pm pixels along x (fast index), pn along y
+  int pre_off, pm, pn;    // target: pm pixels along x (fast index), pn along y; row prefixes
+                          // at pre_off (pn * pm entries)
   int bkt_off, gx, gy;    // bucket grid: gx * gy buckets starting at bkt_off
   int blk_off, nblk;      // cell/vector blocks of this problem
-  int pad0, pad1;
+  int tot_off, nchunk;    // chunk totals at tot_off (pn * nchunk entries)
   double x0, x1, y0, y1;  // target rectangle (proj_domain)
   double dx, dy;          // pixel size
   double bx0, by0;        // bucket grid origin
@@ -46,7 +68,8 @@ struct CellArgs {
   const double* step;       // per-problem step (used when dir != nullptr)
   const double* nu;         // source masses
   const int2* buckets;      // (start, count) per bucket
-  const void* img;          // float or double image, per-problem offsets
+  const double4* pre;       // row prefixes (x, y, z: the three sums; w: the pixel's rho)
+  const double4* tot;       // chunk totals (x, y, z)
   uint8_t* ring;            // per-site candidate ring radius (read, raised on repair)
   uint8_t* redo;            // repair flags; every processed site's flag is cleared
   int redo_mode;            // 0: all sites of active problems; 1: only sites with redo[i] != 0
@@ -68,6 +91,14 @@ struct CellArgs {
   int* overflow;            // per-problem count of cells that exceeded kMaxVerts
   // FP32 geometry only: clipping frame origin (the cell's last geometric centroid)
   double2* org;
+  // candidate source (kCand*) and the previous evaluation's adjacency
+  int cand;
+  int ring_max;
+  const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
+  const uint8_t* adjn;      // edge count of the previous cell (0: none)
+  int* xcnt;                // per site: queued candidates (cleared when the site is built)
+  const int* xcand;         // [site * kMaxExtra + k]
+  unsigned long long* timer;  // kTimerWords words, or nullptr
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -110,6 +141,9 @@ struct VerifyArgs {
   uint8_t* redo;            // set for both cells of a mismatched edge
   int* fails;               // per-problem mismatch count
   double tol_rel;           // tolerance relative to the site spacing of the pair
+  int* xcnt;                // candidate queues of the repair (see kMaxExtra)
+  int* xcand;
+  unsigned long long* timer;  // kTimerWords words, or nullptr
 };
 
 }  // namespace cot

@@ -60,6 +60,15 @@ __device__ double block_max(double v, double* red) {
   return s;
 }
 
+// Atomic increment usable from the per-site code on the device and in its host build.
+__host__ __device__ __forceinline__ int fetch_add(int* p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(p, v);
+#else
+  return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+}
+
 // Neumaier compensated accumulation.
 struct KSum {
   double s = 0.0, c = 0.0;
@@ -152,31 +161,68 @@ __host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy,
   return out >= 3 ? out : 0;
 }
 
-// Row-prefix cache for the Green walk: sums over full pixels k = 0 .. col-1 (local columns) of
-// rho_k * (1, k + 1/2, k^2 + k + 1/3), i.e. the integrals of 1, U, U^2 over whole pixels.
-template <typename TI>
-struct RowCache {
-  const TI* base;  // image + koy * pm + kox
-  int pm;
-  int row = -1, col = 0;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+// Row-prefix lookup for the Green walk.  Columns are local to the cell (local column 0 = global
+// column kox); at(iy, ix) gives the sums over the whole pixels of local columns 0 .. ix-1 of
+// rho_k * (1, k + 1/2, k^2 + k + 1/3) (the integrals of 1, U, U^2 over those pixels, local
+// coordinates) and rho at (iy, ix).  The sums are assembled from the chunked prefixes: the part
+// of kox's chunk right of kox (cached per row), whole chunks, and the part of ix's chunk left of
+// ix, each moved to the local origin by at most a chunk plus the cell's width.
+struct Prefix {
+  const double4* pre;  // this problem's row prefixes
+  const double4* tot;  // this problem's chunk totals
+  int pm, nchunk, kox, koy;
+  int row = -1;
+  double q0, q1, q2;   // prefix of kox within its chunk (chunk-local)
+  double f0, f1, f2;   // kox .. end of its chunk, local coordinates (when ix is past it)
 
-  __host__ __device__ __forceinline__ void add_pixel(const TI* r, int k, double sgn) {
-    double v = sgn * (double)r[k];
-    double kk = (double)k;
-    a0 += v;
-    a1 += v * (kk + 0.5);
-    a2 += v * (kk * kk + kk + 1.0 / 3.0);
+  // moments s (coordinates k') moved to coordinates k' + d
+  __host__ __device__ static __forceinline__ void shift(double& s0, double& s1, double& s2,
+                                                        double d) {
+    s2 += 2.0 * d * s1 + d * d * s0;
+    s1 += d * s0;
   }
-  __host__ __device__ __forceinline__ void seek(int iy, int ix) {
-    const TI* r = base + (size_t)iy * pm;
+  __host__ __device__ __forceinline__ void at(int iy, int ix, double& a0, double& a1, double& a2,
+                                              double& r) {
+    const int grow = koy + iy, g = kox + ix;
+    const double4* qr = pre + (size_t)grow * pm;
+    const int ck = kox / kChunk, cg = g / kChunk;
     if (iy != row) {
       row = iy;
-      col = 0;
-      a0 = a1 = a2 = 0.0;
+      const double4 qk = qr[kox];
+      q0 = qk.x;
+      q1 = qk.y;
+      q2 = qk.z;
+      const double4 tk = tot[(size_t)grow * nchunk + ck];
+      f0 = tk.x - q0;
+      f1 = tk.y - q1;
+      f2 = tk.z - q2;
+      shift(f0, f1, f2, (double)(ck * kChunk - kox));
     }
-    while (col < ix) add_pixel(r, col++, 1.0);
-    while (col > ix) add_pixel(r, --col, -1.0);
+    const double4 t = qr[g];
+    r = t.w;
+    if (cg == ck) {
+      a0 = t.x - q0;
+      a1 = t.y - q1;
+      a2 = t.z - q2;
+      shift(a0, a1, a2, (double)(ck * kChunk - kox));
+      return;
+    }
+    a0 = f0;
+    a1 = f1;
+    a2 = f2;
+    for (int c = ck + 1; c < cg; ++c) {
+      const double4 tc = tot[(size_t)grow * nchunk + c];
+      double b0 = tc.x, b1 = tc.y, b2 = tc.z;
+      shift(b0, b1, b2, (double)(c * kChunk - kox));
+      a0 += b0;
+      a1 += b1;
+      a2 += b2;
+    }
+    double b0 = t.x, b1 = t.y, b2 = t.z;
+    shift(b0, b1, b2, (double)(cg * kChunk - kox));
+    a0 += b0;
+    a1 += b1;
+    a2 += b2;
   }
 };
 
@@ -190,15 +236,16 @@ struct Moments {
 // TP = double: everything in FP64.  TP = float: the polynomial terms of the piece are evaluated
 // in FP32 in coordinates local to the pixel (u, v in [0, 1]); the row prefixes A0..A2, the
 // integer offsets and every accumulation stay FP64.
-template <typename TP, typename TI>
-__host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix, int iy,
-                                      RowCache<TI>& rc, Moments& M) {
+template <typename TP>
+__host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix,
+                                               int iy, Prefix& px, Moments& M) {
   const double dv = V1 - V0;
   if (dv == 0.0) return;
-  rc.seek(iy, ix);
+  double a0, a1, a2, rv;
+  px.at(iy, ix, a0, a1, a2, rv);
   const double fx = (double)ix, fy = (double)iy;
   if constexpr (sizeof(TP) == 4) {
-    const float r = (float)rc.base[(size_t)iy * rc.pm + ix];
+    const float r = (float)rv;
     const float u0 = (float)(U0 - fx), u1 = (float)(U1 - fx);
     const float v0 = (float)(V0 - fy), v1 = (float)(V1 - fy);
     const float dvf = (float)dv;
@@ -210,50 +257,51 @@ __host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, 
     const float Ivv = (v0 * v0 + v0 * v1 + v1 * v1) * (1.0f / 3.0f) * dvf;        // int v^2
     const float Ivu = (v0 * u0 + 4.0f * vm * um + v1 * u1) * (1.0f / 6.0f) * dvf;
     const float Ivvu = (v0 * v0 * u0 + 4.0f * vm * vm * um + v1 * v1 * u1) * (1.0f / 6.0f) * dvf;
-    const double pm = rc.a0 * dv + (double)(r * Iu);  // int F0 dV over the piece
+    const double pm = a0 * dv + (double)(r * Iu);  // int F0 dV over the piece
     M.m0.add(pm);
-    M.m1u += rc.a1 * dv + fx * (double)(r * Iu) + 0.5 * (double)(r * Iuu);
-    const double vF0 = rc.a0 * (double)Iv + (double)(r * Ivu);  // int v F0 dV
+    M.m1u += a1 * dv + fx * (double)(r * Iu) + 0.5 * (double)(r * Iuu);
+    const double vF0 = a0 * (double)Iv + (double)(r * Ivu);  // int v F0 dV
     M.m1v += fy * pm + vF0;
-    M.m2u += rc.a2 * dv + fx * fx * (double)(r * Iu) + fx * (double)(r * Iuu) +
+    M.m2u += a2 * dv + fx * fx * (double)(r * Iu) + fx * (double)(r * Iuu) +
              (1.0 / 3.0) * (double)(r * Iuuu);
-    const double v2F0 = rc.a0 * (double)Ivv + (double)(r * Ivvu);  // int v^2 F0 dV
+    const double v2F0 = a0 * (double)Ivv + (double)(r * Ivvu);  // int v^2 F0 dV
     M.m2v += fy * fy * pm + 2.0 * fy * vF0 + v2F0;
   } else {
-    const double rc_ = (double)rc.base[(size_t)iy * rc.pm + ix];
+    const double rc_ = rv;
     const double Um = 0.5 * (U0 + U1), Vm = 0.5 * (V0 + V1);
     // mass: int (A0 + rho (U - ix)) dV
-    M.m0.add((rc.a0 + rc_ * (Um - fx)) * dv);
+    M.m0.add((a0 + rc_ * (Um - fx)) * dv);
     // int rho U: int (A1 + rho (U^2 - ix^2)/2) dV
     const double u2avg = (U0 * U0 + U0 * U1 + U1 * U1) * (1.0 / 3.0);
-    M.m1u += (rc.a1 + 0.5 * rc_ * (u2avg - fx * fx)) * dv;
+    M.m1u += (a1 + 0.5 * rc_ * (u2avg - fx * fx)) * dv;
     // int rho V: int V (A0 + rho (U - ix)) dV
     const double vu = (V0 * U0 + 4.0 * Vm * Um + V1 * U1) * (1.0 / 6.0) * dv;
-    M.m1v += (rc.a0 - rc_ * fx) * 0.5 * (V1 * V1 - V0 * V0) + rc_ * vu;
+    M.m1v += (a0 - rc_ * fx) * 0.5 * (V1 * V1 - V0 * V0) + rc_ * vu;
     // int rho U^2: int (A2 + rho (U^3 - ix^3)/3) dV
     const double u3 = (U0 * U0 * U0 + 4.0 * Um * Um * Um + U1 * U1 * U1) * (1.0 / 6.0) * dv;
-    M.m2u += (rc.a2 - rc_ * fx * fx * fx * (1.0 / 3.0)) * dv + rc_ * (1.0 / 3.0) * u3;
+    M.m2u += (a2 - rc_ * fx * fx * fx * (1.0 / 3.0)) * dv + rc_ * (1.0 / 3.0) * u3;
     // int rho V^2: int V^2 (A0 + rho (U - ix)) dV
     const double v2u = (V0 * V0 * U0 + 4.0 * Vm * Vm * Um + V1 * V1 * U1) * (1.0 / 6.0) * dv;
-    M.m2v += (rc.a0 - rc_ * fx) * (V1 * V1 * V1 - V0 * V0 * V0) * (1.0 / 3.0) + rc_ * v2u;
+    M.m2v += (a0 - rc_ * fx) * (V1 * V1 * V1 - V0 * V0 * V0) * (1.0 / 3.0) + rc_ * v2u;
   }
 }
 
 // Walk the edge (Ua,Va)->(Ub,Vb) across the pixel lattice (local pixel units, lattice limits
 // lim_u x lim_v) and add each piece.
-template <typename T, typename TI>
+template <typename T>
 __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, int lim_u, int lim_v,
-                          RowCache<TI>& rc, Moments& M) {
+                                   Prefix& px, Moments& M) {
   const double du = ub - ua, dvv = vb - va;
   if (dvv == 0.0) return;
   int iu = (int)floor(ua), iv = (int)floor(va);
   const int iu_end = (int)floor(ub), iv_end = (int)floor(vb);
   const int su = du > 0.0 ? 1 : -1, sv = dvv > 0.0 ? 1 : -1;
+  const double idu = du != 0.0 ? 1.0 / du : 0.0, idv = 1.0 / dvv;
   // next lattice lines crossed
   int nu_line = du > 0.0 ? iu + 1 : iu;
   int nv_line = dvv > 0.0 ? iv + 1 : iv;
-  double tu = du != 0.0 ? ((double)nu_line - ua) / du : 2.0;
-  double tv = ((double)nv_line - va) / dvv;
+  double tu = du != 0.0 ? ((double)nu_line - ua) * idu : 2.0;
+  double tv = ((double)nv_line - va) * idv;
   const int maxsteps = abs(iu_end - iu) + abs(iv_end - iv) + 2;
   double t = 0.0, U0 = ua, V0 = va;
   for (int s = 0; s <= maxsteps; ++s) {
@@ -271,14 +319,14 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
     int iy = (int)floor(0.5 * (V0 + V1));
     ix = min(max(ix, 0), lim_u - 1);
     iy = min(max(iy, 0), lim_v - 1);
-    piece<T, TI>(U0, V0, U1, V1, ix, iy, rc, M);
+    piece<T>(U0, V0, U1, V1, ix, iy, px, M);
     if (tn >= 1.0) break;
     if (tu <= tv) {
       nu_line += su;
-      tu = ((double)nu_line - ua) / du;
+      tu = ((double)nu_line - ua) * idu;
     } else {
       nv_line += sv;
-      tv = ((double)nv_line - va) / dvv;
+      tv = ((double)nv_line - va) * idv;
     }
     t = tn;
     U0 = U1;
@@ -290,16 +338,17 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
 // cell kernel
 // ---------------------------------------------------------------------------------------------
 
-// One site of the cell kernel (lane `ln` of block `blk`); returns true when the cell exceeded
-// kMaxVerts.  Shared by the kernel and the host build used to test this code on the CPU.
-template <typename T, typename TI>
-__host__ __device__ bool cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>& P) {
+// One site of the cell kernel (lane `ln` of block `blk`); returns -1 when the lane has no site
+// to build, 1 when the cell exceeded kMaxVerts, 0 otherwise.  Shared by the kernel and the host
+// build used to test this code on the CPU.
+template <typename T>
+__host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>& P) {
   const BlockDesc bd = a.blocks[blk];
-  if (ln >= bd.count) return false;
-  if (!a.active[bd.prob]) return false;
-  if (a.gate && a.gate[bd.prob] == 0) return false;
+  if (ln >= bd.count) return -1;
+  if (!a.active[bd.prob]) return -1;
+  if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + ln;
-  if (a.redo_mode && !a.redo[i]) return false;
+  if (a.redo_mode && !a.redo[i]) return -1;
   if (a.redo) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   // trial weights w + t * d, with d = -g for a steepest-descent trial
@@ -350,42 +399,90 @@ __host__ __device__ bool cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T
   };
   double bcx, bcy, br2;
   bound(bcx, bcy, br2);
+  int used = 0;
+  // clip by the bisector of site j (candidate from bucket ring r); `dedupe`: j may already bound
+  // an edge (a candidate list with repeats), then clipping by the same line again is skipped
+  auto consider = [&](int j, int r, bool dedupe) {
+    const double2 sj = a.xy[j];
+    const double wj = wtrial(j);
+    const double ddx = sj.x - s.x, ddy = sj.y - s.y;
+    // half-plane u.d <= c in the clipping frame (u = x - o)
+    const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj) - (osx * ddx + osy * ddy);
+    // the half-plane u.d <= c contains the bounding circle: nothing to clip
+    const double slack = c - (bcx * ddx + bcy * ddy);
+    if (slack >= 0.0 && slack * slack >= br2 * (ddx * ddx + ddy * ddy)) return;
+    if (dedupe) {
+      bool have = false;
+      for (int e = 0; e < nv; ++e) have = have || P.lab[e][ln] == j;
+      if (have) return;
+    }
+    bool changed;
+    nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed);
+    if (changed) {
+      used = max(used, r);
+      if (nv > 0) bound(bcx, bcy, br2);
+    }
+  };
+  const int R0 = a.ring[i];
+  const int nadj = a.cand != kCandBucket ? (int)a.adjn[i] : 0;
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
-  const int R = a.ring[i];
-  int used = 0;
-  const int2* bk = a.buckets + pd.bkt_off;
-
-  for (int r = 0; r <= R && nv > 0; ++r) {
-    const int ylo = by - r, yhi = by + r;
-    for (int cy = max(ylo, 0); cy <= min(yhi, pd.gy - 1) && nv > 0; ++cy) {
-      const bool full = (cy == ylo) || (cy == yhi);
-      const int stepx = full ? 1 : 2 * r;
-      for (int cx = bx - r; cx <= bx + r && nv > 0; cx += stepx) {
-        if (cx < 0 || cx >= pd.gx) continue;
-        const int2 b = bk[cy * pd.gx + cx];
-        for (int jj = 0; jj < b.y && nv > 0; ++jj) {
-          const int j = pd.site_off + b.x + jj;
-          if (j == i) continue;
-          const double2 sj = a.xy[j];
-          const double wj = wtrial(j);
-          const double ddx = sj.x - s.x, ddy = sj.y - s.y;
-          // half-plane u.d <= c in the clipping frame (u = x - o)
-          const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj) - (osx * ddx + osy * ddy);
-          // the half-plane u.d <= c contains the bounding circle: nothing to clip
-          const double slack = c - (bcx * ddx + bcy * ddy);
-          if (slack >= 0.0 && slack * slack >= br2 * (ddx * ddx + ddy * ddy)) continue;
-          bool changed;
-          nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed);
-          if (changed) {
-            used = r;
-            if (nv > 0) bound(bcx, bcy, br2);
+  // candidates queued by the conformity check
+  if (a.xcnt) {
+    const int nx = min(a.xcnt[i], kMaxExtra);
+    for (int k = 0; k < nx && nv > 0; ++k) consider(a.xcand[(size_t)i * kMaxExtra + k], 0, true);
+    a.xcnt[i] = 0;
+  }
+  if (nadj > 0) {
+    // the previous cell's neighbours ...
+    const int* row = a.adj + (size_t)i * kMaxVerts;
+    const bool queued = a.xcnt != nullptr;
+    for (int k = 0; k < nadj && nv > 0; ++k) {
+      const int j = row[k];
+      if (j >= 0 && j != i) consider(j, 0, queued);
+    }
+    // ... and theirs (an edge flip makes a neighbour's neighbour adjacent)
+    if (a.cand == kCandAdj2) {
+      for (int k = 0; k < nadj && nv > 0; ++k) {
+        const int j = row[k];
+        if (j < 0) continue;
+        const int nj = a.adjn[j];
+        const int* rj = a.adj + (size_t)j * kMaxVerts;
+        for (int q = 0; q < nj && nv > 0; ++q) {
+          const int jj = rj[q];
+          if (jj >= 0 && jj != i) consider(jj, 0, true);
+        }
+      }
+    }
+    // the bucket ring that holds every neighbour of the cell (the fallback search's start)
+    for (int k = 0; k < nv; ++k) {
+      const int j = P.lab[k][ln];
+      if (j < 0) continue;
+      const double2 sj = a.xy[j];
+      const int qx = min(max((int)floor((sj.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
+      const int qy = min(max((int)floor((sj.y - pd.by0) * pd.binv), 0), pd.gy - 1);
+      used = max(used, max(abs(qx - bx), abs(qy - by)));
+    }
+  } else {
+    const int R = a.cand == kCandAdj2 ? min(2 * R0 + 1, a.ring_max) : R0;
+    const int2* bk = a.buckets + pd.bkt_off;
+    for (int r = 0; r <= R && nv > 0; ++r) {
+      const int ylo = by - r, yhi = by + r;
+      for (int cy = max(ylo, 0); cy <= min(yhi, pd.gy - 1) && nv > 0; ++cy) {
+        const bool full = (cy == ylo) || (cy == yhi);
+        const int stepx = full ? 1 : 2 * r;
+        for (int cx = bx - r; cx <= bx + r && nv > 0; cx += stepx) {
+          if (cx < 0 || cx >= pd.gx) continue;
+          const int2 b = bk[cy * pd.gx + cx];
+          for (int jj = 0; jj < b.y && nv > 0; ++jj) {
+            const int j = pd.site_off + b.x + jj;
+            if (j != i) consider(j, r, false);
           }
         }
       }
     }
   }
-  a.ring_used[i] = (uint8_t)used;
+  a.ring_used[i] = (uint8_t)min(used, 255);
   a.nedge[i] = (uint8_t)nv;
 
   double m0 = 0.0, fi = -wi * a.nu[i] , ar = 0.0;
@@ -503,15 +600,18 @@ __host__ __device__ bool cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T
     const double fv0 = (o.y - pd.y0) * idy_ - (double)koy;
     const double ou = (s.x - pd.x0) * idx_ - (double)kox;  // site, local pixel units
     const double ov = (s.y - pd.y0) * idy_ - (double)koy;
-    const TI* img = reinterpret_cast<const TI*>(a.img) + pd.img_off;
-    RowCache<TI> rc;
-    rc.base = img + (size_t)koy * pd.pm + kox;
-    rc.pm = pd.pm;
+    Prefix px;
+    px.pre = a.pre + pd.pre_off;
+    px.tot = a.tot + pd.tot_off;
+    px.pm = pd.pm;
+    px.nchunk = pd.nchunk;
+    px.kox = kox;
+    px.koy = koy;
     Moments M;
     const int lim_u = pd.pm - kox, lim_v = pd.pn - koy;
     for_each_edge([&](int, double2 va, double2 vb) {
-      walk_edge<T, TI>(fu0 + va.x * idx_, fv0 + va.y * idy_, fu0 + vb.x * idx_,
-                       fv0 + vb.y * idy_, lim_u, lim_v, rc, M);
+      walk_edge<T>(fu0 + va.x * idx_, fv0 + va.y * idy_, fu0 + vb.x * idx_, fv0 + vb.y * idy_,
+                   lim_u, lim_v, px, M);
     });
     m0 = M.m0.value();
     // second moment about the site, physical units
@@ -533,26 +633,74 @@ __host__ __device__ bool cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T
   a.cen[i] = cen;
   a.geo[i] = geo;
   a.exist[i] = nv >= 3 ? 1 : 0;
-  return overflow;
+  return overflow ? 1 : 0;
 }
 
-template <typename T, typename TI>
+// In-kernel launch timer (kTimerWords words, see cot_internal.h), read with the constant-rate
+// wall clock: the launch's duration runs from the earliest block start to the latest end of a
+// block that did work; launches in which no block did work are not counted.
+__device__ __forceinline__ void timer_start(unsigned long long* tm) {
+  if (tm && threadIdx.x == 0) atomicMin(&tm[0], (unsigned long long)wall_clock64());
+}
+
+__device__ void timer_stop(unsigned long long* tm, bool worked) {
+  if (!tm) return;
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  if (worked) any = 1;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (any) {
+    atomicMax(&tm[1], (unsigned long long)wall_clock64());
+    atomicAdd(&tm[3], 1ull);
+  }
+  __threadfence();
+  const unsigned long long prev = atomicAdd(&tm[2], 1ull);
+  if (prev != gridDim.x - 1) return;
+  __threadfence();
+  const unsigned long long t0 = atomicAdd(&tm[0], 0ull), t1 = atomicAdd(&tm[1], 0ull);
+  if (atomicAdd(&tm[3], 0ull) > 0 && t1 > t0) {
+    atomicAdd(&tm[4], t1 - t0);
+    atomicAdd(&tm[5], 1ull);
+  }
+  atomicExch(&tm[0], ~0ull);
+  atomicExch(&tm[1], 0ull);
+  atomicExch(&tm[2], 0ull);
+  atomicExch(&tm[3], 0ull);
+}
+
+template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   __shared__ PolyLDS<T> P;
-  if (cell_site<T, TI>(a, blockIdx.x, threadIdx.x, P))
-    atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
+  timer_start(a.timer);
+  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
+  if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
+  timer_stop(a.timer, rc >= 0);
+}
+
+// The same code for a repair pass (only sites flagged in redo[]); a kernel of its own so that
+// profiles and timers keep full evaluations and repairs apart.
+template <typename T>
+__global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
+  __shared__ PolyLDS<T> P;
+  timer_start(a.timer);
+  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
+  if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
+  timer_stop(a.timer, rc >= 0);
 }
 
 // ---------------------------------------------------------------------------------------------
 // conformity check
 // ---------------------------------------------------------------------------------------------
 
-// One site of the conformity check; returns true when one of its edges is unmatched.
-__host__ __device__ bool verify_site(const VerifyArgs& a, int blk, int ln) {
+// One site of the conformity check; returns -1 when the lane has no site to check, 1 when one
+// of its edges is unmatched, 0 otherwise.
+__host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
   const BlockDesc bd = a.blocks[blk];
-  if (ln >= bd.count) return false;
-  if (!a.active[bd.prob]) return false;
-  if (a.gate && a.gate[bd.prob] == 0) return false;
+  if (ln >= bd.count) return -1;
+  if (!a.active[bd.prob]) return -1;
+  if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + ln;
   const int ne = a.nedge[i];
   const double2 si = a.xy[i];
@@ -578,14 +726,39 @@ __host__ __device__ bool verify_site(const VerifyArgs& a, int blk, int ln) {
     if (found ? !match : (len > tol)) {
       bad = true;
       a.redo[j] = 1;
+      if (!found && a.xcnt) {
+        // j's cell lacks the edge: i is a candidate j missed
+        const int q = fetch_add(&a.xcnt[j], 1);
+        if (q < kMaxExtra) a.xcand[(size_t)j * kMaxExtra + q] = i;
+      }
     }
   }
   if (bad) a.redo[i] = 1;
-  return bad;
+  return bad ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
-  if (verify_site(a, blockIdx.x, threadIdx.x)) atomicAdd(&a.fails[a.blocks[blockIdx.x].prob], 1);
+  timer_start(a.timer);
+  const int rc = verify_site(a, blockIdx.x, threadIdx.x);
+  if (rc == 1) atomicAdd(&a.fails[a.blocks[blockIdx.x].prob], 1);
+  timer_stop(a.timer, rc >= 0);
+}
+
+// Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
+// repair).  gate: skip problems whose counter is zero (nullptr: none).
+__global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
+                                      const uint8_t* nedge, const int* elab, uint8_t* adjn,
+                                      int* adj) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (gate && gate[bd.prob] == 0) return;
+  const int ln = threadIdx.x;
+  if (ln >= bd.count) return;
+  const int i = bd.first + ln;
+  const int ne = nedge[i];
+  adjn[i] = (uint8_t)ne;
+  const int* src = elab + (size_t)i * kMaxVerts;
+  int* dst = adj + (size_t)i * kMaxVerts;
+  for (int k = 0; k < ne; ++k) dst[k] = src[k];
 }
 
 // Grow the ring of every cell marked for a rebuild (r -> 2r + 1).
@@ -747,15 +920,31 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 // ---------------------------------------------------------------------------------------------
 
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st) {
-  if (fp32)
-    hipLaunchKernelGGL((cell_kernel<float, float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((cell_kernel<double, double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  if (a.redo_mode) {
+    if (fp32)
+      hipLaunchKernelGGL((cell_repair_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0,
+                         st, a);
+    else
+      hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0,
+                         st, a);
+  } else if (fp32) {
+    hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(verify_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
+                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(save_adjacency_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, gate,
+                     nedge, elab, adjn, adj);
   return hipGetLastError();
 }
 

@@ -274,7 +274,6 @@ __global__ __launch_bounds__(kCellBlock) void dev_accept_kernel(
     const ProbDesc* probs, const BlockDesc* blocks, DevState* st, SolveParams sp,
     const double* mass, const double* nu, const double* d, double* w, double* g, double* S,
     double* Y, size_t ld, double* part, size_t pstride, int* counter, uint8_t* active) {
-  __shared__ double red[kCellBlock / 64];
   const BlockDesc bd = blocks[blockIdx.x];
   DevState& s = st[bd.prob];
   if (s.phase != kPhaseTrial || !s.accept) return;

@@ -133,7 +133,7 @@ class OTBatch:
     def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop: bool = True,
-              accept_rule: str = "auto", repair_rounds: int = 2):
+              accept_rule: str = "auto", repair_rounds: int = 3):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a

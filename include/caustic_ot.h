@@ -67,14 +67,15 @@ typedef struct {
   int stop_rule;            /* COT_STOP_REFERENCE: |g|_2 < tol with no empty cell (ot_op.cuh:510);
                                COT_STOP_MAXREL: max_i |mass_i - nu_i| / nu_i < tol */
   double h0_threshold;      /* H0 = y.s / y.y when y.y > h0_threshold, else 1 (1e-5 reference) */
-  int verbose;              /* print a line every `verbose` iterations (0: silent) */
+  int verbose;              /* host loop: a line every `verbose` iterations; device loop: a
+                               progress line every 2 s (0: silent) */
   int device_loop;          /* 1: device-resident rounds replayed from a hipGraph (default);
                                0: host-driven rounds (one host sync per phase) */
   int accept_rule;          /* 0: accept a trial on f decrease (ot_op.cuh:397);
                                1: also accept when the directional derivative at the trial is
                                   <= 0 (the convex objective decreased) -- for FP32 geometry,
                                   whose f is too coarse to resolve the last decades */
-  int repair_rounds;        /* conformity repairs inside one device round (default 2) */
+  int repair_rounds;        /* conformity repair rounds of a line-search trial (default 3) */
 } cot_solve_opts;
 
 typedef struct {
@@ -139,20 +140,23 @@ int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centr
 /* Area of each cell (inside proj_domain) at weights w. */
 int cot_cell_areas(cot_context* ctx, const double* w, double* area);
 
-/* Kernel time accumulated since the last reset, measured with HIP events on the library's
- * stream around each full-batch launch of the cell kernel (construction + integration) and of
- * the conformity check.  Repair launches (only the cells that failed) are counted separately. */
+/* Kernel time accumulated since the last reset.  Each launch of the cell kernel (construction
+ * + integration), of its repair pass (cells that failed the conformity check) and of the check
+ * itself is timed inside the kernel with the device's constant-rate wall clock, from the
+ * earliest block start to the latest end of a block that did work; launches in which no block
+ * had work (all problems finished) are not counted. */
 typedef struct {
-  double cell_seconds;        /* full-batch cell-kernel time */
-  long long cell_launches;    /* full-batch cell-kernel launches */
-  double verify_seconds;      /* conformity-check time */
-  long long repair_rounds;    /* repair launches */
-  long long evaluations;      /* evaluations (trials included) */
-  long long sites_evaluated;  /* sites covered by the full-batch launches */
-  long long pixels_covered;   /* target pixels of the problems covered by those launches */
+  double cell_seconds;        /* full cell-kernel launches */
+  long long cell_launches;
+  double repair_seconds;      /* repair launches of the cell kernel */
+  long long repair_launches;
+  double verify_seconds;      /* conformity-check launches */
+  long long verify_launches;
+  long long evaluations;      /* evaluations (line-search trials included) */
+  long long repair_rounds;    /* repair rounds run by host-driven evaluations */
 } cot_stats;
 
-int cot_get_stats(const cot_context* ctx, cot_stats* out);
+int cot_get_stats(cot_context* ctx, cot_stats* out);
 int cot_reset_stats(cot_context* ctx);
 
 #ifdef __cplusplus

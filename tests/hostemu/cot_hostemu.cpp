@@ -1,8 +1,10 @@
 // Host build of the cell and conformity-check kernels (test infrastructure).
 //
 // Runs the exact per-site code of cot_kernels.hip (cell_site, verify_site) on the CPU, one lane
-// at a time, with the library's layout (cot_layout.h) and repair rounds, so that the GPU code
-// path can be checked against the oracle without a GPU.  Built by tests/hostemu/build.sh.
+// at a time, with the library's layout (cot_layout.h) and its evaluation sequence: a full pass
+// from the previous evaluation's two-ring adjacency, the conformity check, and repair rounds
+// (the current cells' two-ring and the check's queued candidates, then bucket rings).  A session keeps rings, adjacency and FP32
+// frames between evaluations like a library context.  Built by tests/hostemu/build.sh.
 #include <cstring>
 #include <vector>
 
@@ -13,23 +15,43 @@ using namespace cot;
 
 namespace {
 
+struct Session {
+  int n = 0, pm = 0, pn = 0, fp32 = 0, max_repairs = 12;
+  double verify_tol = 1e-4;
+  ProbDesc pd{};
+  std::vector<int> order;
+  std::vector<int2> bk;
+  std::vector<BlockDesc> blocks;
+  std::vector<double2> xy, org;
+  std::vector<double> nu, w, mass, fcell, area;
+  std::vector<double4> pre, tot;
+  std::vector<double2> cen, geo;
+  std::vector<uint8_t> exist, ne, ring, used, redo, adjn;
+  std::vector<int> elab, adj, xcnt, xcand;
+  std::vector<float2> espan;
+  bool have_adj = false;
+  int ring_max = 32;  // the library's default
+};
+
 template <typename T>
-struct Emu {
+struct Lanes {
   PolyLDS<T>* P = new PolyLDS<T>();
-  ~Emu() { delete P; }
+  ~Lanes() { delete P; }
 };
 
 }  // namespace
 
 extern "C" {
 
-// Evaluate one problem at w.  Outputs in caller order; any may be NULL.
-// stats[0] = conformity failures left, stats[1] = repair rounds, stats[2] = overflow cells,
-// stats[3] = largest ring used.
-int emu_evaluate(int n, const double* sites, const double* nu, const double* rho, int pm, int pn,
-                 const double* proj, const double* w, int fp32, int ring_init, int max_repairs,
-                 double verify_tol, double* mass, double* fcell, double* cen, double* geo,
-                 unsigned char* exist, int* stats, int* ring_io) {
+void* emu_create(int n, const double* sites, const double* nu, const double* rho, int pm, int pn,
+                 const double* proj, int fp32, int ring_init, int max_repairs, double verify_tol) {
+  Session* S = new Session();
+  S->n = n;
+  S->pm = pm;
+  S->pn = pn;
+  S->fp32 = fp32;
+  S->max_repairs = max_repairs;
+  S->verify_tol = verify_tol;
   cot_problem pr{};
   pr.n_sites = n;
   pr.sites = sites;
@@ -38,149 +60,212 @@ int emu_evaluate(int n, const double* sites, const double* nu, const double* rho
   pr.pm = pm;
   pr.pn = pn;
   for (int k = 0; k < 4; ++k) pr.proj_domain[k] = proj[k];
-  ProbDesc pd{};
-  pd.site_off = 0;
+  ProbDesc& pd = S->pd;
   pd.n = n;
   pd.pm = pm;
   pd.pn = pn;
-  pd.img_off = 0;
   pd.x0 = proj[0];
   pd.x1 = proj[1];
   pd.y0 = proj[2];
   pd.y1 = proj[3];
   pd.dx = (pd.x1 - pd.x0) / pm;
   pd.dy = (pd.y1 - pd.y0) / pn;
-  std::vector<int> order;
-  std::vector<int2> bk;
-  bucket_sort(pr, 1.0, pd, order, bk);
-  pd.bkt_off = 0;
-  std::vector<BlockDesc> blocks;
+  bucket_sort(pr, 1.0, pd, S->order, S->bk);
   for (int f = 0; f < n; f += kCellBlock) {
     BlockDesc b{};
     b.prob = 0;
     b.first = f;
     b.count = std::min(kCellBlock, n - f);
-    blocks.push_back(b);
+    S->blocks.push_back(b);
   }
-  pd.blk_off = 0;
-  pd.nblk = (int)blocks.size();
-  const int nb = pd.nblk;
-  std::vector<double2> xy(n), org(n);
-  std::vector<double> vnu(n), vw(n), vmass(n), vf(n), va(n);
-  std::vector<double2> vcen(n), vgeo(n);
-  std::vector<uint8_t> vex(n), ne(n), ring(n, (uint8_t)ring_init), used(n), redo(n, 0), box(n, 0);
-  std::vector<int> elab((size_t)n * kMaxVerts);
-  std::vector<float2> espan((size_t)n * kMaxVerts);
+  pd.nblk = (int)S->blocks.size();
+  S->xy.resize(n);
+  S->nu.resize(n);
   for (int r = 0; r < n; ++r) {
-    const int o = order[r];
-    xy[r] = make_double2(sites[2 * o], sites[2 * o + 1]);
-    org[r] = xy[r];
-    vnu[r] = nu[o];
-    vw[r] = w[o];
-    // ring_io: the per-site candidate rings carried between evaluations (0: start at ring_init)
-    if (ring_io && ring_io[o] > 0) ring[r] = (uint8_t)ring_io[o];
+    const int o = S->order[r];
+    S->xy[r] = make_double2(sites[2 * o], sites[2 * o + 1]);
+    S->nu[r] = nu[o];
   }
-  std::vector<float> rho32;
-  if (fp32) rho32.assign(rho, rho + (size_t)pm * pn);
+  S->org = S->xy;
+  S->w.assign(n, 0.0);
+  S->mass.assign(n, 0.0);
+  S->fcell.assign(n, 0.0);
+  S->area.assign(n, 0.0);
+  S->cen.assign(n, make_double2(0, 0));
+  S->geo.assign(n, make_double2(0, 0));
+  S->exist.assign(n, 0);
+  S->ne.assign(n, 0);
+  S->ring.assign(n, (uint8_t)ring_init);
+  S->used.assign(n, 0);
+  S->redo.assign(n, 0);
+  S->adjn.assign(n, 0);
+  S->elab.assign((size_t)n * kMaxVerts, 0);
+  S->adj.assign((size_t)n * kMaxVerts, 0);
+  S->xcnt.assign(n, 0);
+  S->xcand.assign((size_t)n * kMaxExtra, 0);
+  S->espan.assign((size_t)n * kMaxVerts, make_float2(0.f, 0.f));
+  pd.nchunk = (pm + kChunk - 1) / kChunk;
+  S->pre.resize((size_t)pm * pn);
+  S->tot.resize((size_t)pd.nchunk * pn);
+  build_prefix(rho, pm, pn, fp32 != 0, S->pre.data(), S->tot.data());
+  return S;
+}
+
+void emu_destroy(void* h) { delete static_cast<Session*>(h); }
+
+// Per-site state after the last evaluation (caller order; any may be NULL): the conformity
+// flags of the last check, edge counts, edge labels (caller indices, sides below zero) and spans
+// (kMaxVerts per site), and the bucket rings.
+void emu_state(void* h, unsigned char* redo, int* nedge, int* elab, float* espan, int* ring) {
+  Session& S = *static_cast<Session*>(h);
+  for (int r = 0; r < S.n; ++r) {
+    const int o = S.order[r];
+    if (redo) redo[o] = S.redo[r];
+    if (nedge) nedge[o] = S.ne[r];
+    if (ring) ring[o] = S.ring[r];
+    for (int k = 0; k < kMaxVerts; ++k) {
+      const int lab = S.elab[(size_t)r * kMaxVerts + k];
+      if (elab) elab[(size_t)o * kMaxVerts + k] = lab >= 0 ? S.order[lab] : lab;
+      if (espan) {
+        espan[2 * ((size_t)o * kMaxVerts + k)] = S.espan[(size_t)r * kMaxVerts + k].x;
+        espan[2 * ((size_t)o * kMaxVerts + k) + 1] = S.espan[(size_t)r * kMaxVerts + k].y;
+      }
+    }
+  }
+}
+
+// Evaluate the session's problem at w (caller order).  Outputs in caller order; any may be NULL.
+// stats[0] = conformity failures left, stats[1] = repair rounds, stats[2] = overflow cells,
+// stats[3] = largest bucket ring, stats[4] = cells rebuilt by repairs.
+int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen, double* geo,
+             unsigned char* exist, int* stats) {
+  Session& S = *static_cast<Session*>(h);
+  const int n = S.n, nb = S.pd.nblk;
+  for (int r = 0; r < n; ++r) S.w[r] = w[S.order[r]];
   uint8_t active = 1;
   int overflow = 0;
   CellArgs a{};
-  a.probs = &pd;
-  a.blocks = blocks.data();
-  a.xy = xy.data();
-  a.w = vw.data();
-  a.dir = nullptr;
-  a.nu = vnu.data();
-  a.buckets = bk.data();
-  a.img = fp32 ? (const void*)rho32.data() : (const void*)rho;
-  a.ring = ring.data();
-  a.redo = redo.data();
-  a.redo_mode = 0;
+  a.probs = &S.pd;
+  a.blocks = S.blocks.data();
+  a.xy = S.xy.data();
+  a.w = S.w.data();
+  a.nu = S.nu.data();
+  a.buckets = S.bk.data();
+  a.pre = S.pre.data();
+  a.tot = S.tot.data();
+  a.ring = S.ring.data();
+  a.redo = S.redo.data();
   a.active = &active;
-  a.mass = vmass.data();
-  a.fcell = vf.data();
-  a.area = va.data();
-  a.cen = vcen.data();
-  a.geo = vgeo.data();
-  a.exist = vex.data();
-  a.nedge = ne.data();
-  a.elab = elab.data();
-  a.espan = espan.data();
-  a.ring_used = used.data();
+  a.mass = S.mass.data();
+  a.fcell = S.fcell.data();
+  a.area = S.area.data();
+  a.cen = S.cen.data();
+  a.geo = S.geo.data();
+  a.exist = S.exist.data();
+  a.nedge = S.ne.data();
+  a.elab = S.elab.data();
+  a.espan = S.espan.data();
+  a.ring_used = S.used.data();
   a.overflow = &overflow;
-  a.org = org.data();
+  a.org = S.org.data();
+  a.ring_max = S.ring_max;
+  a.adj = S.adj.data();
+  a.adjn = S.adjn.data();
+  a.cand = S.have_adj ? kCandAdj2 : kCandBucket;
+  a.xcnt = S.xcnt.data();
+  a.xcand = S.xcand.data();
   VerifyArgs v{};
-  v.probs = &pd;
-  v.blocks = blocks.data();
-  v.xy = xy.data();
-  v.nedge = ne.data();
-  v.elab = elab.data();
-  v.espan = espan.data();
+  v.probs = &S.pd;
+  v.blocks = S.blocks.data();
+  v.xy = S.xy.data();
+  v.nedge = S.ne.data();
+  v.elab = S.elab.data();
+  v.espan = S.espan.data();
   v.active = &active;
-  v.gate = nullptr;
-  v.redo = redo.data();
-  v.tol_rel = verify_tol;
+  v.redo = S.redo.data();
+  v.tol_rel = S.verify_tol;
+  v.xcnt = S.xcnt.data();
+  v.xcand = S.xcand.data();
 
+  int rebuilt = 0;
   auto run_cells = [&]() {
+    int built = 0;
 #pragma omp parallel
     {
-      Emu<float> ef;
-      Emu<double> ed;
-#pragma omp for schedule(dynamic, 1) reduction(+ : overflow)
+      Lanes<float> lf;
+      Lanes<double> ld;
+#pragma omp for schedule(dynamic, 1) reduction(+ : overflow, built)
       for (int b = 0; b < nb; ++b)
         for (int ln = 0; ln < kCellBlock; ++ln) {
-          bool o = fp32 ? cell_site<float, float>(a, b, ln, *ef.P)
-                        : cell_site<double, double>(a, b, ln, *ed.P);
-          overflow += o ? 1 : 0;
+          const int rc = S.fp32 ? cell_site<float>(a, b, ln, *lf.P)
+                                : cell_site<double>(a, b, ln, *ld.P);
+          overflow += rc == 1 ? 1 : 0;
+          built += rc >= 0 ? 1 : 0;
         }
     }
+    return built;
   };
   auto run_verify = [&]() {
     int fails = 0;
     for (int b = 0; b < nb; ++b)
-      for (int ln = 0; ln < kCellBlock; ++ln) fails += verify_site(v, b, ln) ? 1 : 0;
+      for (int ln = 0; ln < kCellBlock; ++ln) fails += verify_site(v, b, ln) == 1 ? 1 : 0;
     return fails;
+  };
+  auto save_adjacency = [&]() {
+    for (int r = 0; r < n; ++r) {
+      S.adjn[r] = S.ne[r];
+      for (int k = 0; k < S.ne[r]; ++k)
+        S.adj[(size_t)r * kMaxVerts + k] = S.elab[(size_t)r * kMaxVerts + k];
+    }
   };
   run_cells();
   int fails = run_verify();
   int rounds = 0;
-  while (fails > 0 && rounds < max_repairs) {
+  a.redo_mode = 1;
+  while (fails > 0 && rounds < S.max_repairs) {
     ++rounds;
-    for (int r = 0; r < n; ++r)
-      if (redo[r]) {
-        ring[r] = (uint8_t)std::min(2 * (int)ring[r] + 1, 250);
-      }
-    a.redo_mode = 1;
-    run_cells();
-    a.redo_mode = 0;
-    std::fill(redo.begin(), redo.end(), 0);
+    // as the library: two rounds from the current cells' two-ring and the queued candidates,
+    // then bucket rings grown 2r + 1
+    a.cand = rounds <= 2 ? kCandAdj2 : kCandBucket;
+    if (a.cand == kCandBucket) {
+      for (int r = 0; r < n; ++r)
+        if (S.redo[r]) S.ring[r] = (uint8_t)std::min(2 * (int)S.ring[r] + 1, S.ring_max);
+    } else {
+      save_adjacency();
+    }
+    rebuilt += run_cells();
+    std::fill(S.redo.begin(), S.redo.end(), 0);
     fails = run_verify();
   }
   int rmax = 0;
-  for (int r = 0; r < n; ++r) rmax = std::max(rmax, (int)ring[r]);
-  // as the library: after a clean check the next ring is the outermost used ring plus one
+  for (int r = 0; r < n; ++r) rmax = std::max(rmax, (int)S.ring[r]);
+  // as the library: the cells just built are the next candidates; after a clean check the next
+  // bucket ring is the outermost used ring plus one
+  save_adjacency();
+  S.have_adj = true;
   if (fails == 0)
-    for (int r = 0; r < n; ++r) ring[r] = (uint8_t)std::max((int)used[r] + 1, 1);
+    for (int r = 0; r < n; ++r)
+      S.ring[r] = (uint8_t)std::min(std::max((int)S.used[r] + 1, 1), S.ring_max);
   if (stats) {
     stats[0] = fails;
     stats[1] = rounds;
     stats[2] = overflow;
     stats[3] = rmax;
+    stats[4] = rebuilt;
   }
   for (int r = 0; r < n; ++r) {
-    const int o = order[r];
-    if (mass) mass[o] = vmass[r];
-    if (fcell) fcell[o] = vf[r];
+    const int o = S.order[r];
+    if (mass) mass[o] = S.mass[r];
+    if (fcell) fcell[o] = S.fcell[r];
     if (cen) {
-      cen[2 * o] = vcen[r].x;
-      cen[2 * o + 1] = vcen[r].y;
+      cen[2 * o] = S.cen[r].x;
+      cen[2 * o + 1] = S.cen[r].y;
     }
     if (geo) {
-      geo[2 * o] = vgeo[r].x;
-      geo[2 * o + 1] = vgeo[r].y;
+      geo[2 * o] = S.geo[r].x;
+      geo[2 * o + 1] = S.geo[r].y;
     }
-    if (exist) exist[o] = vex[r];
-    if (ring_io) ring_io[o] = (int)ring[r];
+    if (exist) exist[o] = S.exist[r];
   }
   return 0;
 }

@@ -21,35 +21,78 @@ def lib():
         if not os.path.exists(_PATH):
             build()
         _lib = ctypes.CDLL(_PATH)
-        _lib.emu_evaluate.restype = ctypes.c_int
+        _lib.emu_create.restype = ctypes.c_void_p
+        _lib.emu_create.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        _lib.emu_destroy.argtypes = [ctypes.c_void_p]
+        _lib.emu_state.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
+        _lib.emu_eval.restype = ctypes.c_int
+        _lib.emu_eval.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p,
+                                  ctypes.c_void_p]
     return _lib
 
 
-def evaluate(p, w, fp32=False, ring_init=2, max_repairs=12, verify_tol=1e-4, rings=None):
-    """Evaluate problem p (workloads.Problem) at w with the host build of the cell kernel.
+def _P(a):
+    return a.ctypes.data_as(_dp)
 
-    rings: optional int32 array carried between calls (the library keeps per-site rings)."""
-    n = p.n
-    sites = np.ascontiguousarray(p.sites, dtype=np.float64)
-    nu = np.ascontiguousarray(p.nu, dtype=np.float64)
-    rho = np.ascontiguousarray(p.rho, dtype=np.float64)
-    proj = np.ascontiguousarray(p.proj_domain, dtype=np.float64)
-    w = np.ascontiguousarray(w, dtype=np.float64)
-    out = {k: np.zeros(n) for k in ("mass", "fcell")}
-    out["centroid"] = np.zeros((n, 2))
-    out["geo"] = np.zeros((n, 2))
-    ex = np.zeros(n, dtype=np.uint8)
-    stats = np.zeros(4, dtype=np.int32)
-    ring = rings if rings is not None else np.zeros(n, dtype=np.int32)
-    P = lambda a: a.ctypes.data_as(_dp)
-    lib().emu_evaluate(ctypes.c_int(n), P(sites), P(nu), P(rho), ctypes.c_int(rho.shape[1]),
-                       ctypes.c_int(rho.shape[0]), P(proj), P(w), ctypes.c_int(int(fp32)),
-                       ctypes.c_int(ring_init), ctypes.c_int(max_repairs),
-                       ctypes.c_double(verify_tol), P(out["mass"]), P(out["fcell"]),
-                       P(out["centroid"]), P(out["geo"]), ex.ctypes.data_as(ctypes.c_void_p),
-                       stats.ctypes.data_as(ctypes.c_void_p), ring.ctypes.data_as(ctypes.c_void_p))
-    out["exist"] = ex.astype(bool)
-    out["f"] = float(out["fcell"].sum())
-    out["fails"], out["repairs"], out["overflow"], out["ring_max"] = (int(x) for x in stats)
-    out["ring"] = ring
-    return out
+
+class Session:
+    """One problem with the library's per-site state (bucket rings, previous adjacency, FP32
+    frames) carried from one evaluation to the next."""
+
+    def __init__(self, p, fp32=False, ring_init=2, max_repairs=12, verify_tol=1e-4):
+        self.n = p.n
+        self._keep = [np.ascontiguousarray(p.sites, dtype=np.float64),
+                      np.ascontiguousarray(p.nu, dtype=np.float64),
+                      np.ascontiguousarray(p.rho, dtype=np.float64),
+                      np.ascontiguousarray(p.proj_domain, dtype=np.float64)]
+        s, nu, rho, proj = self._keep
+        self.h = lib().emu_create(p.n, _P(s), _P(nu), _P(rho), rho.shape[1], rho.shape[0],
+                                  _P(proj), int(fp32), ring_init, max_repairs, verify_tol)
+
+    def evaluate(self, w):
+        n = self.n
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = {k: np.zeros(n) for k in ("mass", "fcell")}
+        out["centroid"] = np.zeros((n, 2))
+        out["geo"] = np.zeros((n, 2))
+        ex = np.zeros(n, dtype=np.uint8)
+        stats = np.zeros(5, dtype=np.int32)
+        lib().emu_eval(self.h, _P(w), _P(out["mass"]), _P(out["fcell"]), _P(out["centroid"]),
+                       _P(out["geo"]), ex.ctypes.data_as(ctypes.c_void_p),
+                       stats.ctypes.data_as(ctypes.c_void_p))
+        out["exist"] = ex.astype(bool)
+        out["f"] = float(out["fcell"].sum())
+        (out["fails"], out["repairs"], out["overflow"], out["ring_max"],
+         out["rebuilt"]) = (int(x) for x in stats)
+        return out
+
+    def state(self):
+        """Per-site state after the last evaluation: conformity flags, edge counts, edge labels
+        (caller indices; sides below zero), spans, bucket rings."""
+        n, K = self.n, 24
+        redo = np.zeros(n, dtype=np.uint8)
+        ne = np.zeros(n, dtype=np.int32)
+        lab = np.zeros((n, K), dtype=np.int32)
+        span = np.zeros((n, K, 2), dtype=np.float32)
+        ring = np.zeros(n, dtype=np.int32)
+        v = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        lib().emu_state(self.h, v(redo), v(ne), v(lab), v(span), v(ring))
+        return {"redo": redo.astype(bool), "nedge": ne, "elab": lab, "espan": span, "ring": ring}
+
+    def close(self):
+        if self.h:
+            lib().emu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def evaluate(p, w, fp32=False, ring_init=2, max_repairs=12, verify_tol=1e-4):
+    """One evaluation in a fresh session (bucket-ring candidates)."""
+    s = Session(p, fp32, ring_init, max_repairs, verify_tol)
+    try:
+        return s.evaluate(w)
+    finally:
+        s.close()

@@ -46,16 +46,44 @@ def test_hostemu_large_shapes_mixed():
     _check(p, w, True, 1e-6, 1e-6)
 
 
+def _displaced(p):
+    x, y = p.sites[:, 0], p.sites[:, 1]
+    return 0.08 * (x - 0.5) ** 2 + 0.05 * np.sin(6 * y)  # smooth, large: displaced cells
+
+
 def test_hostemu_displaced_cells_need_repairs():
     """Weights that move cells several site spacings away: the first rings miss neighbours,
     the conformity check catches it and the repaired cells equal the oracle's."""
     p = W.plain_fp64(seed=9, n_side=32, res=128)
-    x, y = p.sites[:, 0], p.sites[:, 1]
-    w = 0.08 * (x - 0.5) ** 2 + 0.05 * np.sin(6 * y)  # smooth, large: displaced cells
+    w = _displaced(p)
     orc = O.evaluate(p.sites, w, p.nu, p.rho, p.proj_domain, centroid_min_piece=0.0)
     e = E.evaluate(p, w, fp32=False, ring_init=1)
     assert e["repairs"] > 0 and e["fails"] == 0
     np.testing.assert_allclose(e["mass"], orc.mass, rtol=0, atol=1e-12 * p.nu.max())
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_hostemu_adjacency_candidates_follow_moving_weights(fp32):
+    """Candidates from the previous evaluation's cells: walking the weights from zero to the
+    displaced field in steps changes the adjacency (edge flips, cells that vanish and return),
+    the two-ring candidates, the check and its repairs (bucket rings) keep every evaluation equal
+    to the oracle's."""
+    p = W.plain_fp64(seed=9, n_side=32, res=128)
+    w1 = _displaced(p)
+    s = E.Session(p, fp32=fp32, ring_init=1, max_repairs=4)
+    tol = 1e-6 if fp32 else 1e-12
+    rebuilt = 0
+    for k, a in enumerate([0.0, 0.1, 0.25, 0.5, 0.55, 0.8, 1.0, 0.7, 1.0]):
+        w = a * w1
+        e = s.evaluate(w)
+        assert e["fails"] == 0, (k, a)
+        orc = O.evaluate(p.sites, w, p.nu, p.rho, p.proj_domain, centroid_min_piece=0.0)
+        assert np.array_equal(e["exist"], orc.exist.astype(bool)), (k, a)
+        dm = np.abs(e["mass"] - orc.mass) / np.maximum(orc.mass, p.nu)
+        assert dm.max() <= tol, (k, a, dm.max())
+        rebuilt += e["rebuilt"] if k > 0 else 0
+    assert rebuilt > 0  # the adjacency did change and was repaired
+    s.close()
 
 
 @pytest.mark.parametrize("fp32,accept", [(False, "decrease"), (True, "derivative")])
@@ -63,10 +91,10 @@ def test_hostemu_lbfgs_converges(fp32, accept):
     """The L-BFGS rules driving the host-built evaluation reach max-rel 1e-5 on the N = 4,096
     workload; FP32 geometry needs the derivative-aware acceptance."""
     p = W.plain_fp64()
-    rings = np.zeros(p.n, dtype=np.int32)
+    s = E.Session(p, fp32=fp32, max_repairs=3)
 
     def ev(w):
-        e = E.evaluate(p, w, fp32=fp32, max_repairs=3, rings=rings)
+        e = s.evaluate(w)
         if e["fails"]:
             return np.inf, np.full(p.n, np.inf), 0
         return e["f"], e["mass"], 0
