@@ -342,6 +342,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.step = c->d_step;
   ca.cand = c->have_adj ? kCandAdj2 : kCandBucket;
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
+  ca.coarse = c->d_fails;
 
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
   HIPCHK(launch_verify(va, c->nblk, st));
@@ -355,6 +356,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   c->stats.evaluations += 1;
 
   ca.redo_mode = 1;
+  ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   int rounds = 0;
   for (;;) {
@@ -505,10 +507,12 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   ca.st = c->d_state;
   ca.cand = kCandAdj2;
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
+  ca.coarse = c->d_rfails;
   va.fails = c->d_rfails;
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
   HIPCHK(launch_verify(va, c->nblk, st));
   ca.redo_mode = 1;
+  ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;

@@ -29,6 +29,11 @@ constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4;
 // cell, or no previous evaluation) uses its bucket rings, widened to 2 ring + 1 for kCandAdj2.
 // After an adjacency pass ring_used[i] is the bucket ring holding all of the cell's neighbours.
 enum : int { kCandBucket = 0, kCandAdj1 = 1, kCandAdj2 = 2 };
+// FP32 geometry hands a cell to an FP64 rebuild (the repair pass) when its area is below
+// kSliverRel bucket areas, or when a clip emptied it while a vertex lay within kMarginRel bucket
+// sizes of the clipping line.
+constexpr double kSliverRel = 1e-3;
+constexpr double kMarginRel = 1e-3;
 // Candidates the conformity check hands to a repair: when site i has an edge with site j that
 // j's cell lacks, i is queued for j (at most kMaxExtra per site and round).
 constexpr int kMaxExtra = 4;
@@ -89,6 +94,8 @@ struct CellArgs {
   float2* espan;            // [site * kMaxVerts + k]: edge span along the canonical bisector axis
   uint8_t* ring_used;       // outermost ring that cut the cell
   int* overflow;            // per-problem count of cells that exceeded kMaxVerts
+  int* coarse;              // per-problem count of FP32 cells handed to the FP64 repair (the
+                            // conformity-failure counter of the check that follows)
   // FP32 geometry only: clipping frame origin (the cell's last geometric centroid)
   double2* org;
   // candidate source (kCand*) and the previous evaluation's adjacency

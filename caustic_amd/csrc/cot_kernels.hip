@@ -90,17 +90,20 @@ struct PolyLDS {
 // Keep the part of the lane's convex polygon with u.d <= c; the new edge gets `label`.
 // Returns the new vertex count (0 when empty).  Clipping is in place: at step k the writes land
 // at positions <= k + 1, and vertex k + 1 has already been read into registers.
+// When the polygon is emptied, sdmin is the smallest u.d - c over its vertices.
 template <typename T>
 __host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy, T c, int label,
-                              bool& overflow, bool& changed) {
+                                       bool& overflow, bool& changed, T& sdmin) {
   int nout = 0, trans = 0;
   {
     T sd_first = P.x[0][ln] * dx + P.y[0][ln] * dy - c;
+    sdmin = sd_first;
     bool prev = sd_first > T(0);
     nout += prev;
     bool first = prev;
     for (int k = 1; k < n; ++k) {
       T sd = P.x[k][ln] * dx + P.y[k][ln] * dy - c;
+      sdmin = sd < sdmin ? sd : sdmin;
       bool o = sd > T(0);
       nout += o;
       trans += (o != prev);
@@ -380,6 +383,9 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
   P.x[3][ln] = (T)(pd.x0 - o.x); P.y[3][ln] = (T)(pd.y1 - o.y); P.lab[3][ln] = kSideLeft;
   int nv = 4;
   bool overflow = false;
+  // FP32 geometry: the cell is near a degeneracy FP32 cannot resolve (a sliver, a marginal
+  // emptying, a vertex the FP64 refinement could not place) and is rebuilt by an FP64 pass
+  bool coarse = false;
 
   // bounding circle of the polygon (centre cx, cy relative to the frame; radius^2 r2)
   auto bound = [&](double& cx, double& cy, double& r2) {
@@ -417,10 +423,15 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
       if (have) return;
     }
     bool changed;
-    nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed);
+    T sdmin;
+    nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed, sdmin);
     if (changed) {
       used = max(used, r);
-      if (nv > 0) bound(bcx, bcy, br2);
+      if (nv > 0) {
+        bound(bcx, bcy, br2);
+      } else if (kF32 && (double)sdmin < kMarginRel * pd.bsize * sqrt(ddx * ddx + ddy * ddy)) {
+        coarse = true;  // emptied by a margin FP32 cannot resolve: may be a sliver
+      }
     }
   };
   const int R0 = a.ring[i];
@@ -525,12 +536,18 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
       if (!kF32) return make_double2(fx, fy);
       const double det = ax * by - ay * bx;
       const double sc = sqrt((ax * ax + ay * ay) * (bx * bx + by * by));
-      if (!(fabs(det) > 1e-7 * sc)) return make_double2(fx, fy);
+      if (!(fabs(det) > 1e-7 * sc)) {
+        coarse = true;
+        return make_double2(fx, fy);
+      }
       const double rx = (ac * by - bc * ay) / det, ry = (ax * bc - bx * ac) / det;
       // a near-parallel pair that FP32 rounding made adjacent can meet far away: keep the
       // clipped vertex when the refinement moves it by more than 1e-3 of the cell's extent
       const double ex = rx - fx, ey = ry - fy;
-      if (ex * ex + ey * ey > 1e-6 * ext2) return make_double2(fx, fy);
+      if (ex * ex + ey * ey > 1e-6 * ext2) {
+        coarse = true;
+        return make_double2(fx, fy);
+      }
       return make_double2(rx, ry);
     };
     // calls f(k, va, vb) for every edge k from vertex k to vertex k+1 (frame coordinates)
@@ -626,6 +643,11 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
       cen = geo;
     }
     if (kF32) a.org[i] = geo;  // next frame: this centroid
+    if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
+  }
+  if (kF32 && coarse && a.coarse) {
+    a.redo[i] = 1;
+    fetch_add(&a.coarse[bd.prob], 1);
   }
   a.mass[i] = m0;
   a.fcell[i] = fi;
@@ -921,12 +943,9 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st) {
   if (a.redo_mode) {
-    if (fp32)
-      hipLaunchKernelGGL((cell_repair_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0,
-                         st, a);
-    else
-      hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0,
-                         st, a);
+    // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
+    // of the cells the FP32 pass could not resolve)
+    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   } else {

@@ -187,9 +187,12 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   v.xcnt = S.xcnt.data();
   v.xcand = S.xcand.data();
 
-  int rebuilt = 0;
+  int rebuilt = 0, coarse = 0;
+  a.coarse = &coarse;
+  // the full pass in the context's precision, repairs in FP64 (as the library)
   auto run_cells = [&]() {
     int built = 0;
+    const bool f32 = S.fp32 && !a.redo_mode;
 #pragma omp parallel
     {
       Lanes<float> lf;
@@ -197,8 +200,8 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
 #pragma omp for schedule(dynamic, 1) reduction(+ : overflow, built)
       for (int b = 0; b < nb; ++b)
         for (int ln = 0; ln < kCellBlock; ++ln) {
-          const int rc = S.fp32 ? cell_site<float>(a, b, ln, *lf.P)
-                                : cell_site<double>(a, b, ln, *ld.P);
+          const int rc = f32 ? cell_site<float>(a, b, ln, *lf.P)
+                             : cell_site<double>(a, b, ln, *ld.P);
           overflow += rc == 1 ? 1 : 0;
           built += rc >= 0 ? 1 : 0;
         }
@@ -219,7 +222,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
     }
   };
   run_cells();
-  int fails = run_verify();
+  int fails = run_verify() + coarse;
   int rounds = 0;
   a.redo_mode = 1;
   while (fails > 0 && rounds < S.max_repairs) {
