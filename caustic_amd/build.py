@@ -1,6 +1,7 @@
 """Builds the native pieces of caustic_amd in-tree for gfx950.
 
-  caustic_amd/libcaustic_ot.so   HIP kernels + C ABI (include/caustic_ot.h)
+  caustic_amd/libcaustic_ot.so   HIP kernels + C ABI of the OT solver (include/caustic_ot.h)
+  caustic_amd/libcaustic_wsn.so  warm-start network kernels + C ABI (include/caustic_wsn.h)
   caustic_amd/bin/caustic_gpu    drop-in CLI for the reference's caustic_GPU executable
 
 Run ``python -m caustic_amd.build`` (hipcc cross-compiles without a GPU).
@@ -15,6 +16,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libcaustic_ot.so")
+WSN_LIB = os.path.join(PKG, "libcaustic_wsn.so")
 CLI = os.path.join(PKG, "bin", "caustic_gpu")
 ARCH = os.environ.get("CAUSTIC_OFFLOAD_ARCH", "gfx950")
 
@@ -36,17 +38,22 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_library():
+def _shared(sources, out):
     objs = []
-    for src in ("cot_kernels.hip", "cot_solver.hip", "cot_reflect.hip", "cot_api.cpp"):
+    for src in sources:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         extra = ["-x", "hip"] if src.endswith(".cpp") else []
         _run(["hipcc", *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    _run(["hipcc", *HIP_FLAGS, "-shared", *objs, "-o", LIB + ".tmp"])
-    os.replace(LIB + ".tmp", LIB)
+    _run(["hipcc", *HIP_FLAGS, "-shared", *objs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
     for o in objs:
         os.remove(o)
+
+
+def build_library():
+    _shared(("cot_kernels.hip", "cot_solver.hip", "cot_reflect.hip", "cot_api.cpp"), LIB)
+    _shared(("wsn_kernels.hip", "wsn_api.cpp"), WSN_LIB)
 
 
 def build_cli():
