@@ -85,7 +85,8 @@ class cot_stats(ctypes.Structure):
     _fields_ = [("cell_seconds", ctypes.c_double), ("cell_launches", ctypes.c_longlong),
                 ("repair_seconds", ctypes.c_double), ("repair_launches", ctypes.c_longlong),
                 ("verify_seconds", ctypes.c_double), ("verify_launches", ctypes.c_longlong),
-                ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong)]
+                ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong),
+                ("integ_seconds", ctypes.c_double), ("integ_launches", ctypes.c_longlong)]
 
 _lib = None
 

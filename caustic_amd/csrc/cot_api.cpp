@@ -65,6 +65,9 @@ hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, Dev
                              const double* d, double* w, double* g, double* S, double* Y,
                              size_t ld, double* part, size_t pstride, int* counter,
                              uint8_t* active, int nblk, hipStream_t stream);
+hipError_t launch_dev_accept_finish(const ProbDesc* probs, DevState* st, const SolveParams& sp,
+                                    const double* part, size_t pstride, uint8_t* active, int np,
+                                    hipStream_t stream);
 hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
                                  uint8_t* redo, uint8_t* ring, int ring_max, int nblk,
                                  hipStream_t stream);
@@ -165,6 +168,8 @@ struct cot_context {
   int* d_adj = nullptr;          // previous evaluation's edge labels (candidates)
   uint8_t* d_adjn = nullptr;     // previous evaluation's edge counts (0: none yet)
   bool have_adj = false;
+  double2* d_vtx = nullptr;      // cell vertices (construction -> integration)
+  int2* d_kbox = nullptr;
   int* d_xcnt = nullptr;         // candidates queued by the conformity check
   int* d_xcand = nullptr;
   unsigned long long* d_timer = nullptr;  // kTimers records of kTimerWords
@@ -277,6 +282,8 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.adjn = c->d_adjn;
   ca.xcnt = c->d_xcnt;
   ca.xcand = c->d_xcand;
+  ca.vtx = c->d_vtx;
+  ca.kbox = c->d_kbox;
   va = VerifyArgs{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
@@ -290,17 +297,20 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   va.tol_rel = c->opt.verify_tol;
   va.xcnt = c->d_xcnt;
   va.xcand = c->d_xcand;
-  va.timer = c->d_timer + kTimerVerify * kTimerWords;
+  va.timer_close = c->d_timer + kTimerInteg * kTimerWords;
 }
 
-// Candidates: the full pass takes the previous evaluation's cells' two-ring (an edge flip makes
-// a neighbour's neighbour adjacent, and the edge it creates starts too short for the conformity
-// check to see).  The first two repair rounds take the two-ring of the cells just built (larger
+// Candidates: the FP64 full pass takes the previous evaluation's cells' two-ring (an edge flip
+// makes a neighbour's neighbour adjacent, and the edge it creates starts too short for the
+// conformity check to see).  The first two repair rounds take the two-ring of the cells just built (larger
 // than the true cells, so their neighbour lists hold more) and the sites the check queued; later
 // rounds take the bucket rings, grown 2r + 1 before each round from the ring that held the
 // cell's last neighbours.  Without a previous evaluation the full pass uses the rings.
 constexpr int kAdjRepairRounds = 2;
 inline int repair_cand(int r) { return r <= kAdjRepairRounds ? kCandAdj2 : kCandBucket; }
+// FP32 geometry's full pass takes the one-ring: an edge a flip creates and the check cannot yet
+// see changes a mass by ~1e-9 relative, far inside FP32's own error; FP64 keeps the two-ring.
+inline int full_pass_cand(const cot_context* c) { return c->fp32 ? kCandAdj1 : kCandAdj2; }
 
 // Fold the in-kernel timers into the context's statistics and clear their accumulators.
 int pull_timers(cot_context* c) {
@@ -308,15 +318,17 @@ int pull_timers(cot_context* c) {
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(tm, c->d_timer, sizeof(tm), hipMemcpyDeviceToHost));
   const double inv = 1.0 / c->clock_hz;
-  c->stats.cell_seconds += tm[kTimerCell * kTimerWords + 4] * inv;
-  c->stats.cell_launches += (long long)tm[kTimerCell * kTimerWords + 5];
-  c->stats.repair_seconds += tm[kTimerRepair * kTimerWords + 4] * inv;
-  c->stats.repair_launches += (long long)tm[kTimerRepair * kTimerWords + 5];
-  c->stats.verify_seconds += tm[kTimerVerify * kTimerWords + 4] * inv;
-  c->stats.verify_launches += (long long)tm[kTimerVerify * kTimerWords + 5];
+  constexpr int kAcc = 2 * kTimerSlots, kCnt = 2 * kTimerSlots + 1;
+  c->stats.cell_seconds += tm[kTimerCell * kTimerWords + kAcc] * inv;
+  c->stats.cell_launches += (long long)tm[kTimerCell * kTimerWords + kCnt];
+  c->stats.repair_seconds += tm[kTimerRepair * kTimerWords + kAcc] * inv;
+  c->stats.repair_launches += (long long)tm[kTimerRepair * kTimerWords + kCnt];
+  c->stats.integ_seconds += tm[kTimerInteg * kTimerWords + kAcc] * inv;
+  c->stats.integ_launches += (long long)tm[kTimerInteg * kTimerWords + kCnt];
+  c->stats.repair_seconds += tm[kTimerIntegRepair * kTimerWords + kAcc] * inv;
   for (int k = 0; k < kTimers; ++k) {
-    tm[k * kTimerWords + 4] = 0;
-    tm[k * kTimerWords + 5] = 0;
+    tm[k * kTimerWords + kAcc] = 0;
+    tm[k * kTimerWords + kCnt] = 0;
   }
   HIPCHK(hipMemcpy(c->d_timer, tm, sizeof(tm), hipMemcpyHostToDevice));
   return COT_OK;
@@ -340,8 +352,9 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.w = d_w;
   ca.dir = d_dir;
   ca.step = c->d_step;
-  ca.cand = c->have_adj ? kCandAdj2 : kCandBucket;
+  ca.cand = c->have_adj ? full_pass_cand(c) : kCandBucket;
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
+  ca.timer_int = c->d_timer + kTimerInteg * kTimerWords;
   ca.coarse = c->d_fails;
 
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
@@ -358,6 +371,8 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.redo_mode = 1;
   ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
+  ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
+  va.timer_close = ca.timer_int;
   int rounds = 0;
   for (;;) {
     int tot = 0;
@@ -505,8 +520,9 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   base_args(c, ca, va);
   ca.dir = c->d_dir;
   ca.st = c->d_state;
-  ca.cand = kCandAdj2;
+  ca.cand = full_pass_cand(c);
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
+  ca.timer_int = c->d_timer + kTimerInteg * kTimerWords;
   ca.coarse = c->d_rfails;
   va.fails = c->d_rfails;
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
@@ -514,6 +530,8 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   ca.redo_mode = 1;
   ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
+  ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
+  va.timer_close = ca.timer_int;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
     ca.cand = repair_cand(r);
@@ -540,6 +558,7 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   HIPCHK(launch_dev_accept(c->d_probs, c->d_blocks, c->d_state, sp, c->d_mass, c->d_nu,
                            c->d_dir, c->d_w, c->d_g, c->d_S, c->d_Y, ld, c->d_pacc, ps,
                            c->d_cnt + 2 * np, c->d_active, c->nblk, st));
+  HIPCHK(launch_dev_accept_finish(c->d_probs, c->d_state, sp, c->d_pacc, ps, c->d_active, np, st));
   return COT_OK;
 }
 
@@ -593,19 +612,22 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
   sp.t0 = o.initial_step;
   sp.h0_thr = o.h0_threshold;
 
-  // capture one round
+  // capture one round (device_loop 2: launch the same rounds directly, no graph)
+  const bool eager = o.device_loop == 2;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  int rc = launch_round(c, sp, R);
-  hipError_t ce = hipStreamEndCapture(c->st, &graph);
-  if (rc != COT_OK) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
+  if (!eager) {
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = launch_round(c, sp, R);
+    hipError_t ce = hipStreamEndCapture(c->st, &graph);
+    if (rc != COT_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ce);
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  HIPCHK(ce);
-  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   struct GraphGuard {
     hipGraph_t g;
     hipGraphExec_t e;
@@ -621,7 +643,13 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
   long long rounds = 0;
   auto last_print = std::chrono::steady_clock::now();
   for (;;) {
-    for (int k = 0; k < kBatch; ++k) HIPCHK(hipGraphLaunch(exec, c->st));
+    for (int k = 0; k < kBatch; ++k) {
+      if (eager) {
+        CHK(launch_round(c, sp, R));
+      } else {
+        HIPCHK(hipGraphLaunch(exec, c->st));
+      }
+    }
     rounds += kBatch;
     HIPCHK(launch_dev_count(c->d_state, np, c->d_ndone, c->st));
     HIPCHK(hipMemcpyAsync(c->h_ndone, c->d_ndone, sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -815,6 +843,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
       (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
+      (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
       (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
     return bail(rc);
@@ -852,9 +881,10 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   {
     // FP32 clipping frames start at the sites
     ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N);
-    // timers: no launch in flight (earliest start = max), accumulators zero
+    // timers: every slot's start at the maximum, ends and accumulators zero
     unsigned long long tm[kTimers * kTimerWords] = {};
-    for (int k = 0; k < kTimers; ++k) tm[k * kTimerWords] = ~0ull;
+    for (int k = 0; k < kTimers; ++k)
+      for (int r = 0; r < kTimerSlots; ++r) tm[k * kTimerWords + 2 * r] = ~0ull;
     ok = ok && up(c->d_timer, tm, sizeof(tm));
   }
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));

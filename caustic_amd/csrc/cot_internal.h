@@ -10,6 +10,9 @@ constexpr int kMaxVerts = 24;
 // Threads per block of the cell kernel; one block covers kCellBlock consecutive sites of one
 // problem (sites are stored in bucket order, so a block is a compact patch of the plane).
 constexpr int kCellBlock = 128;
+// Integration pass: lanes per site and threads per block.
+constexpr int kIntLanes = 8;
+constexpr int kIntBlock = 256;
 // Threads per block of the vector kernels.
 constexpr int kVecBlock = 256;
 // Largest L-BFGS history.
@@ -37,11 +40,14 @@ constexpr double kMarginRel = 1e-3;
 // Candidates the conformity check hands to a repair: when site i has an edge with site j that
 // j's cell lacks, i is queued for j (at most kMaxExtra per site and round).
 constexpr int kMaxExtra = 4;
-// Kernel timer (in-kernel wall clock, one record per kernel kind): [0] earliest block start of
-// the current launch, [1] latest block end, [2] blocks arrived, [3] blocks that did work,
-// [4] accumulated ticks of launches that did work, [5] such launches.
-constexpr int kTimerWords = 6;
-enum : int { kTimerCell = 0, kTimerRepair = 1, kTimerVerify = 2, kTimers = 3 };
+// Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
+// folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
+// the next kernel on the stream closes the launch: [2 * kTimerSlots] accumulates the ticks from
+// the earliest start to the latest end, [2 * kTimerSlots + 1] counts such launches, and the
+// slots are reset.
+constexpr int kTimerSlots = 16;
+constexpr int kTimerWords = 2 * kTimerSlots + 2;
+enum : int { kTimerCell = 0, kTimerRepair = 1, kTimerInteg = 2, kTimerIntegRepair = 3, kTimers = 4 };
 
 // One OT problem of a batch.  All per-site arrays are indexed by global sorted site index.
 struct ProbDesc {
@@ -105,7 +111,11 @@ struct CellArgs {
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)
   const int* xcand;         // [site * kMaxExtra + k]
-  unsigned long long* timer;  // kTimerWords words, or nullptr
+  unsigned long long* timer;  // the construction pass's timer (kTimerWords words) or nullptr
+  unsigned long long* timer_int;  // the integration pass's timer, closed by the next kernel
+  // construction -> integration
+  double2* vtx;             // [site * kMaxVerts + k]: vertex k of the cell (absolute, FP64)
+  int2* kbox;               // pixel of the cell's lower-left corner (integration origin)
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -150,7 +160,7 @@ struct VerifyArgs {
   double tol_rel;           // tolerance relative to the site spacing of the pair
   int* xcnt;                // candidate queues of the repair (see kMaxExtra)
   int* xcand;
-  unsigned long long* timer;  // kTimerWords words, or nullptr
+  unsigned long long* timer_close;  // the integration timer this check closes, or nullptr
 };
 
 }  // namespace cot

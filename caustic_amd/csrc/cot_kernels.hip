@@ -1,12 +1,14 @@
 // HIP kernels of the semi-discrete OT dual evaluation and the L-BFGS vector work, for gfx950.
 //
 // cell_kernel   one site per lane.  Builds the restricted power cell of the site by clipping the
-//               target rectangle with the half-planes of candidate sites taken ring by ring from
-//               the bucket grid (polygon held per lane in LDS), then integrates the
-//               piecewise-constant target over the cell exactly with Green's theorem: each edge
-//               is walked across the pixel lattice and every piece adds the closed-form integral
-//               of the row prefix of the density.  Mass, first and second moments are summed in
-//               FP64 (mass with a compensated sum).
+//               target rectangle with the half-planes of candidate sites (the previous cells'
+//               neighbours, or bucket rings), polygon held per lane in LDS; stores the cell's
+//               vertices (FP64), edge records, area and centroid.
+// integrate_kernel  kIntLanes lanes per site: integrates the piecewise-constant target over the
+//               cell exactly with Green's theorem -- each edge is walked across the pixel
+//               lattice and every piece adds the closed-form integral of the row prefix of the
+//               density.  Mass, first and second moments are summed in FP64 (mass with a
+//               compensated sum), across the site's lanes with shuffles.
 //               Reference semantics: polygons_in_VD + compute_inte + compute_f + compute_center
 //               (power_diagram.cpp:8-134, basic_function.cuh:743-1191).
 // verify_kernel checks that every edge of every cell is matched by the reciprocal edge of the
@@ -165,18 +167,26 @@ __host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy,
 }
 
 // Row-prefix lookup for the Green walk.  Columns are local to the cell (local column 0 = global
-// column kox); at(iy, ix) gives the sums over the whole pixels of local columns 0 .. ix-1 of
+// column kox); the sums over the whole pixels of local columns 0 .. ix-1 of
 // rho_k * (1, k + 1/2, k^2 + k + 1/3) (the integrals of 1, U, U^2 over those pixels, local
-// coordinates) and rho at (iy, ix).  The sums are assembled from the chunked prefixes: the part
-// of kox's chunk right of kox (cached per row), whole chunks, and the part of ix's chunk left of
-// ix, each moved to the local origin by at most a chunk plus the cell's width.
+// coordinates) and rho at (iy, ix) come from the chunked prefixes: the part of kox's chunk right
+// of kox, whole chunks between, and the part of ix's chunk left of ix, each moved to the local
+// origin by at most a chunk plus the cell's width.  A lookup is split into load() -- three
+// independent reads, issued one piece ahead by the walk -- and eval().
+struct PrefixTaps {
+  double4 t;    // entry (row, kox + ix)
+  double4 qk;   // entry (row, kox)            (fresh rows only)
+  double4 tk;   // total of kox's chunk in row  (fresh rows only)
+  bool fresh;   // the piece starts a new row
+};
+
 struct Prefix {
   const double4* pre;  // this problem's row prefixes
   const double4* tot;  // this problem's chunk totals
   int pm, nchunk, kox, koy;
-  int row = -1;
-  double q0, q1, q2;   // prefix of kox within its chunk (chunk-local)
-  double f0, f1, f2;   // kox .. end of its chunk, local coordinates (when ix is past it)
+  int lrow = -1;       // row of the last load
+  double q0 = 0, q1 = 0, q2 = 0;  // prefix of kox within its chunk (chunk-local), current row
+  double f0 = 0, f1 = 0, f2 = 0;  // kox .. end of its chunk, local coordinates, current row
 
   // moments s (coordinates k') moved to coordinates k' + d
   __host__ __device__ static __forceinline__ void shift(double& s0, double& s1, double& s2,
@@ -184,35 +194,49 @@ struct Prefix {
     s2 += 2.0 * d * s1 + d * d * s0;
     s1 += d * s0;
   }
-  __host__ __device__ __forceinline__ void at(int iy, int ix, double& a0, double& a1, double& a2,
-                                              double& r) {
-    const int grow = koy + iy, g = kox + ix;
+  // Reads for the piece at (iy, ix); the row's two references only when the row changes.
+  __host__ __device__ __forceinline__ PrefixTaps load(int iy, int ix) {
+    const int grow = koy + iy;
     const double4* qr = pre + (size_t)grow * pm;
-    const int ck = kox / kChunk, cg = g / kChunk;
-    if (iy != row) {
-      row = iy;
-      const double4 qk = qr[kox];
-      q0 = qk.x;
-      q1 = qk.y;
-      q2 = qk.z;
-      const double4 tk = tot[(size_t)grow * nchunk + ck];
-      f0 = tk.x - q0;
-      f1 = tk.y - q1;
-      f2 = tk.z - q2;
-      shift(f0, f1, f2, (double)(ck * kChunk - kox));
+    PrefixTaps p;
+    p.t = qr[kox + ix];
+    p.fresh = iy != lrow;
+    if (p.fresh) {
+      p.qk = qr[kox];
+      p.tk = tot[(size_t)grow * nchunk + kox / kChunk];
+      lrow = iy;
     }
-    const double4 t = qr[g];
-    r = t.w;
+    return p;
+  }
+  // Take the row references of a piece that starts a new row (every piece, in walk order, also
+  // those that add nothing).
+  __host__ __device__ __forceinline__ void refresh(const PrefixTaps& p) {
+    if (!p.fresh) return;
+    q0 = p.qk.x;
+    q1 = p.qk.y;
+    q2 = p.qk.z;
+    f0 = p.tk.x - q0;
+    f1 = p.tk.y - q1;
+    f2 = p.tk.z - q2;
+    const int ck = kox / kChunk;
+    shift(f0, f1, f2, (double)(ck * kChunk - kox));
+  }
+  __host__ __device__ __forceinline__ void eval(const PrefixTaps& p, int iy, int ix, double& a0,
+                                                double& a1, double& a2, double& r) const {
+    const int g = kox + ix;
+    const int ck = kox / kChunk, cg = g / kChunk;
+    r = p.t.w;
     if (cg == ck) {
-      a0 = t.x - q0;
-      a1 = t.y - q1;
-      a2 = t.z - q2;
+      a0 = p.t.x - q0;
+      a1 = p.t.y - q1;
+      a2 = p.t.z - q2;
       shift(a0, a1, a2, (double)(ck * kChunk - kox));
       return;
     }
     a0 = f0;
     a1 = f1;
     a2 = f2;
+    const int grow = koy + iy;
     for (int c = ck + 1; c < cg; ++c) {
       const double4 tc = tot[(size_t)grow * nchunk + c];
       double b0 = tc.x, b1 = tc.y, b2 = tc.z;
@@ -221,7 +245,7 @@ struct Prefix {
       a1 += b1;
       a2 += b2;
     }
-    double b0 = t.x, b1 = t.y, b2 = t.z;
+    double b0 = p.t.x, b1 = p.t.y, b2 = p.t.z;
     shift(b0, b1, b2, (double)(cg * kChunk - kox));
     a0 += b0;
     a1 += b1;
@@ -241,11 +265,13 @@ struct Moments {
 // integer offsets and every accumulation stay FP64.
 template <typename TP>
 __host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix,
-                                               int iy, Prefix& px, Moments& M) {
+                                               int iy, Prefix& px, const PrefixTaps& taps,
+                                               Moments& M) {
+  px.refresh(taps);
   const double dv = V1 - V0;
   if (dv == 0.0) return;
   double a0, a1, a2, rv;
-  px.at(iy, ix, a0, a1, a2, rv);
+  px.eval(taps, iy, ix, a0, a1, a2, rv);
   const double fx = (double)ix, fy = (double)iy;
   if constexpr (sizeof(TP) == 4) {
     const float r = (float)rv;
@@ -290,26 +316,39 @@ __host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, 
 }
 
 // Walk the edge (Ua,Va)->(Ub,Vb) across the pixel lattice (local pixel units, lattice limits
-// lim_u x lim_v) and add each piece.
-template <typename T>
-__host__ __device__ void walk_edge(double ua, double va, double ub, double vb, int lim_u, int lim_v,
-                                   Prefix& px, Moments& M) {
-  const double du = ub - ua, dvv = vb - va;
-  if (dvv == 0.0) return;
-  int iu = (int)floor(ua), iv = (int)floor(va);
-  const int iu_end = (int)floor(ub), iv_end = (int)floor(vb);
-  const int su = du > 0.0 ? 1 : -1, sv = dvv > 0.0 ? 1 : -1;
-  const double idu = du != 0.0 ? 1.0 / du : 0.0, idv = 1.0 / dvv;
-  // next lattice lines crossed
-  int nu_line = du > 0.0 ? iu + 1 : iu;
-  int nv_line = dvv > 0.0 ? iv + 1 : iv;
-  double tu = du != 0.0 ? ((double)nu_line - ua) * idu : 2.0;
-  double tv = ((double)nv_line - va) * idv;
-  const int maxsteps = abs(iu_end - iu) + abs(iv_end - iv) + 2;
-  double t = 0.0, U0 = ua, V0 = va;
-  for (int s = 0; s <= maxsteps; ++s) {
+// lim_u x lim_v) and add each piece.  The table reads of the next piece are issued before the
+// current piece is evaluated.
+struct EdgeWalker {
+  double ua, va, ub, vb, du, dvv, idu, idv, tu, tv, t;
+  int su, sv, nu_line, nv_line, lim_u, lim_v, steps_left;
+  double U0, V0;
+
+  __host__ __device__ __forceinline__ void init(double a_u, double a_v, double b_u, double b_v,
+                                                int lu, int lv) {
+    ua = a_u; va = a_v; ub = b_u; vb = b_v; lim_u = lu; lim_v = lv;
+    du = ub - ua;
+    dvv = vb - va;
+    const int iu = (int)floor(ua), iv = (int)floor(va);
+    const int iu_end = (int)floor(ub), iv_end = (int)floor(vb);
+    su = du > 0.0 ? 1 : -1;
+    sv = dvv > 0.0 ? 1 : -1;
+    idu = du != 0.0 ? 1.0 / du : 0.0;
+    idv = 1.0 / dvv;
+    nu_line = du > 0.0 ? iu + 1 : iu;
+    nv_line = dvv > 0.0 ? iv + 1 : iv;
+    tu = du != 0.0 ? ((double)nu_line - ua) * idu : 2.0;
+    tv = ((double)nv_line - va) * idv;
+    steps_left = abs(iu_end - iu) + abs(iv_end - iv) + 2;
+    t = 0.0;
+    U0 = ua;
+    V0 = va;
+  }
+  // The next piece: its end point and pixel; returns true when it is the edge's last piece.
+  __host__ __device__ __forceinline__ bool next(double& P0u, double& P0v, double& P1u,
+                                                double& P1v, int& ix, int& iy) {
     double tn = fmin(fmin(tu, tv), 1.0);
     if (tn < t) tn = t;
+    const bool last = tn >= 1.0 || --steps_left < 0;
     double U1, V1;
     if (tn >= 1.0) {
       U1 = ub;
@@ -318,22 +357,41 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
       U1 = ua + tn * du;
       V1 = va + tn * dvv;
     }
-    int ix = (int)floor(0.5 * (U0 + U1));
-    int iy = (int)floor(0.5 * (V0 + V1));
-    ix = min(max(ix, 0), lim_u - 1);
-    iy = min(max(iy, 0), lim_v - 1);
-    piece<T>(U0, V0, U1, V1, ix, iy, px, M);
-    if (tn >= 1.0) break;
-    if (tu <= tv) {
-      nu_line += su;
-      tu = ((double)nu_line - ua) * idu;
-    } else {
-      nv_line += sv;
-      tv = ((double)nv_line - va) * idv;
+    ix = min(max((int)floor(0.5 * (U0 + U1)), 0), lim_u - 1);
+    iy = min(max((int)floor(0.5 * (V0 + V1)), 0), lim_v - 1);
+    P0u = U0;
+    P0v = V0;
+    P1u = U1;
+    P1v = V1;
+    if (!last) {
+      if (tu <= tv) {
+        nu_line += su;
+        tu = ((double)nu_line - ua) * idu;
+      } else {
+        nv_line += sv;
+        tv = ((double)nv_line - va) * idv;
+      }
+      t = tn;
+      U0 = U1;
+      V0 = V1;
     }
-    t = tn;
-    U0 = U1;
-    V0 = V1;
+    return last;
+  }
+};
+
+template <typename T>
+__host__ __device__ void walk_edge(double ua, double va, double ub, double vb, int lim_u, int lim_v,
+                                   Prefix& px, Moments& M) {
+  if (vb - va == 0.0) return;
+  EdgeWalker w;
+  w.init(ua, va, ub, vb, lim_u, lim_v);
+  for (;;) {
+    double a0u, a0v, a1u, a1v;
+    int ix, iy;
+    const bool last = w.next(a0u, a0v, a1u, a1v, ix, iy);
+    const PrefixTaps taps = px.load(iy, ix);
+    piece<T>(a0u, a0v, a1u, a1v, ix, iy, px, taps, M);
+    if (last) break;
   }
 }
 
@@ -352,7 +410,9 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + ln;
   if (a.redo_mode && !a.redo[i]) return -1;
-  if (a.redo) a.redo[i] = 0;
+  // the full pass clears the flag (an FP32 cell may set it again below); in a repair pass the
+  // integration pass that follows clears it
+  if (!a.redo_mode && a.redo) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   // trial weights w + t * d, with d = -g for a steepest-descent trial
   double tstep = 0.0;
@@ -496,8 +556,9 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
   a.ring_used[i] = (uint8_t)min(used, 255);
   a.nedge[i] = (uint8_t)nv;
 
-  double m0 = 0.0, fi = -wi * a.nu[i] , ar = 0.0;
-  double2 cen = make_double2(0.0, 0.0), geo = make_double2(0.0, 0.0);
+  double ar = 0.0;
+  double2 geo = make_double2(0.0, 0.0);
+  int2 kb = make_int2(0, 0);
   if (nv >= 3) {
     // The final polygon in FP64, edge by edge.  FP64 geometry: the vertices as clipped.  FP32
     // geometry: the clipping only decides which constraint bounds each edge; every vertex is
@@ -599,6 +660,8 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
       }
       a.elab[(size_t)i * kMaxVerts + k] = lab;
       a.espan[(size_t)i * kMaxVerts + k] = span;
+      // the vertex for the integration pass, absolute coordinates
+      a.vtx[(size_t)i * kMaxVerts + k] = make_double2(o.x + va.x, o.y + va.y);
     });
     ar = 0.5 * A2;
     if (A2 != 0.0) {
@@ -607,98 +670,154 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
       geo = o;
     }
 
-    // Green integration in local pixel units, integer origin (kox, koy) at the cell's corner
-    const double idx_ = 1.0 / pd.dx, idy_ = 1.0 / pd.dy;
-    int kox = (int)floor((o.x + mnu - pd.x0) * idx_);
-    int koy = (int)floor((o.y + mnv - pd.y0) * idy_);
-    kox = min(max(kox, 0), pd.pm - 1);
-    koy = min(max(koy, 0), pd.pn - 1);
-    const double fu0 = (o.x - pd.x0) * idx_ - (double)kox;  // frame origin, local pixel units
-    const double fv0 = (o.y - pd.y0) * idy_ - (double)koy;
-    const double ou = (s.x - pd.x0) * idx_ - (double)kox;  // site, local pixel units
-    const double ov = (s.y - pd.y0) * idy_ - (double)koy;
-    Prefix px;
-    px.pre = a.pre + pd.pre_off;
-    px.tot = a.tot + pd.tot_off;
-    px.pm = pd.pm;
-    px.nchunk = pd.nchunk;
-    px.kox = kox;
-    px.koy = koy;
-    Moments M;
-    const int lim_u = pd.pm - kox, lim_v = pd.pn - koy;
-    for_each_edge([&](int, double2 va, double2 vb) {
-      walk_edge<T>(fu0 + va.x * idx_, fv0 + va.y * idy_, fu0 + vb.x * idx_, fv0 + vb.y * idy_,
-                   lim_u, lim_v, px, M);
-    });
-    m0 = M.m0.value();
-    // second moment about the site, physical units
-    const double su2 = M.m2u - 2.0 * ou * M.m1u + ou * ou * m0;
-    const double sv2 = M.m2v - 2.0 * ov * M.m1v + ov * ov * m0;
-    const double m2 = pd.dx * pd.dx * su2 + pd.dy * pd.dy * sv2;
-    fi += wi * m0 - m2;
-    if (m0 != 0.0) {
-      cen = make_double2(pd.x0 + pd.dx * ((double)kox + M.m1u / m0),
-                         pd.y0 + pd.dy * ((double)koy + M.m1v / m0));
-    } else {
-      cen = geo;
-    }
+    // integer pixel origin of the cell's box, for the integration pass
+    int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
+    int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
+    kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
     if (kF32) a.org[i] = geo;  // next frame: this centroid
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
+  a.kbox[i] = kb;
   if (kF32 && coarse && a.coarse) {
     a.redo[i] = 1;
     fetch_add(&a.coarse[bd.prob], 1);
   }
-  a.mass[i] = m0;
-  a.fcell[i] = fi;
   a.area[i] = ar;
-  a.cen[i] = cen;
   a.geo[i] = geo;
   a.exist[i] = nv >= 3 ? 1 : 0;
   return overflow ? 1 : 0;
 }
 
-// In-kernel launch timer (kTimerWords words, see cot_internal.h), read with the constant-rate
-// wall clock: the launch's duration runs from the earliest block start to the latest end of a
-// block that did work; launches in which no block did work are not counted.
-__device__ __forceinline__ void timer_start(unsigned long long* tm) {
-  if (tm && threadIdx.x == 0) atomicMin(&tm[0], (unsigned long long)wall_clock64());
+// ---------------------------------------------------------------------------------------------
+// integration pass: kIntLanes lanes per site; lane q walks edges q, q + kIntLanes, ...
+// ---------------------------------------------------------------------------------------------
+
+// The site of integration lane `t` of block (blk, quarter); -1 when none.
+__host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, int quarter,
+                                                   int t) {
+  const BlockDesc bd = a.blocks[blk];
+  const int ln = quarter * (kIntBlock / kIntLanes) + t / kIntLanes;
+  if (ln >= bd.count) return -1;
+  if (!a.active[bd.prob]) return -1;
+  if (a.gate && a.gate[bd.prob] == 0) return -1;
+  const int i = bd.first + ln;
+  if (a.redo_mode && !a.redo[i]) return -1;
+  return i;
 }
 
-__device__ void timer_stop(unsigned long long* tm, bool worked) {
+// Moments of lane q's share of site i's edges (local pixel units, origin kbox[i]).
+template <typename T>
+__host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, int i, int q,
+                                        Moments& M) {
+  const int nv = a.nedge[i];
+  if (nv < 3) return;
+  const int2 kb = a.kbox[i];
+  Prefix px;
+  px.pre = a.pre + pd.pre_off;
+  px.tot = a.tot + pd.tot_off;
+  px.pm = pd.pm;
+  px.nchunk = pd.nchunk;
+  px.kox = kb.x;
+  px.koy = kb.y;
+  const double idx_ = 1.0 / pd.dx, idy_ = 1.0 / pd.dy;
+  const double fu0 = (double)kb.x, fv0 = (double)kb.y;
+  const int lim_u = pd.pm - kb.x, lim_v = pd.pn - kb.y;
+  const double2* v = a.vtx + (size_t)i * kMaxVerts;
+  for (int k = q; k < nv; k += kIntLanes) {
+    const double2 va = v[k], vb = v[k + 1 < nv ? k + 1 : 0];
+    walk_edge<T>((va.x - pd.x0) * idx_ - fu0, (va.y - pd.y0) * idy_ - fv0,
+                 (vb.x - pd.x0) * idx_ - fu0, (vb.y - pd.y0) * idy_ - fv0, lim_u, lim_v, px, M);
+  }
+}
+
+// Add moments b into a (compensated for the mass).
+__host__ __device__ __forceinline__ void combine(Moments& a, const Moments& b) {
+  a.m0.add(b.m0.s);
+  a.m0.c += b.m0.c;
+  a.m1u += b.m1u;
+  a.m1v += b.m1v;
+  a.m2u += b.m2u;
+  a.m2v += b.m2v;
+}
+
+// Outputs of site i from its cell's total moments: mass, objective term, centroid.
+__host__ __device__ void finish_site(const CellArgs& a, const ProbDesc& pd, int prob, int i,
+                                     const Moments& M) {
+  double tstep = 0.0;
+  bool steep = false;
+  if (a.st) {
+    tstep = a.st[prob].t;
+    steep = a.st[prob].steep != 0;
+  } else if (a.dir) {
+    tstep = a.step[prob];
+  }
+  const double wi = a.dir ? a.w[i] + tstep * (steep ? -a.g[i] : a.dir[i]) : a.w[i];
+  double m0 = 0.0, fi = -wi * a.nu[i];
+  double2 cen = make_double2(0.0, 0.0);
+  if (a.nedge[i] >= 3) {
+    const int2 kb = a.kbox[i];
+    const double2 s = a.xy[i];
+    m0 = M.m0.value();
+    // second moment about the site, physical units
+    const double ou = (s.x - pd.x0) / pd.dx - (double)kb.x;
+    const double ov = (s.y - pd.y0) / pd.dy - (double)kb.y;
+    const double su2 = M.m2u - 2.0 * ou * M.m1u + ou * ou * m0;
+    const double sv2 = M.m2v - 2.0 * ov * M.m1v + ov * ov * m0;
+    fi += wi * m0 - (pd.dx * pd.dx * su2 + pd.dy * pd.dy * sv2);
+    cen = m0 != 0.0 ? make_double2(pd.x0 + pd.dx * ((double)kb.x + M.m1u / m0),
+                                   pd.y0 + pd.dy * ((double)kb.y + M.m1v / m0))
+                    : a.geo[i];
+  }
+  a.mass[i] = m0;
+  a.fcell[i] = fi;
+  a.cen[i] = cen;
+  if (a.redo_mode) a.redo[i] = 0;
+}
+
+
+// In-kernel launch timer (kTimerWords words, see cot_internal.h), read with the constant-rate
+// wall clock.  timer_begin: the block's start (thread 0); timer_mark: after the block's work, a
+// block in which some thread worked folds (start, now) into its slot; timer_close: thread 0 of
+// block 0 of the kernel that follows on the stream accumulates the closed launch.
+__device__ __forceinline__ unsigned long long timer_begin() {
+  return threadIdx.x == 0 ? (unsigned long long)wall_clock64() : 0ull;
+}
+
+__device__ void timer_mark(unsigned long long* tm, bool worked, unsigned long long t0) {
   if (!tm) return;
   __shared__ int any;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
   if (worked) any = 1;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  if (any) {
-    atomicMax(&tm[1], (unsigned long long)wall_clock64());
-    atomicAdd(&tm[3], 1ull);
+  if (threadIdx.x != 0 || !any) return;
+  const int r = (blockIdx.x + gridDim.x * blockIdx.y) % kTimerSlots;
+  atomicMin(&tm[2 * r], t0);
+  atomicMax(&tm[2 * r + 1], (unsigned long long)wall_clock64());
+}
+
+__device__ void timer_close(unsigned long long* tm) {
+  if (!tm || threadIdx.x != 0 || blockIdx.x != 0 || blockIdx.y != 0) return;
+  unsigned long long t0 = ~0ull, t1 = 0ull;
+  for (int r = 0; r < kTimerSlots; ++r) {
+    t0 = min(t0, tm[2 * r]);
+    t1 = max(t1, tm[2 * r + 1]);
+    tm[2 * r] = ~0ull;
+    tm[2 * r + 1] = 0ull;
   }
-  __threadfence();
-  const unsigned long long prev = atomicAdd(&tm[2], 1ull);
-  if (prev != gridDim.x - 1) return;
-  __threadfence();
-  const unsigned long long t0 = atomicAdd(&tm[0], 0ull), t1 = atomicAdd(&tm[1], 0ull);
-  if (atomicAdd(&tm[3], 0ull) > 0 && t1 > t0) {
-    atomicAdd(&tm[4], t1 - t0);
-    atomicAdd(&tm[5], 1ull);
+  if (t0 != ~0ull && t1 > t0) {
+    tm[2 * kTimerSlots] += t1 - t0;
+    tm[2 * kTimerSlots + 1] += 1;
   }
-  atomicExch(&tm[0], ~0ull);
-  atomicExch(&tm[1], 0ull);
-  atomicExch(&tm[2], 0ull);
-  atomicExch(&tm[3], 0ull);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   __shared__ PolyLDS<T> P;
-  timer_start(a.timer);
+  const unsigned long long t0 = timer_begin();
   const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
   if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
-  timer_stop(a.timer, rc >= 0);
+  timer_mark(a.timer, rc >= 0, t0);
 }
 
 // The same code for a repair pass (only sites flagged in redo[]); a kernel of its own so that
@@ -706,10 +825,53 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   __shared__ PolyLDS<T> P;
-  timer_start(a.timer);
+  const unsigned long long t0 = timer_begin();
   const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
   if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
-  timer_stop(a.timer, rc >= 0);
+  timer_mark(a.timer, rc >= 0, t0);
+}
+
+// Integration pass.  Grid (cell blocks, kCellBlock * kIntLanes / kIntBlock), block kIntBlock:
+// kIntLanes consecutive lanes per site; the lanes' moments are summed with shuffles within their
+// aligned group.
+template <typename T>
+__device__ __forceinline__ void integrate_block(const CellArgs& a) {
+  const int t = threadIdx.x;
+  const int i = integ_site(a, blockIdx.x, blockIdx.y, t);
+  const int prob = a.blocks[blockIdx.x].prob;
+  Moments M;
+  if (i >= 0) integrate_lane<T>(a, a.probs[prob], i, t % kIntLanes, M);
+#pragma unroll
+  for (int off = kIntLanes / 2; off > 0; off >>= 1) {
+    Moments o;
+    o.m0.s = __shfl_xor(M.m0.s, off, 64);
+    o.m0.c = __shfl_xor(M.m0.c, off, 64);
+    o.m1u = __shfl_xor(M.m1u, off, 64);
+    o.m1v = __shfl_xor(M.m1v, off, 64);
+    o.m2u = __shfl_xor(M.m2u, off, 64);
+    o.m2v = __shfl_xor(M.m2v, off, 64);
+    combine(M, o);
+  }
+  if (i >= 0 && t % kIntLanes == 0) finish_site(a, a.probs[prob], prob, i, M);
+}
+
+// a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
+template <typename T>
+__global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a) {
+  timer_close(a.timer);
+  const unsigned long long t0 = timer_begin();
+  const bool worked = integ_site(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
+  integrate_block<T>(a);
+  timer_mark(a.timer_int, worked, t0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a) {
+  timer_close(a.timer);
+  const unsigned long long t0 = timer_begin();
+  const bool worked = integ_site(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
+  integrate_block<T>(a);
+  timer_mark(a.timer_int, worked, t0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -760,10 +922,9 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
 }
 
 __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
-  timer_start(a.timer);
+  timer_close(a.timer_close);
   const int rc = verify_site(a, blockIdx.x, threadIdx.x);
   if (rc == 1) atomicAdd(&a.fails[a.blocks[blockIdx.x].prob], 1);
-  timer_stop(a.timer, rc >= 0);
 }
 
 // Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
@@ -941,15 +1102,21 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 // launchers
 // ---------------------------------------------------------------------------------------------
 
+// Construction (one lane per site) then integration (kIntLanes lanes per site).
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st) {
+  const CellArgs& b = a;  // the integration pass closes a.timer and marks a.timer_int
+  const dim3 ig(nblocks, kCellBlock * kIntLanes / kIntBlock);
   if (a.redo_mode) {
     // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
     // of the cells the FP32 pass could not resolve)
     hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    hipLaunchKernelGGL((integrate_repair_kernel<double>), ig, dim3(kIntBlock), 0, st, b);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    hipLaunchKernelGGL((integrate_kernel<float>), ig, dim3(kIntBlock), 0, st, b);
   } else {
     hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    hipLaunchKernelGGL((integrate_kernel<double>), ig, dim3(kIntBlock), 0, st, b);
   }
   return hipGetLastError();
 }

@@ -99,37 +99,44 @@ __device__ bool arrive_last(int* counter, int nblk) {
 }
 
 // Two-loop recursion of ot_op.cuh:298-337 in coefficient space (d = dg g + sum qy y + sum qs s),
-// stored as coef = [cg, cy[], cs[]] with d = -(cg g + sum cy y + sum cs s).
-__device__ void plan_direction(DevState& S, int m) {
+// stored as coef = [cg, cy[], cs[]] with d = -(cg g + sum cy y + sum cs s).  Runs on one thread
+// over a shared-memory copy of the Gram matrices (L), with its work arrays in shared memory too.
+struct LdsHist {
+  double SY[kMaxHist][kMaxHist];
+  double YY[kMaxHist][kMaxHist];
+  double Sg[kMaxHist], Yg[kMaxHist], rho[kMaxHist], alpha[kMaxHist], qy[kMaxHist], qs[kMaxHist];
+};
+
+__device__ void plan_direction(LdsHist& L, int loop_iter, bool restart, double H0, int m,
+                               double* coef, int& hcount) {
   constexpr int K = kMaxHist;
-  double qy[K], qs[K];
-  for (int b = 0; b < K; ++b) qy[b] = qs[b] = 0.0;
-  const int maxidx = min(S.loop_iter, m);
+  for (int b = 0; b < K; ++b) L.qy[b] = L.qs[b] = 0.0;
+  const int maxidx = min(loop_iter, m);
   double dg = -1.0;
-  if (!S.restart) {
+  if (!restart) {
     for (int i = 1; i <= maxidx; ++i) {
-      const int idx = ((S.loop_iter - i) % m + m) % m;
-      double sd = dg * S.Sg[idx];
-      for (int b = 0; b < maxidx; ++b) sd += qy[b] * S.SY[idx][b];
-      S.alpha[idx] = sd * S.rho[idx];
-      qy[idx] -= S.alpha[idx];
+      const int idx = ((loop_iter - i) % m + m) % m;
+      double sd = dg * L.Sg[idx];
+      for (int b = 0; b < maxidx; ++b) sd += L.qy[b] * L.SY[idx][b];
+      L.alpha[idx] = sd * L.rho[idx];
+      L.qy[idx] -= L.alpha[idx];
     }
-    dg *= S.H0;
-    for (int b = 0; b < maxidx; ++b) qy[b] *= S.H0;
+    dg *= H0;
+    for (int b = 0; b < maxidx; ++b) L.qy[b] *= H0;
     for (int i = maxidx; i >= 1; --i) {
-      const int idx = ((S.loop_iter - i) % m + m) % m;
-      double yd = dg * S.Yg[idx];
-      for (int b = 0; b < maxidx; ++b) yd += qy[b] * S.YY[idx][b] + qs[b] * S.SY[b][idx];
-      const double beta = S.rho[idx] * yd;
-      qs[idx] += S.alpha[idx] - beta;
+      const int idx = ((loop_iter - i) % m + m) % m;
+      double yd = dg * L.Yg[idx];
+      for (int b = 0; b < maxidx; ++b) yd += L.qy[b] * L.YY[idx][b] + L.qs[b] * L.SY[b][idx];
+      const double beta = L.rho[idx] * yd;
+      L.qs[idx] += L.alpha[idx] - beta;
     }
   }
-  S.coef[0] = -dg;
+  coef[0] = -dg;
   for (int b = 0; b < K; ++b) {
-    S.coef[1 + b] = -qy[b];
-    S.coef[1 + K + b] = -qs[b];
+    coef[1 + b] = -L.qy[b];
+    coef[1 + K + b] = -L.qs[b];
   }
-  S.hcount = S.restart ? 0 : maxidx;
+  hcount = restart ? 0 : maxidx;
 }
 
 }  // namespace
@@ -218,11 +225,10 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   __shared__ double tot[5];
   sum_partials(part, pstride, pd.blk_off, pd.nblk, 5, lw, tot);
   // the max relative error is a max, not a sum
-  if (ln == 0) {
-    double m = 0.0;
-    for (int b = 0; b < pd.nblk; ++b) m = fmax(m, part[1 * pstride + pd.blk_off + b]);
-    tot[1] = m;
-  }
+  double mloc = 0.0;
+  for (int b = ln; b < pd.nblk; b += blockDim.x) mloc = fmax(mloc, part[1 * pstride + pd.blk_off + b]);
+  mloc = block_max1(mloc, red);
+  if (ln == 0) tot[1] = mloc;
   __syncthreads();
   if (ln != 0) return;
   const int p = bd.prob;
@@ -333,23 +339,61 @@ __global__ __launch_bounds__(kCellBlock) void dev_accept_kernel(
       __syncthreads();
     }
   }
-  if (!arrive_last(&counter[bd.prob], pd.nblk)) return;
-  __shared__ double lw[4 * 16];
-  __shared__ double P[5 * kMaxHist];
-  for (int q = 0; q < 5; ++q)
-    sum_partials(part + (size_t)q * kMaxHist * pstride, pstride, pd.blk_off, pd.nblk, H, lw,
-                 P + q * kMaxHist);
-  if (ln != 0) return;
-  for (int b = 0; b < H; ++b) {
-    s.SY[slot][b] = P[b];
-    s.SY[b][slot] = P[kMaxHist + b];
-    s.YY[slot][b] = s.YY[b][slot] = P[2 * kMaxHist + b];
-    s.Sg[b] = P[3 * kMaxHist + b];
-    s.Yg[b] = P[4 * kMaxHist + b];
+  (void)counter;
+  (void)pd;
+}
+
+// The accepted problems' reductions of the accept partials (one block of kFinishBlock threads
+// per problem: each wave sums some of the 5H values over the problem's blocks), then the Gram
+// update, H0 (ot_op.cuh:434-466), stopping test (:510) and the next direction's coefficients.
+constexpr int kFinishBlock = 1024;
+__global__ __launch_bounds__(kFinishBlock) void dev_accept_finish_kernel(
+    const ProbDesc* probs, DevState* st, SolveParams sp, const double* part, size_t pstride,
+    uint8_t* active) {
+  DevState& s = st[blockIdx.x];
+  if (s.phase != kPhaseTrial || !s.accept) return;
+  const ProbDesc pd = probs[blockIdx.x];
+  const int ln = threadIdx.x, wv = ln >> 6, wl = ln & 63;
+  const int m = sp.m;
+  const int slot = s.loop_iter % m;
+  const int H = min(s.loop_iter + 1, m);
+  constexpr int K = kMaxHist;
+  __shared__ double P[5 * K];
+  __shared__ LdsHist L;
+  for (int v = wv; v < 5 * K; v += kFinishBlock / 64) {
+    if (v % K >= H) continue;
+    double acc = 0.0;
+    for (int b = wl; b < pd.nblk; b += 64) acc += part[(size_t)v * pstride + pd.blk_off + b];
+    acc = wsum(acc);
+    if (wl == 0) P[v] = acc;
   }
-  s.SY[slot][slot] = P[slot];
-  const double ys = s.SY[slot][slot], yy = s.YY[slot][slot];
-  s.rho[slot] = 1.0 / ys;
+  __syncthreads();
+  // Gram update in shared memory (the whole block), written back to the state
+  for (int e = ln; e < K * K; e += blockDim.x) {
+    const int r = e / K, c = e % K;
+    double sy = s.SY[r][c], yy = s.YY[r][c];
+    if (r == slot && c < H) sy = P[c];
+    if (c == slot && r < H) sy = P[K + r];
+    if (r == slot && c < H) yy = P[2 * K + c];
+    if (c == slot && r < H) yy = P[2 * K + r];
+    if (r == slot && c == slot) sy = P[slot];
+    L.SY[r][c] = sy;
+    L.YY[r][c] = yy;
+    s.SY[r][c] = sy;
+    s.YY[r][c] = yy;
+  }
+  for (int b = ln; b < K; b += blockDim.x) {
+    L.Sg[b] = b < H ? P[3 * K + b] : s.Sg[b];
+    L.Yg[b] = b < H ? P[4 * K + b] : s.Yg[b];
+    L.rho[b] = s.rho[b];
+    s.Sg[b] = L.Sg[b];
+    s.Yg[b] = L.Yg[b];
+  }
+  __syncthreads();
+  if (ln != 0) return;
+  const double ys = L.SY[slot][slot], yy = L.YY[slot][slot];
+  L.rho[slot] = 1.0 / ys;
+  s.rho[slot] = L.rho[slot];
   s.H0 = yy > sp.h0_thr ? ys / yy : 1.0;
   s.loop_iter += 1;
   s.restart = 0;
@@ -369,12 +413,14 @@ __global__ __launch_bounds__(kCellBlock) void dev_accept_kernel(
   if (done) {
     s.conv = 1;
     s.phase = kPhaseDone;
-    active[bd.prob] = 0;
+    active[blockIdx.x] = 0;
     return;
   }
-  plan_direction(s, m);
+  int hc = 0;
+  plan_direction(L, s.loop_iter, s.restart != 0, s.H0, m, s.coef, hc);
+  s.hcount = hc;
   s.phase = kPhaseDir;
-  active[bd.prob] = 0;
+  active[blockIdx.x] = 0;
 }
 
 // Repair-round helpers driven by the per-problem failure counter of the previous check.
@@ -449,6 +495,14 @@ hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, Dev
                              uint8_t* active, int nblk, hipStream_t stream) {
   hipLaunchKernelGGL(dev_accept_kernel, dim3(nblk), dim3(kCellBlock), 0, stream, probs, blocks,
                      st, sp, mass, nu, d, w, g, S, Y, ld, part, pstride, counter, active);
+  return hipGetLastError();
+}
+
+hipError_t launch_dev_accept_finish(const ProbDesc* probs, DevState* st, const SolveParams& sp,
+                                    const double* part, size_t pstride, uint8_t* active, int np,
+                                    hipStream_t stream) {
+  hipLaunchKernelGGL(dev_accept_finish_kernel, dim3(np), dim3(kFinishBlock), 0, stream, probs, st,
+                     sp, part, pstride, active);
   return hipGetLastError();
 }
 

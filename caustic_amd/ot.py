@@ -132,7 +132,7 @@ class OTBatch:
 
     def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
-              verbose: int = 0, raise_on_fail: bool = True, device_loop: bool = True,
+              verbose: int = 0, raise_on_fail: bool = True, device_loop=True,
               accept_rule: str = "auto", repair_rounds: int = 3):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
@@ -151,7 +151,9 @@ class OTBatch:
         o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL}[stop]
         o.h0_threshold = h0_threshold
         o.verbose = verbose
-        o.device_loop = int(bool(device_loop))
+        # True/1: graph-replayed device rounds; 2: the same rounds launched directly; False/0:
+        # host-driven rounds
+        o.device_loop = int(device_loop) if not isinstance(device_loop, bool) else int(device_loop)
         o.accept_rule = {"decrease": 0, "derivative": 1}[accept_rule]
         o.repair_rounds = repair_rounds
         reps = (L.cot_report * self.n_problems)()

@@ -70,6 +70,7 @@ typedef struct {
   int verbose;              /* host loop: a line every `verbose` iterations; device loop: a
                                progress line every 2 s (0: silent) */
   int device_loop;          /* 1: device-resident rounds replayed from a hipGraph (default);
+                               2: the same device-resident rounds launched without a graph;
                                0: host-driven rounds (one host sync per phase) */
   int accept_rule;          /* 0: accept a trial on f decrease (ot_op.cuh:397);
                                1: also accept when the directional derivative at the trial is
@@ -140,20 +141,22 @@ int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centr
 /* Area of each cell (inside proj_domain) at weights w. */
 int cot_cell_areas(cot_context* ctx, const double* w, double* area);
 
-/* Kernel time accumulated since the last reset.  Each launch of the cell kernel (construction
- * + integration), of its repair pass (cells that failed the conformity check) and of the check
- * itself is timed inside the kernel with the device's constant-rate wall clock, from the
- * earliest block start to the latest end of a block that did work; launches in which no block
- * had work (all problems finished) are not counted. */
+/* Kernel time accumulated since the last reset.  Each launch of the cell-construction kernel,
+ * of the integration kernel and of their repair passes (cells that failed the conformity check)
+ * is timed inside the kernel with the device's constant-rate wall clock, from the earliest
+ * start to the latest end of a block that did work; launches in which no block had work (all
+ * problems finished) are not counted.  The conformity check is not timed (verify_* stay 0). */
 typedef struct {
-  double cell_seconds;        /* full cell-kernel launches */
+  double cell_seconds;        /* full passes of the construction kernel */
   long long cell_launches;
-  double repair_seconds;      /* repair launches of the cell kernel */
+  double repair_seconds;      /* repair passes (construction + integration) */
   long long repair_launches;
   double verify_seconds;      /* conformity-check launches */
   long long verify_launches;
   long long evaluations;      /* evaluations (line-search trials included) */
   long long repair_rounds;    /* repair rounds run by host-driven evaluations */
+  double integ_seconds;       /* full passes of the integration kernel */
+  long long integ_launches;
 } cot_stats;
 
 int cot_get_stats(cot_context* ctx, cot_stats* out);

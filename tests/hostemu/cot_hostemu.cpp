@@ -28,6 +28,8 @@ struct Session {
   std::vector<double2> cen, geo;
   std::vector<uint8_t> exist, ne, ring, used, redo, adjn;
   std::vector<int> elab, adj, xcnt, xcand;
+  std::vector<double2> vtx;
+  std::vector<int2> kbox;
   std::vector<float2> espan;
   bool have_adj = false;
   int ring_max = 32;  // the library's default
@@ -102,6 +104,8 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
   S->elab.assign((size_t)n * kMaxVerts, 0);
   S->adj.assign((size_t)n * kMaxVerts, 0);
   S->xcnt.assign(n, 0);
+  S->vtx.assign((size_t)n * kMaxVerts, make_double2(0, 0));
+  S->kbox.assign(n, make_int2(0, 0));
   S->xcand.assign((size_t)n * kMaxExtra, 0);
   S->espan.assign((size_t)n * kMaxVerts, make_float2(0.f, 0.f));
   pd.nchunk = (pm + kChunk - 1) / kChunk;
@@ -171,9 +175,11 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   a.ring_max = S.ring_max;
   a.adj = S.adj.data();
   a.adjn = S.adjn.data();
-  a.cand = S.have_adj ? kCandAdj2 : kCandBucket;
+  a.cand = S.have_adj ? (S.fp32 ? kCandAdj1 : kCandAdj2) : kCandBucket;  // as the library
   a.xcnt = S.xcnt.data();
   a.xcand = S.xcand.data();
+  a.vtx = S.vtx.data();
+  a.kbox = S.kbox.data();
   VerifyArgs v{};
   v.probs = &S.pd;
   v.blocks = S.blocks.data();
@@ -206,6 +212,27 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
           built += rc >= 0 ? 1 : 0;
         }
     }
+    // integration pass: kIntLanes lanes per site, summed in the order of the GPU's butterfly
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; ++b)
+      for (int quarter = 0; quarter < kCellBlock * kIntLanes / kIntBlock; ++quarter)
+        for (int t0 = 0; t0 < kIntBlock; t0 += kIntLanes) {
+          const int i = integ_site(a, b, quarter, t0);
+          if (i < 0) continue;
+          Moments M[kIntLanes];
+          for (int q = 0; q < kIntLanes; ++q)
+            f32 ? integrate_lane<float>(a, S.pd, i, q, M[q])
+                : integrate_lane<double>(a, S.pd, i, q, M[q]);
+          for (int off = kIntLanes / 2; off > 0; off >>= 1) {
+            Moments nxt[kIntLanes];
+            for (int q = 0; q < kIntLanes; ++q) {
+              nxt[q] = M[q];
+              combine(nxt[q], M[q ^ off]);
+            }
+            for (int q = 0; q < kIntLanes; ++q) M[q] = nxt[q];
+          }
+          finish_site(a, S.pd, 0, i, M[0]);
+        }
     return built;
   };
   auto run_verify = [&]() {
