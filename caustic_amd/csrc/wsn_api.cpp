@@ -48,8 +48,8 @@ hipError_t launch_up_back(const float* dout, int B, int H, int W, int C, int Cd,
                           float* din, hipStream_t st);
 hipError_t launch_copy_channels(const float* src, int Cs, int s0, float* dst, int Cd, int d0,
                                 int nc, size_t P, int acc, hipStream_t st);
-hipError_t launch_loss(const float* out, const float* t, int S2, int B, float* loss, float* dout,
-                       hipStream_t st);
+hipError_t launch_loss(const float* out, const float* t, int S, int B, float lam, float* loss,
+                       float* dout, hipStream_t st);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, size_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t st);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t st);
@@ -133,6 +133,7 @@ struct wsn_net {
   int ds_n = 0;
   double train_seconds = 0.0;
   long long train_samples = 0;
+  float grad_weight = 0.f;  // weight of the neighbouring-difference term of the loss
 
   ~wsn_net() {
     if (st) hipStreamSynchronize(st);
@@ -320,9 +321,8 @@ int check_batch(wsn_net* n, int B) {
 
 // forward + loss (+ gradients into n->g); in and tgt already hold the batch
 int loss_and_grad(wsn_net* n, int B, float* loss_out) {
-  const int S2 = n->S * n->S;
   CHK(forward(n, B));
-  HIPCHK(launch_loss(n->act[OUT], n->tgt, S2, B, n->loss, n->gact[OUT], n->st));
+  HIPCHK(launch_loss(n->act[OUT], n->tgt, n->S, B, n->grad_weight, n->loss, n->gact[OUT], n->st));
   CHK(backward(n, B));
   if (loss_out) {
     std::vector<float> ls(B);
@@ -462,6 +462,12 @@ int wsn_gradients(wsn_net* n, const float* x, const float* t, int B, float* grad
   CHK(loss_and_grad(n, B, loss));
   HIPCHK(hipMemcpyAsync(grad, n->g, sizeof(float) * n->np, hipMemcpyDeviceToHost, n->st));
   HIPCHK(hipStreamSynchronize(n->st));
+  return WSN_OK;
+}
+
+int wsn_set_loss(wsn_net* n, float grad_weight) {
+  if (!n || !(grad_weight >= 0.f)) return fail(WSN_EINVAL, "bad loss weight");
+  n->grad_weight = grad_weight;
   return WSN_OK;
 }
 

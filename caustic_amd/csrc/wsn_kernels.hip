@@ -307,12 +307,14 @@ __global__ void copy_channels_kernel(const float* src, int Cs, int s0, float* ds
   *d = accumulate ? *d + v : v;
 }
 
-// Per sample b (one block): e = out - mean(out) - t; loss_b = sum e^2; dout = 2 (e - mean e) /
-// (B * S2).  Block 256.
-__global__ void loss_kernel(const float* out, const float* t, int S2, int B, float* loss,
-                            float* dout) {
+// Per sample b (one block of 256): e = out - mean(out) - t over the S x S image;
+// loss_b = (sum e^2 + lam * sum over neighbouring pixel pairs (e_a - e_b)^2) / (B S^2);
+// dout = de - mean(de) with de_k = 2 (e_k + lam * sum over k's neighbours (e_k - e_nbr)) / (B S^2).
+__global__ void loss_kernel(const float* out, const float* t, int S, int B, float lam,
+                            float* loss, float* dout) {
   __shared__ double red[256];
   const int b = blockIdx.x;
+  const int S2 = S * S;
   const float* o = out + (size_t)b * S2;
   const float* tt = t + (size_t)b * S2;
   auto block_sum = [&](double v) {
@@ -331,20 +333,36 @@ __global__ void loss_kernel(const float* out, const float* t, int S2, int B, flo
     so += o[i];
     st += tt[i];
   }
-  const double mo = block_sum(so) / S2, me = mo + block_sum(st) / S2;  // mean of o + t
+  const double mo = block_sum(so) / S2, mt = block_sum(st) / S2;
+  auto err = [&](int k) { return (double)o[k] - mo - tt[k]; };
   double se = 0.0;
   for (int i = threadIdx.x; i < S2; i += blockDim.x) {
-    const double e = (double)o[i] - mo - tt[i];
+    const double e = err(i);
     se += e * e;
+    const int x = i % S, y = i / S;
+    if (x + 1 < S) {
+      const double d = err(i + 1) - e;
+      se += lam * d * d;
+    }
+    if (y + 1 < S) {
+      const double d = err(i + S) - e;
+      se += lam * d * d;
+    }
   }
   const double ls = block_sum(se);
-  if (threadIdx.x == 0) loss[b] = (float)(ls / ((double)B * S2));
-  // mean(e) = mean(o) - mo - mean(t) = -mean(t)
-  const double mean_e = mo - me;
   const double sc = 2.0 / ((double)B * S2);
+  if (threadIdx.x == 0) loss[b] = (float)(ls / ((double)B * S2));
+  // mean over the image of de: the difference terms cancel; mean(e) = -mean(t)
+  const double mean_de = sc * (-mt);
   for (int i = threadIdx.x; i < S2; i += blockDim.x) {
-    const double e = (double)o[i] - mo - tt[i];
-    dout[(size_t)b * S2 + i] = (float)(sc * (e - mean_e));
+    const double e = err(i);
+    const int x = i % S, y = i / S;
+    double lap = 0.0;
+    if (x > 0) lap += e - err(i - 1);
+    if (x + 1 < S) lap += e - err(i + 1);
+    if (y > 0) lap += e - err(i - S);
+    if (y + 1 < S) lap += e - err(i + S);
+    dout[(size_t)b * S2 + i] = (float)(sc * (e + lam * lap) - mean_de);
   }
 }
 
@@ -431,9 +449,9 @@ hipError_t launch_copy_channels(const float* src, int Cs, int s0, float* dst, in
                      dst, Cd, d0, nc, P, acc);
   return hipGetLastError();
 }
-hipError_t launch_loss(const float* out, const float* t, int S2, int B, float* loss, float* dout,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(loss_kernel, dim3(B), dim3(256), 0, st, out, t, S2, B, loss, dout);
+hipError_t launch_loss(const float* out, const float* t, int S, int B, float lam, float* loss,
+                       float* dout, hipStream_t st) {
+  hipLaunchKernelGGL(loss_kernel, dim3(B), dim3(256), 0, st, out, t, S, B, lam, loss, dout);
   return hipGetLastError();
 }
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, size_t n, float lr, float b1,

@@ -96,16 +96,20 @@ class OTBatch:
     def split(self, v):
         return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n_problems)]
 
-    def evaluate(self, w=None):
-        """Masses, per-problem objective, rho-weighted centroids and existence flags at w."""
+    def evaluate(self, w=None, allow_unverified: bool = False):
+        """Masses, per-problem objective, rho-weighted centroids and existence flags at w.
+        allow_unverified: return the cells as built (with "verified": False) when the
+        conformity check still fails after the repair rounds, instead of raising."""
         w = self._w(w)
         mass = np.zeros(self.n_total)
         f = np.zeros(self.n_problems)
         cen = np.zeros((self.n_total, 2))
         ex = np.zeros(self.n_total, dtype=np.uint8)
-        L.check(self._lib.cot_evaluate(self._ctx, _ptr(w), _ptr(mass), _ptr(f), _ptr(cen),
-                                       ex.ctypes.data_as(_up)))
-        return {"mass": mass, "f": f, "centroid": cen, "exist": ex.astype(bool)}
+        rc = self._lib.cot_evaluate(self._ctx, _ptr(w), _ptr(mass), _ptr(f), _ptr(cen),
+                                    ex.ctypes.data_as(_up))
+        L.check(rc, allow=(L.COT_EUNVERIFIED,) if allow_unverified else ())
+        return {"mass": mass, "f": f, "centroid": cen, "exist": ex.astype(bool),
+                "verified": rc == L.COT_OK}
 
     def centroids(self, w=None, weighted: bool = True):
         w = self._w(w)

@@ -58,6 +58,7 @@ def _load():
         "wsn_forward": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int, _fp]),
         "wsn_gradients": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int, _fp, _fp]),
         "wsn_set_dataset": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int]),
+        "wsn_set_loss": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
         "wsn_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                      ctypes.c_int, ctypes.c_float, _fp]),
         "wsn_train_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
@@ -143,6 +144,10 @@ class WarmStartNet:
         _check(self._lib.wsn_gradients(self._h, _p(x), _p(t), x.shape[0], _p(g),
                                        ctypes.byref(loss)))
         return g, float(loss.value)
+
+    def set_loss(self, grad_weight: float):
+        """Weight of the neighbouring-difference term of the training loss (0: plain MSE)."""
+        _check(self._lib.wsn_set_loss(self._h, ctypes.c_float(grad_weight)))
 
     def set_dataset(self, x, t):
         x = _f32(x).reshape(-1, self.S, self.S)
@@ -235,15 +240,39 @@ def features(rho: np.ndarray, S: int = S_DEFAULT) -> np.ndarray:
     return np.log(np.maximum(d, 1e-6)).astype(np.float32)
 
 
-def weights_to_target(w: np.ndarray, S: int) -> np.ndarray:
-    """Gauge-free weights (mean removed) in units of the site spacing squared."""
-    w = np.asarray(w, dtype=np.float64)
-    return ((w - w.mean()) * S * S).reshape(S, S).astype(np.float32)
+def grid_nodes(S: int) -> np.ndarray:
+    """Cell centres of an S x S grid of the unit square, in site order (row-major, y first)."""
+    k = np.arange(S * S)
+    return np.stack([(k % S + 0.5) / S, (k // S + 0.5) / S], axis=1)
 
 
-def target_to_weights(t: np.ndarray, S: int) -> np.ndarray:
-    t = np.asarray(t, dtype=np.float64).reshape(S, S)
-    return (t - t.mean()) / (S * S)
+def field_from_solution(sites: np.ndarray, w: np.ndarray, cen: np.ndarray, S: int) -> np.ndarray:
+    """The weight field U (w_i = -U(s_i)) sampled at the S x S grid nodes, for a jittered grid
+    whose site k sits near node k.  With psi the Brenier potential, w(s) = |s|^2 - 2 psi(s), so
+    U(s) = 2 psi(s) - |s|^2 is smooth with gradient -2 (s - c(s)), c(s) the centroid of the cell
+    of s; hence U(node_k) = -w_k + 2 (s_k - c_k).(s_k - node_k) to first order.  The mean is
+    removed (the gauge)."""
+    nodes = grid_nodes(S)
+    u = -np.asarray(w, dtype=np.float64) + 2.0 * np.sum((sites - cen) * (sites - nodes), axis=1)
+    return (u - u.mean()).reshape(S, S)
+
+
+def weights_from_field(U: np.ndarray, sites: np.ndarray) -> np.ndarray:
+    """w_i = -U(s_i), bilinear in the field, mean removed."""
+    w = -resample_field(np.asarray(U, dtype=np.float64), sites)
+    return w - w.mean()
+
+
+def field_to_target(U: np.ndarray) -> np.ndarray:
+    """Network target: the field in units of the grid spacing squared."""
+    S = U.shape[0]
+    return (np.asarray(U) * S * S).astype(np.float32)
+
+
+def target_to_field(t: np.ndarray) -> np.ndarray:
+    S = t.shape[-1]
+    U = np.asarray(t, dtype=np.float64).reshape(S, S) / (S * S)
+    return U - U.mean()
 
 
 def resample_field(f: np.ndarray, xy: np.ndarray) -> np.ndarray:
@@ -259,22 +288,19 @@ def resample_field(f: np.ndarray, xy: np.ndarray) -> np.ndarray:
             + (1 - a) * b * f[j0 + 1, i0] + a * b * f[j0 + 1, i0 + 1])
 
 
+def predict_fields(net: WarmStartNet, problems) -> list:
+    """The network's weight fields (S x S, see field_from_solution) for the problems' targets."""
+    x = np.stack([features(p.rho, net.S) for p in problems])
+    y = net.forward(x)
+    return [target_to_field(yk * net.t_scale) for yk in y]
+
+
 def predict_weights(net: WarmStartNet, problems) -> list:
     """Predicted dual weights for problems on jittered square grids in the unit square (any
-    side); one call of the network for all of them."""
+    side): the network's field sampled at the sites."""
     single = not isinstance(problems, (list, tuple))
     probs = [problems] if single else list(problems)
-    x = np.stack([features(p.rho, net.S) for p in probs])
-    y = net.forward(x)
-    out = []
-    for p, yk in zip(probs, y):
-        field = target_to_weights(yk * net.t_scale, net.S)
-        n_side = int(round(np.sqrt(p.n)))
-        if n_side == net.S:
-            w = field.ravel().copy()
-        else:
-            w = resample_field(field, p.sites)
-        out.append(w - w.mean())
+    out = [weights_from_field(U, p.sites) for U, p in zip(predict_fields(net, probs), probs)]
     return out[0] if single else out
 
 
@@ -292,44 +318,48 @@ def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
     return W.make_problem(sites, rho / rho.sum(), p.tol, p.name + f"_c{n_side}", seed)
 
 
-def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
-                     coarse_tol=1e-3, tol=1e-5, max_iter=20000, w0=None, log=print):
-    """Solve every problem to max-rel `tol` in one batch per level: coarse grids first (to
-    `coarse_tol`), each level's weights interpolated to the next level's sites.  w0: optional
-    initial weights for the finest level (skips the coarse levels).  Returns (weights, reports)."""
+def _solve_level(problems, init, tol, precision, max_iter):
     from .ot import OTBatch
 
+    with OTBatch(list(problems), precision=precision) as b:
+        w, rep = b.solve(None if init is None else np.concatenate(init), history=10, tol=tol,
+                         stop="maxrel", h0_threshold=0.0, max_iter=max_iter,
+                         raise_on_fail=False)
+        cen, _ = b.centroids(w, weighted=True)
+        ws = [x.copy() for x in b.split(w)]
+        cs = [c.copy() for c in b.split(cen.reshape(-1, 2))]
+    return ws, cs, rep
+
+
+def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
+                     coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print):
+    """Solve every problem to max-rel `tol`, one batch per level: coarse grids first (to
+    `coarse_tol`), each level's weights carried to the next level's sites through the weight
+    field (field_from_solution / weights_from_field).  fields: optional per-problem fields that
+    start the first level (a learned start); levels=() with fields starts the final level from
+    them.  Returns (weights, centroids, reports) of the final level."""
     probs = list(problems)
-    fields = None
-    if w0 is None:
-        for li, n_side in enumerate(levels):
-            cps = [coarse_problem(p, n_side, 7919 * (li + 1) + k) for k, p in enumerate(probs)]
-            init = None
-            if fields is not None:
-                init = np.concatenate([resample_field(f, cp.sites) for f, cp in zip(fields, cps)])
-            t0 = time.perf_counter()
-            with OTBatch(cps, precision=precision) as b:
-                w, rep = b.solve(init, history=10, tol=coarse_tol, stop="maxrel",
-                                 h0_threshold=0.0, max_iter=max_iter, raise_on_fail=False)
-                ws = b.split(w)
-            fields = [wk.reshape(n_side, n_side) for wk in ws]
-            log(f"[multiscale] level {n_side}: {len(cps)} problems, "
-                f"iters max {max(r['iterations'] for r in rep)} "
-                f"mean {np.mean([r['iterations'] for r in rep]):.0f}, "
-                f"{time.perf_counter() - t0:.1f}s")
-        init = np.concatenate([resample_field(f, p.sites) for f, p in zip(fields, probs)])
-    else:
-        init = np.concatenate([np.asarray(x, dtype=np.float64) for x in w0])
+    for li, n_side in enumerate(levels):
+        cps = [coarse_problem(p, n_side, 7919 * (li + 1) + k) for k, p in enumerate(probs)]
+        init = None if fields is None else [weights_from_field(U, cp.sites)
+                                            for U, cp in zip(fields, cps)]
+        t0 = time.perf_counter()
+        ws, cs, rep = _solve_level(cps, init, coarse_tol, precision, max_iter)
+        fields = [field_from_solution(cp.sites, wk, ck, n_side)
+                  for cp, wk, ck in zip(cps, ws, cs)]
+        log(f"[multiscale] level {n_side}: {len(cps)} problems, "
+            f"iters max {max(r['iterations'] for r in rep)} "
+            f"mean {np.mean([r['iterations'] for r in rep]):.0f}, "
+            f"{time.perf_counter() - t0:.1f}s")
+    init = None if fields is None else [weights_from_field(U, p.sites)
+                                        for U, p in zip(fields, probs)]
     t0 = time.perf_counter()
-    with OTBatch(probs, precision=precision) as b:
-        w, rep = b.solve(init, history=10, tol=tol, stop="maxrel", h0_threshold=0.0,
-                         max_iter=max_iter, raise_on_fail=False)
-        ws = b.split(w)
+    ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter)
     log(f"[multiscale] final: {len(probs)} problems, "
         f"iters max {max(r['iterations'] for r in rep)} "
         f"mean {np.mean([r['iterations'] for r in rep]):.0f}, converged "
         f"{sum(r['converged'] for r in rep)}, {time.perf_counter() - t0:.1f}s")
-    return [x.copy() for x in ws], rep
+    return ws, cs, rep
 
 
 # ---------------------------------------------------------------------------------------------
@@ -358,12 +388,12 @@ def generate(n: int, seed0: int, batch: int = 64, budget_s: float = 1e9, S: int 
             break
         probs = [W.warm_start_sample(seed0 + k, n_side=S, res=res)
                  for k in range(b0, min(b0 + batch, n))]
-        ws, rep = multiscale_solve(probs, levels=(S // 4, S // 2), log=log)
-        for p, w, r in zip(probs, ws, rep):
+        ws, cs, rep = multiscale_solve(probs, levels=(S // 4, S // 2), log=log)
+        for p, w, c, r in zip(probs, ws, cs, rep):
             if not r["converged"]:
                 continue
             xs.append(features(p.rho, S))
-            ts.append(weights_to_target(w, S))
+            ts.append(field_to_target(field_from_solution(p.sites, w, c, S)))
             seeds.append(p.seed)
         log(f"[generate] {len(xs)} / {n} problems, {time.perf_counter() - t0:.0f}s")
     return np.stack(xs), np.stack(ts), np.array(seeds)
@@ -371,10 +401,13 @@ def generate(n: int, seed0: int, batch: int = 64, budget_s: float = 1e9, S: int 
 
 def train(net: WarmStartNet, x: np.ndarray, t: np.ndarray, epochs: int, batch: int = 16,
           lr: float = 1e-3, lr_min: float = 1e-5, augment: bool = True, budget_s: float = 1e9,
-          seed: int = 0, log=print):
-    """Adam with a cosine learning-rate schedule; the 8 symmetries of the square as augmentation
-    (the training set is stored with all of them).  Returns the per-epoch mean losses."""
+          grad_weight: float = 100.0, seed: int = 0, log=print):
+    """Adam with a cosine learning-rate schedule on the field loss (values plus grad_weight times
+    neighbouring differences: the differences decide the cells); the 8 symmetries of the square
+    as augmentation (the training set is stored with all of them).  Returns the per-epoch mean
+    losses."""
     rng = np.random.default_rng(seed)
+    net.set_loss(grad_weight)
     if augment:
         x = np.concatenate([dihedral(x, k) for k in range(8)])
         t = np.concatenate([dihedral(t, k) for k in range(8)])
@@ -400,8 +433,10 @@ def residual(problems, ws, precision="mixed"):
     from .ot import OTBatch
 
     with OTBatch(list(problems), precision=precision) as b:
-        ev = b.evaluate(np.concatenate(ws))
+        ev = b.evaluate(np.concatenate(ws), allow_unverified=True)
         ms = b.split(ev["mass"])
+    if not ev["verified"]:  # cells too wild to resolve: no better than anything
+        return [float("inf")] * len(problems)
     return [float(np.max(np.abs(m - p.nu) / p.nu)) for m, p in zip(ms, problems)]
 
 
@@ -416,15 +451,17 @@ def choose_start(problems, w_pred, precision="mixed"):
 
 def evaluate_warm_start(net: WarmStartNet, seeds, S: int = S_DEFAULT, res: int = 512,
                         precision="mixed", log=print):
-    """Held-out problems solved to max-rel 1e-5 one at a time from zeros and from the network's
-    weights (with the fallback); iterations and wall-clock of each."""
+    """Held-out problems solved to max-rel 1e-5 one at a time: from zeros, from the network's
+    weights (zeros instead when the prediction's mass residual is the larger), and from a
+    coarse-to-fine schedule; iterations and wall-clock of each (network inference and the
+    coarse levels included in their times)."""
     from .ot import OTBatch
 
     rows = []
+    kw = dict(history=10, tol=1e-5, stop="maxrel", h0_threshold=0.0, max_iter=20000,
+              raise_on_fail=False)
     for sd in seeds:
         p = W.warm_start_sample(int(sd), n_side=S, res=res)
-        kw = dict(history=10, tol=1e-5, stop="maxrel", h0_threshold=0.0, max_iter=20000,
-                  raise_on_fail=False)
         with OTBatch([p], precision=precision) as b:
             t0 = time.perf_counter()
             _, rc = b.solve(None, **kw)
@@ -433,12 +470,17 @@ def evaluate_warm_start(net: WarmStartNet, seeds, S: int = S_DEFAULT, res: int =
             (w0,), (rp,), (rz,) = choose_start([p], [predict_weights(net, p)], precision)
             _, rw = b.solve(w0, **kw)
             t_warm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, rm = multiscale_solve([p], levels=(S // 4, S // 2), log=lambda *_: None)
+        t_ms = time.perf_counter() - t0
         rows.append({"seed": int(sd), "cold_iters": rc[0]["iterations"], "cold_s": t_cold,
                      "warm_iters": rw[0]["iterations"], "warm_s": t_warm,
+                     "multiscale_final_iters": rm[0]["iterations"], "multiscale_s": t_ms,
                      "pred_residual": rp, "zero_residual": rz,
                      "converged": bool(rc[0]["converged"] and rw[0]["converged"])})
-        log(f"[eval] seed {sd}: cold {rows[-1]['cold_iters']} it {t_cold:.2f}s, warm "
-            f"{rows[-1]['warm_iters']} it {t_warm:.2f}s (start residual {rp:.3g} vs {rz:.3g})")
+        log(f"[eval] seed {sd}: cold {rows[-1]['cold_iters']} it {t_cold:.2f}s, learned "
+            f"{rows[-1]['warm_iters']} it {t_warm:.2f}s (start residual {rp:.3g} vs {rz:.3g}), "
+            f"multiscale {rm[0]['iterations']} it {t_ms:.2f}s")
     return rows
 
 
@@ -454,14 +496,25 @@ def main(argv=None):
     ap.add_argument("--train-budget", type=float, default=300.0)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--grad-weight", type=float, default=100.0)
     ap.add_argument("--eval", type=int, default=16, help="held-out problems to evaluate")
     ap.add_argument("--log", default=None, help="JSON summary path")
+    ap.add_argument("--eval-only", default=None, help="evaluate this saved model, no training")
     args = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
+    if args.eval_only:
+        rows = evaluate_warm_start(WarmStartNet.load(args.eval_only),
+                                   range(900000, 900000 + args.eval), log=log)
+        if args.log:
+            with open(args.log, "w") as f:
+                json.dump({"eval": rows}, f, indent=1)
+        return 0
     x, t, seeds = generate(args.samples, seed0=100000, budget_s=args.gen_budget, log=log)
     net = WarmStartNet(S=S_DEFAULT, max_batch=args.batch, seed=0)
-    hist = train(net, x, t, args.epochs, args.batch, args.lr, budget_s=args.train_budget, log=log)
-    net.save(args.out, train_seeds=seeds, loss_history=np.array(hist))
+    hist = train(net, x, t, args.epochs, args.batch, args.lr, budget_s=args.train_budget,
+                 grad_weight=args.grad_weight, log=log)
+    net.save(args.out, train_seeds=seeds, loss_history=np.array(hist),
+             grad_weight=args.grad_weight)
     st = net.train_stats()
     log(f"[train] {st['samples']} samples in {st['seconds']:.1f}s: "
         f"{st['samples'] * st['flops_per_sample'] / st['seconds'] / 1e12:.1f} TFLOP/s")
@@ -471,7 +524,9 @@ def main(argv=None):
                "loss_history": hist, "train_stats": st, "eval": rows,
                "median_iter_ratio": float(np.median([r["cold_iters"] / max(r["warm_iters"], 1)
                                                      for r in rows])),
-               "median_time_ratio": float(np.median([r["cold_s"] / r["warm_s"] for r in rows]))}
+               "median_time_ratio": float(np.median([r["cold_s"] / r["warm_s"] for r in rows])),
+               "median_multiscale_time_ratio": float(np.median([r["cold_s"] / r["multiscale_s"]
+                                                                for r in rows]))}
     log(f"[eval] median cold/warm iterations {summary['median_iter_ratio']:.2f}, "
         f"time {summary['median_time_ratio']:.2f}")
     if args.log:

@@ -34,8 +34,12 @@ int wsn_get_params(wsn_net* net, float* params);
 /* y[B][S][S] = network(x[B][S][S]) */
 int wsn_forward(wsn_net* net, const float* x, int B, float* y);
 
-/* Loss (mean squared error of the prediction minus its per-sample mean against t, summed over
- * nothing: the mean over all B*S*S values) and its gradient with respect to every parameter. */
+/* The loss: with e = y - mean(y) - t per sample (the prediction's mean is a free gauge),
+ * L = (sum e^2 + grad_weight * sum over neighbouring pixel pairs (e_a - e_b)^2) / (B*S*S).
+ * grad_weight defaults to 0 (mean squared error). */
+int wsn_set_loss(wsn_net* net, float grad_weight);
+
+/* The loss and its gradient with respect to every parameter. */
 int wsn_gradients(wsn_net* net, const float* x, const float* t, int B, float* grad, float* loss);
 
 /* Keep `count` training pairs (x, t) on the device for wsn_train. */

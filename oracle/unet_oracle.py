@@ -222,15 +222,29 @@ def backward(P, acts, dout, bf16=True):
     return G
 
 
-def mse_loss(out, target):
-    """Mean squared error between the prediction (gauge removed: its per-sample mean) and a
-    zero-mean target; returns (loss, dL/dout)."""
-    B = out.shape[0]
-    o = out.reshape(B, -1).astype(np.float64)
-    t = target.reshape(B, -1).astype(np.float64)
-    e = o - o.mean(axis=1, keepdims=True) - t
-    n = o.shape[1]
-    loss = float((e * e).sum() / (B * n))
-    de = 2.0 * e / (B * n)
-    dout = de - de.mean(axis=1, keepdims=True)
+def field_loss(out, target, grad_weight=0.0):
+    """Loss of a predicted field against a zero-mean target, the prediction's per-sample mean
+    removed (the gauge): with e = out - mean(out) - target over the S x S image,
+        L = (sum e^2 + grad_weight * sum over neighbouring pixel pairs (e_a - e_b)^2) / (B S^2).
+    Returns (loss, dL/dout)."""
+    B, S = out.shape[0], out.shape[1]
+    o = out.reshape(B, S, S).astype(np.float64)
+    t = target.reshape(B, S, S).astype(np.float64)
+    e = o - o.mean(axis=(1, 2), keepdims=True) - t
+    n = S * S
+    dx = e[:, :, 1:] - e[:, :, :-1]
+    dy = e[:, 1:, :] - e[:, :-1, :]
+    loss = float(((e * e).sum() + grad_weight * ((dx * dx).sum() + (dy * dy).sum())) / (B * n))
+    lap = np.zeros_like(e)
+    lap[:, :, 1:] += dx
+    lap[:, :, :-1] -= dx
+    lap[:, 1:, :] += dy
+    lap[:, :-1, :] -= dy
+    de = (2.0 * e + 2.0 * grad_weight * lap) / (B * n)
+    dout = de - de.mean(axis=(1, 2), keepdims=True)
     return loss, dout.reshape(out.shape).astype(out.dtype)
+
+
+def mse_loss(out, target):
+    """field_loss without the difference term."""
+    return field_loss(out, target, 0.0)

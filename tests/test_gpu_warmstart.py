@@ -45,12 +45,15 @@ def test_gradients_match_oracle(gpu_available):
     P, x, t = _setup(2)
     WS, net = _net()
     net.set_params(WS.dict_to_flat(P))
+    net.set_loss(7.0)
     g, loss = net.gradients(x[..., 0], t[..., 0])
+    lam = 7.0  # the loss's neighbouring-difference weight (set on the network above)
+
     def oracle(acc):
         U.ACC = acc
         try:
             out, acts = U.forward(P, x, bf16=True, keep=True)
-            ref_loss, dout = U.mse_loss(out, t)
+            ref_loss, dout = U.field_loss(out, t, lam)
             return ref_loss, U.backward(P, acts, dout, bf16=True)
         finally:
             U.ACC = np.float64

@@ -109,3 +109,22 @@ def test_bf16_oracle_differs_from_fp32_only_at_bf16_level():
     b = U.forward(P, x, bf16=False)
     rel = np.abs(a - b).max() / np.abs(b).max()
     assert 1e-5 < rel < 5e-2
+
+
+def test_field_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(11)
+    out = rng.standard_normal((2, 8, 8, 1))
+    t = rng.standard_normal((2, 8, 8, 1))
+    t -= t.mean(axis=(1, 2, 3), keepdims=True)
+    loss, d = U.field_loss(out, t, grad_weight=3.0)
+    for _ in range(10):
+        idx = tuple(int(rng.integers(0, s)) for s in out.shape)
+        h = 1e-6
+        o2 = out.copy()
+        o2[idx] += h
+        lp = U.field_loss(o2, t, 3.0)[0]
+        o2[idx] -= 2 * h
+        lm = U.field_loss(o2, t, 3.0)[0]
+        assert abs((lp - lm) / (2 * h) - d[idx]) <= 1e-6 * max(1.0, abs(d[idx]))
+    # a constant shift of the prediction changes nothing (the gauge)
+    assert abs(U.field_loss(out + 5.0, t, 3.0)[0] - loss) < 1e-12

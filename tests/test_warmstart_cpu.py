@@ -43,12 +43,40 @@ def test_features_are_log_of_mean_normalised_box_average():
     np.testing.assert_allclose(np.exp(f[32:]), 0.5, rtol=1e-6)
 
 
-def test_weight_normalisation_round_trip_removes_the_gauge():
+def test_field_round_trip_on_a_regular_grid_removes_the_gauge():
+    """Sites exactly at the nodes: U = -w and w = -U(s), up to the mean."""
     rng = np.random.default_rng(0)
-    w = rng.normal(0, 1e-4, 64 * 64) + 3.0
-    t = WS.weights_to_target(w, 64)
-    assert abs(float(t.mean())) < 1e-5
-    np.testing.assert_allclose(WS.target_to_weights(t, 64).ravel(), w - w.mean(), atol=1e-10)
+    S = 64
+    sites = WS.grid_nodes(S)
+    w = rng.normal(0, 1e-4, S * S) + 3.0
+    U = WS.field_from_solution(sites, w, rng.uniform(0, 1, (S * S, 2)), S)
+    assert abs(float(U.mean())) < 1e-12
+    np.testing.assert_allclose(WS.weights_from_field(U, sites), w - w.mean(), atol=1e-12)
+    np.testing.assert_allclose(WS.target_to_field(WS.field_to_target(U)), U, rtol=1e-6,
+                               atol=1e-6 * np.abs(U).max())
+
+
+def test_field_correction_recovers_a_smooth_field_from_jittered_sites():
+    """For w_i = -U(s_i) with a smooth U and centroids from its gradient (c = s + grad U / 2),
+    the first-order correction gives U at the nodes to second order in the jitter."""
+    S = 64
+    rng = np.random.default_rng(1)
+    sites = WS.grid_nodes(S) + rng.uniform(-0.25, 0.25, (S * S, 2)) / S
+    Uf = lambda x, y: 0.01 * np.sin(3 * x) * np.cos(2 * y)
+    gx = lambda x, y: 0.03 * np.cos(3 * x) * np.cos(2 * y)
+    gy = lambda x, y: -0.02 * np.sin(3 * x) * np.sin(2 * y)
+    w = -Uf(sites[:, 0], sites[:, 1])
+    cen = sites + 0.5 * np.stack([gx(sites[:, 0], sites[:, 1]), gy(sites[:, 0], sites[:, 1])], 1)
+    U = WS.field_from_solution(sites, w, cen, S)
+    nodes = WS.grid_nodes(S)
+    ref = Uf(nodes[:, 0], nodes[:, 1]).reshape(S, S)
+    naive = (-w - (-w).mean()).reshape(S, S)
+    err = np.abs(U - (ref - ref.mean())).max()
+    # first order in the jitter uncorrected (|grad U| |ds| ~ 1e-4), second order corrected:
+    # |U''| |ds|^2 / 2 with |U''| <= 0.13 and |ds|^2 <= 2 (0.25 / 64)^2
+    assert np.abs(naive - (ref - ref.mean())).max() > 5e-5
+    assert err < 0.13 * 2 * (0.25 / S) ** 2
+    assert err < 2e-2 * np.abs(naive - (ref - ref.mean())).max()
 
 
 def test_resample_field_reproduces_linear_fields_and_nodes():
