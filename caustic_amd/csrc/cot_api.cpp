@@ -29,8 +29,8 @@ hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
                                  const int* elab, uint8_t* adjn, int* adj, int nblocks,
                                  hipStream_t st);
-hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             int nblocks, hipStream_t st);
+hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
+                             uint8_t* ring, int rmax, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
                              int rmax, int nblocks, hipStream_t st);
 hipError_t launch_reduce_eval(const BlockDesc* b, const double* mass, const double* fcell,
@@ -68,9 +68,7 @@ hipError_t launch_dev_accept(const ProbDesc* probs, const BlockDesc* blocks, Dev
 hipError_t launch_dev_accept_finish(const ProbDesc* probs, DevState* st, const SolveParams& sp,
                                     const double* part, size_t pstride, uint8_t* active, int np,
                                     hipStream_t stream);
-hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
-                                 uint8_t* redo, uint8_t* ring, int ring_max, int nblk,
-                                 hipStream_t stream);
+
 hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points, const double* ins,
                                  const double* height, const double* clip_area,
                                  const double2* centroid, double z_len, double z_recv,
@@ -161,7 +159,10 @@ struct cot_context {
   double *d_fcell = nullptr, *d_area = nullptr, *d_g = nullptr;
   double2 *d_cen = nullptr, *d_geo = nullptr;
   uint8_t *d_exist = nullptr, *d_nedge = nullptr, *d_ring = nullptr, *d_used = nullptr;
-  uint8_t *d_redo = nullptr, *d_active = nullptr;
+  uint8_t* d_active = nullptr;
+  int* d_redo = nullptr;         // repair flags (see CellArgs)
+  int* d_list = nullptr;         // two work lists of flagged sites, N slots each
+  int* d_lcount = nullptr;       // their lengths
   double2* d_org = nullptr;      // FP32 geometry: clipping frame origins
   int* d_elab = nullptr;
   float2* d_espan = nullptr;
@@ -250,6 +251,10 @@ void sum_partials(cot_context* c, std::vector<EvalSum>& out) {
   }
 }
 
+// Work list k % 2 of flagged sites and its length.
+inline int* list(cot_context* c, int k) { return c->d_list + (size_t)(k & 1) * c->ntot; }
+inline int* list_count(cot_context* c, int k) { return c->d_lcount + (k & 1); }
+
 // Kernel arguments shared by every cell / conformity-check launch of the context.
 void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca = CellArgs{};
@@ -302,10 +307,10 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
 
 // Candidates: the FP64 full pass takes the previous evaluation's cells' two-ring (an edge flip
 // makes a neighbour's neighbour adjacent, and the edge it creates starts too short for the
-// conformity check to see).  The first two repair rounds take the two-ring of the cells just built (larger
-// than the true cells, so their neighbour lists hold more) and the sites the check queued; later
-// rounds take the bucket rings, grown 2r + 1 before each round from the ring that held the
-// cell's last neighbours.  Without a previous evaluation the full pass uses the rings.
+// conformity check to see).  Repairs take the cells just built (larger than the true cells, so
+// their neighbour lists hold more) plus the sites the check queued, then the bucket rings, grown
+// 2r + 1 before each round from the ring that held the cell's last neighbours.  Without a
+// previous evaluation the full pass uses the rings.
 constexpr int kAdjRepairRounds = 2;
 inline int repair_cand(int r) { return r <= kAdjRepairRounds ? kCandAdj2 : kCandBucket; }
 // FP32 geometry's full pass takes the one-ring: an edge a flip creates and the check cannot yet
@@ -344,7 +349,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   std::memcpy(c->h_active, active.data(), c->np);
   HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, st));
   HIPCHK(launch_fill_i32(c->d_fails, 2 * (size_t)c->np, 0, st));
-  HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
+  HIPCHK(launch_fill_i32(c->d_redo, c->ntot, 0, st));
 
   CellArgs ca;
   VerifyArgs va;
@@ -356,6 +361,9 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
   ca.timer_int = c->d_timer + kTimerInteg * kTimerWords;
   ca.coarse = c->d_fails;
+  ca.reset_count = list_count(c, 0);
+  va.list = list(c, 0);
+  va.list_count = list_count(c, 0);
 
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
   HIPCHK(launch_verify(va, c->nblk, st));
@@ -379,14 +387,20 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
     for (int p = 0; p < c->np; ++p) tot += active[p] ? c->h_counts[p] : 0;
     if (tot == 0 || rounds >= max_rounds) break;
     ++rounds;
+    // round r rebuilds the list the previous check filled and its check fills the other one
+    ca.list = list(c, rounds - 1);
+    ca.list_count = list_count(c, rounds - 1);
+    ca.reset_count = list_count(c, rounds);
+    va.list = list(c, rounds);
+    va.list_count = list_count(c, rounds);
     ca.cand = repair_cand(rounds);
     if (ca.cand == kCandBucket)
-      HIPCHK(launch_raise_ring(c->d_blocks, c->d_redo, c->d_ring, c->opt.ring_max, c->nblk, st));
+      HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
+                               st));
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
                                    c->d_adj, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
-    HIPCHK(launch_fill_u8(c->d_redo, c->ntot, 0, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
     HIPCHK(launch_verify(va, c->nblk, st));
     HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu,
@@ -524,7 +538,10 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   ca.timer = c->d_timer + kTimerCell * kTimerWords;
   ca.timer_int = c->d_timer + kTimerInteg * kTimerWords;
   ca.coarse = c->d_rfails;
+  ca.reset_count = list_count(c, 0);
   va.fails = c->d_rfails;
+  va.list = list(c, 0);
+  va.list_count = list_count(c, 0);
   HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
   HIPCHK(launch_verify(va, c->nblk, st));
   ca.redo_mode = 1;
@@ -534,14 +551,19 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   va.timer_close = ca.timer_int;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
+    // round r rebuilds the list the previous check filled and its check fills the other one
+    ca.list = list(c, r - 1);
+    ca.list_count = list_count(c, r - 1);
+    ca.reset_count = list_count(c, r);
+    va.list = list(c, r);
+    va.list_count = list_count(c, r);
     ca.cand = repair_cand(r);
     if (ca.cand == kCandBucket)
-      HIPCHK(launch_dev_raise_ring(c->d_blocks, c->d_state, gate, c->d_redo, c->d_ring,
-                                   c->opt.ring_max, c->nblk, st));
+      HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
+                               st));
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                    c->nblk, st));
-    ca.gate = gate;
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
@@ -835,6 +857,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_geo, N)) || (rc = dalloc(c, &c->d_exist, N)) ||
       (rc = dalloc(c, &c->d_nedge, N)) || (rc = dalloc(c, &c->d_ring, N)) ||
       (rc = dalloc(c, &c->d_used, N)) || (rc = dalloc(c, &c->d_redo, N)) ||
+      (rc = dalloc(c, &c->d_list, 2 * N)) || (rc = dalloc(c, &c->d_lcount, 2)) ||
       (rc = dalloc(c, &c->d_active, c->np)) || (rc = dalloc(c, &c->d_elab, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_espan, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_overflow, c->np)) || (rc = dalloc(c, &c->d_fails, 2 * c->np)) ||
@@ -890,6 +913,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
       launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
+      launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
+      launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
       hipMemset(c->d_fails, 0, sizeof(int) * 2 * c->np) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)

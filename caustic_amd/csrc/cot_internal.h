@@ -10,6 +10,8 @@ constexpr int kMaxVerts = 24;
 // Threads per block of the cell kernel; one block covers kCellBlock consecutive sites of one
 // problem (sites are stored in bucket order, so a block is a compact patch of the plane).
 constexpr int kCellBlock = 128;
+// Blocks of the repair passes, which walk the work list of flagged sites (one per CU).
+constexpr int kRepairBlocks = 256;
 // Integration pass: lanes per site and threads per block.
 constexpr int kIntLanes = 8;
 constexpr int kIntBlock = 256;
@@ -82,8 +84,14 @@ struct CellArgs {
   const double4* pre;       // row prefixes (x, y, z: the three sums; w: the pixel's rho)
   const double4* tot;       // chunk totals (x, y, z)
   uint8_t* ring;            // per-site candidate ring radius (read, raised on repair)
-  uint8_t* redo;            // repair flags; every processed site's flag is cleared
-  int redo_mode;            // 0: all sites of active problems; 1: only sites with redo[i] != 0
+  int* redo;                // repair flags: 0 none, 1 listed for a repair, 2 an FP32 cell that
+                            // the next check lists; a full pass clears its sites' flags, a
+                            // repair pass those of the sites it rebuilt
+  int redo_mode;            // 0: all sites of active problems; 1: the sites of the work list
+  const int* list;          // repair pass: work list of site slots (block * kCellBlock + lane)
+  const int* list_count;    //   and its length
+  int* reset_count;         // integration pass: the length of the list the next check fills,
+                            // cleared here (nullptr: none)
   const uint8_t* active;    // per-problem: evaluate only problems with active[p] != 0
   const struct DevState* st;  // device-driven solve: step, direction sign, activity per problem
   const double* g;          // gradient (steepest-descent trials)
@@ -155,7 +163,9 @@ struct VerifyArgs {
   const float2* espan;
   const uint8_t* active;    // per-problem mask
   const int* gate;          // skip problems whose counter is zero (nullptr: none)
-  uint8_t* redo;            // set for both cells of a mismatched edge
+  int* redo;                // set for both cells of a mismatched edge (see CellArgs)
+  int* list;                // work list the check appends newly flagged sites to (nullptr: none)
+  int* list_count;
   int* fails;               // per-problem mismatch count
   double tol_rel;           // tolerance relative to the site spacing of the pair
   int* xcnt;                // candidate queues of the repair (see kMaxExtra)

@@ -62,12 +62,20 @@ __device__ double block_max(double v, double* red) {
   return s;
 }
 
-// Atomic increment usable from the per-site code on the device and in its host build.
+// Atomic increment and exchange usable from the per-site code on the device and in its host
+// build.
 __host__ __device__ __forceinline__ int fetch_add(int* p, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return atomicAdd(p, v);
 #else
   return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+}
+__host__ __device__ __forceinline__ int exchange(int* p, int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicExch(p, v);
+#else
+  return __atomic_exchange_n(p, v, __ATOMIC_RELAXED);
 #endif
 }
 
@@ -399,16 +407,18 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
 // cell kernel
 // ---------------------------------------------------------------------------------------------
 
-// One site of the cell kernel (lane `ln` of block `blk`); returns -1 when the lane has no site
-// to build, 1 when the cell exceeded kMaxVerts, 0 otherwise.  Shared by the kernel and the host
-// build used to test this code on the CPU.
+// One site of the cell kernel (lane `site_ln` of block `blk`); returns -1 when there is no site
+// to build, 1 when the cell exceeded kMaxVerts, 0 otherwise.  Shared by the kernels and the
+// host build used to test this code on the CPU.
 template <typename T>
-__host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>& P) {
+__host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyLDS<T>& P,
+                                  int ln) {
+  // site_ln: the site's lane in its cell block; ln: the LDS column of the polygon
   const BlockDesc bd = a.blocks[blk];
-  if (ln >= bd.count) return -1;
+  if (site_ln >= bd.count) return -1;
   if (!a.active[bd.prob]) return -1;
   if (a.gate && a.gate[bd.prob] == 0) return -1;
-  const int i = bd.first + ln;
+  const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
   // the full pass clears the flag (an FP32 cell may set it again below); in a repair pass the
   // integration pass that follows clears it
@@ -679,7 +689,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int ln, PolyLDS<T>
   }
   a.kbox[i] = kb;
   if (kF32 && coarse && a.coarse) {
-    a.redo[i] = 1;
+    a.redo[i] = 2;  // the check that follows lists it
     fetch_add(&a.coarse[bd.prob], 1);
   }
   a.area[i] = ar;
@@ -815,7 +825,7 @@ template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   __shared__ PolyLDS<T> P;
   const unsigned long long t0 = timer_begin();
-  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
+  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P, threadIdx.x);
   if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
   timer_mark(a.timer, rc >= 0, t0);
 }
@@ -826,9 +836,17 @@ template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   __shared__ PolyLDS<T> P;
   const unsigned long long t0 = timer_begin();
-  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P);
-  if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
-  timer_mark(a.timer, rc >= 0, t0);
+  const int n = *a.list_count;
+  bool worked = false;
+  // entries dealt round-robin over the blocks, so that a short list still spreads over the CUs
+  for (int e = threadIdx.x * gridDim.x + blockIdx.x; e < n; e += gridDim.x * kCellBlock) {
+    const int slot = a.list[e];
+    const int blk = slot / kCellBlock;
+    const int rc = cell_site<T>(a, blk, slot % kCellBlock, P, threadIdx.x);
+    if (rc == 1) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
+    worked = worked || rc >= 0;
+  }
+  timer_mark(a.timer, worked, t0);
 }
 
 // Integration pass.  Grid (cell blocks, kCellBlock * kIntLanes / kIntBlock), block kIntBlock:
@@ -855,22 +873,59 @@ __device__ __forceinline__ void integrate_block(const CellArgs& a) {
   if (i >= 0 && t % kIntLanes == 0) finish_site(a, a.probs[prob], prob, i, M);
 }
 
+// Clears the length of the work list the next check fills (nothing reads or writes that list
+// while this kernel runs).
+__device__ __forceinline__ void reset_list(const CellArgs& a) {
+  if (a.reset_count && threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) *a.reset_count = 0;
+}
+
 // a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
 template <typename T>
 __global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a) {
   timer_close(a.timer);
+  reset_list(a);
   const unsigned long long t0 = timer_begin();
   const bool worked = integ_site(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
   integrate_block<T>(a);
   timer_mark(a.timer_int, worked, t0);
 }
 
+// Repair integration over the work list: kIntLanes lanes per listed site.  Grid kRepairBlocks.
 template <typename T>
 __global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a) {
   timer_close(a.timer);
+  reset_list(a);
   const unsigned long long t0 = timer_begin();
-  const bool worked = integ_site(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
-  integrate_block<T>(a);
+  const int n = *a.list_count;
+  constexpr int kSites = kIntBlock / kIntLanes;
+  const int t = threadIdx.x;
+  bool worked = false;
+  // entries dealt round-robin over the blocks (as in cell_repair_kernel)
+  for (int base = 0; base < n; base += gridDim.x * kSites) {
+    const int e = base + (t / kIntLanes) * gridDim.x + blockIdx.x;
+    int i = -1, prob = 0;
+    if (e < n) {
+      const int slot = a.list[e];
+      const BlockDesc bd = a.blocks[slot / kCellBlock];
+      i = bd.first + slot % kCellBlock;
+      prob = bd.prob;
+    }
+    Moments M;
+    if (i >= 0) integrate_lane<T>(a, a.probs[prob], i, t % kIntLanes, M);
+#pragma unroll
+    for (int off = kIntLanes / 2; off > 0; off >>= 1) {
+      Moments o;
+      o.m0.s = __shfl_xor(M.m0.s, off, 64);
+      o.m0.c = __shfl_xor(M.m0.c, off, 64);
+      o.m1u = __shfl_xor(M.m1u, off, 64);
+      o.m1v = __shfl_xor(M.m1v, off, 64);
+      o.m2u = __shfl_xor(M.m2u, off, 64);
+      o.m2v = __shfl_xor(M.m2v, off, 64);
+      combine(M, o);
+    }
+    if (i >= 0 && t % kIntLanes == 0) finish_site(a, a.probs[prob], prob, i, M);
+    worked = worked || i >= 0;
+  }
   timer_mark(a.timer_int, worked, t0);
 }
 
@@ -889,6 +944,15 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
   const int ne = a.nedge[i];
   const double2 si = a.xy[i];
   bool bad = false;
+  // flag site k for a repair; the first to flag it appends it to the work list
+  const ProbDesc& pd = a.probs[bd.prob];
+  auto mark = [&](int k) {
+    if (exchange(&a.redo[k], 1) == 1 || !a.list) return;
+    const int local = k - pd.site_off;
+    const int slot = (pd.blk_off + local / kCellBlock) * kCellBlock + local % kCellBlock;
+    a.list[fetch_add(a.list_count, 1)] = slot;
+  };
+  if (a.redo[i] == 2) mark(i);  // an FP32 cell handed to the FP64 repair
   for (int k = 0; k < ne; ++k) {
     const int j = a.elab[(size_t)i * kMaxVerts + k];
     if (j < 0) continue;
@@ -909,7 +973,7 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
     }
     if (found ? !match : (len > tol)) {
       bad = true;
-      a.redo[j] = 1;
+      mark(j);
       if (!found && a.xcnt) {
         // j's cell lacks the edge: i is a candidate j missed
         const int q = fetch_add(&a.xcnt[j], 1);
@@ -917,7 +981,7 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
       }
     }
   }
-  if (bad) a.redo[i] = 1;
+  if (bad) mark(i);
   return bad ? 1 : 0;
 }
 
@@ -945,14 +1009,14 @@ __global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
 }
 
 // Grow the ring of every cell marked for a rebuild (r -> 2r + 1).
-__global__ void raise_ring_kernel(const BlockDesc* blocks, const uint8_t* redo, uint8_t* ring,
-                                  int ring_max) {
-  const BlockDesc bd = blocks[blockIdx.x];
-  const int ln = threadIdx.x;
-  if (ln >= bd.count) return;
-  const int i = bd.first + ln;
-  if (!redo[i]) return;
-  ring[i] = (uint8_t)min(2 * (int)ring[i] + 1, ring_max);
+__global__ void raise_ring_kernel(const BlockDesc* blocks, const int* list, const int* list_count,
+                                  uint8_t* ring, int ring_max) {
+  const int n = *list_count;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int slot = list[e];
+    const int i = blocks[slot / kCellBlock].first + slot % kCellBlock;
+    ring[i] = (uint8_t)min(2 * (int)ring[i] + 1, ring_max);
+  }
 }
 
 // Next evaluation's ring: the outermost ring that shaped the cell plus one, at least ring_min.
@@ -1108,9 +1172,11 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st
   const dim3 ig(nblocks, kCellBlock * kIntLanes / kIntBlock);
   if (a.redo_mode) {
     // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
-    // of the cells the FP32 pass could not resolve)
-    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-    hipLaunchKernelGGL((integrate_repair_kernel<double>), ig, dim3(kIntBlock), 0, st, b);
+    // of the cells the FP32 pass could not resolve), on a small grid over the work list
+    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(kRepairBlocks), dim3(kCellBlock), 0, st,
+                       a);
+    hipLaunchKernelGGL((integrate_repair_kernel<double>), dim3(kRepairBlocks), dim3(kIntBlock), 0,
+                       st, b);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
     hipLaunchKernelGGL((integrate_kernel<float>), ig, dim3(kIntBlock), 0, st, b);
@@ -1134,10 +1200,11 @@ hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint
   return hipGetLastError();
 }
 
-hipError_t launch_raise_ring(const BlockDesc* b, const uint8_t* redo, uint8_t* ring, int rmax,
-                             int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(raise_ring_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, redo, ring,
-                     rmax);
+// Grow the ring (r -> 2r + 1) of every site on the work list.
+hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
+                             uint8_t* ring, int rmax, hipStream_t st) {
+  hipLaunchKernelGGL(raise_ring_kernel, dim3(kRepairBlocks), dim3(kCellBlock), 0, st, b, list,
+                     list_count, ring, rmax);
   return hipGetLastError();
 }
 

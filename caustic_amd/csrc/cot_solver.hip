@@ -423,19 +423,7 @@ __global__ __launch_bounds__(kFinishBlock) void dev_accept_finish_kernel(
   active[blockIdx.x] = 0;
 }
 
-// Repair-round helpers driven by the per-problem failure counter of the previous check.
-__global__ void dev_raise_ring_kernel(const BlockDesc* blocks, const DevState* st,
-                                      const int* gate, uint8_t* redo, uint8_t* ring,
-                                      int ring_max) {
-  const BlockDesc bd = blocks[blockIdx.x];
-  if (st[bd.prob].phase != kPhaseTrial || gate[bd.prob] == 0) return;
-  const int ln = threadIdx.x;
-  if (ln >= bd.count) return;
-  const int i = bd.first + ln;
-  if (!redo[i]) return;
-  ring[i] = (uint8_t)min(2 * (int)ring[i] + 1, ring_max);
-}
-
+// Next ring of the sites of problems whose final check was clean (see adapt_ring_kernel).
 __global__ void dev_adapt_ring_kernel(const BlockDesc* blocks, const DevState* st,
                                       const int* final_fails, const uint8_t* used, uint8_t* ring,
                                       int ring_min, int ring_max) {
@@ -506,13 +494,6 @@ hipError_t launch_dev_accept_finish(const ProbDesc* probs, DevState* st, const S
   return hipGetLastError();
 }
 
-hipError_t launch_dev_raise_ring(const BlockDesc* blocks, const DevState* st, const int* gate,
-                                 uint8_t* redo, uint8_t* ring, int ring_max, int nblk,
-                                 hipStream_t stream) {
-  hipLaunchKernelGGL(dev_raise_ring_kernel, dim3(nblk), dim3(kCellBlock), 0, stream, blocks, st,
-                     gate, redo, ring, ring_max);
-  return hipGetLastError();
-}
 
 hipError_t launch_dev_adapt_ring(const BlockDesc* blocks, const DevState* st,
                                  const int* final_fails, const uint8_t* used, uint8_t* ring,

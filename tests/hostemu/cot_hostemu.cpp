@@ -26,7 +26,8 @@ struct Session {
   std::vector<double> nu, w, mass, fcell, area;
   std::vector<double4> pre, tot;
   std::vector<double2> cen, geo;
-  std::vector<uint8_t> exist, ne, ring, used, redo, adjn;
+  std::vector<uint8_t> exist, ne, ring, used, adjn;
+  std::vector<int> redo;
   std::vector<int> elab, adj, xcnt, xcand;
   std::vector<double2> vtx;
   std::vector<int2> kbox;
@@ -124,7 +125,7 @@ void emu_state(void* h, unsigned char* redo, int* nedge, int* elab, float* espan
   Session& S = *static_cast<Session*>(h);
   for (int r = 0; r < S.n; ++r) {
     const int o = S.order[r];
-    if (redo) redo[o] = S.redo[r];
+    if (redo) redo[o] = (unsigned char)(S.redo[r] != 0);
     if (nedge) nedge[o] = S.ne[r];
     if (ring) ring[o] = S.ring[r];
     for (int k = 0; k < kMaxVerts; ++k) {
@@ -206,8 +207,8 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
 #pragma omp for schedule(dynamic, 1) reduction(+ : overflow, built)
       for (int b = 0; b < nb; ++b)
         for (int ln = 0; ln < kCellBlock; ++ln) {
-          const int rc = f32 ? cell_site<float>(a, b, ln, *lf.P)
-                             : cell_site<double>(a, b, ln, *ld.P);
+          const int rc = f32 ? cell_site<float>(a, b, ln, *lf.P, ln)
+                             : cell_site<double>(a, b, ln, *ld.P, ln);
           overflow += rc == 1 ? 1 : 0;
           built += rc >= 0 ? 1 : 0;
         }
