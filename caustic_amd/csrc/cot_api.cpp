@@ -228,6 +228,26 @@ int dalloc(cot_context* c, T** p, size_t count) {
   return COT_OK;
 }
 
+// Device buffers that live for one call: freed, after the stream drains, when the owner ends.
+struct CallBuffers {
+  hipStream_t st;
+  std::vector<void*> held;
+  explicit CallBuffers(hipStream_t s) : st(s) {}
+  template <typename T>
+  int get(T** out, size_t count) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+      return fail(COT_ENOMEM, "hipMalloc failed");
+    held.push_back(q);
+    *out = reinterpret_cast<T*>(q);
+    return COT_OK;
+  }
+  ~CallBuffers() {
+    if (!held.empty()) hipStreamSynchronize(st);
+    for (void* q : held) hipFree(q);
+  }
+};
+
 // Sum the per-block partials of the last reduce into per-problem summaries.
 void sum_partials(cot_context* c, std::vector<EvalSum>& out) {
   out.assign(c->np, EvalSum());
@@ -1030,15 +1050,16 @@ int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
   double2* d_pts = nullptr;
   double *d_ins = nullptr, *d_h = nullptr, *d_area = nullptr, *d_wt = nullptr, *d_G = nullptr;
   double *d_R = nullptr, *d_alpha = nullptr, *d_pp = nullptr;
-  CHK(dalloc(c, &d_pts, N));
-  CHK(dalloc(c, &d_ins, 3 * (size_t)N));
-  CHK(dalloc(c, &d_h, N));
-  CHK(dalloc(c, &d_area, N));
-  CHK(dalloc(c, &d_wt, N));
-  CHK(dalloc(c, &d_G, (size_t)mA * N));
-  CHK(dalloc(c, &d_R, (size_t)mA * N));
-  CHK(dalloc(c, &d_alpha, mA));
-  CHK(dalloc(c, &d_pp, (size_t)std::max(mA, 1) * nb));
+  CallBuffers buf(st);
+  CHK(buf.get(&d_pts, N));
+  CHK(buf.get(&d_ins, 3 * (size_t)N));
+  CHK(buf.get(&d_h, N));
+  CHK(buf.get(&d_area, N));
+  CHK(buf.get(&d_wt, N));
+  CHK(buf.get(&d_G, (size_t)mA * N));
+  CHK(buf.get(&d_R, (size_t)mA * N));
+  CHK(buf.get(&d_alpha, mA));
+  CHK(buf.get(&d_pp, (size_t)std::max(mA, 1) * nb));
   {
     std::vector<double2> pts(N);
     std::vector<double> ins(3 * (size_t)N), h(N), ar(N);

@@ -167,6 +167,51 @@ class OTBatch:
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
 
+    def solve_reflection(self, w0, lens, history: int = 7, max_iter: int = 50001,
+                         tol: float = 1e-2, initial_step: float = 1.0,
+                         h0_threshold: float = 1e-5, aa_history_size: int = 7,
+                         aa_fixed_para: float = -2.0, if_aa_op_initial: bool = True,
+                         device_loop=True, verbose: int = 0, raise_on_fail: bool = True):
+        """The t >= 1 solve (aarl_direct_ot, ot_op.cuh:591-1323) of a one-problem batch: source
+        masses from the lens's Fresnel transmittance at the current cells' geometric centroids,
+        L-BFGS with those masses frozen (if_aa_op_initial), then Anderson acceleration of
+        w + aa_fixed_para * (mass - nu) with the masses recomputed at every trial, until
+        |mass - nu|_2 < tol.  lens: dict with points (N, 2), ins (N, 3), height (N,),
+        clip_area (N,), z_of_len, z_of_receiver, refraction_index, area_scalar.
+        Returns (w, report)."""
+        if self.n_problems != 1:
+            raise ValueError("the reflection solve takes one-problem batches")
+        w = self._w(w0).copy()
+        n = self.n_total
+        pts = np.ascontiguousarray(lens["points"], dtype=np.float64).reshape(n, 2)
+        ins = np.ascontiguousarray(lens["ins"], dtype=np.float64).reshape(n, 3)
+        hgt = np.ascontiguousarray(lens["height"], dtype=np.float64).reshape(n)
+        clip = np.ascontiguousarray(lens["clip_area"], dtype=np.float64).reshape(n)
+        r = L.cot_reflection()
+        r.points, r.inside_dir, r.height, r.clip_area = _ptr(pts), _ptr(ins), _ptr(hgt), _ptr(clip)
+        r.z_of_len = float(lens["z_of_len"])
+        r.z_of_receiver = float(lens["z_of_receiver"])
+        r.refraction_index = float(lens["refraction_index"])
+        r.area_scalar = float(lens["area_scalar"])
+        r.aa_history_size = int(aa_history_size)
+        r.aa_fixed_para = float(aa_fixed_para)
+        r.if_aa_op_initial = int(bool(if_aa_op_initial))
+        o = L.cot_solve_opts()
+        self._lib.cot_default_solve_opts(ctypes.byref(o))
+        o.history = history
+        o.max_iter = max_iter
+        o.tol = tol
+        o.initial_step = initial_step
+        o.h0_threshold = h0_threshold
+        o.verbose = verbose
+        o.device_loop = int(device_loop)
+        rep = L.cot_report()
+        rc = self._lib.cot_solve_reflection(self._ctx, _ptr(w), ctypes.byref(o), ctypes.byref(r),
+                                            ctypes.byref(rep))
+        if raise_on_fail:
+            L.check(rc, allow=(L.COT_ENOTCONV,))
+        return w, {k: getattr(rep, k) for k, _ in L.cot_report._fields_}
+
 
 def direct_ot(proj_xy, weights, picture, picture_size, total_iter, convergence_tolerance,
               initial_step, history_size, proj_domain, carry_energy, precision="fp64",

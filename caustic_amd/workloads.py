@@ -147,3 +147,30 @@ def box_downsample(rho: np.ndarray, out: int) -> np.ndarray:
     """Box-filter a square image to out x out (sum-preserving)."""
     f = rho.shape[0] // out
     return rho.reshape(out, f, out, f).sum(axis=(1, 3))
+
+
+def lens_problem(seed: int = 4000, n_side: int = 16, res: int = 64, tilt: float = 0.3,
+                 z_of_receiver: float = 1.0, refraction_index: float = 1.5):
+    """A t >= 1 (refraction-aware) problem: the plain_fp64-style target on a res^2 image and
+    n_side^2 jittered sites, plus the lens data the Fresnel-weighted masses need.
+
+    Lens plane z = 0, receiver z = z_of_receiver; source points = the sites (parallel light);
+    inside directions (u, v, 1) normalised with u, v ~ U[-tilt, tilt]; back-surface heights
+    0.1 + U[0, 0.02]; clip areas (1 + U[-0.1, 0.1]) / N.  Returns (Problem, lens dict)."""
+    rng = rng_for(seed)
+    sites = jittered_grid(n_side, rng)
+    rho = gaussian_mixture(res, rng, 3, (0.05, 0.15), (0.5, 1.0), (0.2, 0.8), 0.05)
+    n = sites.shape[0]
+    ins = np.concatenate([rng.uniform(-tilt, tilt, size=(n, 2)), np.ones((n, 1))], axis=1)
+    ins /= np.linalg.norm(ins, axis=1, keepdims=True)
+    lens = {
+        "points": sites.copy(),
+        "ins": ins,
+        "height": 0.1 + rng.uniform(0.0, 0.02, size=n),
+        "clip_area": (1.0 + rng.uniform(-0.1, 0.1, size=n)) / n,
+        "z_of_len": 0.0,
+        "z_of_receiver": float(z_of_receiver),
+        "refraction_index": float(refraction_index),
+        "area_scalar": 1.0,
+    }
+    return make_problem(sites, rho, 1e-6, f"lens_n{n}_{res}px", seed), lens

@@ -14,6 +14,8 @@ Pieces and what they follow:
   * ``evaluate`` -- compute_inte / compute_f / compute_center (basic_function.cuh:743-1191).
   * ``lbfgs`` -- the L-BFGS driver of direct_ot (ot_op.cuh:289-537) with the reference's rules;
     ``stop="maxrel"`` swaps only the stopping test for max_i |mass_i - nu_i| / nu_i < tol.
+  * ``anderson`` / ``reflection_solve`` -- the t >= 1 path of aarl_direct_ot (ot_op.cuh:591-1323):
+    Fresnel-weighted source masses, L-BFGS with frozen masses, then Anderson acceleration.
   * ``read_pic_txt`` / ``format_weight`` -- text formats (basic_function.cuh:145-204,
     main.cu:295-299).
 """
@@ -349,6 +351,149 @@ def solve(problem, w0=None, m=10, stop="maxrel", tol=None, total_iter=5000, meth
     w0 = np.zeros(problem.n) if w0 is None else w0
     return lbfgs(ev, w0, nu, m=m, total_iter=total_iter,
                  tol=problem.tol if tol is None else tol, stop=stop, h0_threshold=h0_threshold)
+
+
+# --------------------------------------------------------------------------------------------
+# Anderson acceleration and the t >= 1 driver (ot_op.cuh:591-1323)
+# --------------------------------------------------------------------------------------------
+
+def anderson_coefficients(A):
+    """alpha = M^-1 1 / (1 . M^-1 1), M = A + 1e-10 I, A the Gram matrix of the stored
+    residuals (ot_op.cuh:1147-1163; the reference factorises M by LDL^T)."""
+    A = np.asarray(A, dtype=np.float64)
+    H = A.shape[0]
+    a = np.linalg.solve(A + 1e-10 * np.eye(H), np.ones(H))
+    return a / a.sum()
+
+
+class AndersonResult:
+    def __init__(self):
+        self.w = None
+        self.g = None
+        self.iterations = 0
+        self.evaluations = 0
+        self.converged = False
+        self.restarts = 0
+        self.history = []  # per-iteration (|g|_2 after the step, step)
+
+
+def anderson(residual_fn, w0, g0, zeros0, m, beta, total_iter, tol, initial_step=1.0,
+             max_vertices=200) -> AndersonResult:
+    """Anderson acceleration of the fixed point G(w) = w + beta g(w) (ot_op.cuh:1112-1323).
+
+    residual_fn(w) -> (g, zeros, max_len): the residual mass - nu at w (nu recomputed from
+    w's own cells), the number of cells of zero mass and the largest cell's vertex count.
+    (g0, zeros0) are the residual and zero count at w0.  Each iteration:
+
+    * stores g and G(w) = w + beta g in slot aa % m and uses the H = min(aa + 1, m) slots
+      0..H-1 (:1131-1145);
+    * alpha from the H x H Gram matrix of the stored g (anderson_coefficients);
+    * d = sum_h alpha_h G_h - w; trials w + t d from t = 1, halved while a cell exceeds
+      max_vertices; the first admissible trial is accepted, residual and all (:1176-1243);
+    * t <= 1e-4 initial_step keeps w and g and restarts the history (aa = 0) (:1244-1253);
+    * stop when |g|_2 < tol after the step and -- on every 10th iteration only, counted at the
+      start of the iteration -- no cell has zero mass (:1257-1268, 1296).
+    """
+    w = _f64(w0).copy()
+    g = _f64(g0).copy()
+    zeros = int(zeros0)
+    Gs = [None] * m
+    Fs = [None] * m
+    aa = 0
+    res = AndersonResult()
+    for it in range(total_iter):
+        res.iterations = it + 1
+        H = min(aa + 1, m)
+        slot = aa % m
+        Gs[slot] = g.copy()
+        Fs[slot] = w + beta * g
+        A = np.array([[float(np.dot(Gs[i], Gs[j])) for j in range(H)] for i in range(H)])
+        alpha = anderson_coefficients(A)
+        d = sum(alpha[h] * Fs[h] for h in range(H)) - w
+        zeros_start = zeros
+        step = 1.0
+        while True:
+            w_try = w + step * d
+            g_try, z_try, max_len = residual_fn(w_try)
+            res.evaluations += 1
+            if max_len > max_vertices:
+                step *= 0.5
+            else:
+                w, g, zeros = w_try, _f64(g_try), int(z_try)
+                aa += 1
+                break
+            if step <= 1e-4 * initial_step:
+                aa = 0
+                res.restarts += 1
+                break
+        gn = float(np.linalg.norm(g))
+        res.history.append((gn, step))
+        degenerate = zeros_start if it % 10 == 0 else 0
+        if gn < tol and degenerate == 0:
+            res.converged = True
+            break
+    res.w = w
+    res.g = g
+    return res
+
+
+class ReflectionResult:
+    def __init__(self):
+        self.w = None
+        self.nu = None
+        self.stage_a = None
+        self.stage_b = None
+
+
+def fresnel_masses(ev, lens):
+    """nu_i = clip_area_i T1_i T2_i / sum * area_scalar, the transmittances taken at the cells'
+    geometric centroids ev.geo (ot_op.cuh:779-795, parallel light: wi = (0, 0, 1))."""
+    n = lens["points"].shape[0]
+    wi = np.tile([0.0, 0.0, 1.0], (n, 1))
+    rate = fresnel_rate(lens["points"], wi, lens["ins"], lens["height"], ev.geo,
+                        lens["z_of_len"], lens["z_of_receiver"], lens["refraction_index"])
+    we = _f64(lens["clip_area"]) * rate
+    return we / we.sum() * lens["area_scalar"]
+
+
+def reflection_solve(xy, w0, rho, proj, lens, m=7, total_iter=50001, tol=1e-2,
+                     initial_step=1.0, h0_threshold=1e-5, aa_history_size=7, aa_fixed_para=-2.0,
+                     if_aa_op_initial=True, max_vertices=200) -> ReflectionResult:
+    """The t >= 1 solve of aarl_direct_ot (ot_op.cuh:591-1323) for a parallel light.
+
+    lens: dict of points (n, 2) lens-plane source points, ins (n, 3) unit directions inside
+    the lens, height (n,), clip_area (n,), z_of_len, z_of_receiver, refraction_index,
+    area_scalar.  Stage A (if_aa_op_initial): L-BFGS with the Fresnel masses of w0's cells
+    frozen, the reference's stopping rule (:787-1073).  Then the masses are recomputed at the
+    result (:1076-1103) and stage B runs ``anderson`` with masses recomputed at every trial."""
+    xy = _f64(xy)
+    rho = _f64(rho)
+    proj = _f64(proj)
+    w = _f64(w0).copy()
+    out = ReflectionResult()
+    ev = evaluate(xy, w, np.zeros(xy.shape[0]), rho, proj)
+    nu = fresnel_masses(ev, lens)
+    if if_aa_op_initial:
+        def ev_a(wv):
+            e = evaluate(xy, wv, nu, rho, proj)
+            return e.f, e.mass, e.max_len
+
+        out.stage_a = lbfgs(ev_a, w, nu, m=m, total_iter=total_iter, tol=tol,
+                            initial_step=initial_step, stop="reference",
+                            h0_threshold=h0_threshold, max_vertices=max_vertices)
+        w = out.stage_a.w
+
+    def residual(wv):
+        e = evaluate(xy, wv, np.zeros(xy.shape[0]), rho, proj)
+        nu_v = fresnel_masses(e, lens)
+        return e.mass - nu_v, int(np.sum(e.mass == 0.0)), e.max_len
+
+    g, zeros, _ = residual(w)
+    out.stage_b = anderson(residual, w, g, zeros, aa_history_size, aa_fixed_para, total_iter, tol,
+                           initial_step, max_vertices)
+    out.w = out.stage_b.w
+    out.nu = fresnel_masses(evaluate(xy, out.w, np.zeros(xy.shape[0]), rho, proj), lens)
+    return out
 
 
 # --------------------------------------------------------------------------------------------

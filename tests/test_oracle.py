@@ -200,3 +200,98 @@ def test_reference_centroid_piece_rule_is_below_1e8():
     b = O.evaluate(c1.sites, np.zeros(c1.n), c1.nu, c1.rho, c1.proj_domain,
                    centroid_min_piece=0.0)
     assert np.abs(a.centroid - b.centroid).max() < 1e-8
+
+
+def test_fresnel_brewster_angle():
+    """Air -> glass at Brewster's angle tan(i) = n: Rp = 0, so T1 = 1 - Rs/2 with
+    Rs = ((cos i - n cos t) / (cos i + n cos t))^2; the exit ray leaves along the inside
+    direction (normal incidence on the back surface): T2 = 1 - ((n - 1)/(n + 1))^2."""
+    n_idx = 1.5
+    i = np.arctan(n_idx)
+    t = np.arcsin(np.sin(i) / n_idx)
+    wi = np.array([[np.sin(i), 0.0, np.cos(i)]])
+    d = np.array([[np.sin(t), 0.0, np.cos(t)]])
+    z_recv = 10.0
+    tgt = np.array([[z_recv * np.tan(t), 0.0]])  # on the inside ray from the origin
+    out = O.fresnel_rate(np.zeros((1, 2)), wi, d, np.zeros(1), tgt, 0.0, z_recv, n_idx)
+    rs = ((np.cos(i) - n_idx * np.cos(t)) / (np.cos(i) + n_idx * np.cos(t))) ** 2
+    t2 = 1.0 - ((n_idx - 1.0) / (n_idx + 1.0)) ** 2
+    assert out[0] == pytest.approx((1.0 - 0.5 * rs) * t2, rel=1e-13)
+    # the same rays without the Brewster cancellation: Rp > 0 lowers T1
+    wi2 = np.array([[np.sin(0.5 * i), 0.0, np.cos(0.5 * i)]])
+    out2 = O.fresnel_rate(np.zeros((1, 2)), wi2, d, np.zeros(1), tgt, 0.0, z_recv, n_idx)
+    assert out2[0] < t2
+
+
+def test_anderson_coefficients_closed_forms():
+    """One residual: alpha = 1.  Orthogonal residuals of norms a, b: the minimum-norm affine
+    combination has alpha proportional to (1/a^2, 1/b^2)."""
+    np.testing.assert_allclose(O.anderson_coefficients([[4.0]]), [1.0])
+    a, b = 2.0, 0.5
+    al = O.anderson_coefficients([[a * a, 0.0], [0.0, b * b]])
+    np.testing.assert_allclose(al, np.array([1 / a**2, 1 / b**2]) / (1 / a**2 + 1 / b**2),
+                               rtol=1e-9)
+
+
+def _linear_residual(K, b):
+    def fn(w):
+        return b - K @ w, 0, 3
+    return fn
+
+
+def test_anderson_history_one_is_the_damped_fixed_point_iteration():
+    """m = 1: every step is w <- w + beta g(w)."""
+    rng = np.random.default_rng(3)
+    M = rng.normal(size=(5, 5))
+    K = M @ M.T / 5 + np.eye(5)
+    b = rng.normal(size=5)
+    beta = 0.1
+    fn = _linear_residual(K, b)
+    w0 = np.zeros(5)
+    r = O.anderson(fn, w0, fn(w0)[0], 0, 1, beta, 6, 0.0)
+    w = w0.copy()
+    for _ in range(6):
+        w = w + beta * (b - K @ w)
+    np.testing.assert_allclose(r.w, w, rtol=1e-13, atol=1e-15)
+    assert r.iterations == 6 and not r.converged
+
+
+def test_anderson_solves_a_linear_problem_in_n_plus_one_steps():
+    """On an affine residual in R^n, Anderson acceleration with an untruncated history is
+    equivalent to GMRES (Walker & Ni, SIAM J. Numer. Anal. 49 (2011)) and reaches the solution
+    at iteration n + 1, far ahead of the fixed-point iteration.  The residuals are scaled to
+    ~1e3 so that the Gram matrix's fixed 1e-10 regularisation stays negligible."""
+    rng = np.random.default_rng(4)
+    n = 5
+    M = rng.normal(size=(n, n))
+    K = M @ M.T / n + 0.5 * np.eye(n)
+    b = 1e3 * rng.normal(size=n)
+    fn = _linear_residual(K, b)
+    w0 = np.zeros(n)
+    beta = 1.0 / np.linalg.norm(K, 2)
+    r = O.anderson(fn, w0, fn(w0)[0], 0, n + 3, beta, 50, 1e-6)
+    assert r.converged and r.iterations == n + 1
+    np.testing.assert_allclose(r.w, np.linalg.solve(K, b), rtol=1e-9)
+    slow = O.anderson(fn, w0, fn(w0)[0], 0, 1, beta, 50, 1e-6)
+    assert not slow.converged
+
+
+def test_anderson_rejected_trials_restart_and_the_zero_count_rule():
+    """Trials whose cells exceed max_vertices halve the step down to 1e-4 (14 halvings from 1),
+    then w is kept and the history restarts.  The zero-mass count at the start of an iteration
+    blocks convergence only on every 10th iteration."""
+    calls = {"n": 0}
+
+    def too_big(w):
+        calls["n"] += 1
+        return np.zeros(2), 0, 500
+
+    r = O.anderson(too_big, np.ones(2), np.full(2, 1.0), 0, 3, -1.0, 2, 1e-12, max_vertices=24)
+    assert calls["n"] == 2 * 14 and r.restarts == 2
+    np.testing.assert_array_equal(r.w, np.ones(2))
+
+    def tiny(w):
+        return np.full(2, 1e-9), 1, 3
+
+    r = O.anderson(tiny, np.zeros(2), np.full(2, 1e-9), 1, 3, -1.0, 10, 1e-6)
+    assert r.converged and r.iterations == 2  # iteration 0 sees a zero cell; 1 does not count
