@@ -1,15 +1,20 @@
-"""Benchmark: semi-discrete OT dual solves to 1e-5 max relative cell-mass error at N = 65,536.
+"""Benchmark: semi-discrete OT dual solves to 1e-5 max relative cell-mass error.
 
-Workload: 256x256 jittered sites (N = 65,536), 1024x1024 target of 40 random disks and
-rectangles at 100:1 contrast, uniform source masses 1/N, mixed precision (FP32 cell geometry
-and integration, FP64 accumulation and L-BFGS, m = 10), until max_i |mass_i - nu_i| / nu_i < 1e-5.
+Default workload (--workload shapes65k): 256x256 jittered sites (N = 65,536), 1024x1024 target
+of 40 random disks and rectangles at 100:1 contrast, uniform source masses 1/N, mixed precision
+(FP32 cell geometry and integration, FP64 accumulation and L-BFGS, m = 10), until
+max_i |mass_i - nu_i| / nu_i < 1e-5.
 
-One step = one complete solve from nothing but the problem, inputs resident in HBM: the warm
-start (coarse-to-fine: the problem on 64x64 and 128x128 jittered grids with box-downsampled
-targets, solved to 1e-3, each solution carried up through the weight field; or the network's
-field, --start learned), then the full-resolution solve.  Every level's work is inside the timed
-region.  The plain solve from zero weights (the reference's start) is timed once, outside the
-timed region, for the speedup.
+--workload batch64: 64 uneven problems (N_k from 2,048 to 32,768, 512x512 Gaussian-mixture
+targets) solved together in one batched context, each to max-rel 1e-5.
+
+One step = complete solves from nothing but the problems, inputs resident in HBM: the warm start
+(coarse-to-fine: each problem on coarse jittered grids -- 64x64 and 128x128 for shapes65k, 32x32
+for batch64 -- with box-downsampled targets, solved to 1e-3, each solution carried up through the
+weight field; or the network's field, --start learned), then the full-resolution solve.  Every
+level's work is inside the timed region.  The plain solve from zero weights (the reference's
+start) is timed once, outside the timed region, for the speedup.  Launch shapes come from the
+autotuner (caustic_amd.autotune), run before the timed region.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -27,6 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "shapes65k": {
+        "levels": (64, 128),
+        "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
+                "(100:1), L-BFGS m=10 to max-rel 1e-5, mixed precision",
+        "metric": "solves/s to 1e-5 cell-mass err at N=65k",
+    },
+    "batch64": {
+        "levels": (32,),
+        "desc": "64 uneven problems in one batch: N_k in [2048, 32768] jittered sites, 512x512 "
+                "Gaussian-mixture targets, L-BFGS m=10 to max-rel 1e-5 each, mixed precision",
+        "metric": "solves/s to 1e-5 cell-mass err, 64 uneven problems batched",
+    },
+}
 
 
 def cpu_baseline(problem, evals_per_solve, budget_s=20.0):
@@ -62,12 +81,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="shapes65k", choices=sorted(WORKLOADS))
     ap.add_argument("--start", default="multiscale", choices=["multiscale", "learned", "zero"])
     ap.add_argument("--no-cold", action="store_true", help="skip the plain zero-start solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--device-loop", default="graph", choices=["graph", "eager"],
                     help="eager: the same device rounds launched without graph capture")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shapes instead of the autotuner's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
@@ -83,47 +105,73 @@ def main():
     from caustic_amd import workloads as W
     from caustic_amd.ot import OTBatch
 
+    wl = WORKLOADS[args.workload]
     loop = {"graph": 1, "eager": 2}[args.device_loop]
-    prob = W.large_shapes()
+    probs = [W.large_shapes()] if args.workload == "shapes65k" else W.batched_uneven()
     tol = 1e-5
     log = lambda s: print(s, flush=True)
     net = WS.WarmStartNet.load(WS.DEFAULT_MODEL) if args.start == "learned" else None
     kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
               device_loop=loop)
 
-    def one_solve(batch):
-        """One timed step.  Returns (report of the full-resolution solve, evaluations)."""
+    levels = {"multiscale": wl["levels"], "learned": (wl["levels"][-1],), "zero": ()}[args.start]
+    coarse = WS.coarse_problems(probs, levels)
+    tuning = {"final": None, "levels": [None] * len(levels)}
+
+    def start_weights():
+        """The warm start's weights for the full-resolution problems and its evaluations."""
         if args.start == "zero":
-            w, rep = batch.solve(None, **kw)
-            return rep[0], rep[0]["evaluations"]
-        if args.start == "multiscale":
-            fields, levels = None, (64, 128)
-        else:
-            fields, levels = WS.predict_fields(net, [prob]), (128,)
+            return None, 0
+        fields = None if args.start == "multiscale" else WS.predict_fields(net, probs)
         evals = 0
-        for li, n_side in enumerate(levels):
-            cp = WS.coarse_problem(prob, n_side, 7919 * (li + 1))
-            init = None if fields is None else WS.weights_from_field(fields[0], cp.sites)
-            with OTBatch([cp], precision="mixed") as cb:
+        for li, (n_side, cps) in enumerate(zip(levels, coarse)):
+            init = None if fields is None else np.concatenate(
+                [WS.weights_from_field(U, cp.sites) for U, cp in zip(fields, cps)])
+            with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
                 wc, rc = cb.solve(init, **dict(kw, tol=1e-3))
                 cen, _ = cb.centroids(wc)
-            evals += rc[0]["evaluations"]
-            fields = [WS.field_from_solution(cp.sites, wc, cen, n_side)]
-        w0 = WS.weights_from_field(fields[0], prob.sites)
-        w, rep = batch.solve(w0, **kw)
-        return rep[0], evals + rep[0]["evaluations"]
+                wcs, ccs = cb.split(wc), cb.split(cen.reshape(-1, 2))
+            evals += sum(r["evaluations"] for r in rc)
+            fields = [WS.field_from_solution(cp.sites, wk, ck, n_side)
+                      for cp, wk, ck in zip(cps, wcs, ccs)]
+        return np.concatenate([WS.weights_from_field(U, p.sites)
+                               for U, p in zip(fields, probs)]), evals
 
-    batch = OTBatch([prob], precision="mixed")
+    def one_solve(batch):
+        """One timed step.  Returns (reports of the full-resolution solves, evaluations)."""
+        w0, evals = start_weights()
+        w, rep = batch.solve(w0, **kw)
+        return rep, evals + sum(r["evaluations"] for r in rep)
+
+    tune_report = None
+    if not args.no_tune:
+        # launch shapes per level: the coarse levels measured from zero weights, the final
+        # level from the warm start's weights (outside the timed region)
+        from caustic_amd import autotune as AT
+
+        tune_report = {}
+        for li, cps in enumerate(coarse):
+            tuning["levels"][li], r = AT.tune(cps, "mixed", iters=40, log=log)
+            tune_report[f"level{levels[li]}"] = r
+        w_probe, _ = start_weights()
+        tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, iters=40, log=log)
+        tune_report["final"] = r
+        log(f"[autotune] {json.dumps(tune_report)}")
+
+    batch = OTBatch(probs, precision="mixed", tuning=tuning["final"])
     cold = None
     if not args.no_cold:
         t0 = time.perf_counter()
         _, rc = batch.solve(None, **dict(kw, verbose=1))
-        cold = {"seconds": time.perf_counter() - t0, "iterations": rc[0]["iterations"],
-                "evaluations": rc[0]["evaluations"], "converged": bool(rc[0]["converged"])}
+        cold = {"seconds": time.perf_counter() - t0,
+                "iterations": max(r["iterations"] for r in rc),
+                "evaluations": sum(r["evaluations"] for r in rc),
+                "converged": all(bool(r["converged"]) for r in rc)}
         log(f"[plain zero-start solve] {cold}")
     for _ in range(args.warmup):
         r, e = one_solve(batch)
-        log(f"[warmup] iters {r['iterations']} evals {e} maxrel {r['max_rel_err']:.3e}")
+        log(f"[warmup] iters {max(x['iterations'] for x in r)} evals {e} "
+            f"maxrel {max(x['max_rel_err'] for x in r):.3e}")
     torch.cuda.synchronize()
     batch.reset_stats()
     reps = []
@@ -132,21 +180,23 @@ def main():
         ts = time.perf_counter()
         r, e = one_solve(batch)
         reps.append((r, e, time.perf_counter() - ts))
-        log(f"[step {k}] iters {r['iterations']} evals {e} maxrel {r['max_rel_err']:.3e} "
-            f"{reps[-1][2]:.3f}s")
+        log(f"[step {k}] iters {max(x['iterations'] for x in r)} evals {e} "
+            f"maxrel {max(x['max_rel_err'] for x in r):.3e} {reps[-1][2]:.3f}s")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = batch.stats()
-    conv = all(r["converged"] for r, _, _ in reps)
-    # dominant work of an evaluation: construction + integration passes of the full problem
+    conv = all(x["converged"] for r, _, _ in reps for x in r)
+    # dominant work of an evaluation: construction + integration passes of the full batch
     cell_avg = st["cell_seconds"] / max(st["cell_launches"], 1)
     integ_avg = st["integ_seconds"] / max(st["integ_launches"], 1)
-    P = prob.rho.size
-    alg_bytes = P * 4 + 48 * prob.n  # FP32 density read once; sites, w, nu, mass, f
+    P = sum(p.rho.size for p in probs)
+    N = sum(p.n for p in probs)
+    alg_bytes = P * 4 + 48 * N  # FP32 density read once; sites, w, nu, mass, f
     achieved = alg_bytes / (cell_avg + integ_avg) / 1e9
+    n_solves = len(probs) * args.steps
     out = {
-        "metric": "solves/s to 1e-5 cell-mass err at N=65k",
-        "value": args.steps / dt if conv else 0.0,
+        "metric": wl["metric"],
+        "value": n_solves / dt if conv else 0.0,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -156,21 +206,24 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (seeded jittered grid + disks/rectangles target)",
+        "data": "synthetic (seeded jittered grids + disks/rectangles or Gaussian-mixture targets)",
         "config": {
-            "workload": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects "
-                        "target (100:1), L-BFGS m=10 to max-rel 1e-5, mixed precision",
-            "start": args.start, "n_sites": prob.n, "target": [1024, 1024],
-            "final_iterations": int(np.mean([r["iterations"] for r, _, _ in reps])),
+            "workload": wl["desc"],
+            "start": args.start, "problems": len(probs), "n_sites": N,
+            "target_pixels": P,
+            "final_iterations_max": int(np.mean([max(x["iterations"] for x in r)
+                                                 for r, _, _ in reps])),
             "evaluations_all_levels": int(np.mean([e for _, e, _ in reps])),
-            "converged": conv, "max_rel_err": max(r["max_rel_err"] for r, _, _ in reps),
+            "converged": conv,
+            "max_rel_err": max(x["max_rel_err"] for r, _, _ in reps for x in r),
             "device_loop": args.device_loop,
+            "launch_shapes": tuning,
         },
         "plain_zero_start": cold,
         "speedup_vs_zero_start": (cold["seconds"] / (dt / args.steps)) if cold else None,
         "roofline": {
             "kernel": "cell evaluation: cell_kernel (construction) + integrate_kernel (exact "
-                      "mass integral), full N=65536 passes of the timed solves",
+                      "mass integral), full-batch passes of the timed solves",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "avg_launch_us": {"cell_kernel": cell_avg * 1e6, "integrate_kernel": integ_avg * 1e6},
@@ -182,10 +235,11 @@ def main():
             "integrate": [st["integ_seconds"], st["integ_launches"]],
             "repairs": [st["repair_seconds"], st["repair_launches"]],
         },
+        "autotune": tune_report,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "shapes65k":
         n_eval = cold["evaluations"] if cold else out["config"]["evaluations_all_levels"]
-        out["cpu_baseline"] = cpu_baseline(prob, n_eval, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(probs[0], n_eval, args.cpu_budget)
     print(json.dumps(out), flush=True)
     return 0
 

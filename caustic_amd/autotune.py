@@ -1,0 +1,151 @@
+"""Autotuner of the evaluation kernels' launch shapes, measured on the problems to be solved.
+
+Knobs (fields of cot_options):
+
+  int_lanes         integration lanes per site (4, 8, 16).  Lane q of a site walks edges q,
+                    q + L, ... across the pixel lattice, so more lanes mean shorter serial walks
+                    and more waves in flight, at the price of idle lanes on cells with few edges
+                    and a longer shuffle reduction.  Large cells (coarse sites over a fine image)
+                    favour more lanes.
+  repair_blocks     grid of the repair passes that walk the work list of flagged sites
+                    (128 ... 1024 blocks: short lists want few blocks, long ones one per CU or
+                    more).
+  sites_per_bucket  bucket-grid density (0.5, 1, 2).  Sites are stored in 8x8-bucket tiles, so
+                    this also sets how compact the patch of plane under one 128-site block is,
+                    and the size of the fallback candidate rings.
+
+Objective: device time per full evaluation -- construction, integration and repair kernels, read
+from the in-kernel timers (cot_get_stats) -- over a short stretch of the real solve: `iters`
+L-BFGS iterations from the caller's weights, replayed for every candidate.  Coordinate descent
+over the knobs (a change must win by 2%), at most two sweeps.  Results are cached per device,
+precision and problem shape in a JSON file ($CAUSTIC_TUNE_CACHE, default
+~/.cache/caustic_amd/tuning.json).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+KNOBS = {
+    "int_lanes": (4, 8, 16),
+    "repair_blocks": (128, 256, 512, 1024),
+    "sites_per_bucket": (0.5, 1.0, 2.0),
+}
+DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0}
+MIN_GAIN = 0.02
+
+
+def cache_path() -> str:
+    return os.environ.get("CAUSTIC_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "caustic_amd",
+                                       "tuning.json"))
+
+
+def shape_key(problems, precision: str) -> str:
+    import torch
+
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
+    n = sum(int(p.n) for p in problems)
+    px = sum(int(p.rho.size) for p in problems)
+    return f"{dev}|{precision}|problems={len(problems)}|sites={n}|pixels={px}"
+
+
+def _load_cache() -> dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key: str, entry: dict) -> None:
+    data = _load_cache()
+    data[key] = entry
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # an unwritable cache only costs a re-tune
+
+
+def measure(problems, precision: str, cfg: dict, w0, iters: int, solve_kw: dict) -> dict:
+    """Device seconds per full evaluation of `iters` L-BFGS iterations from w0 under cfg."""
+    from .ot import OTBatch
+
+    kw = dict(solve_kw, raise_on_fail=False)
+    with OTBatch(problems, precision=precision, tuning=cfg) as b:
+        b.solve(w0, **dict(kw, max_iter=2))  # adjacency of the start, graph capture
+        b.reset_stats()
+        t0 = time.perf_counter()
+        b.solve(w0, **dict(kw, max_iter=iters))
+        wall = time.perf_counter() - t0
+        st = b.stats()
+    evals = max(int(st["cell_launches"]), 1)
+    dev = st["cell_seconds"] + st["integ_seconds"] + st["repair_seconds"]
+    return {"per_eval": dev / evals, "evaluations": evals, "wall": wall,
+            "cell": st["cell_seconds"] / evals, "integrate": st["integ_seconds"] / evals,
+            "repair": st["repair_seconds"] / evals}
+
+
+def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw: dict = None,
+         use_cache: bool = True, log=None):
+    """Launch shapes for `problems` (a list of workloads.Problem).  w0: weights to start the
+    measured iterations from (zeros by default; a warm start is more representative of a real
+    solve).  Returns (config dict for OTBatch(tuning=...), report dict)."""
+    key = shape_key(problems, precision)
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            return dict(hit["config"]), dict(hit, cached=True)
+    solve_kw = dict(solve_kw or {})
+    solve_kw.setdefault("history", 10)
+    solve_kw.setdefault("tol", 1e-12)  # run the full stretch
+    solve_kw.setdefault("stop", "maxrel")
+    solve_kw.setdefault("h0_threshold", 0.0)
+    n = sum(int(p.n) for p in problems)
+    w0 = np.zeros(n) if w0 is None else np.asarray(w0, dtype=np.float64)
+    seen = {}
+
+    def run(cfg):
+        k = tuple(sorted(cfg.items()))
+        if k not in seen:
+            seen[k] = measure(problems, precision, cfg, w0, iters, solve_kw)
+            if log:
+                log(f"[autotune] {cfg} -> {seen[k]['per_eval'] * 1e6:.1f} us/eval")
+        return seen[k]["per_eval"]
+
+    t_start = time.perf_counter()
+    cfg = dict(DEFAULT)
+    best = run(cfg)
+    for _ in range(2):
+        changed = False
+        for knob, values in KNOBS.items():
+            for v in values:
+                if v == cfg[knob]:
+                    continue
+                trial = dict(cfg, **{knob: v})
+                t = run(trial)
+                if t < best * (1.0 - MIN_GAIN):
+                    best, cfg, changed = t, trial, True
+        if not changed:
+            break
+    base = seen[tuple(sorted(DEFAULT.items()))]["per_eval"]
+    report = {
+        "config": cfg,
+        "per_eval_us": best * 1e6,
+        "default_per_eval_us": base * 1e6,
+        "gain_vs_default": base / best if best > 0 else 1.0,
+        "candidates": len(seen),
+        "seconds": time.perf_counter() - t_start,
+        "cached": False,
+    }
+    if use_cache:
+        _store_cache(key, report)
+    return dict(cfg), report

@@ -24,13 +24,14 @@
 #include "cot_layout.h"
 
 namespace cot {
-hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st);
+hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
+                       int repair_blocks, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
                                  const int* elab, uint8_t* adjn, int* adj, int nblocks,
                                  hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
-                             uint8_t* ring, int rmax, hipStream_t st);
+                             uint8_t* ring, int rmax, int repair_blocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
                              int rmax, int nblocks, hipStream_t st);
 hipError_t launch_reduce_eval(const BlockDesc* b, const double* mass, const double* fcell,
@@ -385,7 +386,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   va.list = list(c, 0);
   va.list_count = list_count(c, 0);
 
-  HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
   HIPCHK(launch_verify(va, c->nblk, st));
   HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu, c->d_exist,
                             d_dir, c->d_part, c->nblk, st));
@@ -416,11 +417,11 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
     ca.cand = repair_cand(rounds);
     if (ca.cand == kCandBucket)
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
-                               st));
+                               c->opt.repair_blocks, st));
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
                                    c->d_adj, c->nblk, st));
-    HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
     HIPCHK(launch_verify(va, c->nblk, st));
     HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu,
@@ -562,7 +563,7 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   va.fails = c->d_rfails;
   va.list = list(c, 0);
   va.list_count = list_count(c, 0);
-  HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
   HIPCHK(launch_verify(va, c->nblk, st));
   ca.redo_mode = 1;
   ca.coarse = nullptr;
@@ -580,11 +581,11 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
     ca.cand = repair_cand(r);
     if (ca.cand == kCandBucket)
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
-                               st));
+                               c->opt.repair_blocks, st));
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                    c->nblk, st));
-    HIPCHK(launch_cell(ca, c->nblk, c->fp32, st));
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
     HIPCHK(launch_verify(va, c->nblk, st));
@@ -757,6 +758,8 @@ void cot_default_options(cot_options* o) {
   o->ring_max = 32;
   o->verify_tol = 1e-4;
   o->max_repairs = 12;
+  o->int_lanes = kIntLanes;
+  o->repair_blocks = kRepairBlocks;
 }
 
 void cot_default_solve_opts(cot_solve_opts* o) {
@@ -802,6 +805,10 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!(c->opt.sites_per_bucket > 0)) c->opt.sites_per_bucket = 1.0;
   if (!(c->opt.verify_tol > 0)) c->opt.verify_tol = 1e-4;
   if (c->opt.max_repairs < 0) c->opt.max_repairs = 0;
+  if (c->opt.int_lanes != 4 && c->opt.int_lanes != 8 && c->opt.int_lanes != 16)
+    return bail(fail(COT_EINVAL, "int_lanes must be 4, 8 or 16"));
+  if (c->opt.repair_blocks < 1 || c->opt.repair_blocks > 65535)
+    return bail(fail(COT_EINVAL, "repair_blocks must be in [1, 65535]"));
   c->fp32 = c->opt.precision == COT_PREC_MIXED;
   c->np = n_problems;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)

@@ -10,9 +10,9 @@ constexpr int kMaxVerts = 24;
 // Threads per block of the cell kernel; one block covers kCellBlock consecutive sites of one
 // problem (sites are stored in bucket order, so a block is a compact patch of the plane).
 constexpr int kCellBlock = 128;
-// Blocks of the repair passes, which walk the work list of flagged sites (one per CU).
+// Default blocks of the repair passes, which walk the work list of flagged sites (one per CU).
 constexpr int kRepairBlocks = 256;
-// Integration pass: lanes per site and threads per block.
+// Integration pass: default lanes per site (4, 8 and 16 are built) and threads per block.
 constexpr int kIntLanes = 8;
 constexpr int kIntBlock = 256;
 // Threads per block of the vector kernels.

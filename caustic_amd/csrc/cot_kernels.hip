@@ -4,7 +4,7 @@
 //               target rectangle with the half-planes of candidate sites (the previous cells'
 //               neighbours, or bucket rings), polygon held per lane in LDS; stores the cell's
 //               vertices (FP64), edge records, area and centroid.
-// integrate_kernel  kIntLanes lanes per site: integrates the piecewise-constant target over the
+// integrate_kernel  4, 8 or 16 lanes per site (a tuning option): integrates the piecewise-constant target over the
 //               cell exactly with Green's theorem -- each edge is walked across the pixel
 //               lattice and every piece adds the closed-form integral of the row prefix of the
 //               density.  Mass, first and second moments are summed in FP64 (mass with a
@@ -699,14 +699,15 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
 }
 
 // ---------------------------------------------------------------------------------------------
-// integration pass: kIntLanes lanes per site; lane q walks edges q, q + kIntLanes, ...
+// integration pass: L lanes per site (4, 8 or 16); lane q walks edges q, q + L, ...
 // ---------------------------------------------------------------------------------------------
 
-// The site of integration lane `t` of block (blk, quarter); -1 when none.
+// The site of integration lane `t` of block (blk, quarter) with L lanes per site; -1 when none.
+template <int L = kIntLanes>
 __host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, int quarter,
                                                    int t) {
   const BlockDesc bd = a.blocks[blk];
-  const int ln = quarter * (kIntBlock / kIntLanes) + t / kIntLanes;
+  const int ln = quarter * (kIntBlock / L) + t / L;
   if (ln >= bd.count) return -1;
   if (!a.active[bd.prob]) return -1;
   if (a.gate && a.gate[bd.prob] == 0) return -1;
@@ -715,8 +716,9 @@ __host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, i
   return i;
 }
 
-// Moments of lane q's share of site i's edges (local pixel units, origin kbox[i]).
-template <typename T>
+// Moments of lane q's share (edges q, q + L, ...) of site i's edges (local pixel units, origin
+// kbox[i]).
+template <typename T, int L = kIntLanes>
 __host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, int i, int q,
                                         Moments& M) {
   const int nv = a.nedge[i];
@@ -733,7 +735,7 @@ __host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, i
   const double fu0 = (double)kb.x, fv0 = (double)kb.y;
   const int lim_u = pd.pm - kb.x, lim_v = pd.pn - kb.y;
   const double2* v = a.vtx + (size_t)i * kMaxVerts;
-  for (int k = q; k < nv; k += kIntLanes) {
+  for (int k = q; k < nv; k += L) {
     const double2 va = v[k], vb = v[k + 1 < nv ? k + 1 : 0];
     walk_edge<T>((va.x - pd.x0) * idx_ - fu0, (va.y - pd.y0) * idy_ - fv0,
                  (vb.x - pd.x0) * idx_ - fu0, (vb.y - pd.y0) * idy_ - fv0, lim_u, lim_v, px, M);
@@ -849,18 +851,18 @@ __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   timer_mark(a.timer, worked, t0);
 }
 
-// Integration pass.  Grid (cell blocks, kCellBlock * kIntLanes / kIntBlock), block kIntBlock:
-// kIntLanes consecutive lanes per site; the lanes' moments are summed with shuffles within their
-// aligned group.
-template <typename T>
+// Integration pass.  Grid (cell blocks, kCellBlock * L / kIntBlock), block kIntBlock: L
+// consecutive lanes per site; the lanes' moments are summed with shuffles within their aligned
+// group.
+template <typename T, int L>
 __device__ __forceinline__ void integrate_block(const CellArgs& a) {
   const int t = threadIdx.x;
-  const int i = integ_site(a, blockIdx.x, blockIdx.y, t);
+  const int i = integ_site<L>(a, blockIdx.x, blockIdx.y, t);
   const int prob = a.blocks[blockIdx.x].prob;
   Moments M;
-  if (i >= 0) integrate_lane<T>(a, a.probs[prob], i, t % kIntLanes, M);
+  if (i >= 0) integrate_lane<T, L>(a, a.probs[prob], i, t % L, M);
 #pragma unroll
-  for (int off = kIntLanes / 2; off > 0; off >>= 1) {
+  for (int off = L / 2; off > 0; off >>= 1) {
     Moments o;
     o.m0.s = __shfl_xor(M.m0.s, off, 64);
     o.m0.c = __shfl_xor(M.m0.c, off, 64);
@@ -870,7 +872,7 @@ __device__ __forceinline__ void integrate_block(const CellArgs& a) {
     o.m2v = __shfl_xor(M.m2v, off, 64);
     combine(M, o);
   }
-  if (i >= 0 && t % kIntLanes == 0) finish_site(a, a.probs[prob], prob, i, M);
+  if (i >= 0 && t % L == 0) finish_site(a, a.probs[prob], prob, i, M);
 }
 
 // Clears the length of the work list the next check fills (nothing reads or writes that list
@@ -880,29 +882,29 @@ __device__ __forceinline__ void reset_list(const CellArgs& a) {
 }
 
 // a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a) {
   timer_close(a.timer);
   reset_list(a);
   const unsigned long long t0 = timer_begin();
-  const bool worked = integ_site(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
-  integrate_block<T>(a);
+  const bool worked = integ_site<L>(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
+  integrate_block<T, L>(a);
   timer_mark(a.timer_int, worked, t0);
 }
 
-// Repair integration over the work list: kIntLanes lanes per listed site.  Grid kRepairBlocks.
-template <typename T>
+// Repair integration over the work list: L lanes per listed site, any grid size.
+template <typename T, int L>
 __global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a) {
   timer_close(a.timer);
   reset_list(a);
   const unsigned long long t0 = timer_begin();
   const int n = *a.list_count;
-  constexpr int kSites = kIntBlock / kIntLanes;
+  constexpr int kSites = kIntBlock / L;
   const int t = threadIdx.x;
   bool worked = false;
   // entries dealt round-robin over the blocks (as in cell_repair_kernel)
   for (int base = 0; base < n; base += gridDim.x * kSites) {
-    const int e = base + (t / kIntLanes) * gridDim.x + blockIdx.x;
+    const int e = base + (t / L) * gridDim.x + blockIdx.x;
     int i = -1, prob = 0;
     if (e < n) {
       const int slot = a.list[e];
@@ -911,9 +913,9 @@ __global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a)
       prob = bd.prob;
     }
     Moments M;
-    if (i >= 0) integrate_lane<T>(a, a.probs[prob], i, t % kIntLanes, M);
+    if (i >= 0) integrate_lane<T, L>(a, a.probs[prob], i, t % L, M);
 #pragma unroll
-    for (int off = kIntLanes / 2; off > 0; off >>= 1) {
+    for (int off = L / 2; off > 0; off >>= 1) {
       Moments o;
       o.m0.s = __shfl_xor(M.m0.s, off, 64);
       o.m0.c = __shfl_xor(M.m0.c, off, 64);
@@ -923,7 +925,7 @@ __global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a)
       o.m2v = __shfl_xor(M.m2v, off, 64);
       combine(M, o);
     }
-    if (i >= 0 && t % kIntLanes == 0) finish_site(a, a.probs[prob], prob, i, M);
+    if (i >= 0 && t % L == 0) finish_site(a, a.probs[prob], prob, i, M);
     worked = worked || i >= 0;
   }
   timer_mark(a.timer_int, worked, t0);
@@ -1166,23 +1168,39 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 // launchers
 // ---------------------------------------------------------------------------------------------
 
-// Construction (one lane per site) then integration (kIntLanes lanes per site).
-hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, hipStream_t st) {
-  const CellArgs& b = a;  // the integration pass closes a.timer and marks a.timer_int
-  const dim3 ig(nblocks, kCellBlock * kIntLanes / kIntBlock);
+// Integration pass with L lanes per site: the full grid, or the repair grid over the work list.
+template <int L>
+void launch_integrate(const CellArgs& a, int nblocks, bool fp32, int repair_blocks,
+                      hipStream_t st) {
+  const dim3 ig(nblocks, kCellBlock * L / kIntBlock);
+  if (a.redo_mode) {
+    hipLaunchKernelGGL((integrate_repair_kernel<double, L>), dim3(repair_blocks), dim3(kIntBlock),
+                       0, st, a);
+  } else if (fp32) {
+    hipLaunchKernelGGL((integrate_kernel<float, L>), ig, dim3(kIntBlock), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((integrate_kernel<double, L>), ig, dim3(kIntBlock), 0, st, a);
+  }
+}
+
+// Construction (one lane per site) then integration (int_lanes lanes per site: 4, 8 or 16).
+// The integration pass closes a.timer and marks a.timer_int.
+hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
+                       int repair_blocks, hipStream_t st) {
   if (a.redo_mode) {
     // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
     // of the cells the FP32 pass could not resolve), on a small grid over the work list
-    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(kRepairBlocks), dim3(kCellBlock), 0, st,
+    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(repair_blocks), dim3(kCellBlock), 0, st,
                        a);
-    hipLaunchKernelGGL((integrate_repair_kernel<double>), dim3(kRepairBlocks), dim3(kIntBlock), 0,
-                       st, b);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-    hipLaunchKernelGGL((integrate_kernel<float>), ig, dim3(kIntBlock), 0, st, b);
   } else {
     hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-    hipLaunchKernelGGL((integrate_kernel<double>), ig, dim3(kIntBlock), 0, st, b);
+  }
+  switch (int_lanes) {
+    case 4: launch_integrate<4>(a, nblocks, fp32, repair_blocks, st); break;
+    case 16: launch_integrate<16>(a, nblocks, fp32, repair_blocks, st); break;
+    default: launch_integrate<8>(a, nblocks, fp32, repair_blocks, st); break;
   }
   return hipGetLastError();
 }
@@ -1202,8 +1220,8 @@ hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint
 
 // Grow the ring (r -> 2r + 1) of every site on the work list.
 hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
-                             uint8_t* ring, int rmax, hipStream_t st) {
-  hipLaunchKernelGGL(raise_ring_kernel, dim3(kRepairBlocks), dim3(kCellBlock), 0, st, b, list,
+                             uint8_t* ring, int rmax, int repair_blocks, hipStream_t st) {
+  hipLaunchKernelGGL(raise_ring_kernel, dim3(repair_blocks), dim3(kCellBlock), 0, st, b, list,
                      list_count, ring, rmax);
   return hipGetLastError();
 }

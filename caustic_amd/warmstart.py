@@ -309,19 +309,23 @@ def predict_weights(net: WarmStartNet, problems) -> list:
 # ---------------------------------------------------------------------------------------------
 
 def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
-    """The problem on an n_side x n_side jittered grid with the target box-downsampled to keep
-    the same pixels per site (at least 32 x 32 pixels)."""
+    """The problem on an n_side x n_side jittered grid with the target box-downsampled by the
+    largest factor that divides the image side and keeps at least the fine problem's pixels per
+    site and 32 x 32 pixels."""
     fine_side = int(round(np.sqrt(p.n)))
-    res = max(p.rho.shape[0] * n_side // fine_side, 32)
-    rho = W.box_downsample(p.rho, res) if res < p.rho.shape[0] else p.rho
+    side = p.rho.shape[0]
+    f = max(1, min(fine_side // max(n_side, 1), side // 32))
+    while side % f:
+        f -= 1
+    rho = W.box_downsample(p.rho, side // f) if f > 1 else p.rho
     sites = W.jittered_grid(n_side, W.rng_for(seed))
     return W.make_problem(sites, rho / rho.sum(), p.tol, p.name + f"_c{n_side}", seed)
 
 
-def _solve_level(problems, init, tol, precision, max_iter):
+def _solve_level(problems, init, tol, precision, max_iter, tuning=None):
     from .ot import OTBatch
 
-    with OTBatch(list(problems), precision=precision) as b:
+    with OTBatch(list(problems), precision=precision, tuning=tuning) as b:
         w, rep = b.solve(None if init is None else np.concatenate(init), history=10, tol=tol,
                          stop="maxrel", h0_threshold=0.0, max_iter=max_iter,
                          raise_on_fail=False)
@@ -331,20 +335,30 @@ def _solve_level(problems, init, tol, precision, max_iter):
     return ws, cs, rep
 
 
+def coarse_problems(problems, levels):
+    """The coarse problems of every level, as multiscale_solve builds them: a list per level."""
+    return [[coarse_problem(p, n_side, 7919 * (li + 1) + k) for k, p in enumerate(problems)]
+            for li, n_side in enumerate(levels)]
+
+
 def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
-                     coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print):
+                     coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print,
+                     tunings=None):
     """Solve every problem to max-rel `tol`, one batch per level: coarse grids first (to
     `coarse_tol`), each level's weights carried to the next level's sites through the weight
     field (field_from_solution / weights_from_field).  fields: optional per-problem fields that
     start the first level (a learned start); levels=() with fields starts the final level from
-    them.  Returns (weights, centroids, reports) of the final level."""
+    them.  tunings: optional launch shapes (caustic_amd.autotune) keyed by level side or
+    "final".  Returns (weights, centroids, reports) of the final level."""
+    tunings = tunings or {}
     probs = list(problems)
     for li, n_side in enumerate(levels):
         cps = [coarse_problem(p, n_side, 7919 * (li + 1) + k) for k, p in enumerate(probs)]
         init = None if fields is None else [weights_from_field(U, cp.sites)
                                             for U, cp in zip(fields, cps)]
         t0 = time.perf_counter()
-        ws, cs, rep = _solve_level(cps, init, coarse_tol, precision, max_iter)
+        ws, cs, rep = _solve_level(cps, init, coarse_tol, precision, max_iter,
+                                   tunings.get(n_side))
         fields = [field_from_solution(cp.sites, wk, ck, n_side)
                   for cp, wk, ck in zip(cps, ws, cs)]
         log(f"[multiscale] level {n_side}: {len(cps)} problems, "
@@ -354,7 +368,7 @@ def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="
     init = None if fields is None else [weights_from_field(U, p.sites)
                                         for U, p in zip(fields, probs)]
     t0 = time.perf_counter()
-    ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter)
+    ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter, tunings.get("final"))
     log(f"[multiscale] final: {len(probs)} problems, "
         f"iters max {max(r['iterations'] for r in rep)} "
         f"mean {np.mean([r['iterations'] for r in rep]):.0f}, converged "

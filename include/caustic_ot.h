@@ -57,6 +57,8 @@ typedef struct {
   int ring_max;             /* largest ring radius (default 32) */
   double verify_tol;        /* edge-match tolerance relative to the site spacing (default 1e-4) */
   int max_repairs;          /* conformity repair rounds per evaluation (default 12) */
+  int int_lanes;            /* integration lanes per site: 4, 8 or 16 (default 8) */
+  int repair_blocks;        /* blocks of the repair passes over the work list (default 256) */
 } cot_options;
 
 typedef struct {

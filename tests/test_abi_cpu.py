@@ -32,10 +32,11 @@ def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "caustic_ot.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
         "sizeof(cot_options), sizeof(cot_solve_opts), sizeof(cot_report), sizeof(cot_stats), "
         "offsetof(cot_solve_opts, repair_rounds), sizeof(cot_reflection), "
-        "offsetof(cot_reflection, if_aa_op_initial));return 0;}\n")
+        "offsetof(cot_reflection, if_aa_op_initial), offsetof(cot_options, repair_blocks));"
+        "return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
@@ -43,7 +44,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     want = [ctypes.sizeof(L.cot_problem), ctypes.sizeof(L.cot_options),
             ctypes.sizeof(L.cot_solve_opts), ctypes.sizeof(L.cot_report),
             ctypes.sizeof(L.cot_stats), L.cot_solve_opts.repair_rounds.offset,
-            ctypes.sizeof(L.cot_reflection), L.cot_reflection.if_aa_op_initial.offset]
+            ctypes.sizeof(L.cot_reflection), L.cot_reflection.if_aa_op_initial.offset,
+            L.cot_options.repair_blocks.offset]
     assert got == want
 
 

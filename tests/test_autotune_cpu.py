@@ -1,0 +1,39 @@
+"""The autotuner's search and cache, with the measurement replaced by a known cost model."""
+from caustic_amd import autotune as AT
+from caustic_amd import workloads as W
+
+
+def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setenv("CAUSTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
+    calls = []
+
+    def cost(problems, precision, cfg, w0, iters, solve_kw):
+        calls.append(dict(cfg))
+        t = {4: 3.0, 8: 2.0, 16: 1.0}[cfg["int_lanes"]]
+        t += {128: 0.5, 256: 0.3, 512: 0.1, 1024: 0.2}[cfg["repair_blocks"]]
+        t += {0.5: 0.05, 1.0: 0.0, 2.0: 0.3}[cfg["sites_per_bucket"]]
+        return {"per_eval": t * 1e-4, "evaluations": 10, "wall": 0.0, "cell": 0.0,
+                "integrate": 0.0, "repair": 0.0}
+
+    monkeypatch.setattr(AT, "measure", cost)
+    probs = [W.plain_fp64(n_side=8, res=16)]
+    cfg, rep = AT.tune(probs, "mixed")
+    assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0}
+    assert abs(rep["gain_vs_default"] - 2.3 / 1.1) < 1e-9
+    assert len(calls) == len({tuple(sorted(c.items())) for c in calls})  # no repeats
+    n = len(calls)
+    cfg2, rep2 = AT.tune(probs, "mixed")  # from the cache: nothing measured
+    assert cfg2 == cfg and rep2["cached"] and len(calls) == n
+
+
+def test_small_gains_keep_the_default(tmp_path, monkeypatch):
+    monkeypatch.setenv("CAUSTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
+
+    def flat(problems, precision, cfg, w0, iters, solve_kw):
+        t = 1.0 - (0.01 if cfg["int_lanes"] == 16 else 0.0)  # 1% better: below the bar
+        return {"per_eval": t, "evaluations": 1, "wall": 0.0, "cell": 0.0, "integrate": 0.0,
+                "repair": 0.0}
+
+    monkeypatch.setattr(AT, "measure", flat)
+    cfg, _ = AT.tune([W.plain_fp64(n_side=8, res=16)], "mixed", use_cache=False)
+    assert cfg == AT.DEFAULT

@@ -149,3 +149,30 @@ def test_eval_large_shapes_full_size(gpu_available):
             ev = b.evaluate(w)
         _compare(ev, orc, p.nu, tol, 1e-8 if prec == "fp64" else 1e-6)
 
+
+
+@pytest.mark.parametrize("int_lanes,repair_blocks,spb", [(4, 128, 0.5), (16, 1024, 2.0)])
+def test_launch_shapes_give_the_same_cells(gpu_available, c1, int_lanes, repair_blocks, spb):
+    """Every launch shape the autotuner may pick computes the same evaluation (the lane count
+    only reorders the FP64 sums): masses within 1e-12 of max(mass, nu) of the defaults'.  A
+    solve under the shape converges, and the oracle confirms its cells and its max-rel error.
+    (The L-BFGS trajectories of two shapes need not coincide step for step: 30 iterations with
+    the bucket density at 2 have ended elsewhere than with the default.)"""
+    print(f"[case] launch shape lanes={int_lanes} repair_blocks={repair_blocks} spb={spb}",
+          flush=True)
+    rng = np.random.default_rng(7)
+    w = rng.normal(0.0, 2e-4, c1.n)
+    with _batch([c1], precision="fp64") as b:
+        ref = b.evaluate(w)
+    tuning = {"int_lanes": int_lanes, "repair_blocks": repair_blocks, "sites_per_bucket": spb}
+    with _batch([c1], precision="fp64", tuning=tuning) as b:
+        ev = b.evaluate(w)
+        wt, rep = b.solve(None, tol=1e-4)
+        evt = b.evaluate(wt)
+    dm = np.abs(ev["mass"] - ref["mass"]) / np.maximum(ref["mass"], c1.nu)
+    assert dm.max() < 1e-12
+    assert np.array_equal(ev["exist"], ref["exist"])
+    assert rep[0]["converged"]
+    orc = _exact(c1, wt)
+    _compare(evt, orc, c1.nu, 1e-10, 1e-11)
+    assert O.max_rel_error(orc.mass, c1.nu) < 1e-4
