@@ -5,13 +5,17 @@ of 40 random disks and rectangles at 100:1 contrast, uniform source masses 1/N, 
 (FP32 cell geometry and integration, FP64 accumulation and L-BFGS, m = 10), until
 max_i |mass_i - nu_i| / nu_i < 1e-5.
 
+--workload curves262k: N = 262,144 sites on a 30-degree tilted source plane, 2048x2048 target
+of 30 cubic Bezier strokes; coarse levels 64, 128 and 256.
+
 --workload batch64: 64 uneven problems (N_k from 2,048 to 32,768, 512x512 Gaussian-mixture
 targets) solved together in one batched context, each to max-rel 1e-5.
 
 One step = complete solves from nothing but the problems, inputs resident in HBM: the warm start
 (coarse-to-fine: each problem on coarse jittered grids -- 64x64 and 128x128 for shapes65k, 32x32
 for batch64 -- with box-downsampled targets, solved to 1e-3, each solution carried up through the
-weight field; or the network's field, --start learned), then the full-resolution solve.  Every
+weight field; or the network's field, --start learned), then the full-resolution solve.  Each
+level runs L-BFGS until max-rel 1e-2 and damped Newton steps after (--newton-from).  Every
 level's work is inside the timed region.  The plain solve from zero weights (the reference's
 start) is timed once, outside the timed region, for the speedup.  Launch shapes come from the
 autotuner (caustic_amd.autotune), run before the timed region.
@@ -38,6 +42,13 @@ WORKLOADS = {
         "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
                 "(100:1), L-BFGS m=10 to max-rel 1e-5, mixed precision",
         "metric": "solves/s to 1e-5 cell-mass err at N=65k",
+    },
+    "curves262k": {
+        "levels": (64, 128, 256),
+        "desc": "single large solve: N=262144 jittered sites on a 30-degree tilted plane, "
+                "2048x2048 target of 30 Bezier strokes (100:1), L-BFGS m=10 to max-rel 1e-5, "
+                "mixed precision",
+        "metric": "solves/s to 1e-5 cell-mass err at N=262k",
     },
     "batch64": {
         "levels": (32,),
@@ -90,6 +101,9 @@ def main():
                     help="eager: the same device rounds launched without graph capture")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--newton-from", type=float, default=1e-2,
+                    help="max-rel at which each level hands over from L-BFGS to damped Newton "
+                         "steps (0: L-BFGS to the end)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
@@ -107,12 +121,14 @@ def main():
 
     wl = WORKLOADS[args.workload]
     loop = {"graph": 1, "eager": 2}[args.device_loop]
-    probs = [W.large_shapes()] if args.workload == "shapes65k" else W.batched_uneven()
+    probs = {"shapes65k": lambda: [W.large_shapes()], "curves262k": lambda: [W.tilted_curves()],
+             "batch64": W.batched_uneven}[args.workload]()
     tol = 1e-5
     log = lambda s: print(s, flush=True)
     net = WS.WarmStartNet.load(WS.DEFAULT_MODEL) if args.start == "learned" else None
+    # verbose: the solver prints a progress line every 2 s of a solve
     kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
-              device_loop=loop)
+              device_loop=loop, verbose=1, newton_from=args.newton_from or None)
 
     levels = {"multiscale": wl["levels"], "learned": (wl["levels"][-1],), "zero": ()}[args.start]
     coarse = WS.coarse_problems(probs, levels)
@@ -126,15 +142,15 @@ def main():
         evals = 0
         for li, (n_side, cps) in enumerate(zip(levels, coarse)):
             init = None if fields is None else np.concatenate(
-                [WS.weights_from_field(U, cp.sites) for U, cp in zip(fields, cps)])
+                [WS.weights_from_field(U, cp.sites, cp.frame) for U, cp in zip(fields, cps)])
             with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
                 wc, rc = cb.solve(init, **dict(kw, tol=1e-3))
                 cen, _ = cb.centroids(wc)
                 wcs, ccs = cb.split(wc), cb.split(cen.reshape(-1, 2))
             evals += sum(r["evaluations"] for r in rc)
-            fields = [WS.field_from_solution(cp.sites, wk, ck, n_side)
+            fields = [WS.field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                       for cp, wk, ck in zip(cps, wcs, ccs)]
-        return np.concatenate([WS.weights_from_field(U, p.sites)
+        return np.concatenate([WS.weights_from_field(U, p.sites, p.frame)
                                for U, p in zip(fields, probs)]), evals
 
     def one_solve(batch):
@@ -162,7 +178,7 @@ def main():
     cold = None
     if not args.no_cold:
         t0 = time.perf_counter()
-        _, rc = batch.solve(None, **dict(kw, verbose=1))
+        _, rc = batch.solve(None, **kw)
         cold = {"seconds": time.perf_counter() - t0,
                 "iterations": max(r["iterations"] for r in rc),
                 "evaluations": sum(r["evaluations"] for r in rc),
@@ -217,6 +233,9 @@ def main():
             "converged": conv,
             "max_rel_err": max(x["max_rel_err"] for r, _, _ in reps for x in r),
             "device_loop": args.device_loop,
+            "newton_from": args.newton_from or None,
+            "newton_iterations": int(np.mean([max(x.get("newton_iterations", 0) for x in r)
+                                              for r, _, _ in reps])),
             "launch_shapes": tuning,
         },
         "plain_zero_start": cold,

@@ -83,6 +83,23 @@ hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const doubl
 hipError_t launch_aa_direction(const BlockDesc* blocks, const double* alpha, int H,
                                const double* G, size_t ld, const double* w, double* d, int nblk,
                                hipStream_t st);
+hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
+                          const double2* xy, const double2* vtx, const uint8_t* nedge,
+                          const int* elab, const double4* pre, double* h, double* diag, int nblk,
+                          hipStream_t st);
+hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
+                          const double* shift, const double* diag, double* x, double* r,
+                          double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st);
+hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+                               const double* h, const double* diag, double* x, double* r,
+                               double* z, double* p, double* q, double* part_rz_cur,
+                               double* part_rz_other, double* part_pq, double* part_rr, int first,
+                               int nblk, hipStream_t st);
+hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
+                            const double* d, double* w, int nblk, hipStream_t st);
+hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+                                const double* h, const double* diag, const double* p, double* q,
+                                double* part_pq, int nblk, hipStream_t st);
 hipError_t launch_axpy(const BlockDesc* blocks, double t, const double* d, double* w, int nblk,
                        hipStream_t st);
 hipError_t launch_dev_adapt_ring(const BlockDesc* blocks, const DevState* st,
@@ -182,6 +199,11 @@ struct cot_context {
   int hist_alloc = 0;
   double* d_coef = nullptr;
   int *d_slot = nullptr, *d_hcount = nullptr;
+  // Newton steps: edge Hessian entries, diagonal, CG vectors and partials
+  double *d_hh = nullptr, *d_hdiag = nullptr, *d_nx = nullptr, *d_nr = nullptr, *d_nz = nullptr;
+  double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
+  double* d_nshift = nullptr;
+  std::vector<double> nu_sum;  // per-problem sum of the source masses
   // device-resident solver
   DevState* d_state = nullptr;
   int* d_cnt = nullptr;        // last-block counters: direction, decide, accept (np each)
@@ -858,12 +880,15 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       c->blocks.push_back(b);
     }
     pd.nblk = (int)c->blocks.size() - pd.blk_off;
+    double nus = 0.0;
     for (int r = 0; r < pr.n_sites; ++r) {
       int o = order[r];
       c->perm.push_back(site_off + o);
       xy.push_back(make_double2(pr.sites[2 * o], pr.sites[2 * o + 1]));
       nu.push_back(pr.nu[o]);
+      nus += pr.nu[o];
     }
+    c->nu_sum.push_back(nus);
     img_total += (size_t)pr.pm * pr.pn;
     tot_total += (size_t)pd.nchunk * pr.pn;
     if (img_total > (size_t)INT32_MAX) return bail(fail(COT_EINVAL, "images too large"));
@@ -1211,6 +1236,196 @@ int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
   }
   if (status != COT_OK) return fail(status, "cells failed the conformity check");
   if (!conv) return fail(COT_ENOTCONV, "iteration limit reached");
+  return COT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Buffers of the Newton steps, allocated on first use.
+int ensure_newton(cot_context* c) {
+  if (c->d_hh) return COT_OK;
+  const size_t N = c->ntot, nb = c->nblk;
+  CHK(dalloc(c, &c->d_hh, N * kMaxVerts));
+  CHK(dalloc(c, &c->d_hdiag, N));
+  CHK(dalloc(c, &c->d_nx, N));
+  CHK(dalloc(c, &c->d_nr, N));
+  CHK(dalloc(c, &c->d_nz, N));
+  CHK(dalloc(c, &c->d_np, N));
+  CHK(dalloc(c, &c->d_nq, N));
+  CHK(dalloc(c, &c->d_prz, 2 * nb));
+  CHK(dalloc(c, &c->d_ppq, nb));
+  CHK(dalloc(c, &c->d_prr, nb));
+  CHK(dalloc(c, &c->d_nshift, c->np));
+  return COT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cot_hessian_product(cot_context* c, const double* w, const double* x, double* y) {
+  if (!c || !w || !x || !y) return fail(COT_EINVAL, "null argument");
+  CHK(ensure_newton(c));
+  const int nb = c->nblk;
+  std::vector<uint8_t> act(c->np, 1);
+  std::vector<EvalSum> cur;
+  CHK(upload_sorted(c, w, c->d_w));
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  for (int p = 0; p < c->np; ++p)
+    if (cur[p].fails) return fail(COT_EUNVERIFIED, "cells failed the conformity check");
+  std::memset(c->h_active, 1, c->np);
+  HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
+                        c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, c->st));
+  CHK(upload_sorted(c, x, c->d_np));
+  HIPCHK(launch_hessian_apply(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_np,
+                              c->d_nq, c->d_ppq, nb, c->st));
+  CHK(download_sorted(c, c->d_nq, y));
+  return COT_OK;
+}
+
+int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_report* reports) {
+  if (!c || !w) return fail(COT_EINVAL, "null argument");
+  cot_solve_opts o;
+  if (so) {
+    o = *so;
+  } else {
+    cot_default_solve_opts(&o);
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  const int np = c->np, nb = c->nblk;
+  hipStream_t st = c->st;
+  CHK(ensure_newton(c));
+  const int rounds = std::min(c->opt.max_repairs, std::max(o.repair_rounds, 1));
+  CHK(upload_sorted(c, w, c->d_w));
+  std::vector<uint8_t> act(np, 1), conv(np, 0), failed(np, 0);
+  std::vector<EvalSum> cur, trial;
+  std::vector<int> iters(np, 0), evals(np, 1);
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+  std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
+  // CG stops at |r| <= kEta |rhs| (an inexact Newton step) or after kMaxCG iterations
+  constexpr double kEta = 1e-3;
+  constexpr int kMaxCG = 4000, kCheckCG = 16;
+  auto sum_rr = [&](double& rr) -> int {
+    HIPCHK(hipMemcpyAsync(rr_part.data(), c->d_prr, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    rr = 0.0;
+    for (double v : rr_part) rr += v;
+    return COT_OK;
+  };
+  for (int it = 0; it < o.max_iter; ++it) {
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+      if (!conv[p] && !failed[p]) {
+        if (cur[p].maxrel < o.tol) {
+          conv[p] = 1;
+        } else if (cur[p].zeros > 0 || cur[p].fails > 0) {
+          failed[p] = 1;  // Newton needs every cell non-empty and verified
+        }
+      }
+      act[p] = !conv[p] && !failed[p];
+      any |= act[p] != 0;
+      shift[p] = act[p] ? (cur[p].mass - c->nu_sum[p]) / c->probs[p].n : 0.0;
+    }
+    if (!any) break;
+    std::memcpy(c->h_active, act.data(), np);
+    HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, np, hipMemcpyHostToDevice, st));
+    std::memcpy(c->h_small, shift.data(), sizeof(double) * np);
+    HIPCHK(hipMemcpyAsync(c->d_nshift, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice,
+                          st));
+    // Hessian at the current cells, then H x = -(g - mean g)
+    HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
+                          c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, st));
+    HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nx,
+                          c->d_nr, c->d_nz, c->d_prz, c->d_prr, nb, st));
+    double rr0 = 0.0, rr = 0.0;
+    CHK(sum_rr(rr0));
+    int cg = 0;
+    for (; cg < kMaxCG; ++cg) {
+      double* cur_rz = c->d_prz + (size_t)(cg & 1) * nb;
+      double* oth_rz = c->d_prz + (size_t)((cg + 1) & 1) * nb;
+      HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
+                                 c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz, oth_rz,
+                                 c->d_ppq, c->d_prr, cg == 0 ? 1 : 0, nb, st));
+      if ((cg + 1) % kCheckCG == 0) {
+        CHK(sum_rr(rr));
+        if (!(rr > kEta * kEta * rr0)) {
+          ++cg;
+          break;
+        }
+      }
+    }
+    // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
+    std::vector<uint8_t> tr = act, acc(np, 0);
+    std::fill(t.begin(), t.end(), 1.0);
+    for (;;) {
+      bool any_trial = false;
+      for (int p = 0; p < np; ++p) {
+        any_trial |= tr[p] != 0;
+        c->h_small[p] = t[p];
+      }
+      if (!any_trial) break;
+      HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice,
+                            st));
+      CHK(evaluate_device(c, c->d_w, c->d_nx, tr, trial, rounds));
+      for (int p = 0; p < np; ++p) {
+        if (!tr[p]) continue;
+        ++evals[p];
+        const bool ok = trial[p].fails == 0 && trial[p].overflow == 0 && trial[p].zeros == 0 &&
+                        std::sqrt(trial[p].g2) <= (1.0 - 0.5 * t[p]) * std::sqrt(cur[p].g2);
+        if (ok) {
+          acc[p] = 1;
+          tr[p] = 0;
+          cur[p] = trial[p];
+        } else {
+          t[p] *= 0.5;
+          if (t[p] < 1.0 / 4096.0) {
+            tr[p] = 0;
+            failed[p] = 1;
+          }
+        }
+      }
+    }
+    std::memcpy(c->h_active, acc.data(), np);
+    for (int p = 0; p < np; ++p) c->h_small[p] = t[p];
+    HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_axpy_step(c->d_blocks, c->d_active, c->d_step, c->d_nx, c->d_w, nb, st));
+    HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+    for (int p = 0; p < np; ++p) iters[p] += acc[p];
+    if (o.verbose > 0) {
+      for (int p = 0; p < np; ++p)
+        if (act[p])
+          std::printf("[cot] newton prob %d iter %d cg %d |r|/|r0| %.2e step %g maxrel %.3e\n", p,
+                      iters[p], cg, std::sqrt(rr / std::max(rr0, 1e-300)), t[p],
+                      cur[p].maxrel);
+      std::fflush(stdout);
+    }
+  }
+  CHK(download_sorted(c, c->d_w, w));
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  bool all = true;
+  for (int p = 0; p < np; ++p) {
+    if (!conv[p] && !failed[p] && cur[p].maxrel < o.tol) conv[p] = 1;
+    all = all && conv[p];
+    if (reports) {
+      cot_report& r = reports[p];
+      r.iterations = iters[p];
+      r.evaluations = evals[p];
+      r.converged = conv[p];
+      r.status = conv[p] ? COT_OK : COT_ENOTCONV;
+      r.f = cur[p].f;
+      r.grad_norm = std::sqrt(cur[p].g2);
+      r.max_rel_err = cur[p].maxrel;
+      r.seconds = secs;
+    }
+  }
+  if (!all) return fail(COT_ENOTCONV, "Newton steps did not reach the tolerance");
   return COT_OK;
 }
 

@@ -1,0 +1,265 @@
+// Damped Newton steps for the semi-discrete OT dual (the tail of a max-rel solve), for gfx950.
+//
+// The Hessian of the dual objective is the cell-adjacency matrix
+//   dm_i/dw_j = -h_ij,  h_ij = int_{C_i n C_j} rho ds / (2 |s_i - s_j|)   (i != j),
+//   dm_i/dw_i = sum_j h_ij,
+// a weighted graph Laplacian (Kitagawa, Merigot and Thibert, "Convergence of a Newton algorithm
+// for semi-discrete optimal transport", JEMS 2019).  hessian_kernel integrates the target along
+// every cell edge of the last evaluation (the edge labels name the neighbour), cg_* kernels solve
+// H d = -g by Jacobi-preconditioned conjugate gradients with H applied edge by edge (the
+// evaluation's edge table is the sparse pattern), and the host driver (cot_api.cpp) damps the
+// step so that every cell keeps mass and |g| decreases.
+#include "cot_internal.h"
+
+namespace cot {
+
+namespace {
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+  __syncthreads();
+  return t;
+}
+
+// Sum of n partials, the same value (same order) in every block.
+__device__ __forceinline__ double sum_partials(const double* part, int n, double* red) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v += part[k];
+  return block_sum(v, red);
+}
+
+// Sum over pieces of (parameter length) * rho(pixel) along the segment a -> b given in pixel
+// units of the problem's lattice (pm x pn pixels).
+__device__ double edge_rho(double ua, double va, double ub, double vb, const double4* pre,
+                           int pm, int pn) {
+  const double du = ub - ua, dv = vb - va;
+  const double iu0 = floor(ua), iv0 = floor(va);
+  double tu = du > 0.0 ? (iu0 + 1.0 - ua) / du : (du < 0.0 ? (iu0 - ua) / du : 2.0);
+  double tv = dv > 0.0 ? (iv0 + 1.0 - va) / dv : (dv < 0.0 ? (iv0 - va) / dv : 2.0);
+  const double su = du != 0.0 ? 1.0 / fabs(du) : 0.0, sv = dv != 0.0 ? 1.0 / fabs(dv) : 0.0;
+  int guard = (int)(fabs(floor(ub) - iu0) + fabs(floor(vb) - iv0)) + 3;
+  double t = 0.0, acc = 0.0;
+  while (t < 1.0 && guard-- > 0) {
+    const double tn = fmin(fmin(tu, tv), 1.0);
+    const double tm = 0.5 * (t + tn);
+    const int ix = min(max((int)floor(ua + tm * du), 0), pm - 1);
+    const int iy = min(max((int)floor(va + tm * dv), 0), pn - 1);
+    acc += (tn - t) * pre[(size_t)iy * pm + ix].w;
+    t = tn;
+    if (tu <= tv) {
+      tu += su;
+    } else {
+      tv += sv;
+    }
+  }
+  return acc;
+}
+
+}  // namespace
+
+// h[site * kMaxVerts + k] = h_ij of edge k of the site's cell (0 for sides and missing edges),
+// diag[site] = sum_k h.  Sites of inactive problems and empty cells get zeros.
+__global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
+                                                      const BlockDesc* blocks,
+                                                      const uint8_t* active, const double2* xy,
+                                                      const double2* vtx, const uint8_t* nedge,
+                                                      const int* elab, const double4* pre,
+                                                      double* h, double* diag) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const ProbDesc pd = probs[bd.prob];
+  const bool on = active[bd.prob] != 0;
+  const double inv_pix = 1.0 / (pd.dx * pd.dy);
+  for (int e = threadIdx.x; e < bd.count * kMaxVerts; e += blockDim.x) {
+    const int i = bd.first + e / kMaxVerts, k = e % kMaxVerts;
+    const int nv = nedge[i];
+    double hv = 0.0;
+    const int j = k < nv ? elab[(size_t)i * kMaxVerts + k] : -1;
+    if (on && nv >= 3 && j >= 0) {
+      const double2 a = vtx[(size_t)i * kMaxVerts + k];
+      const double2 b = vtx[(size_t)i * kMaxVerts + (k + 1 < nv ? k + 1 : 0)];
+      const double len = sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y));
+      if (len > 0.0) {
+        const double r = edge_rho((a.x - pd.x0) / pd.dx, (a.y - pd.y0) / pd.dy,
+                                  (b.x - pd.x0) / pd.dx, (b.y - pd.y0) / pd.dy,
+                                  pre + pd.pre_off, pd.pm, pd.pn);
+        const double2 si = xy[i], sj = xy[j];
+        const double d = sqrt((si.x - sj.x) * (si.x - sj.x) + (si.y - sj.y) * (si.y - sj.y));
+        hv = r * len * inv_pix / (2.0 * d);
+      }
+    }
+    h[(size_t)i * kMaxVerts + k] = hv;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < bd.count) {
+    const int i = bd.first + threadIdx.x;
+    double s = 0.0;
+    for (int k = 0; k < kMaxVerts; ++k) s += h[(size_t)i * kMaxVerts + k];
+    diag[i] = s;
+  }
+}
+
+// CG start: x = 0, r = -(g - shift[prob]) on the active problems' non-empty cells (0 elsewhere;
+// shift = the problem's mean of g, which puts the right-hand side in the range of the singular
+// H), z = r / diag, partial r.z and r.r.
+__global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
+    const BlockDesc* blocks, const uint8_t* active, const double* g, const double* shift,
+    const double* diag, double* x, double* r, double* z, double* part_rz, double* part_rr) {
+  __shared__ double red[kCellBlock / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int ln = threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (ln < bd.count) {
+    const int i = bd.first + ln;
+    const double dg = diag[i];
+    const double ri = (active[bd.prob] && dg > 0.0) ? shift[bd.prob] - g[i] : 0.0;
+    const double zi = dg > 0.0 ? ri / dg : 0.0;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    rz = ri * zi;
+    rr = ri * ri;
+  }
+  rz = block_sum(rz, red);
+  rr = block_sum(rr, red);
+  if (ln == 0) {
+    part_rz[blockIdx.x] = rz;
+    part_rr[blockIdx.x] = rr;
+  }
+}
+
+// p = z + beta p, beta = (r.z) / (r.z of the previous iteration); first: p = z.
+__global__ __launch_bounds__(kCellBlock) void cg_direction_kernel(
+    const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rz_prev,
+    int first, const double* z, double* p) {
+  __shared__ double red[kCellBlock / 64];
+  double beta = 0.0;
+  if (!first) {
+    const double rz = sum_partials(part_rz, nblk, red);
+    const double rz0 = sum_partials(part_rz_prev, nblk, red);
+    beta = rz0 != 0.0 ? rz / rz0 : 0.0;
+  }
+  const BlockDesc bd = blocks[blockIdx.x];
+  if ((int)threadIdx.x < bd.count) {
+    const int i = bd.first + threadIdx.x;
+    p[i] = z[i] + beta * (first ? 0.0 : p[i]);
+  }
+}
+
+// q = H p (edge by edge), partial p.q.
+__global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
+    const BlockDesc* blocks, const uint8_t* nedge, const int* elab, const double* h,
+    const double* diag, const double* p, double* q, double* part_pq) {
+  __shared__ double red[kCellBlock / 64];
+  const BlockDesc bd = blocks[blockIdx.x];
+  double pq = 0.0;
+  if ((int)threadIdx.x < bd.count) {
+    const int i = bd.first + threadIdx.x;
+    const double pi = p[i];
+    double s = diag[i] * pi;
+    const int nv = nedge[i];
+    for (int k = 0; k < nv; ++k) {
+      const int j = elab[(size_t)i * kMaxVerts + k];
+      if (j >= 0) s -= h[(size_t)i * kMaxVerts + k] * p[j];
+    }
+    q[i] = s;
+    pq = pi * s;
+  }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = pq;
+}
+
+// alpha = (r.z) / (p.q); x += alpha p; r -= alpha q; z = r / diag; partial r.z (new), r.r.
+__global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
+    const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_pq,
+    const double* diag, const double* p, const double* q, double* x, double* r, double* z,
+    double* part_rz_new, double* part_rr) {
+  __shared__ double red[kCellBlock / 64];
+  const double rz = sum_partials(part_rz, nblk, red);
+  const double pq = sum_partials(part_pq, nblk, red);
+  const double alpha = pq > 0.0 ? rz / pq : 0.0;
+  const BlockDesc bd = blocks[blockIdx.x];
+  double rzn = 0.0, rr = 0.0;
+  if ((int)threadIdx.x < bd.count) {
+    const int i = bd.first + threadIdx.x;
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double dg = diag[i];
+    const double zi = dg > 0.0 ? ri / dg : 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    rzn = ri * zi;
+    rr = ri * ri;
+  }
+  rzn = block_sum(rzn, red);
+  rr = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    part_rz_new[blockIdx.x] = rzn;
+    part_rr[blockIdx.x] = rr;
+  }
+}
+
+// w += step[prob] * d for the problems with active[prob] != 0.
+__global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* blocks,
+                                                               const uint8_t* active,
+                                                               const double* step,
+                                                               const double* d, double* w) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (!active[bd.prob] || (int)threadIdx.x >= bd.count) return;
+  const int i = bd.first + threadIdx.x;
+  w[i] += step[bd.prob] * d[i];
+}
+
+hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
+                          const double2* xy, const double2* vtx, const uint8_t* nedge,
+                          const int* elab, const double4* pre, double* h, double* diag, int nblk,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(hessian_kernel, dim3(nblk), dim3(256), 0, st, probs, blocks, active, xy, vtx,
+                     nedge, elab, pre, h, diag);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
+                          const double* shift, const double* diag, double* x, double* r,
+                          double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(cg_init_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, g, shift,
+                     diag, x, r, z, part_rz, part_rr);
+  return hipGetLastError();
+}
+
+// One CG iteration: direction, H p, update.  part_rz[cur] holds r.z of the current residual;
+// the update writes the new r.z into part_rz[1 - cur].
+hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+                               const double* h, const double* diag, double* x, double* r,
+                               double* z, double* p, double* q, double* part_rz_cur,
+                               double* part_rz_other, double* part_pq, double* part_rr, int first,
+                               int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(cg_direction_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
+                     part_rz_cur, part_rz_other, first, z, p);
+  hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, elab, h,
+                     diag, p, q, part_pq);
+  hipLaunchKernelGGL(cg_update_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
+                     part_rz_cur, part_pq, diag, p, q, x, r, z, part_rz_other, part_rr);
+  return hipGetLastError();
+}
+
+hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+                                const double* h, const double* diag, const double* p, double* q,
+                                double* part_pq, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, elab, h,
+                     diag, p, q, part_pq);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
+                            const double* d, double* w, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_step_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, step,
+                     d, w);
+  return hipGetLastError();
+}
+
+}  // namespace cot

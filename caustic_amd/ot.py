@@ -130,6 +130,14 @@ class OTBatch:
                                         ex.ctypes.data_as(_up)))
         return cen, ex.astype(bool)
 
+    def hessian_product(self, w, x):
+        """H x with H = d mass / d w at w (the Newton steps' matrix)."""
+        w = self._w(w)
+        x = self._w(x)
+        y = np.zeros(self.n_total)
+        L.check(self._lib.cot_hessian_product(self._ctx, _ptr(w), _ptr(x), _ptr(y)))
+        return y
+
     def cell_areas(self, w=None):
         w = self._w(w)
         a = np.zeros(self.n_total)
@@ -148,12 +156,41 @@ class OTBatch:
     def solve(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop=True,
-              accept_rule: str = "auto", repair_rounds: int = 3):
+              accept_rule: str = "auto", repair_rounds: int = 3, newton_from: float = None,
+              newton_max_iter: int = 50):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
         trial whose directional derivative is <= 0 -- for FP32 geometry, whose f is too coarse
-        for the last decades), or "auto" (derivative for mixed precision, decrease for fp64)."""
+        for the last decades), or "auto" (derivative for mixed precision, decrease for fp64).
+        newton_from (stop="maxrel" only): L-BFGS until max-rel newton_from, then damped Newton
+        steps (cot_solve_newton) to tol; problems the Newton steps leave unconverged finish
+        with L-BFGS.  Reports then add "newton_iterations"."""
+        kw = dict(history=history, max_iter=max_iter, initial_step=initial_step, stop=stop,
+                  h0_threshold=h0_threshold, verbose=verbose, device_loop=device_loop,
+                  accept_rule=accept_rule, repair_rounds=repair_rounds)
+        if newton_from is not None and stop == "maxrel" and newton_from > tol:
+            w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False, **kw)
+            w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds)
+            if all(r["converged"] for r in r2):
+                w3, r3 = w2, None
+            else:
+                w3, r3 = self.solve(w2, tol=tol, raise_on_fail=raise_on_fail, **kw)
+            out = []
+            for k in range(self.n_problems):
+                last = r3[k] if r3 is not None else r2[k]
+                rep = dict(last)
+                rep["iterations"] = r1[k]["iterations"] + r2[k]["iterations"] + (
+                    r3[k]["iterations"] if r3 is not None else 0)
+                rep["evaluations"] = r1[k]["evaluations"] + r2[k]["evaluations"] + (
+                    r3[k]["evaluations"] if r3 is not None else 0)
+                rep["newton_iterations"] = r2[k]["iterations"]
+                rep["seconds"] = r1[k]["seconds"] + r2[k]["seconds"] + (
+                    r3[k]["seconds"] if r3 is not None else 0.0)
+                out.append(rep)
+            if raise_on_fail and not all(r["converged"] for r in out):
+                raise RuntimeError("solve did not converge")
+            return w3, out
         if accept_rule == "auto":
             accept_rule = "derivative" if self.precision == "mixed" else "decrease"
         w = self._w(w0).copy()
@@ -177,6 +214,21 @@ class OTBatch:
             L.check(rc, allow=(L.COT_ENOTCONV,))
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
+
+    def _newton(self, w0, tol, max_iter, verbose, repair_rounds):
+        """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol."""
+        w = self._w(w0).copy()
+        o = L.cot_solve_opts()
+        self._lib.cot_default_solve_opts(ctypes.byref(o))
+        o.tol = tol
+        o.max_iter = max_iter
+        o.stop_rule = L.COT_STOP_MAXREL
+        o.verbose = verbose
+        o.repair_rounds = repair_rounds
+        reps = (L.cot_report * self.n_problems)()
+        rc = self._lib.cot_solve_newton(self._ctx, _ptr(w), ctypes.byref(o), reps)
+        L.check(rc, allow=(L.COT_ENOTCONV,))
+        return w, [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
 
     def solve_reflection(self, w0, lens, history: int = 7, max_iter: int = 50001,
                          tol: float = 1e-2, initial_step: float = 1.0,

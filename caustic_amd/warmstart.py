@@ -240,26 +240,36 @@ def features(rho: np.ndarray, S: int = S_DEFAULT) -> np.ndarray:
     return np.log(np.maximum(d, 1e-6)).astype(np.float32)
 
 
-def grid_nodes(S: int) -> np.ndarray:
-    """Cell centres of an S x S grid of the unit square, in site order (row-major, y first)."""
+def grid_nodes(S: int, frame=W.UNIT_DOMAIN) -> np.ndarray:
+    """Cell centres of an S x S grid of the frame (x0, x1, y0, y1), in site order (row-major,
+    y first)."""
     k = np.arange(S * S)
-    return np.stack([(k % S + 0.5) / S, (k // S + 0.5) / S], axis=1)
+    x0, x1, y0, y1 = frame
+    return np.stack([x0 + (x1 - x0) * (k % S + 0.5) / S, y0 + (y1 - y0) * (k // S + 0.5) / S],
+                    axis=1)
 
 
-def field_from_solution(sites: np.ndarray, w: np.ndarray, cen: np.ndarray, S: int) -> np.ndarray:
-    """The weight field U (w_i = -U(s_i)) sampled at the S x S grid nodes, for a jittered grid
-    whose site k sits near node k.  With psi the Brenier potential, w(s) = |s|^2 - 2 psi(s), so
+def to_frame(xy: np.ndarray, frame=W.UNIT_DOMAIN) -> np.ndarray:
+    """Coordinates in the frame's unit square."""
+    x0, x1, y0, y1 = frame
+    return np.stack([(xy[:, 0] - x0) / (x1 - x0), (xy[:, 1] - y0) / (y1 - y0)], axis=1)
+
+
+def field_from_solution(sites: np.ndarray, w: np.ndarray, cen: np.ndarray, S: int,
+                        frame=W.UNIT_DOMAIN) -> np.ndarray:
+    """The weight field U (w_i = -U(s_i)) sampled at the S x S grid nodes of the frame (the
+    region the sites sample, Problem.frame), for a jittered grid whose site k sits near node k.  With psi the Brenier potential, w(s) = |s|^2 - 2 psi(s), so
     U(s) = 2 psi(s) - |s|^2 is smooth with gradient -2 (s - c(s)), c(s) the centroid of the cell
     of s; hence U(node_k) = -w_k + 2 (s_k - c_k).(s_k - node_k) to first order.  The mean is
     removed (the gauge)."""
-    nodes = grid_nodes(S)
+    nodes = grid_nodes(S, frame)
     u = -np.asarray(w, dtype=np.float64) + 2.0 * np.sum((sites - cen) * (sites - nodes), axis=1)
     return (u - u.mean()).reshape(S, S)
 
 
-def weights_from_field(U: np.ndarray, sites: np.ndarray) -> np.ndarray:
-    """w_i = -U(s_i), bilinear in the field, mean removed."""
-    w = -resample_field(np.asarray(U, dtype=np.float64), sites)
+def weights_from_field(U: np.ndarray, sites: np.ndarray, frame=W.UNIT_DOMAIN) -> np.ndarray:
+    """w_i = -U(s_i), bilinear in the field given on the frame's grid nodes, mean removed."""
+    w = -resample_field(np.asarray(U, dtype=np.float64), to_frame(sites, frame))
     return w - w.mean()
 
 
@@ -309,26 +319,30 @@ def predict_weights(net: WarmStartNet, problems) -> list:
 # ---------------------------------------------------------------------------------------------
 
 def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
-    """The problem on an n_side x n_side jittered grid with the target box-downsampled by the
-    largest factor that divides the image side and keeps at least the fine problem's pixels per
-    site and 32 x 32 pixels."""
+    """The problem on an n_side x n_side jittered grid of the fine problem's site frame (the
+    region its sites sample), with the target box-downsampled by the largest factor that divides
+    the image side and keeps at least the fine problem's pixels per site and 32 x 32 pixels."""
     fine_side = int(round(np.sqrt(p.n)))
     side = p.rho.shape[0]
     f = max(1, min(fine_side // max(n_side, 1), side // 32))
     while side % f:
         f -= 1
     rho = W.box_downsample(p.rho, side // f) if f > 1 else p.rho
-    sites = W.jittered_grid(n_side, W.rng_for(seed))
-    return W.make_problem(sites, rho / rho.sum(), p.tol, p.name + f"_c{n_side}", seed)
+    x0, x1, y0, y1 = p.frame
+    sites = W.jittered_grid(n_side, W.rng_for(seed)) * [x1 - x0, y1 - y0] + [x0, y0]
+    cp = W.make_problem(sites, rho / rho.sum(), p.tol, p.name + f"_c{n_side}", seed)
+    cp.site_box = p.site_box
+    return cp
 
 
-def _solve_level(problems, init, tol, precision, max_iter, tuning=None):
+def _solve_level(problems, init, tol, precision, max_iter, tuning=None, verbose=0,
+                 newton_from=None):
     from .ot import OTBatch
 
     with OTBatch(list(problems), precision=precision, tuning=tuning) as b:
         w, rep = b.solve(None if init is None else np.concatenate(init), history=10, tol=tol,
                          stop="maxrel", h0_threshold=0.0, max_iter=max_iter,
-                         raise_on_fail=False)
+                         raise_on_fail=False, verbose=verbose, newton_from=newton_from)
         cen, _ = b.centroids(w, weighted=True)
         ws = [x.copy() for x in b.split(w)]
         cs = [c.copy() for c in b.split(cen.reshape(-1, 2))]
@@ -343,32 +357,35 @@ def coarse_problems(problems, levels):
 
 def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
                      coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print,
-                     tunings=None):
+                     tunings=None, verbose=0, newton_from=None):
     """Solve every problem to max-rel `tol`, one batch per level: coarse grids first (to
     `coarse_tol`), each level's weights carried to the next level's sites through the weight
     field (field_from_solution / weights_from_field).  fields: optional per-problem fields that
     start the first level (a learned start); levels=() with fields starts the final level from
     them.  tunings: optional launch shapes (caustic_amd.autotune) keyed by level side or
-    "final".  Returns (weights, centroids, reports) of the final level."""
+    "final".  verbose > 0: the solver prints a progress line every 2 s.  newton_from: every
+    level runs L-BFGS to this max-rel, then damped Newton steps (OTBatch.solve).  Returns
+    (weights, centroids, reports) of the final level."""
     tunings = tunings or {}
     probs = list(problems)
     for li, n_side in enumerate(levels):
         cps = [coarse_problem(p, n_side, 7919 * (li + 1) + k) for k, p in enumerate(probs)]
-        init = None if fields is None else [weights_from_field(U, cp.sites)
+        init = None if fields is None else [weights_from_field(U, cp.sites, cp.frame)
                                             for U, cp in zip(fields, cps)]
         t0 = time.perf_counter()
         ws, cs, rep = _solve_level(cps, init, coarse_tol, precision, max_iter,
-                                   tunings.get(n_side))
-        fields = [field_from_solution(cp.sites, wk, ck, n_side)
+                                   tunings.get(n_side), verbose, newton_from)
+        fields = [field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                   for cp, wk, ck in zip(cps, ws, cs)]
         log(f"[multiscale] level {n_side}: {len(cps)} problems, "
             f"iters max {max(r['iterations'] for r in rep)} "
             f"mean {np.mean([r['iterations'] for r in rep]):.0f}, "
             f"{time.perf_counter() - t0:.1f}s")
-    init = None if fields is None else [weights_from_field(U, p.sites)
+    init = None if fields is None else [weights_from_field(U, p.sites, p.frame)
                                         for U, p in zip(fields, probs)]
     t0 = time.perf_counter()
-    ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter, tunings.get("final"))
+    ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter, tunings.get("final"),
+                               verbose, newton_from)
     log(f"[multiscale] final: {len(probs)} problems, "
         f"iters max {max(r['iterations'] for r in rep)} "
         f"mean {np.mean([r['iterations'] for r in rep]):.0f}, converged "

@@ -27,6 +27,8 @@ class Problem:
             (``picture / gray_multiple``, ot_op.cuh:81-88,154); W pixels along x
     proj_domain: (x0, x1, y0, y1) of the target plane
     tol:    max relative cell-mass error at which the solve stops
+    site_box: (x0, x1, y0, y1) of the region the sites sample when it is not proj_domain (a
+            source plane seen at a tilt); coarse problems of the warm start sample it too
     """
 
     sites: np.ndarray
@@ -36,6 +38,12 @@ class Problem:
     tol: float = 1e-5
     name: str = ""
     seed: int = 0
+    site_box: tuple = None
+
+    @property
+    def frame(self) -> tuple:
+        """The region the sites sample: site_box, else proj_domain."""
+        return tuple(self.site_box) if self.site_box is not None else tuple(self.proj_domain)
 
     @property
     def n(self) -> int:
@@ -174,3 +182,47 @@ def lens_problem(seed: int = 4000, n_side: int = 16, res: int = 64, tilt: float 
         "area_scalar": 1.0,
     }
     return make_problem(sites, rho, 1e-6, f"lens_n{n}_{res}px", seed), lens
+
+
+def bezier_strokes(res: int, rng: np.random.Generator, count: int = 30,
+                   width_px=(2.0, 6.0), high: float = 1.0, floor: float = 0.01) -> np.ndarray:
+    """`count` cubic Bezier strokes (control points U[0,1]^2, width U[width_px] pixels) at
+    `high` on a `floor` background: every pixel whose centre lies within half the width of the
+    curve (sampled at <= 0.25-pixel steps) is set."""
+    img = np.full((res, res), floor, dtype=np.float64)
+    for _ in range(count):
+        P = rng.uniform(0.0, 1.0, size=(4, 2)) * res
+        r = 0.5 * rng.uniform(*width_px)
+        length = (np.linalg.norm(P[1] - P[0]) + np.linalg.norm(P[2] - P[1]) +
+                  np.linalg.norm(P[3] - P[2]))
+        t = np.linspace(0.0, 1.0, int(np.ceil(4 * length)) + 2)[:, None]
+        C = ((1 - t) ** 3 * P[0] + 3 * (1 - t) ** 2 * t * P[1] + 3 * (1 - t) * t ** 2 * P[2] +
+             t ** 3 * P[3])
+        k = int(np.ceil(r)) + 1
+        base = np.floor(C).astype(np.int64)
+        for dy in range(-k, k + 1):
+            for dx in range(-k, k + 1):
+                px = base[:, 0] + dx
+                py = base[:, 1] + dy
+                ok = (px >= 0) & (px < res) & (py >= 0) & (py < res)
+                d2 = (px + 0.5 - C[:, 0]) ** 2 + (py + 0.5 - C[:, 1]) ** 2
+                hit = ok & (d2 <= r * r)
+                img[py[hit], px[hit]] = high
+    return img / img.sum()
+
+
+def tilted_curves(seed: int = 5000, n_side: int = 512, res: int = 2048,
+                  tilt_deg: float = 30.0) -> Problem:
+    """N = 262,144 sites: a 512x512 jittered grid on a source plane tilted by 30 degrees about
+    the y axis and projected straight onto the target, x -> 1/2 + (x - 1/2) cos(30 deg), so the
+    sites cover only the central 87% of the target's width and the border cells must stretch;
+    2048^2 target of 30 cubic Bezier strokes 2-6 px wide at 1.0 on a 0.01 floor; tol 1e-5."""
+    rng = rng_for(seed)
+    sites = jittered_grid(n_side, rng)
+    c = math.cos(math.radians(tilt_deg))
+    sites[:, 0] = 0.5 + (sites[:, 0] - 0.5) * c
+    rho = bezier_strokes(res, rng)
+    p = make_problem(sites, rho, 1e-5, f"curves30_tilt{tilt_deg:g}_n{n_side * n_side}_{res}px",
+                     seed)
+    p.site_box = (0.5 - 0.5 * c, 0.5 + 0.5 * c, 0.0, 1.0)
+    return p

@@ -135,6 +135,21 @@ typedef struct {
 int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
                          const cot_reflection* refl, cot_report* report);
 
+/* Damped Newton steps from w until max_i |mass_i - nu_i| / nu_i < opt->tol (opt->max_iter Newton
+ * steps at most); w in-out, one report per problem.  No counterpart in the reference, whose
+ * solver is L-BFGS only (ot_op.cuh:289-537): it serves the tail of a max-rel solve, after L-BFGS
+ * has brought every cell to non-empty.  Each step builds the Hessian -- the cell-adjacency
+ * Laplacian h_ij = int over the common edge of rho ds / (2 |s_i - s_j|) -- from the current
+ * cells, solves H d = -g by Jacobi-preconditioned conjugate gradients, and halves the step until
+ * every cell keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).
+ * A problem with an empty or unverified cell, or whose step falls below 1/4096, stops with
+ * status COT_ENOTCONV at its last accepted weights. */
+int cot_solve_newton(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
+
+/* y = H x, H the Hessian of the dual objective at w (d mass_i / d w_j, the matrix the Newton
+ * steps solve with); all arrays in caller order.  No counterpart in the reference. */
+int cot_hessian_product(cot_context* ctx, const double* w, const double* x, double* y);
+
 /* Centroids of the cells at weights w: weighted != 0 -> rho-weighted (geometric fallback),
  * weighted == 0 -> geometric centroid of the domain-clipped cell; exist per site. */
 int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centroid,

@@ -1,0 +1,71 @@
+"""The Newton path: the device Hessian against finite differences of the masses, and damped
+Newton solves confirmed by the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+from caustic_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(problems, **kw):
+    from caustic_amd.ot import OTBatch
+
+    return OTBatch(problems, **kw)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return W.plain_fp64()
+
+
+def test_hessian_matches_finite_differences(gpu_available, c1):
+    """H x against (mass(w + e x) - mass(w - e x)) / 2e in FP64; H is symmetric and annihilates
+    the constant vector (the gauge)."""
+    print("[case] Hessian vs central differences, c1 fp64", flush=True)
+    rng = np.random.default_rng(9)
+    w = rng.normal(0.0, 2e-5, c1.n)
+    x = rng.normal(0.0, 1.0, c1.n)
+    y = rng.normal(0.0, 1.0, c1.n)
+    eps = 1e-10
+    with _batch([c1], precision="fp64") as b:
+        hx = b.hessian_product(w, x)
+        hy = b.hessian_product(w, y)
+        h1 = b.hessian_product(w, np.ones(c1.n))
+        mp = b.evaluate(w + eps * x)["mass"]
+        mm = b.evaluate(w - eps * x)["mass"]
+    fd = (mp - mm) / (2 * eps)
+    scale = np.abs(hx).max()
+    assert scale > 0
+    assert np.abs(hx - fd).max() <= 1e-5 * scale, np.abs(hx - fd).max() / scale
+    assert abs(x @ hy - y @ hx) <= 1e-10 * np.abs(x).sum() * scale
+    assert np.abs(h1).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-5)])
+def test_newton_solve_c1(gpu_available, c1, precision, tol):
+    """L-BFGS to max-rel 1e-2, then damped Newton steps to tol; the oracle confirms the masses
+    (FP32 geometry: to within 1e-6 relative)."""
+    print(f"[case] Newton solve c1 {precision} to {tol}", flush=True)
+    with _batch([c1], precision=precision) as b:
+        w, rep = b.solve(tol=tol, h0_threshold=0.0, newton_from=1e-2, verbose=1)
+    assert rep[0]["converged"], rep
+    assert 1 <= rep[0]["newton_iterations"] <= 12, rep
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    slack = 1e-9 if precision == "fp64" else 1e-6
+    assert O.max_rel_error(orc.mass, c1.nu) < tol + slack
+
+
+def test_newton_batch_of_uneven_problems(gpu_available):
+    """One CG system for the whole batch; per-problem damping and stopping."""
+    print("[case] Newton batched solve of 3 uneven problems", flush=True)
+    probs = [W.plain_fp64(seed=21, n_side=24, res=96), W.plain_fp64(seed=22, n_side=48, res=192),
+             W.plain_fp64(seed=23, n_side=32, res=128)]
+    with _batch(probs, precision="fp64") as b:
+        w, rep = b.solve(tol=1e-7, h0_threshold=0.0, newton_from=1e-2)
+        ws = b.split(w)
+    for p, wk, r in zip(probs, ws, rep):
+        assert r["converged"] and r["newton_iterations"] >= 1, r
+        orc = O.evaluate(p.sites, wk, p.nu, p.rho, p.proj_domain)
+        assert O.max_rel_error(orc.mass, p.nu) < 1e-7 + 1e-9

@@ -150,6 +150,41 @@ def test_eval_large_shapes_full_size(gpu_available):
         _compare(ev, orc, p.nu, tol, 1e-8 if prec == "fp64" else 1e-6)
 
 
+@pytest.fixture(scope="module")
+def c5():
+    return W.tilted_curves()
+
+
+def test_eval_tilted_curves_full_size(gpu_available, c5):
+    """N = 262,144 sites on a 30-degree tilted source plane, 2048^2 target of Bezier strokes."""
+    print("[case] eval tilted curves N=262144 2048^2", flush=True)
+    rng = np.random.default_rng(6)
+    w = rng.normal(0.0, 1e-6, c5.n)
+    orc = _exact(c5, w)
+    assert (~orc.exist.astype(bool)).sum() > 0  # hidden cells at this weight spread
+    for prec, tol in (("fp64", 1e-9), ("mixed", 1e-6)):
+        with _batch([c5], precision=prec) as b:
+            ev = b.evaluate(w)
+        _compare(ev, orc, c5.nu, tol, 1e-8 if prec == "fp64" else 1e-6)
+        # mixed: the image is held in FP32, whose rounding of the two grey levels shifts the
+        # total by up to ~3e-8
+        assert abs(ev["mass"].sum() - 1.0) < (1e-10 if prec == "fp64" else 1e-7)
+
+
+def test_solve_tilted_curves_coarse_to_fine(gpu_available, c5):
+    """The full N = 262,144 solve to max-rel 1e-5 in mixed precision from the coarse-to-fine
+    start (64, 128, 256 levels), each level L-BFGS to 1e-2 and damped Newton steps after; the
+    FP64 oracle confirms the masses at the returned weights."""
+    from caustic_amd import warmstart as WS
+
+    print("[case] solve tilted curves N=262144 coarse-to-fine", flush=True)
+    ws, _, rep = WS.multiscale_solve([c5], levels=(64, 128, 256), tol=1e-5, max_iter=20000,
+                                     log=lambda s: print(s, flush=True), verbose=1,
+                                     newton_from=1e-2)
+    assert rep[0]["converged"], rep
+    orc = O.evaluate(c5.sites, ws[0], c5.nu, c5.rho, c5.proj_domain)
+    assert O.max_rel_error(orc.mass, c5.nu) < 1e-5 + 1e-6
+
 
 @pytest.mark.parametrize("int_lanes,repair_blocks,spb", [(4, 128, 0.5), (16, 1024, 2.0)])
 def test_launch_shapes_give_the_same_cells(gpu_available, c1, int_lanes, repair_blocks, spb):

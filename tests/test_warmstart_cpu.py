@@ -97,3 +97,17 @@ def test_coarse_problem_keeps_pixels_per_site_and_mass():
     cp = WS.coarse_problem(p, 32, seed=9)
     assert cp.n == 1024 and cp.rho.shape == (128, 128)
     assert abs(cp.rho.sum() - 1.0) < 1e-12 and abs(cp.nu.sum() - 1.0) < 1e-12
+
+
+def test_field_round_trip_in_a_frame():
+    """Sites sampling a sub-rectangle (a tilted source plane): the field lives on the frame's
+    grid nodes, and exact node weights come back from it."""
+    frame = (0.1, 0.8, 0.2, 1.0)
+    S = 8
+    nodes = WS.grid_nodes(S, frame)
+    assert nodes[:, 0].min() > 0.1 and nodes[:, 0].max() < 0.8
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=S * S)
+    cen = rng.uniform(0, 1, (S * S, 2))
+    U = WS.field_from_solution(nodes, w, cen, S, frame)
+    np.testing.assert_allclose(WS.weights_from_field(U, nodes, frame), w - w.mean(), atol=1e-12)
