@@ -6,16 +6,17 @@ of 40 random disks and rectangles at 100:1 contrast, uniform source masses 1/N, 
 max_i |mass_i - nu_i| / nu_i < 1e-5.
 
 --workload curves262k: N = 262,144 sites on a 30-degree tilted source plane, 2048x2048 target
-of 30 cubic Bezier strokes; coarse levels 64, 128 and 256.
+of 30 cubic Bezier strokes.
 
 --workload batch64: 64 uneven problems (N_k from 2,048 to 32,768, 512x512 Gaussian-mixture
 targets) solved together in one batched context, each to max-rel 1e-5.
 
 One step = complete solves from nothing but the problems, inputs resident in HBM: the warm start
-(coarse-to-fine: each problem on coarse jittered grids -- 64x64 and 128x128 for shapes65k, 32x32
-for batch64 -- with box-downsampled targets, solved to 1e-3, each solution carried up through the
-weight field; or the network's field, --start learned), then the full-resolution solve.  Each
-level runs L-BFGS until max-rel 1e-2 and damped Newton steps after (--newton-from).  Every
+(coarse-to-fine: each problem on coarse jittered grids -- 16, 32, 64 and 128 sites a side for
+shapes65k, up to 256 for curves262k, 16 and 32 for batch64 -- with box-downsampled targets,
+solved to 1e-3, each solution carried up through the weight field; or the network's field,
+--start learned), then the full-resolution solve.  Each level runs L-BFGS until max-rel 1 and
+damped Newton steps after (--newton-from).  Every
 level's work is inside the timed region.  The plain solve from zero weights (the reference's
 start) is timed once, outside the timed region, for the speedup.  Launch shapes come from the
 autotuner (caustic_amd.autotune), run before the timed region.
@@ -38,20 +39,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "shapes65k": {
-        "levels": (64, 128),
+        "levels": (16, 32, 64, 128),
         "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
                 "(100:1), L-BFGS m=10 to max-rel 1e-5, mixed precision",
         "metric": "solves/s to 1e-5 cell-mass err at N=65k",
     },
     "curves262k": {
-        "levels": (64, 128, 256),
+        "levels": (16, 32, 64, 128, 256),
         "desc": "single large solve: N=262144 jittered sites on a 30-degree tilted plane, "
                 "2048x2048 target of 30 Bezier strokes (100:1), L-BFGS m=10 to max-rel 1e-5, "
                 "mixed precision",
         "metric": "solves/s to 1e-5 cell-mass err at N=262k",
     },
     "batch64": {
-        "levels": (32,),
+        "levels": (16, 32),
         "desc": "64 uneven problems in one batch: N_k in [2048, 32768] jittered sites, 512x512 "
                 "Gaussian-mixture targets, L-BFGS m=10 to max-rel 1e-5 each, mixed precision",
         "metric": "solves/s to 1e-5 cell-mass err, 64 uneven problems batched",
@@ -101,7 +102,10 @@ def main():
                     help="eager: the same device rounds launched without graph capture")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of the autotuner's")
-    ap.add_argument("--newton-from", type=float, default=1e-2,
+    ap.add_argument("--levels", default=None,
+                    help="comma-separated coarse grid sides of the warm start (default: the "
+                         "workload's)")
+    ap.add_argument("--newton-from", type=float, default=1.0,
                     help="max-rel at which each level hands over from L-BFGS to damped Newton "
                          "steps (0: L-BFGS to the end)")
     args = ap.parse_args()
@@ -130,12 +134,14 @@ def main():
     kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
               device_loop=loop, verbose=1, newton_from=args.newton_from or None)
 
-    levels = {"multiscale": wl["levels"], "learned": (wl["levels"][-1],), "zero": ()}[args.start]
+    ms_levels = tuple(int(v) for v in args.levels.split(",")) if args.levels else wl["levels"]
+    levels = {"multiscale": ms_levels, "learned": (ms_levels[-1],), "zero": ()}[args.start]
     coarse = WS.coarse_problems(probs, levels)
     tuning = {"final": None, "levels": [None] * len(levels)}
 
-    def start_weights():
-        """The warm start's weights for the full-resolution problems and its evaluations."""
+    def start_weights(record=None):
+        """The warm start's weights for the full-resolution problems and its evaluations;
+        record: a list that receives each coarse level's starting weights (None: zeros)."""
         if args.start == "zero":
             return None, 0
         fields = None if args.start == "multiscale" else WS.predict_fields(net, probs)
@@ -143,6 +149,8 @@ def main():
         for li, (n_side, cps) in enumerate(zip(levels, coarse)):
             init = None if fields is None else np.concatenate(
                 [WS.weights_from_field(U, cp.sites, cp.frame) for U, cp in zip(fields, cps)])
+            if record is not None:
+                record.append(init)
             with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
                 wc, rc = cb.solve(init, **dict(kw, tol=1e-3))
                 cen, _ = cb.centroids(wc)
@@ -161,15 +169,16 @@ def main():
 
     tune_report = None
     if not args.no_tune:
-        # launch shapes per level: the coarse levels measured from zero weights, the final
-        # level from the warm start's weights (outside the timed region)
+        # launch shapes per level, each measured from the weights the warm start hands that
+        # level (a pass with the default shapes), outside the timed region
         from caustic_amd import autotune as AT
 
         tune_report = {}
+        inits = []
+        w_probe, _ = start_weights(record=inits)
         for li, cps in enumerate(coarse):
-            tuning["levels"][li], r = AT.tune(cps, "mixed", iters=40, log=log)
+            tuning["levels"][li], r = AT.tune(cps, "mixed", w0=inits[li], iters=40, log=log)
             tune_report[f"level{levels[li]}"] = r
-        w_probe, _ = start_weights()
         tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, iters=40, log=log)
         tune_report["final"] = r
         log(f"[autotune] {json.dumps(tune_report)}")

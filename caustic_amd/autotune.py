@@ -10,9 +10,10 @@ Knobs (fields of cot_options):
   repair_blocks     grid of the repair passes that walk the work list of flagged sites
                     (128 ... 1024 blocks: short lists want few blocks, long ones one per CU or
                     more).
-  sites_per_bucket  bucket-grid density (0.5, 1, 2).  Sites are stored in 8x8-bucket tiles, so
-                    this also sets how compact the patch of plane under one 128-site block is,
-                    and the size of the fallback candidate rings.
+  sites_per_bucket  bucket-grid density (1, 2).  Sites are stored in 8x8-bucket tiles, so this
+                    also sets how compact the patch of plane under one 128-site block is, and the
+                    size of the fallback candidate rings (half a site per bucket has cost 50x
+                    per evaluation where cells are far from their sites, and is not tried).
 
 Objective: device time per full evaluation -- construction, integration and repair kernels, read
 from the in-kernel timers (cot_get_stats) -- over a short stretch of the real solve: `iters`
@@ -32,7 +33,7 @@ import numpy as np
 KNOBS = {
     "int_lanes": (4, 8, 16),
     "repair_blocks": (128, 256, 512, 1024),
-    "sites_per_bucket": (0.5, 1.0, 2.0),
+    "sites_per_bucket": (1.0, 2.0),
 }
 DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0}
 MIN_GAIN = 0.02

@@ -1306,9 +1306,33 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
   HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
   std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
-  // CG stops at |r| <= kEta |rhs| (an inexact Newton step) or after kMaxCG iterations
-  constexpr double kEta = 1e-3;
+  // CG stops at |r| <= eta |rhs| -- an inexact Newton step, eta = 0.1 max-rel clamped to
+  // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
+  // is launched directly; the rest run in chunks of kCheckCG iterations (a captured graph unless
+  // device_loop == 2), with the residual checked after each chunk.
   constexpr int kMaxCG = 4000, kCheckCG = 16;
+  const bool eager = o.device_loop == 2;
+  hipGraph_t cg_graph = nullptr;
+  hipGraphExec_t cg_exec = nullptr;
+  struct CgGraphGuard {
+    hipGraph_t* g;
+    hipGraphExec_t* e;
+    ~CgGraphGuard() {
+      if (*e) (void)hipGraphExecDestroy(*e);
+      if (*g) (void)hipGraphDestroy(*g);
+    }
+  } cg_guard{&cg_graph, &cg_exec};
+  // iterations k0 .. k0 + kCheckCG - 1 (k0 odd: buffers alternate with k's parity)
+  auto cg_chunk = [&]() -> int {
+    for (int k = 1; k <= kCheckCG; ++k) {
+      double* cur_rz = c->d_prz + (size_t)(k & 1) * nb;
+      double* oth_rz = c->d_prz + (size_t)((k + 1) & 1) * nb;
+      HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
+                                 c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz, oth_rz,
+                                 c->d_ppq, c->d_prr, 0, nb, st));
+    }
+    return COT_OK;
+  };
   auto sum_rr = [&](double& rr) -> int {
     HIPCHK(hipMemcpyAsync(rr_part.data(), c->d_prr, sizeof(double) * nb, hipMemcpyDeviceToHost,
                           st));
@@ -1342,22 +1366,32 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
                           c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, st));
     HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nx,
                           c->d_nr, c->d_nz, c->d_prz, c->d_prr, nb, st));
-    double rr0 = 0.0, rr = 0.0;
+    double rr0 = 0.0, rr = 0.0, worst = 0.0;
+    for (int p = 0; p < np; ++p)
+      if (act[p]) worst = std::max(worst, cur[p].maxrel);
+    const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
     CHK(sum_rr(rr0));
-    int cg = 0;
-    for (; cg < kMaxCG; ++cg) {
-      double* cur_rz = c->d_prz + (size_t)(cg & 1) * nb;
-      double* oth_rz = c->d_prz + (size_t)((cg + 1) & 1) * nb;
-      HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
-                                 c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz, oth_rz,
-                                 c->d_ppq, c->d_prr, cg == 0 ? 1 : 0, nb, st));
-      if ((cg + 1) % kCheckCG == 0) {
-        CHK(sum_rr(rr));
-        if (!(rr > kEta * kEta * rr0)) {
-          ++cg;
-          break;
+    HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_nx,
+                               c->d_nr, c->d_nz, c->d_np, c->d_nq, c->d_prz, c->d_prz + nb,
+                               c->d_ppq, c->d_prr, 1, nb, st));
+    int cg = 1;
+    while (cg < kMaxCG) {
+      if (eager) {
+        CHK(cg_chunk());
+      } else {
+        if (!cg_exec) {
+          HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          const int rc = cg_chunk();
+          const hipError_t ce = hipStreamEndCapture(st, &cg_graph);
+          if (rc != COT_OK) return rc;
+          HIPCHK(ce);
+          HIPCHK(hipGraphInstantiate(&cg_exec, cg_graph, nullptr, nullptr, 0));
         }
+        HIPCHK(hipGraphLaunch(cg_exec, st));
       }
+      cg += kCheckCG;
+      CHK(sum_rr(rr));
+      if (!(rr > eta * eta * rr0)) break;
     }
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
     std::vector<uint8_t> tr = act, acc(np, 0);
@@ -1377,6 +1411,11 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
         ++evals[p];
         const bool ok = trial[p].fails == 0 && trial[p].overflow == 0 && trial[p].zeros == 0 &&
                         std::sqrt(trial[p].g2) <= (1.0 - 0.5 * t[p]) * std::sqrt(cur[p].g2);
+        if (o.verbose > 1 && !ok)
+          std::printf("[cot] newton prob %d rejects step %g: unverified %d overflow %d empty %g "
+                      "|g| ratio %.6f\n",
+                      p, t[p], trial[p].fails, trial[p].overflow, trial[p].zeros,
+                      std::sqrt(trial[p].g2 / std::max(cur[p].g2, 1e-300)));
         if (ok) {
           acc[p] = 1;
           tr[p] = 0;
@@ -1396,6 +1435,9 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice, st));
     HIPCHK(launch_axpy_step(c->d_blocks, c->d_active, c->d_step, c->d_nx, c->d_w, nb, st));
     HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+    // the pinned staging (h_active, h_small) is rewritten at the top of the next iteration:
+    // the copies above must have been made
+    HIPCHK(hipStreamSynchronize(st));
     for (int p = 0; p < np; ++p) iters[p] += acc[p];
     if (o.verbose > 0) {
       for (int p = 0; p < np; ++p)

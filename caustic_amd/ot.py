@@ -171,7 +171,7 @@ class OTBatch:
                   accept_rule=accept_rule, repair_rounds=repair_rounds)
         if newton_from is not None and stop == "maxrel" and newton_from > tol:
             w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False, **kw)
-            w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds)
+            w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds, device_loop)
             if all(r["converged"] for r in r2):
                 w3, r3 = w2, None
             else:
@@ -215,8 +215,9 @@ class OTBatch:
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
 
-    def _newton(self, w0, tol, max_iter, verbose, repair_rounds):
-        """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol."""
+    def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True):
+        """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol; device_loop == 2
+        launches the conjugate-gradient kernels directly instead of replaying a graph."""
         w = self._w(w0).copy()
         o = L.cot_solve_opts()
         self._lib.cot_default_solve_opts(ctypes.byref(o))
@@ -225,6 +226,7 @@ class OTBatch:
         o.stop_rule = L.COT_STOP_MAXREL
         o.verbose = verbose
         o.repair_rounds = repair_rounds
+        o.device_loop = int(device_loop)
         reps = (L.cot_report * self.n_problems)()
         rc = self._lib.cot_solve_newton(self._ctx, _ptr(w), ctypes.byref(o), reps)
         L.check(rc, allow=(L.COT_ENOTCONV,))

@@ -142,8 +142,10 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * Laplacian h_ij = int over the common edge of rho ds / (2 |s_i - s_j|) -- from the current
  * cells, solves H d = -g by Jacobi-preconditioned conjugate gradients, and halves the step until
  * every cell keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).
- * A problem with an empty or unverified cell, or whose step falls below 1/4096, stops with
- * status COT_ENOTCONV at its last accepted weights. */
+ * The conjugate gradients run to |r| <= eta |rhs|, eta = 0.1 max-rel clamped to [1e-4, 0.1],
+ * replayed from a captured graph in chunks of 16 iterations (launched directly when
+ * opt->device_loop == 2).  A problem with an empty or unverified cell, or whose step falls below
+ * 1/4096, stops with status COT_ENOTCONV at its last accepted weights. */
 int cot_solve_newton(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
 
 /* y = H x, H the Hessian of the dual objective at w (d mass_i / d w_j, the matrix the Newton

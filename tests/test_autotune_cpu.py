@@ -11,7 +11,7 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
         calls.append(dict(cfg))
         t = {4: 3.0, 8: 2.0, 16: 1.0}[cfg["int_lanes"]]
         t += {128: 0.5, 256: 0.3, 512: 0.1, 1024: 0.2}[cfg["repair_blocks"]]
-        t += {0.5: 0.05, 1.0: 0.0, 2.0: 0.3}[cfg["sites_per_bucket"]]
+        t += {1.0: 0.0, 2.0: 0.3}[cfg["sites_per_bucket"]]
         return {"per_eval": t * 1e-4, "evaluations": 10, "wall": 0.0, "cell": 0.0,
                 "integrate": 0.0, "repair": 0.0}
 
