@@ -34,7 +34,8 @@ class cot_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("sites_per_bucket", ctypes.c_double),
                 ("ring_init", ctypes.c_int), ("ring_max", ctypes.c_int),
                 ("verify_tol", ctypes.c_double), ("max_repairs", ctypes.c_int),
-                ("int_lanes", ctypes.c_int), ("repair_blocks", ctypes.c_int)]
+                ("int_lanes", ctypes.c_int), ("repair_blocks", ctypes.c_int),
+                ("full_pass_ring", ctypes.c_int)]
 
 
 class cot_solve_opts(ctypes.Structure):

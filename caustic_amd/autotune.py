@@ -14,6 +14,10 @@ Knobs (fields of cot_options):
                     also sets how compact the patch of plane under one 128-site block is, and the
                     size of the fallback candidate rings (half a site per bucket has cost 50x
                     per evaluation where cells are far from their sites, and is not tried).
+  full_pass_ring    candidates of the FP32 full pass: the previous cells' one-ring (1) or
+                    two-ring (2).  The two-ring costs more clipping per cell and leaves fewer
+                    unmatched edges to the repair passes; it pays where weights move far between
+                    evaluations.
 
 Objective: device time per full evaluation -- construction, integration and repair kernels, read
 from the in-kernel timers (cot_get_stats) -- over a short stretch of the real solve: `iters`
@@ -34,8 +38,9 @@ KNOBS = {
     "int_lanes": (4, 8, 16),
     "repair_blocks": (128, 256, 512, 1024),
     "sites_per_bucket": (1.0, 2.0),
+    "full_pass_ring": (1, 2),
 }
-DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0}
+DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 1}
 MIN_GAIN = 0.02
 
 

@@ -356,9 +356,13 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
 // previous evaluation the full pass uses the rings.
 constexpr int kAdjRepairRounds = 2;
 inline int repair_cand(int r) { return r <= kAdjRepairRounds ? kCandAdj2 : kCandBucket; }
-// FP32 geometry's full pass takes the one-ring: an edge a flip creates and the check cannot yet
-// see changes a mass by ~1e-9 relative, far inside FP32's own error; FP64 keeps the two-ring.
-inline int full_pass_cand(const cot_context* c) { return c->fp32 ? kCandAdj1 : kCandAdj2; }
+// FP32 geometry's full pass takes the one-ring by default (full_pass_ring 0 or 1): an edge a
+// flip creates and the check cannot yet see changes a mass by ~1e-9 relative, far inside FP32's
+// own error; full_pass_ring 2 takes the two-ring, which costs more per cell and leaves fewer
+// cells to the repairs where weights move far between evaluations.  FP64 keeps the two-ring.
+inline int full_pass_cand(const cot_context* c) {
+  return (c->fp32 && c->opt.full_pass_ring != 2) ? kCandAdj1 : kCandAdj2;
+}
 
 // Fold the in-kernel timers into the context's statistics and clear their accumulators.
 int pull_timers(cot_context* c) {
@@ -782,6 +786,7 @@ void cot_default_options(cot_options* o) {
   o->max_repairs = 12;
   o->int_lanes = kIntLanes;
   o->repair_blocks = kRepairBlocks;
+  o->full_pass_ring = 0;
 }
 
 void cot_default_solve_opts(cot_solve_opts* o) {
@@ -831,6 +836,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     return bail(fail(COT_EINVAL, "int_lanes must be 4, 8 or 16"));
   if (c->opt.repair_blocks < 1 || c->opt.repair_blocks > 65535)
     return bail(fail(COT_EINVAL, "repair_blocks must be in [1, 65535]"));
+  if (c->opt.full_pass_ring < 0 || c->opt.full_pass_ring > 2)
+    return bail(fail(COT_EINVAL, "full_pass_ring must be 0, 1 or 2"));
   c->fp32 = c->opt.precision == COT_PREC_MIXED;
   c->np = n_problems;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)

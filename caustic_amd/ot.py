@@ -31,13 +31,15 @@ class OTBatch:
     def __init__(self, problems: Sequence, precision: str = "fp64", sites_per_bucket: float = 1.0,
                  ring_init: int = 2, ring_max: int = 32, verify_tol: float = 1e-4,
                  max_repairs: int = 12, int_lanes: int = 8, repair_blocks: int = 256,
-                 tuning: dict = None):
+                 full_pass_ring: int = 0, tuning: dict = None):
         """tuning: launch-shape options as chosen by caustic_amd.autotune (int_lanes,
-        repair_blocks, sites_per_bucket); they override the keyword arguments."""
+        repair_blocks, sites_per_bucket, full_pass_ring); they override the keyword
+        arguments."""
         if tuning:
             int_lanes = int(tuning.get("int_lanes", int_lanes))
             repair_blocks = int(tuning.get("repair_blocks", repair_blocks))
             sites_per_bucket = float(tuning.get("sites_per_bucket", sites_per_bucket))
+            full_pass_ring = int(tuning.get("full_pass_ring", full_pass_ring))
         lib = L.load()
         self._keep = []
         arr = (L.cot_problem * len(problems))()
@@ -67,6 +69,7 @@ class OTBatch:
         opt.max_repairs = max_repairs
         opt.int_lanes = int_lanes
         opt.repair_blocks = repair_blocks
+        opt.full_pass_ring = full_pass_ring
         ctx = ctypes.c_void_p()
         L.check(lib.cot_create(arr, len(problems), ctypes.byref(opt), ctypes.byref(ctx)))
         self._keep = []  # the library copied everything
@@ -74,7 +77,7 @@ class OTBatch:
         self._lib = lib
         self.precision = precision
         self.tuning = {"int_lanes": int_lanes, "repair_blocks": repair_blocks,
-                       "sites_per_bucket": sites_per_bucket}
+                       "sites_per_bucket": sites_per_bucket, "full_pass_ring": full_pass_ring}
         self.n_total = int(sum(self.sizes))
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.n_problems = len(problems)

@@ -59,6 +59,9 @@ typedef struct {
   int max_repairs;          /* conformity repair rounds per evaluation (default 12) */
   int int_lanes;            /* integration lanes per site: 4, 8 or 16 (default 8) */
   int repair_blocks;        /* blocks of the repair passes over the work list (default 256) */
+  int full_pass_ring;       /* candidates of a FP32-geometry full pass: the previous cells'
+                               one-ring (0: default, or 1) or two-ring (2); FP64 always uses
+                               the two-ring */
 } cot_options;
 
 typedef struct {

@@ -12,14 +12,16 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
         t = {4: 3.0, 8: 2.0, 16: 1.0}[cfg["int_lanes"]]
         t += {128: 0.5, 256: 0.3, 512: 0.1, 1024: 0.2}[cfg["repair_blocks"]]
         t += {1.0: 0.0, 2.0: 0.3}[cfg["sites_per_bucket"]]
+        t += {1: 0.2, 2: 0.0}[cfg["full_pass_ring"]]
         return {"per_eval": t * 1e-4, "evaluations": 10, "wall": 0.0, "cell": 0.0,
                 "integrate": 0.0, "repair": 0.0}
 
     monkeypatch.setattr(AT, "measure", cost)
     probs = [W.plain_fp64(n_side=8, res=16)]
     cfg, rep = AT.tune(probs, "mixed")
-    assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0}
-    assert abs(rep["gain_vs_default"] - 2.3 / 1.1) < 1e-9
+    assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0,
+                   "full_pass_ring": 2}
+    assert abs(rep["gain_vs_default"] - 2.5 / 1.1) < 1e-9
     assert len(calls) == len({tuple(sorted(c.items())) for c in calls})  # no repeats
     n = len(calls)
     cfg2, rep2 = AT.tune(probs, "mixed")  # from the cache: nothing measured
