@@ -43,7 +43,7 @@ class cot_solve_opts(ctypes.Structure):
                 ("initial_step", ctypes.c_double), ("stop_rule", ctypes.c_int),
                 ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("device_loop", ctypes.c_int), ("accept_rule", ctypes.c_int),
-                ("repair_rounds", ctypes.c_int)]
+                ("repair_rounds", ctypes.c_int), ("newton_sweeps", ctypes.c_int)]
 
 
 class cot_report(ctypes.Structure):

@@ -88,11 +88,12 @@ hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const 
                           const int* elab, const double4* pre, double* h, double* diag, int nblk,
                           hipStream_t st);
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
-                          const double* shift, const double* diag, double* x, double* r,
+                          const double* shift, const double* diag, const uint8_t* nedge,
+                          const int* elab, const double* h, int sweeps, double* x, double* r,
                           double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st);
 hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
-                               const double* h, const double* diag, double* x, double* r,
-                               double* z, double* p, double* q, double* part_rz_cur,
+                               const double* h, const double* diag, int sweeps, double* x,
+                               double* r, double* z, double* p, double* q, double* part_rz_cur,
                                double* part_rz_other, double* part_pq, double* part_rr, int first,
                                int nblk, hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
@@ -800,6 +801,7 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->device_loop = 1;
   o->accept_rule = 0;
   o->repair_rounds = 3;
+  o->newton_sweeps = 0;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -1319,6 +1321,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   // device_loop == 2), with the residual checked after each chunk.
   constexpr int kMaxCG = 4000, kCheckCG = 16;
   const bool eager = o.device_loop == 2;
+  const int sweeps = std::min(std::max(o.newton_sweeps, 0), 64);
   hipGraph_t cg_graph = nullptr;
   hipGraphExec_t cg_exec = nullptr;
   struct CgGraphGuard {
@@ -1335,8 +1338,8 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       double* cur_rz = c->d_prz + (size_t)(k & 1) * nb;
       double* oth_rz = c->d_prz + (size_t)((k + 1) & 1) * nb;
       HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
-                                 c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz, oth_rz,
-                                 c->d_ppq, c->d_prr, 0, nb, st));
+                                 sweeps, c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz,
+                                 oth_rz, c->d_ppq, c->d_prr, 0, nb, st));
     }
     return COT_OK;
   };
@@ -1371,16 +1374,17 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     // Hessian at the current cells, then H x = -(g - mean g)
     HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
                           c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, st));
-    HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nx,
-                          c->d_nr, c->d_nz, c->d_prz, c->d_prr, nb, st));
+    HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nedge,
+                          c->d_elab, c->d_hh, sweeps, c->d_nx, c->d_nr, c->d_nz, c->d_prz,
+                          c->d_prr, nb, st));
     double rr0 = 0.0, rr = 0.0, worst = 0.0;
     for (int p = 0; p < np; ++p)
       if (act[p]) worst = std::max(worst, cur[p].maxrel);
     const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
     CHK(sum_rr(rr0));
-    HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_nx,
-                               c->d_nr, c->d_nz, c->d_np, c->d_nq, c->d_prz, c->d_prz + nb,
-                               c->d_ppq, c->d_prr, 1, nb, st));
+    HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, sweeps,
+                               c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, c->d_prz,
+                               c->d_prz + nb, c->d_ppq, c->d_prr, 1, nb, st));
     int cg = 1;
     while (cg < kMaxCG) {
       if (eager) {

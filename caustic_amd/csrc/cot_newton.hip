@@ -6,9 +6,9 @@
 // a weighted graph Laplacian (Kitagawa, Merigot and Thibert, "Convergence of a Newton algorithm
 // for semi-discrete optimal transport", JEMS 2019).  hessian_kernel integrates the target along
 // every cell edge of the last evaluation (the edge labels name the neighbour), cg_* kernels solve
-// H d = -g by Jacobi-preconditioned conjugate gradients with H applied edge by edge (the
-// evaluation's edge table is the sparse pattern), and the host driver (cot_api.cpp) damps the
-// step so that every cell keeps mass and |g| decreases.
+// H d = -g by preconditioned conjugate gradients (block-local Jacobi sweeps in LDS) with H
+// applied edge by edge (the evaluation's edge table is the sparse pattern), and the host driver
+// (cot_api.cpp) damps the step so that every cell keeps mass and |g| decreases.
 #include "cot_internal.h"
 
 namespace cot {
@@ -60,6 +60,37 @@ __device__ double edge_rho(double ua, double va, double ub, double vb, const dou
   return acc;
 }
 
+// Preconditioner of the conjugate gradients, applied by a block of kCellBlock threads to the
+// residual r_i of its sites (one per thread): `sweeps` Jacobi sweeps on the block's principal
+// submatrix D - B of H (B: the couplings between two sites of the block),
+//   z_0 = D^-1 r,  z_{k+1} = D^-1 (r + B z_k),
+// i.e. the Neumann polynomial sum_{k<=sweeps} (D^-1 B)^k D^-1 r -- a fixed symmetric positive
+// definite operator (D^-1/2 B D^-1/2 has spectral radius below 1: H is diagonally dominant and a
+// block has neighbours outside it).  sweeps = 0 is Jacobi.  Every thread of the block must call
+// this (it synchronises).
+__device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
+                               const uint8_t* nedge, const int* elab, const double* h,
+                               int sweeps, double* zl) {
+  const int ln = threadIdx.x;
+  const bool own = ln < bd.count && dg > 0.0;
+  double z = own ? ri / dg : 0.0;
+  const int nv = ln < bd.count ? nedge[i] : 0;
+  for (int s = 0; s < sweeps; ++s) {
+    zl[ln] = z;
+    __syncthreads();
+    if (own) {
+      double acc = ri;
+      for (int k = 0; k < nv; ++k) {
+        const int j = elab[(size_t)i * kMaxVerts + k] - bd.first;
+        if (j >= 0 && j < bd.count) acc += h[(size_t)i * kMaxVerts + k] * zl[j];
+      }
+      z = acc / dg;
+    }
+    __syncthreads();
+  }
+  return z;
+}
+
 }  // namespace
 
 // h[site * kMaxVerts + k] = h_ij of edge k of the site's cell (0 for sides and missing edges),
@@ -108,16 +139,21 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
 // H), z = r / diag, partial r.z and r.r.
 __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
     const BlockDesc* blocks, const uint8_t* active, const double* g, const double* shift,
-    const double* diag, double* x, double* r, double* z, double* part_rz, double* part_rr) {
+    const double* diag, const uint8_t* nedge, const int* elab, const double* h, int sweeps,
+    double* x, double* r, double* z, double* part_rz, double* part_rr) {
   __shared__ double red[kCellBlock / 64];
+  __shared__ double zl[kCellBlock];
   const BlockDesc bd = blocks[blockIdx.x];
   const int ln = threadIdx.x;
+  const int i = bd.first + ln;
+  double ri = 0.0, dg = 0.0;
+  if (ln < bd.count) {
+    dg = diag[i];
+    ri = (active[bd.prob] && dg > 0.0) ? shift[bd.prob] - g[i] : 0.0;
+  }
+  const double zi = precondition(bd, i, ri, dg, nedge, elab, h, sweeps, zl);
   double rz = 0.0, rr = 0.0;
   if (ln < bd.count) {
-    const int i = bd.first + ln;
-    const double dg = diag[i];
-    const double ri = (active[bd.prob] && dg > 0.0) ? shift[bd.prob] - g[i] : 0.0;
-    const double zi = dg > 0.0 ? ri / dg : 0.0;
     x[i] = 0.0;
     r[i] = ri;
     z[i] = zi;
@@ -176,20 +212,26 @@ __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
 // alpha = (r.z) / (p.q); x += alpha p; r -= alpha q; z = r / diag; partial r.z (new), r.r.
 __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
     const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_pq,
-    const double* diag, const double* p, const double* q, double* x, double* r, double* z,
-    double* part_rz_new, double* part_rr) {
+    const double* diag, const uint8_t* nedge, const int* elab, const double* h, int sweeps,
+    const double* p, const double* q, double* x, double* r, double* z, double* part_rz_new,
+    double* part_rr) {
   __shared__ double red[kCellBlock / 64];
+  __shared__ double zl[kCellBlock];
   const double rz = sum_partials(part_rz, nblk, red);
   const double pq = sum_partials(part_pq, nblk, red);
   const double alpha = pq > 0.0 ? rz / pq : 0.0;
   const BlockDesc bd = blocks[blockIdx.x];
-  double rzn = 0.0, rr = 0.0;
-  if ((int)threadIdx.x < bd.count) {
-    const int i = bd.first + threadIdx.x;
+  const int ln = threadIdx.x;
+  const int i = bd.first + ln;
+  double ri = 0.0, dg = 0.0;
+  if (ln < bd.count) {
     x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    const double dg = diag[i];
-    const double zi = dg > 0.0 ? ri / dg : 0.0;
+    ri = r[i] - alpha * q[i];
+    dg = diag[i];
+  }
+  const double zi = precondition(bd, i, ri, dg, nedge, elab, h, sweeps, zl);
+  double rzn = 0.0, rr = 0.0;
+  if (ln < bd.count) {
     r[i] = ri;
     z[i] = zi;
     rzn = ri * zi;
@@ -224,18 +266,19 @@ hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const 
 }
 
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
-                          const double* shift, const double* diag, double* x, double* r,
+                          const double* shift, const double* diag, const uint8_t* nedge,
+                          const int* elab, const double* h, int sweeps, double* x, double* r,
                           double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(cg_init_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, g, shift,
-                     diag, x, r, z, part_rz, part_rr);
+                     diag, nedge, elab, h, sweeps, x, r, z, part_rz, part_rr);
   return hipGetLastError();
 }
 
 // One CG iteration: direction, H p, update.  part_rz[cur] holds r.z of the current residual;
 // the update writes the new r.z into part_rz[1 - cur].
 hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
-                               const double* h, const double* diag, double* x, double* r,
-                               double* z, double* p, double* q, double* part_rz_cur,
+                               const double* h, const double* diag, int sweeps, double* x,
+                               double* r, double* z, double* p, double* q, double* part_rz_cur,
                                double* part_rz_other, double* part_pq, double* part_rr, int first,
                                int nblk, hipStream_t st) {
   hipLaunchKernelGGL(cg_direction_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
@@ -243,7 +286,8 @@ hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, co
   hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, elab, h,
                      diag, p, q, part_pq);
   hipLaunchKernelGGL(cg_update_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
-                     part_rz_cur, part_pq, diag, p, q, x, r, z, part_rz_other, part_rr);
+                     part_rz_cur, part_pq, diag, nedge, elab, h, sweeps, p, q, x, r, z,
+                     part_rz_other, part_rr);
   return hipGetLastError();
 }
 

@@ -82,6 +82,10 @@ typedef struct {
                                   <= 0 (the convex objective decreased) -- for FP32 geometry,
                                   whose f is too coarse to resolve the last decades */
   int repair_rounds;        /* conformity repair rounds of a line-search trial (default 3) */
+  int newton_sweeps;        /* cot_solve_newton: block-local Jacobi sweeps of the conjugate
+                               gradients' preconditioner (default 0: Jacobi.  More sweeps
+                               cut the CG iterations ~1.7x at 4 sweeps but cost more per
+                               iteration than they save at N >= 65,536 on MI355X) */
 } cot_solve_opts;
 
 typedef struct {
@@ -143,7 +147,8 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * solver is L-BFGS only (ot_op.cuh:289-537): it serves the tail of a max-rel solve, after L-BFGS
  * has brought every cell to non-empty.  Each step builds the Hessian -- the cell-adjacency
  * Laplacian h_ij = int over the common edge of rho ds / (2 |s_i - s_j|) -- from the current
- * cells, solves H d = -g by Jacobi-preconditioned conjugate gradients, and halves the step until
+ * cells, solves H d = -g by preconditioned conjugate gradients (opt->newton_sweeps Jacobi sweeps
+ * on each 128-site block's submatrix of H), and halves the step until
  * every cell keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).
  * The conjugate gradients run to |r| <= eta |rhs|, eta = 0.1 max-rel clamped to [1e-4, 0.1],
  * replayed from a captured graph in chunks of 16 iterations (launched directly when

@@ -34,7 +34,7 @@ def test_struct_layouts_match_the_header(tmp_path):
         '#include <stdio.h>\n#include <stddef.h>\n#include "caustic_ot.h"\n'
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
         "sizeof(cot_options), sizeof(cot_solve_opts), sizeof(cot_report), sizeof(cot_stats), "
-        "offsetof(cot_solve_opts, repair_rounds), sizeof(cot_reflection), "
+        "offsetof(cot_solve_opts, newton_sweeps), sizeof(cot_reflection), "
         "offsetof(cot_reflection, if_aa_op_initial), offsetof(cot_options, full_pass_ring));"
         "return 0;}\n")
     exe = tmp_path / "sz"
@@ -43,7 +43,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                                           check=True).stdout.split()]
     want = [ctypes.sizeof(L.cot_problem), ctypes.sizeof(L.cot_options),
             ctypes.sizeof(L.cot_solve_opts), ctypes.sizeof(L.cot_report),
-            ctypes.sizeof(L.cot_stats), L.cot_solve_opts.repair_rounds.offset,
+            ctypes.sizeof(L.cot_stats), L.cot_solve_opts.newton_sweeps.offset,
             ctypes.sizeof(L.cot_reflection), L.cot_reflection.if_aa_op_initial.offset,
             L.cot_options.full_pass_ring.offset]
     assert got == want
