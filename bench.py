@@ -37,6 +37,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved per evaluation (construction + integration launch) of the N=65,536 workload,
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each scaled by the same
+# counter's reading for a copy of known size (profiles/shapes65k_eval_traffic_pmc.md)
+TRAFFIC_BYTES = {"shapes65k": 88.98e6}
 WORKLOADS = {
     "shapes65k": {
         "levels": (16, 32, 64, 128),
@@ -253,7 +257,7 @@ def main():
             "kernel": "cell evaluation: cell_kernel (construction) + integrate_kernel (exact "
                       "mass integral), full-batch passes of the timed solves",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(args.workload),
             "avg_launch_us": {"cell_kernel": cell_avg * 1e6, "integrate_kernel": integ_avg * 1e6},
             "launches": st["cell_launches"], "alg_bytes_per_launch": alg_bytes,
         },

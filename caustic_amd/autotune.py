@@ -50,10 +50,20 @@ def cache_path() -> str:
                                        "tuning.json"))
 
 
-def shape_key(problems, precision: str) -> str:
+def device_key() -> str:
+    """The GPU's architecture and compute-unit count (the marketing name is not reported
+    reliably under every tool)."""
     import torch
 
-    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
+    if not torch.cuda.is_available():
+        return "none"
+    pr = torch.cuda.get_device_properties(0)
+    arch = (getattr(pr, "gcnArchName", "") or "gpu").split(":")[0]
+    return f"{arch}-{pr.multi_processor_count}cu"
+
+
+def shape_key(problems, precision: str) -> str:
+    dev = device_key()
     n = sum(int(p.n) for p in problems)
     px = sum(int(p.rho.size) for p in problems)
     return f"{dev}|{precision}|problems={len(problems)}|sites={n}|pixels={px}"
