@@ -93,9 +93,10 @@ hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const 
                           double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st);
 hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
                                const double* h, const double* diag, int sweeps, double* x,
-                               double* r, double* z, double* p, double* q, double* part_rz_cur,
-                               double* part_rz_other, double* part_pq, double* part_rr, int first,
-                               int nblk, hipStream_t st);
+                               double* r, double* z, const double* p_old, double* p_new,
+                               double* q, double* part_rz_cur, double* part_rz_other,
+                               double* part_pq, double* part_rr, int first, int nblk,
+                               hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
                             const double* d, double* w, int nblk, hipStream_t st);
 hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
@@ -203,6 +204,7 @@ struct cot_context {
   // Newton steps: edge Hessian entries, diagonal, CG vectors and partials
   double *d_hh = nullptr, *d_hdiag = nullptr, *d_nx = nullptr, *d_nr = nullptr, *d_nz = nullptr;
   double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
+  double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
   double* d_nshift = nullptr;
   std::vector<double> nu_sum;  // per-problem sum of the source masses
   // device-resident solver
@@ -1262,6 +1264,7 @@ int ensure_newton(cot_context* c) {
   CHK(dalloc(c, &c->d_nr, N));
   CHK(dalloc(c, &c->d_nz, N));
   CHK(dalloc(c, &c->d_np, N));
+  CHK(dalloc(c, &c->d_np2, N));
   CHK(dalloc(c, &c->d_nq, N));
   CHK(dalloc(c, &c->d_prz, 2 * nb));
   CHK(dalloc(c, &c->d_ppq, nb));
@@ -1319,7 +1322,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
   // is launched directly; the rest run in chunks of kCheckCG iterations (a captured graph unless
   // device_loop == 2), with the residual checked after each chunk.
-  constexpr int kMaxCG = 4000, kCheckCG = 16;
+  constexpr int kMaxCG = 4000, kCheckCG = 32;
   const bool eager = o.device_loop == 2;
   const int sweeps = std::min(std::max(o.newton_sweeps, 0), 64);
   hipGraph_t cg_graph = nullptr;
@@ -1332,14 +1335,18 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       if (*g) (void)hipGraphDestroy(*g);
     }
   } cg_guard{&cg_graph, &cg_exec};
-  // iterations k0 .. k0 + kCheckCG - 1 (k0 odd: buffers alternate with k's parity)
+  // iteration k: r.z partials in d_prz[k & 1] (current) and [(k + 1) & 1] (previous), the
+  // direction read from pbuf[k & 1] and written to pbuf[(k + 1) & 1]; a chunk runs iterations
+  // k0 .. k0 + kCheckCG - 1 with k0 odd, so the buffers realign from chunk to chunk
+  double* pbuf[2] = {c->d_np, c->d_np2};
   auto cg_chunk = [&]() -> int {
     for (int k = 1; k <= kCheckCG; ++k) {
       double* cur_rz = c->d_prz + (size_t)(k & 1) * nb;
       double* oth_rz = c->d_prz + (size_t)((k + 1) & 1) * nb;
       HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
-                                 sweeps, c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, cur_rz,
-                                 oth_rz, c->d_ppq, c->d_prr, 0, nb, st));
+                                 sweeps, c->d_nx, c->d_nr, c->d_nz, pbuf[k & 1],
+                                 pbuf[(k + 1) & 1], c->d_nq, cur_rz, oth_rz, c->d_ppq, c->d_prr,
+                                 0, nb, st));
     }
     return COT_OK;
   };
@@ -1383,7 +1390,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
     CHK(sum_rr(rr0));
     HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, sweeps,
-                               c->d_nx, c->d_nr, c->d_nz, c->d_np, c->d_nq, c->d_prz,
+                               c->d_nx, c->d_nr, c->d_nz, pbuf[0], pbuf[1], c->d_nq, c->d_prz,
                                c->d_prz + nb, c->d_ppq, c->d_prr, 1, nb, st));
     int cg = 1;
     while (cg < kMaxCG) {

@@ -168,10 +168,13 @@ __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
   }
 }
 
-// p = z + beta p, beta = (r.z) / (r.z of the previous iteration); first: p = z.
-__global__ __launch_bounds__(kCellBlock) void cg_direction_kernel(
+// One CG direction and product: beta = (r.z) / (r.z of the previous iteration) (0 on the first),
+// p_new = z + beta p_old for the block's sites, q = H p_new with the neighbours' p_new formed on
+// the fly from z and p_old (which no block writes here), partial p.q.
+__global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
     const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rz_prev,
-    int first, const double* z, double* p) {
+    int first, const uint8_t* nedge, const int* elab, const double* h, const double* diag,
+    const double* z, const double* p_old, double* p_new, double* q, double* part_pq) {
   __shared__ double red[kCellBlock / 64];
   double beta = 0.0;
   if (!first) {
@@ -180,10 +183,25 @@ __global__ __launch_bounds__(kCellBlock) void cg_direction_kernel(
     beta = rz0 != 0.0 ? rz / rz0 : 0.0;
   }
   const BlockDesc bd = blocks[blockIdx.x];
+  double pq = 0.0;
   if ((int)threadIdx.x < bd.count) {
     const int i = bd.first + threadIdx.x;
-    p[i] = z[i] + beta * (first ? 0.0 : p[i]);
+    const double pi = first ? z[i] : z[i] + beta * p_old[i];
+    p_new[i] = pi;
+    double s = diag[i] * pi;
+    const int nv = nedge[i];
+    for (int k = 0; k < nv; ++k) {
+      const int j = elab[(size_t)i * kMaxVerts + k];
+      if (j >= 0) {
+        const double pj = first ? z[j] : z[j] + beta * p_old[j];
+        s -= h[(size_t)i * kMaxVerts + k] * pj;
+      }
+    }
+    q[i] = s;
+    pq = pi * s;
   }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = pq;
 }
 
 // q = H p (edge by edge), partial p.q.
@@ -274,19 +292,20 @@ hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const 
   return hipGetLastError();
 }
 
-// One CG iteration: direction, H p, update.  part_rz[cur] holds r.z of the current residual;
-// the update writes the new r.z into part_rz[1 - cur].
+// One CG iteration in two kernels: direction and H p (cg_step), then the update.  part_rz_cur
+// holds r.z of the current residual, part_rz_other the previous one; the update writes the new
+// r.z into part_rz_other.  p_old is the previous direction, p_new receives this one.
 hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
                                const double* h, const double* diag, int sweeps, double* x,
-                               double* r, double* z, double* p, double* q, double* part_rz_cur,
-                               double* part_rz_other, double* part_pq, double* part_rr, int first,
-                               int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(cg_direction_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
-                     part_rz_cur, part_rz_other, first, z, p);
-  hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, elab, h,
-                     diag, p, q, part_pq);
+                               double* r, double* z, const double* p_old, double* p_new,
+                               double* q, double* part_rz_cur, double* part_rz_other,
+                               double* part_pq, double* part_rr, int first, int nblk,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(cg_step_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
+                     part_rz_cur, part_rz_other, first, nedge, elab, h, diag, z, p_old, p_new, q,
+                     part_pq);
   hipLaunchKernelGGL(cg_update_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nblk,
-                     part_rz_cur, part_pq, diag, nedge, elab, h, sweeps, p, q, x, r, z,
+                     part_rz_cur, part_pq, diag, nedge, elab, h, sweeps, p_new, q, x, r, z,
                      part_rz_other, part_rr);
   return hipGetLastError();
 }

@@ -151,7 +151,7 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * on each 128-site block's submatrix of H), and halves the step until
  * every cell keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).
  * The conjugate gradients run to |r| <= eta |rhs|, eta = 0.1 max-rel clamped to [1e-4, 0.1],
- * replayed from a captured graph in chunks of 16 iterations (launched directly when
+ * replayed from a captured graph in chunks of 32 iterations (launched directly when
  * opt->device_loop == 2).  A problem with an empty or unverified cell, or whose step falls below
  * 1/4096, stops with status COT_ENOTCONV at its last accepted weights. */
 int cot_solve_newton(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
