@@ -14,7 +14,8 @@ targets) solved together in one batched context, each to max-rel 1e-5.
 One step = complete solves from nothing but the problems, inputs resident in HBM: the warm start
 (coarse-to-fine: each problem on coarse jittered grids -- 16, 32, 64 and 128 sites a side for
 shapes65k, up to 256 for curves262k, 16 and 32 for batch64 -- with box-downsampled targets,
-solved to 1e-3, each solution carried up through the weight field; or the network's field,
+solved to 1e-3 (--coarse-tol), each solution carried up through the weight field; or the
+network's field,
 --start learned), then the full-resolution solve.  Each level runs L-BFGS until max-rel 1 and
 damped Newton steps after (--newton-from).  Every
 level's work is inside the timed region.  The plain solve from zero weights (the reference's
@@ -109,6 +110,8 @@ def main():
     ap.add_argument("--levels", default=None,
                     help="comma-separated coarse grid sides of the warm start (default: the "
                          "workload's)")
+    ap.add_argument("--coarse-tol", type=float, default=1e-3,
+                    help="max-rel to which each coarse level of the warm start is solved")
     ap.add_argument("--newton-from", type=float, default=1.0,
                     help="max-rel at which each level hands over from L-BFGS to damped Newton "
                          "steps (0: L-BFGS to the end)")
@@ -156,7 +159,7 @@ def main():
             if record is not None:
                 record.append(init)
             with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
-                wc, rc = cb.solve(init, **dict(kw, tol=1e-3))
+                wc, rc = cb.solve(init, **dict(kw, tol=args.coarse_tol))
                 cen, _ = cb.centroids(wc)
                 wcs, ccs = cb.split(wc), cb.split(cen.reshape(-1, 2))
             evals += sum(r["evaluations"] for r in rc)
