@@ -79,6 +79,9 @@ SIGNATURES = {
     "cot_solve_newton": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(cot_solve_opts),
                                         ctypes.POINTER(cot_report)]),
     "cot_hessian_product": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "cot_render": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cot_cell_polygons": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int),
+                                         _dp]),
     "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
     "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

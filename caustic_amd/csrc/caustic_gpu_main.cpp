@@ -12,9 +12,14 @@
 // and, when the settings hold "mi355x": {"full_precision_weights": true}, also
 //         ../../data/weights/<xiushi>/<t>/final_proj_point_weights_full.txt (%.17g).
 //
+// and the diagnostics ../render_result/<xiushi>/<t>/{render_initial,render_<iter>}.txt and
+// ../mesh_output/<xiushi>/<t>/{mesh_initial,mesh_<iter>}.obj (t >= 1: pre_render_initial.txt,
+// pre_initial_mesh.obj, <t-1>_render_<iter>.txt, <t-1>_mesh_<iter>.obj), unless the settings
+// hold "mi355x": {"diagnostics": false}.
+//
 // Differences from the reference executable: errors end the process with a message and a
-// non-zero status (never abort or an unspecified status); the diagnostic render_*.txt, mesh
-// OBJ and periodic weight snapshots are not written.
+// non-zero status (never abort or an unspecified status); the periodic render, mesh and weight
+// snapshots of the iterations are not written; the meshes hold the cells inside proj_domain.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -214,6 +219,42 @@ bool read_doubles(const std::string& path, std::vector<double>& out) {
   return true;
 }
 
+// Picture-shaped text in the reader's orientation: m rows of n values, fixed with 6 decimals,
+// row r column c holding data[c * m + (m - 1 - r)] * scale (save_array_to_txt,
+// basic_function.cuh:205-248).
+void write_picture(const std::string& path, const std::vector<double>& data, int m, int n,
+                   double scale) {
+  fs::create_directories(fs::path(path).parent_path());
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return;
+  for (int r = 0; r < m; ++r) {
+    for (int c = 0; c < n; ++c)
+      std::fprintf(f, c + 1 < n ? "%.6f " : "%.6f", data[(size_t)c * m + (m - 1 - r)] * scale);
+    std::fputc('\n', f);
+  }
+  std::fclose(f);
+}
+
+// Wavefront OBJ of the cells: each non-empty cell's vertices as "v x y 0.0", then its face
+// (writeOBJ, basic_function.cuh:15-59).
+void write_obj(const std::string& path, const std::vector<int>& counts,
+               const std::vector<double>& verts) {
+  fs::create_directories(fs::path(path).parent_path());
+  std::ofstream f(path);
+  int idx = 1;
+  for (size_t i = 0; i < counts.size(); ++i) {
+    const int nv = counts[i];
+    if (nv < 3) continue;
+    for (int k = 0; k < nv; ++k)
+      f << "v " << verts[(i * COT_MAX_VERTS + k) * 2] << " "
+        << verts[(i * COT_MAX_VERTS + k) * 2 + 1] << " 0.0\n";
+    f << "f ";
+    for (int k = 0; k < nv; ++k) f << idx + k << " ";
+    f << "\n";
+    idx += nv;
+  }
+}
+
 // m text rows x n columns; element (ix, iy) = row m-1-ix, column iy, stored at iy*m + ix
 // (basic_function.cuh:145-204).
 bool read_picture(const std::string& path, std::vector<double>& data, int& m, int& n) {
@@ -267,6 +308,7 @@ int main(int argc, char** argv) {
   double tol = 0, initial_step = 1.0;
   std::vector<double> sample_domain, proj_domain;
   bool full_precision = false;
+  bool diagnostics = true;
   int aa_history_size = 7;
   double aa_fixed_para = -2.0, z_of_len = 0.0, z_of_receiver = 0.0, refraction_index = 1.5;
   bool if_aa_op_initial = true;
@@ -294,6 +336,8 @@ int main(int argc, char** argv) {
     picture_name = J.at("picture_para").at("name").as_str();
     if (const JVal* ext = J.find("mi355x"))
       if (const JVal* fp = ext->find("full_precision_weights")) full_precision = fp->as_bool();
+    if (const JVal* ext = J.find("mi355x"))
+      if (const JVal* dg = ext->find("diagnostics")) diagnostics = dg->as_bool();
   } catch (const std::exception& e) {
     return die(std::string("settings: ") + e.what(), 2);
   }
@@ -395,6 +439,25 @@ int main(int argc, char** argv) {
   so.verbose = 20;
   cot_report rep{};
   int rc;
+  // diagnostic render and mesh of the cells (ot_op.cuh:214-219, 281-286, 499-533, 798-820,
+  // 1270-1296): before the solve and at its end
+  const std::string rdir = "../render_result/" + xiushi + "/" + ot_time + "/";
+  const std::string mdir = "../mesh_output/" + xiushi + "/" + ot_time + "/";
+  const std::string pre = ot_time == "0" ? "" : "pre_";
+  auto diagnose = [&](const std::string& render_name, const std::string& mesh_name) -> bool {
+    std::vector<double> img((size_t)pm * pn);
+    std::vector<int> counts(N);
+    std::vector<double> verts((size_t)N * COT_MAX_VERTS * 2);
+    if (cot_render(g.c, w.data(), img.data()) != COT_OK ||
+        cot_cell_polygons(g.c, w.data(), counts.data(), verts.data()) != COT_OK)
+      return false;
+    write_picture(rdir + render_name, img, pm, pn, gray_multiple);
+    write_obj(mdir + mesh_name, counts, verts);
+    return true;
+  };
+  if (diagnostics && !diagnose(pre + "render_initial.txt",
+                               ot_time == "0" ? "mesh_initial.obj" : "pre_initial_mesh.obj"))
+    return die(std::string("render: ") + cot_last_error(), 5);
   if (ot_time == "0") {
     rc = cot_solve(g.c, w.data(), &so, &rep);
   } else {
@@ -414,6 +477,12 @@ int main(int argc, char** argv) {
     rc = cot_solve_reflection(g.c, w.data(), &so, &rf, &rep);
   }
   if (rc != COT_OK && rc != COT_ENOTCONV) return die(std::string("solve: ") + cot_last_error(), 5);
+  if (diagnostics) {
+    const std::string it = std::to_string(rep.iterations);
+    const std::string tag = ot_time == "0" ? "" : std::to_string(std::stoi(ot_time) - 1) + "_";
+    if (!diagnose(tag + "render_" + it + ".txt", tag + "mesh_" + it + ".obj"))
+      return die(std::string("render: ") + cot_last_error(), 5);
+  }
   std::printf("caustic_gpu: %s after %d iterations (%d evaluations), |g| %.6e, %.2f s\n",
               rep.converged ? "converged" : "stopped", rep.iterations, rep.evaluations,
               rep.grad_norm, rep.seconds);

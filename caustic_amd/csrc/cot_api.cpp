@@ -99,6 +99,9 @@ hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, co
                                hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
                             const double* d, double* w, int nblk, hipStream_t st);
+hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
+                         const uint8_t* nedge, const double* nu, double* image, size_t pixels,
+                         int nblk, hipStream_t st);
 hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
                                 const double* h, const double* diag, const double* p, double* q,
                                 double* part_pq, int nblk, hipStream_t st);
@@ -207,6 +210,8 @@ struct cot_context {
   double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
   double* d_nshift = nullptr;
   std::vector<double> nu_sum;  // per-problem sum of the source masses
+  size_t img_total = 0;        // pixels of all targets
+  double* d_img = nullptr;     // diagnostic render (allocated on first use)
   // device-resident solver
   DevState* d_state = nullptr;
   int* d_cnt = nullptr;        // last-block counters: direction, decide, accept (np each)
@@ -935,6 +940,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     return bail(rc);
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
+  c->img_total = img_total;
   if ((rc = dalloc(c, &c->d_pre, img_total)) || (rc = dalloc(c, &c->d_tot, tot_total)))
     return bail(rc);
   if (hipHostMalloc((void**)&c->h_part,
@@ -1276,6 +1282,51 @@ int ensure_newton(cot_context* c) {
 }  // namespace
 
 extern "C" {
+
+int cot_render(cot_context* c, const double* w, double* image) {
+  if (!c || !w || !image) return fail(COT_EINVAL, "null argument");
+  if (!c->d_img) CHK(dalloc(c, &c->d_img, c->img_total));
+  std::vector<uint8_t> act(c->np, 1);
+  std::vector<EvalSum> cur;
+  CHK(upload_sorted(c, w, c->d_w));
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  for (int p = 0; p < c->np; ++p)
+    if (cur[p].fails) return fail(COT_EUNVERIFIED, "cells failed the conformity check");
+  HIPCHK(launch_render(c->d_probs, c->d_blocks, c->d_vtx, c->d_nedge, c->d_nu, c->d_img,
+                       c->img_total, c->nblk, c->st));
+  HIPCHK(hipMemcpyAsync(image, c->d_img, sizeof(double) * c->img_total, hipMemcpyDeviceToHost,
+                        c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return COT_OK;
+}
+
+int cot_cell_polygons(cot_context* c, const double* w, int* counts, double* verts) {
+  if (!c || !w || !counts || !verts) return fail(COT_EINVAL, "null argument");
+  std::vector<uint8_t> act(c->np, 1);
+  std::vector<EvalSum> cur;
+  CHK(upload_sorted(c, w, c->d_w));
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  for (int p = 0; p < c->np; ++p)
+    if (cur[p].fails) return fail(COT_EUNVERIFIED, "cells failed the conformity check");
+  const size_t N = c->ntot;
+  std::vector<uint8_t> ne(N);
+  std::vector<double2> vx(N * kMaxVerts);
+  HIPCHK(hipMemcpyAsync(ne.data(), c->d_nedge, N, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(vx.data(), c->d_vtx, sizeof(double2) * N * kMaxVerts,
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (size_t r = 0; r < N; ++r) {
+    const int o = c->perm[r];
+    const int nv = ne[r] >= 3 ? ne[r] : 0;
+    counts[o] = nv;
+    for (int k = 0; k < kMaxVerts; ++k) {
+      const double2 v = k < nv ? vx[r * kMaxVerts + k] : make_double2(0.0, 0.0);
+      verts[((size_t)o * kMaxVerts + k) * 2] = v.x;
+      verts[((size_t)o * kMaxVerts + k) * 2 + 1] = v.y;
+    }
+  }
+  return COT_OK;
+}
 
 int cot_hessian_product(cot_context* c, const double* w, const double* x, double* y) {
   if (!c || !w || !x || !y) return fail(COT_EINVAL, "null argument");

@@ -1,0 +1,127 @@
+// Diagnostic outputs of the evaluation (the reference's `render` and `writeOBJ`,
+// basic_function.cuh:660-741, 15-59), for gfx950.
+//
+// render_kernel spreads each site's source mass nu_i uniformly over its cell: pixel p receives
+// nu_i |C_i n p| / |C_i|, the pieces found by clipping the cell (FP64 vertices of the last
+// evaluation) against each pixel of its bounding box (Sutherland-Hodgman) and accumulated with
+// FP64 atomics.  One site per lane; a diagnostic, not a hot path.
+#include "cot_internal.h"
+
+namespace cot {
+
+namespace {
+
+constexpr int kClipMax = kMaxVerts + 8;
+
+// Clip polygon (n vertices in x, y) against the half-plane given by side s of the rectangle
+// [x0, x1] x [y0, y1] (0: y >= y0, 1: x <= x1, 2: y <= y1, 3: x >= x0); result in ox, oy.
+__device__ int clip_side(const double* x, const double* y, int n, int s, double x0, double x1,
+                         double y0, double y1, double* ox, double* oy) {
+  auto inside = [&](double px, double py) {
+    switch (s) {
+      case 0: return py >= y0;
+      case 1: return px <= x1;
+      case 2: return py <= y1;
+      default: return px >= x0;
+    }
+  };
+  auto cross = [&](double ax, double ay, double bx, double by, double& cx, double& cy) {
+    if (s == 0 || s == 2) {
+      const double yc = s == 0 ? y0 : y1;
+      const double t = (yc - ay) / (by - ay);
+      cx = ax + t * (bx - ax);
+      cy = yc;
+    } else {
+      const double xc = s == 1 ? x1 : x0;
+      const double t = (xc - ax) / (bx - ax);
+      cx = xc;
+      cy = ay + t * (by - ay);
+    }
+  };
+  int m = 0;
+  for (int k = 0; k < n; ++k) {
+    const double ax = x[k], ay = y[k];
+    const double bx = x[k + 1 < n ? k + 1 : 0], by = y[k + 1 < n ? k + 1 : 0];
+    const bool ia = inside(ax, ay), ib = inside(bx, by);
+    if (ia) {
+      if (m < kClipMax) { ox[m] = ax; oy[m] = ay; ++m; }
+      if (!ib && m < kClipMax) { cross(ax, ay, bx, by, ox[m], oy[m]); ++m; }
+    } else if (ib && m < kClipMax) {
+      cross(ax, ay, bx, by, ox[m], oy[m]);
+      ++m;
+    }
+  }
+  return m;
+}
+
+__device__ double shoelace(const double* x, const double* y, int n) {
+  double a = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const int j = k + 1 < n ? k + 1 : 0;
+    a += x[k] * y[j] - x[j] * y[k];
+  }
+  return 0.5 * fabs(a);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kCellBlock) void render_kernel(const ProbDesc* probs,
+                                                            const BlockDesc* blocks,
+                                                            const double2* vtx,
+                                                            const uint8_t* nedge,
+                                                            const double* nu, double* image) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if ((int)threadIdx.x >= bd.count) return;
+  const ProbDesc pd = probs[bd.prob];
+  const int i = bd.first + threadIdx.x;
+  const int nv = nedge[i];
+  if (nv < 3) return;
+  double px[kMaxVerts], py[kMaxVerts];
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int k = 0; k < nv; ++k) {
+    const double2 v = vtx[(size_t)i * kMaxVerts + k];
+    px[k] = v.x;
+    py[k] = v.y;
+    mnx = fmin(mnx, v.x); mxx = fmax(mxx, v.x);
+    mny = fmin(mny, v.y); mxy = fmax(mxy, v.y);
+  }
+  const double area = shoelace(px, py, nv);
+  if (!(area > 0.0)) return;
+  const int ix0 = max((int)floor((mnx - pd.x0) / pd.dx), 0);
+  const int iy0 = max((int)floor((mny - pd.y0) / pd.dy), 0);
+  const int ix1 = min((int)floor((mxx - pd.x0) / pd.dx) + 1, pd.pm);
+  const int iy1 = min((int)floor((mxy - pd.y0) / pd.dy) + 1, pd.pn);
+  const double scale = nu[i] / area;
+  double* img = image + pd.pre_off;
+  double ax[kClipMax], ay[kClipMax], bx[kClipMax], by[kClipMax];
+  for (int iy = iy0; iy < iy1; ++iy) {
+    const double y0 = pd.y0 + iy * pd.dy, y1 = pd.y0 + (iy + 1) * pd.dy;
+    for (int ix = ix0; ix < ix1; ++ix) {
+      const double x0 = pd.x0 + ix * pd.dx, x1 = pd.x0 + (ix + 1) * pd.dx;
+      int n = clip_side(px, py, nv, 0, x0, x1, y0, y1, ax, ay);
+      n = clip_side(ax, ay, n, 1, x0, x1, y0, y1, bx, by);
+      n = clip_side(bx, by, n, 2, x0, x1, y0, y1, ax, ay);
+      n = clip_side(ax, ay, n, 3, x0, x1, y0, y1, bx, by);
+      if (n < 3) continue;
+      const double a = shoelace(bx, by, n);
+      if (a > 0.0) atomicAdd(&img[(size_t)iy * pd.pm + ix], scale * a);
+    }
+  }
+}
+
+__global__ void fill_f64_kernel(double* p, size_t n, double v) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
+                         const uint8_t* nedge, const double* nu, double* image, size_t pixels,
+                         int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, st,
+                     image, pixels, 0.0);
+  hipLaunchKernelGGL(render_kernel, dim3(nblk), dim3(kCellBlock), 0, st, probs, blocks, vtx, nedge,
+                     nu, image);
+  return hipGetLastError();
+}
+
+}  // namespace cot

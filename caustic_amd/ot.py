@@ -78,6 +78,7 @@ class OTBatch:
         self.precision = precision
         self.tuning = {"int_lanes": int_lanes, "repair_blocks": repair_blocks,
                        "sites_per_bucket": sites_per_bucket, "full_pass_ring": full_pass_ring}
+        self._img_shapes = [tuple(np.shape(p.rho)) for p in problems]
         self.n_total = int(sum(self.sizes))
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.n_problems = len(problems)
@@ -132,6 +133,30 @@ class OTBatch:
         L.check(self._lib.cot_centroids(self._ctx, _ptr(w), int(weighted), _ptr(cen),
                                         ex.ctypes.data_as(_up)))
         return cen, ex.astype(bool)
+
+    def render(self, w=None):
+        """Per-problem images (H, W): each site's source mass spread uniformly over its cell
+        (the reference's diagnostic render)."""
+        w = self._w(w)
+        total = sum(int(np.prod(sh)) for sh in self._img_shapes)
+        img = np.zeros(total)
+        L.check(self._lib.cot_render(self._ctx, _ptr(w), _ptr(img)))
+        out, off = [], 0
+        for sh in self._img_shapes:
+            n = int(np.prod(sh))
+            out.append(img[off:off + n].reshape(sh))
+            off += n
+        return out
+
+    def cell_polygons(self, w=None):
+        """(counts (N,), vertices (N, 24, 2)) of the cells at w, CCW; count 0 for an empty cell."""
+        w = self._w(w)
+        counts = np.zeros(self.n_total, dtype=np.int32)
+        verts = np.zeros((self.n_total, 24, 2))
+        L.check(self._lib.cot_cell_polygons(self._ctx, _ptr(w),
+                                            counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            _ptr(verts)))
+        return counts, verts
 
     def hessian_product(self, w, x):
         """H x with H = d mass / d w at w (the Newton steps' matrix)."""

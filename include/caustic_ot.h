@@ -156,6 +156,18 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * 1/4096, stops with status COT_ENOTCONV at its last accepted weights. */
 int cot_solve_newton(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
 
+/* Diagnostic image (render, basic_function.cuh:660-741): pixel p of each problem's target gets
+ * sum_i nu_i |C_i n p| / |C_i| over the cells at weights w; image holds the problems' targets one
+ * after the other in the reference layout (ix + iy * pm), sum of pm * pn doubles.  The caller
+ * multiplies by its grey scale (gray_multiple, ot_op.cuh:215) if it wants picture units. */
+int cot_render(cot_context* ctx, const double* w, double* image);
+
+/* The cells at weights w (the polygons writeOBJ writes, basic_function.cuh:15-59): counts[i]
+ * vertices (0 for an empty cell, at most COT_MAX_VERTS), CCW, in verts[(i * COT_MAX_VERTS + k) *
+ * 2 + {0, 1}] (caller order).  The cells are the restricted cells inside proj_domain. */
+#define COT_MAX_VERTS 24
+int cot_cell_polygons(cot_context* ctx, const double* w, int* counts, double* verts);
+
 /* y = H x, H the Hessian of the dual objective at w (d mass_i / d w_j, the matrix the Newton
  * steps solve with); all arrays in caller order.  No counterpart in the reference. */
 int cot_hessian_product(cot_context* ctx, const double* w, const double* x, double* y);

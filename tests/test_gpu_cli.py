@@ -107,6 +107,20 @@ def test_cli_t0_then_t1(gpu_available, tmp_path):
     np.testing.assert_allclose(inn, pts[ex], rtol=6e-6, atol=1e-9)
     with open(os.path.join(pdir, "0", "_py_sp_settings.json")) as f:
         assert json.load(f) == T["settings"]
+    # diagnostics: the final render in picture units (x from the bottom text row up) and mesh
+    gray = T["A"].sum() / ((SAMPLE[1] - SAMPLE[0]) * (SAMPLE[3] - SAMPLE[2]))
+    rdir = os.path.join(root, "Optimal_Transport_rl", "render_result", XIUSHI, "0")
+    mdir = os.path.join(root, "Optimal_Transport_rl", "mesh_output", XIUSHI, "0")
+    assert os.path.exists(os.path.join(rdir, "render_initial.txt"))
+    assert os.path.exists(os.path.join(mdir, "mesh_initial.obj"))
+    finals = [f for f in os.listdir(rdir) if f != "render_initial.txt"]
+    assert len(finals) == 1 and finals[0].startswith("render_"), finals
+    img = O.read_pic_txt(os.path.join(rdir, finals[0]))
+    ref = O.render(ev.cells, nu, rho.shape, PROJ) * gray
+    np.testing.assert_allclose(img, ref, rtol=0, atol=6e-7 + 1e-9 * ref.max())
+    faces = [line for line in open(os.path.join(mdir, finals[0].replace("render_", "mesh_")
+                                                .replace(".txt", ".obj"))) if line[0] == "f"]
+    assert len(faces) == ex.sum()
 
     # ---- t = 1: starts from t = 0's 6-digit weights; Fresnel-weighted masses, Anderson
     r = run_cli(T["build"], 1)

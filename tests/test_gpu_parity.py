@@ -211,3 +211,29 @@ def test_launch_shapes_give_the_same_cells(gpu_available, c1, int_lanes, repair_
     orc = _exact(c1, wt)
     _compare(evt, orc, c1.nu, 1e-10, 1e-11)
     assert O.max_rel_error(orc.mass, c1.nu) < 1e-4
+
+
+def test_render_and_cell_polygons_match_oracle(gpu_available, c1):
+    """The diagnostic render (each nu_i spread uniformly over its cell) against the oracle's
+    restatement of the reference's render kernel; the cell polygons tile the domain."""
+    print("[case] render + cell polygons c1 fp64", flush=True)
+    rng = np.random.default_rng(8)
+    w = rng.normal(0.0, 2e-4, c1.n)
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    ref = O.render(orc.cells, c1.nu, c1.rho.shape, c1.proj_domain)
+    with _batch([c1], precision="fp64") as b:
+        img = b.render(w)[0]
+        counts, verts = b.cell_polygons(w)
+    assert img.shape == c1.rho.shape
+    # pixel pieces of a cell are clipped in another order than the oracle's: slivers differ by
+    # FP64 cancellation, ~1e-10 of the largest pixel
+    np.testing.assert_allclose(img, ref, rtol=0, atol=1e-9 * ref.max())
+    ex = orc.exist.astype(bool)
+    assert abs(img.sum() - c1.nu[ex].sum()) < 1e-12
+    assert np.array_equal(counts >= 3, ex)
+    areas = np.zeros(c1.n)
+    for i in np.nonzero(counts)[0]:
+        x, y = verts[i, :counts[i], 0], verts[i, :counts[i], 1]
+        areas[i] = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
+    assert (areas[ex] > 0).all()  # counter-clockwise
+    np.testing.assert_allclose(areas, orc.area, rtol=0, atol=1e-14)
