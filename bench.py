@@ -115,6 +115,11 @@ def main():
     ap.add_argument("--newton-from", type=float, default=1.0,
                     help="max-rel at which each level hands over from L-BFGS to damped Newton "
                          "steps (0: L-BFGS to the end)")
+    ap.add_argument("--newton-sweeps", type=int, default=0,
+                    help="block-local Jacobi sweeps of the Newton steps' preconditioner")
+    ap.add_argument("--newton-coarse", type=int, default=0,
+                    help="patches per problem in the coarse term of the Newton steps' "
+                         "preconditioner (0: Jacobi only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
@@ -139,7 +144,8 @@ def main():
     net = WS.WarmStartNet.load(WS.DEFAULT_MODEL) if args.start == "learned" else None
     # verbose: the solver prints a progress line every 2 s of a solve
     kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
-              device_loop=loop, verbose=1, newton_from=args.newton_from or None)
+              device_loop=loop, verbose=1, newton_from=args.newton_from or None,
+              newton_coarse=args.newton_coarse, newton_sweeps=args.newton_sweeps)
 
     ms_levels = tuple(int(v) for v in args.levels.split(",")) if args.levels else wl["levels"]
     levels = {"multiscale": ms_levels, "learned": (ms_levels[-1],), "zero": ()}[args.start]
@@ -158,11 +164,15 @@ def main():
                 [WS.weights_from_field(U, cp.sites, cp.frame) for U, cp in zip(fields, cps)])
             if record is not None:
                 record.append(init)
+            tl = time.perf_counter()
             with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
                 wc, rc = cb.solve(init, **dict(kw, tol=args.coarse_tol))
                 cen, _ = cb.centroids(wc)
                 wcs, ccs = cb.split(wc), cb.split(cen.reshape(-1, 2))
             evals += sum(r["evaluations"] for r in rc)
+            log(f"[level {n_side}] {1e3 * (time.perf_counter() - tl):.1f} ms, iterations "
+                f"{max(r['iterations'] for r in rc)} (Newton "
+                f"{max(r.get('newton_iterations', 0) for r in rc)})")
             fields = [WS.field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                       for cp, wk, ck in zip(cps, wcs, ccs)]
         return np.concatenate([WS.weights_from_field(U, p.sites, p.frame)
@@ -171,7 +181,11 @@ def main():
     def one_solve(batch):
         """One timed step.  Returns (reports of the full-resolution solves, evaluations)."""
         w0, evals = start_weights()
+        tl = time.perf_counter()
         w, rep = batch.solve(w0, **kw)
+        log(f"[level final] {1e3 * (time.perf_counter() - tl):.1f} ms, iterations "
+            f"{max(r['iterations'] for r in rep)} (Newton "
+            f"{max(r.get('newton_iterations', 0) for r in rep)})")
         return rep, evals + sum(r["evaluations"] for r in rep)
 
     tune_report = None
@@ -250,6 +264,12 @@ def main():
             "max_rel_err": max(x["max_rel_err"] for r, _, _ in reps for x in r),
             "device_loop": args.device_loop,
             "newton_from": args.newton_from or None,
+            "newton_coarse": args.newton_coarse,
+            "newton_sweeps": args.newton_sweeps,
+            "final_level_newton_per_solve": {
+                "steps": st["newton_steps"] / args.steps,
+                "cg_iterations": st["cg_iterations"] / args.steps,
+                "coarse_inverse_host_ms": 1e3 * st["coarse_seconds"] / args.steps},
             "newton_iterations": int(np.mean([max(x.get("newton_iterations", 0) for x in r)
                                               for r, _, _ in reps])),
             "launch_shapes": tuning,

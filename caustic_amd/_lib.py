@@ -43,7 +43,8 @@ class cot_solve_opts(ctypes.Structure):
                 ("initial_step", ctypes.c_double), ("stop_rule", ctypes.c_int),
                 ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("device_loop", ctypes.c_int), ("accept_rule", ctypes.c_int),
-                ("repair_rounds", ctypes.c_int), ("newton_sweeps", ctypes.c_int)]
+                ("repair_rounds", ctypes.c_int), ("newton_sweeps", ctypes.c_int),
+                ("newton_coarse", ctypes.c_int)]
 
 
 class cot_report(ctypes.Structure):
@@ -94,7 +95,9 @@ class cot_stats(ctypes.Structure):
                 ("repair_seconds", ctypes.c_double), ("repair_launches", ctypes.c_longlong),
                 ("verify_seconds", ctypes.c_double), ("verify_launches", ctypes.c_longlong),
                 ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong),
-                ("integ_seconds", ctypes.c_double), ("integ_launches", ctypes.c_longlong)]
+                ("integ_seconds", ctypes.c_double), ("integ_launches", ctypes.c_longlong),
+                ("newton_steps", ctypes.c_longlong), ("cg_iterations", ctypes.c_longlong),
+                ("coarse_seconds", ctypes.c_double)]
 
 _lib = None
 

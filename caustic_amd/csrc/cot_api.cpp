@@ -85,18 +85,22 @@ hipError_t launch_aa_direction(const BlockDesc* blocks, const double* alpha, int
                                hipStream_t st);
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
-                          const int* elab, const double4* pre, double* h, double* diag, int nblk,
-                          hipStream_t st);
+                          const int* elab, const double4* pre, const int* bagg, double* h,
+                          double* diag, int* sagg, int nblk, hipStream_t st);
+hipError_t launch_coarse_assemble(const BlockDesc* blocks, const CoarseDesc* cdesc,
+                                  const int* sagg, const uint8_t* nedge, const int* elab,
+                                  const double* h, double* ac, size_t ac_len, int nblk,
+                                  hipStream_t st);
+hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdesc,
+                            const int* agg_start, const int* agg_blk, const double* rsum,
+                            const double* ainv, double* cvec, double* part_rzc, hipStream_t st);
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
                           const double* shift, const double* diag, const uint8_t* nedge,
                           const int* elab, const double* h, int sweeps, double* x, double* r,
-                          double* z, double* part_rz, double* part_rr, int nblk, hipStream_t st);
-hipError_t launch_cg_iteration(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
-                               const double* h, const double* diag, int sweeps, double* x,
-                               double* r, double* z, const double* p_old, double* p_new,
-                               double* q, double* part_rz_cur, double* part_rz_other,
-                               double* part_pq, double* part_rr, int first, int nblk,
-                               hipStream_t st);
+                          double* z, double* part_rz, double* part_rr, double* rsum, int nblk,
+                          hipStream_t st);
+hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_new, int cur,
+                               int first, hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
                             const double* d, double* w, int nblk, hipStream_t st);
 hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
@@ -209,6 +213,19 @@ struct cot_context {
   double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
   double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
   double* d_nshift = nullptr;
+  // coarse space of the CG preconditioner, built for coarse_max patches per problem
+  std::vector<int2> blk_cell;  // bucket (x, y) of each block's first site
+  int coarse_max = 0, coarse_total = 0, coarse_ntask = 0;
+  size_t coarse_mat = 0;  // doubles of all problems' dense patch matrices
+  std::vector<CoarseDesc> cdesc;
+  int* d_bagg = nullptr;   // block -> global patch (-1: none)
+  int* d_sagg = nullptr;   // site -> global patch, rewritten with every Hessian
+  int *d_agg_start = nullptr, *d_agg_blk = nullptr;
+  CoarseDesc* d_cdesc = nullptr;
+  int4* d_ctask = nullptr;
+  double *d_ac = nullptr, *d_ainv = nullptr, *d_cvec = nullptr, *d_prc = nullptr;
+  double* d_rsum = nullptr;
+  double* h_ac = nullptr;  // pinned: A_c down, its inverse up
   std::vector<double> nu_sum;  // per-problem sum of the source masses
   size_t img_total = 0;        // pixels of all targets
   double* d_img = nullptr;     // diagnostic render (allocated on first use)
@@ -238,6 +255,7 @@ struct cot_context {
     if (h_part) hipHostFree(h_part);
     if (h_counts) hipHostFree(h_counts);
     if (h_vec) hipHostFree(h_vec);
+    if (h_ac) hipHostFree(h_ac);
     if (h_small) hipHostFree(h_small);
     if (h_coef) hipHostFree(h_coef);
     if (h_ismall) hipHostFree(h_ismall);
@@ -809,6 +827,7 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->accept_rule = 0;
   o->repair_rounds = 3;
   o->newton_sweeps = 0;
+  o->newton_coarse = 0;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -894,6 +913,11 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       b.first = site_off + f;
       b.count = std::min(kCellBlock, pr.n_sites - f);
       c->blocks.push_back(b);
+      const int o = order[f];
+      const int bx = (int)std::floor((pr.sites[2 * o] - pd.bx0) * pd.binv);
+      const int by = (int)std::floor((pr.sites[2 * o + 1] - pd.by0) * pd.binv);
+      c->blk_cell.push_back(make_int2(std::min(std::max(bx, 0), pd.gx - 1),
+                                      std::min(std::max(by, 0), pd.gy - 1)));
     }
     pd.nblk = (int)c->blocks.size() - pd.blk_off;
     double nus = 0.0;
@@ -1279,6 +1303,95 @@ int ensure_newton(cot_context* c) {
   return COT_OK;
 }
 
+// Patches of the coarse space for at most maxp per problem (0: none).  A problem's bucket grid is
+// cut into rectangles of ax x ay buckets (multiples of the 8x8 tiles, doubled alternately until
+// at most maxp rectangles remain); a block joins the rectangle of its first site, so a patch is a
+// union of whole blocks and its residual sum is a sum of per-block sums.  A problem with a single
+// patch gets none (the constants are the Hessian's null space).
+int build_coarse(cot_context* c, int maxp) {
+  maxp = std::min(std::max(maxp, 0), kMaxCoarse);
+  if (maxp == c->coarse_max) return COT_OK;
+  const int np = c->np, nb = c->nblk;
+  std::vector<int> bagg(nb, -1), start, blk;
+  std::vector<CoarseDesc> cd(np, CoarseDesc{0, 0, 0, 0});
+  std::vector<int4> tasks;
+  int total = 0;
+  size_t moff = 0;
+  for (int p = 0; p < np && maxp > 1; ++p) {
+    const ProbDesc& pd = c->probs[p];
+    int ax = 8, ay = 8;
+    auto cells = [&](int a, int b) {
+      return (long long)((pd.gx + a - 1) / a) * ((pd.gy + b - 1) / b);
+    };
+    while (cells(ax, ay) > maxp) {
+      if (ax <= ay) {
+        ax *= 2;
+      } else {
+        ay *= 2;
+      }
+    }
+    const int nx = (pd.gx + ax - 1) / ax;
+    std::vector<int> idx((size_t)cells(ax, ay), -1);
+    std::vector<std::vector<int>> members;
+    for (int b = pd.blk_off; b < pd.blk_off + pd.nblk; ++b) {
+      const int2 bc = c->blk_cell[b];
+      const size_t key = (size_t)(bc.y / ay) * nx + bc.x / ax;
+      if (idx[key] < 0) {
+        idx[key] = (int)members.size();
+        members.emplace_back();
+      }
+      members[idx[key]].push_back(b);
+    }
+    const int na = (int)members.size();
+    if (na <= 1) continue;
+    if (moff + (size_t)na * na > (size_t)INT32_MAX)
+      return fail(COT_EINVAL, "coarse space too large for the batch");
+    cd[p] = CoarseDesc{total, na, (int)moff, 0};
+    for (int a = 0; a < na; ++a) {
+      start.push_back((int)blk.size());
+      for (int b : members[a]) {
+        blk.push_back(b);
+        bagg[b] = total + a;
+      }
+    }
+    for (int r0 = 0; r0 < na; r0 += kCoarseRows)
+      tasks.push_back(make_int4(p, r0, std::min(kCoarseRows, na - r0), 0));
+    total += na;
+    moff += (size_t)na * na;
+  }
+  start.push_back((int)blk.size());
+  c->coarse_max = maxp;
+  c->coarse_total = total;
+  c->coarse_ntask = (int)tasks.size();
+  c->coarse_mat = moff;
+  c->cdesc = cd;
+  if (tasks.empty()) return COT_OK;
+  CHK(dalloc(c, &c->d_bagg, nb));
+  if (!c->d_sagg) CHK(dalloc(c, &c->d_sagg, c->ntot));
+  if (!c->d_rsum) CHK(dalloc(c, &c->d_rsum, nb));
+  CHK(dalloc(c, &c->d_agg_start, start.size()));
+  CHK(dalloc(c, &c->d_agg_blk, blk.size()));
+  CHK(dalloc(c, &c->d_cdesc, np));
+  CHK(dalloc(c, &c->d_ctask, tasks.size()));
+  CHK(dalloc(c, &c->d_ac, moff));
+  CHK(dalloc(c, &c->d_ainv, moff));
+  CHK(dalloc(c, &c->d_cvec, total));
+  CHK(dalloc(c, &c->d_prc, 2 * tasks.size()));
+  if (c->h_ac) (void)hipHostFree(c->h_ac);
+  c->h_ac = nullptr;
+  if (hipHostMalloc((void**)&c->h_ac, sizeof(double) * moff) != hipSuccess)
+    return fail(COT_ENOMEM, "hipHostMalloc failed");
+  HIPCHK(hipMemcpy(c->d_bagg, bagg.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_agg_start, start.data(), sizeof(int) * start.size(),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_agg_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_cdesc, cd.data(), sizeof(CoarseDesc) * np, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_ctask, tasks.data(), sizeof(int4) * tasks.size(),
+                   hipMemcpyHostToDevice));
+  return COT_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -1341,7 +1454,7 @@ int cot_hessian_product(cot_context* c, const double* w, const double* x, double
   std::memset(c->h_active, 1, c->np);
   HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, c->st));
   HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                        c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, c->st));
+                        c->d_elab, c->d_pre, nullptr, c->d_hh, c->d_hdiag, nullptr, nb, c->st));
   CHK(upload_sorted(c, x, c->d_np));
   HIPCHK(launch_hessian_apply(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_np,
                               c->d_nq, c->d_ppq, nb, c->st));
@@ -1371,11 +1484,40 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
   // CG stops at |r| <= eta |rhs| -- an inexact Newton step, eta = 0.1 max-rel clamped to
   // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
-  // is launched directly; the rest run in chunks of kCheckCG iterations (a captured graph unless
+  // is launched directly; the rest run in chunks of `check` iterations (a captured graph unless
   // device_loop == 2), with the residual checked after each chunk.
-  constexpr int kMaxCG = 4000, kCheckCG = 32;
+  constexpr int kMaxCG = 4000;
   const bool eager = o.device_loop == 2;
   const int sweeps = std::min(std::max(o.newton_sweeps, 0), 64);
+  CHK(build_coarse(c, o.newton_coarse));
+  const bool coarse = c->coarse_ntask > 0;
+  // the coarse term cuts the iterations ~8x: check the residual more often
+  const int check = coarse ? 16 : 32;
+  CgArgs ca{};
+  ca.blocks = c->d_blocks;
+  ca.nblk = nb;
+  ca.sweeps = sweeps;
+  ca.ntask = coarse ? c->coarse_ntask : 0;
+  ca.nedge = c->d_nedge;
+  ca.elab = c->d_elab;
+  ca.h = c->d_hh;
+  ca.diag = c->d_hdiag;
+  ca.x = c->d_nx;
+  ca.r = c->d_nr;
+  ca.z = c->d_nz;
+  ca.q = c->d_nq;
+  ca.part_rz = c->d_prz;
+  ca.part_pq = c->d_ppq;
+  ca.part_rr = c->d_prr;
+  ca.part_rzc = c->d_prc;
+  ca.tasks = c->d_ctask;
+  ca.cdesc = c->d_cdesc;
+  ca.agg_start = c->d_agg_start;
+  ca.agg_blk = c->d_agg_blk;
+  ca.sagg = c->d_sagg;
+  ca.rsum = c->d_rsum;
+  ca.ainv = c->d_ainv;
+  ca.cvec = c->d_cvec;
   hipGraph_t cg_graph = nullptr;
   hipGraphExec_t cg_exec = nullptr;
   struct CgGraphGuard {
@@ -1386,19 +1528,13 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       if (*g) (void)hipGraphDestroy(*g);
     }
   } cg_guard{&cg_graph, &cg_exec};
-  // iteration k: r.z partials in d_prz[k & 1] (current) and [(k + 1) & 1] (previous), the
+  // iteration k: r.z partials in slot k & 1 (current) and (k + 1) & 1 (previous), the
   // direction read from pbuf[k & 1] and written to pbuf[(k + 1) & 1]; a chunk runs iterations
-  // k0 .. k0 + kCheckCG - 1 with k0 odd, so the buffers realign from chunk to chunk
+  // k0 .. k0 + check - 1 with k0 odd, so the buffers realign from chunk to chunk
   double* pbuf[2] = {c->d_np, c->d_np2};
   auto cg_chunk = [&]() -> int {
-    for (int k = 1; k <= kCheckCG; ++k) {
-      double* cur_rz = c->d_prz + (size_t)(k & 1) * nb;
-      double* oth_rz = c->d_prz + (size_t)((k + 1) & 1) * nb;
-      HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag,
-                                 sweeps, c->d_nx, c->d_nr, c->d_nz, pbuf[k & 1],
-                                 pbuf[(k + 1) & 1], c->d_nq, cur_rz, oth_rz, c->d_ppq, c->d_prr,
-                                 0, nb, st));
-    }
+    for (int k = 1; k <= check; ++k)
+      HIPCHK(launch_cg_iteration(ca, pbuf[k & 1], pbuf[(k + 1) & 1], k & 1, 0, st));
     return COT_OK;
   };
   auto sum_rr = [&](double& rr) -> int {
@@ -1431,18 +1567,43 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
                           st));
     // Hessian at the current cells, then H x = -(g - mean g)
     HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                          c->d_elab, c->d_pre, c->d_hh, c->d_hdiag, nb, st));
+                          c->d_elab, c->d_pre, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hdiag,
+                          coarse ? c->d_sagg : nullptr, nb, st));
+    if (coarse) {
+      // patches' Laplacian down, its inverse (host FP64) up
+      HIPCHK(launch_coarse_assemble(c->d_blocks, c->d_cdesc, c->d_sagg, c->d_nedge, c->d_elab,
+                                    c->d_hh, c->d_ac, c->coarse_mat, nb, st));
+      HIPCHK(hipMemcpyAsync(c->h_ac, c->d_ac, sizeof(double) * c->coarse_mat,
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int p = 0; p < np; ++p) {
+        const CoarseDesc& cd = c->cdesc[p];
+        if (cd.na == 0) continue;
+        double* m = c->h_ac + cd.moff;
+        if (act[p]) {
+          coarse_inverse(m, cd.na);
+        } else {
+          std::fill(m, m + (size_t)cd.na * cd.na, 0.0);
+        }
+      }
+      c->stats.coarse_seconds +=
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      HIPCHK(hipMemcpyAsync(c->d_ainv, c->h_ac, sizeof(double) * c->coarse_mat,
+                            hipMemcpyHostToDevice, st));
+    }
     HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nedge,
                           c->d_elab, c->d_hh, sweeps, c->d_nx, c->d_nr, c->d_nz, c->d_prz,
-                          c->d_prr, nb, st));
+                          c->d_prr, coarse ? c->d_rsum : nullptr, nb, st));
+    if (coarse)
+      HIPCHK(launch_cg_coarse(c->d_ctask, c->coarse_ntask, c->d_cdesc, c->d_agg_start,
+                              c->d_agg_blk, c->d_rsum, c->d_ainv, c->d_cvec, c->d_prc, st));
     double rr0 = 0.0, rr = 0.0, worst = 0.0;
     for (int p = 0; p < np; ++p)
       if (act[p]) worst = std::max(worst, cur[p].maxrel);
     const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
     CHK(sum_rr(rr0));
-    HIPCHK(launch_cg_iteration(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, sweeps,
-                               c->d_nx, c->d_nr, c->d_nz, pbuf[0], pbuf[1], c->d_nq, c->d_prz,
-                               c->d_prz + nb, c->d_ppq, c->d_prr, 1, nb, st));
+    HIPCHK(launch_cg_iteration(ca, pbuf[0], pbuf[1], 0, 1, st));
     int cg = 1;
     while (cg < kMaxCG) {
       if (eager) {
@@ -1458,10 +1619,12 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
         }
         HIPCHK(hipGraphLaunch(cg_exec, st));
       }
-      cg += kCheckCG;
+      cg += check;
       CHK(sum_rr(rr));
       if (!(rr > eta * eta * rr0)) break;
     }
+    c->stats.newton_steps += 1;
+    c->stats.cg_iterations += cg;
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
     std::vector<uint8_t> tr = act, acc(np, 0);
     std::fill(t.begin(), t.end(), 1.0);

@@ -71,6 +71,38 @@ struct BlockDesc {
   int prob, first, count, pad;
 };
 
+// Coarse space of the Newton steps' preconditioner: the patches (unions of whole blocks) of a
+// problem are global indices [a0, a0 + na); their na x na matrices (A_c, its inverse) are dense
+// row-major at offset moff.  At most kMaxCoarse patches per problem; the coarse kernel handles
+// kCoarseRows rows per block.
+constexpr int kMaxCoarse = 1024;
+constexpr int kCoarseRows = 16;
+struct CoarseDesc {
+  int a0, na, moff, pad;
+};
+
+// Buffers of one conjugate-gradient iteration (cot_newton.hip).  part_rz holds two slots of nblk
+// r.z partials, part_rzc two slots of ntask coarse partials; ntask == 0 runs without the coarse
+// term (tasks, cdesc, agg_*, rsum, ainv, cvec, sagg unused).
+struct CgArgs {
+  const BlockDesc* blocks;
+  int nblk, sweeps, ntask, pad;
+  const uint8_t* nedge;
+  const int* elab;
+  const double* h;
+  const double* diag;
+  double *x, *r, *z, *q;
+  double *part_rz, *part_pq, *part_rr, *part_rzc;
+  const int4* tasks;
+  const CoarseDesc* cdesc;
+  const int* agg_start;   // patch -> its blocks (CSR over agg_blk)
+  const int* agg_blk;
+  const int* sagg;        // site -> global patch, -1 where the Hessian row is zero
+  double* rsum;           // per-block sums of r
+  const double* ainv;     // per-problem inverses of A_c (+ rank one)
+  double* cvec;           // per-patch coarse correction
+};
+
 // Everything the cell kernel reads and writes.
 struct CellArgs {
   const ProbDesc* probs;

@@ -83,4 +83,79 @@ inline void build_prefix(const double* rho, int pm, int pn, bool fp32, double4* 
   }
 }
 
+// In place: a (n x n, row-major) holds the off-diagonal entries of the patches' Laplacian as the
+// device accumulated them (each side of an edge adds to its own row, so a[i][j] and a[j][i] differ
+// by rounding); on return it holds (A_c + s 11^T)^-1, A_c symmetrised with its diagonal set to
+// minus the off-diagonal row sums, s = trace / n^2 (the constant vector, A_c's null space, gets
+// the mean diagonal as its eigenvalue).  Cholesky factor L, then L^-1 row by row, then
+// L^-T L^-1; about n^3 / 2 multiply-adds.  A zero or numerically indefinite matrix leaves zeros
+// (no coarse correction).
+inline void coarse_inverse(double* a, int n) {
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const double v = 0.5 * (a[(size_t)i * n + j] + a[(size_t)j * n + i]);
+      a[(size_t)i * n + j] = v;
+      a[(size_t)j * n + i] = v;
+    }
+  double tr = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) sum += a[(size_t)i * n + j];
+    a[(size_t)i * n + i] = -sum;
+    tr -= sum;
+  }
+  const size_t nn = (size_t)n * n;
+  if (!(tr > 0.0)) {
+    std::fill(a, a + nn, 0.0);
+    return;
+  }
+  const double sig = tr / ((double)n * n);
+  for (size_t k = 0; k < nn; ++k) a[k] += sig;
+  // Cholesky, lower triangle in place
+  for (int j = 0; j < n; ++j) {
+    double* rj = a + (size_t)j * n;
+    double d = rj[j];
+    for (int k = 0; k < j; ++k) d -= rj[k] * rj[k];
+    if (!(d > 1e-14 * tr / n)) {
+      std::fill(a, a + nn, 0.0);
+      return;
+    }
+    rj[j] = std::sqrt(d);
+    const double inv = 1.0 / rj[j];
+    for (int i = j + 1; i < n; ++i) {
+      double* ri = a + (size_t)i * n;
+      double v = ri[j];
+      for (int k = 0; k < j; ++k) v -= ri[k] * rj[k];
+      ri[j] = v * inv;
+    }
+  }
+  // M = L^-1 (lower): row i = (e_i - sum_{k<i} L[i][k] row k) / L[i][i]
+  std::vector<double> m(nn, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double* mi = m.data() + (size_t)i * n;
+    const double* li = a + (size_t)i * n;
+    for (int k = 0; k < i; ++k) {
+      const double f = li[k];
+      const double* mk = m.data() + (size_t)k * n;
+      for (int j = 0; j <= k; ++j) mi[j] -= f * mk[j];
+    }
+    mi[i] = 1.0;
+    const double inv = 1.0 / li[i];
+    for (int j = 0; j <= i; ++j) mi[j] *= inv;
+  }
+  // a = M^T M: a[p][q] = sum_{k >= max(p, q)} M[k][p] M[k][q]
+  std::fill(a, a + nn, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const double* mk = m.data() + (size_t)k * n;
+    for (int p = 0; p <= k; ++p) {
+      const double f = mk[p];
+      double* ap = a + (size_t)p * n;
+      for (int q = 0; q <= p; ++q) ap[q] += f * mk[q];
+    }
+  }
+  for (int p = 0; p < n; ++p)
+    for (int q = p + 1; q < n; ++q) a[(size_t)p * n + q] = a[(size_t)q * n + p];
+}
+
 }  // namespace cot

@@ -185,7 +185,7 @@ class OTBatch:
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop=True,
               accept_rule: str = "auto", repair_rounds: int = 3, newton_from: float = None,
-              newton_max_iter: int = 50, newton_sweeps: int = 0):
+              newton_max_iter: int = 50, newton_sweeps: int = 0, newton_coarse: int = 0):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
@@ -194,14 +194,15 @@ class OTBatch:
         newton_from (stop="maxrel" only): L-BFGS until max-rel newton_from, then damped Newton
         steps (cot_solve_newton) to tol; problems the Newton steps leave unconverged finish
         with L-BFGS.  Reports then add "newton_iterations".  newton_sweeps: block-local
-        Jacobi sweeps of the conjugate gradients' preconditioner."""
+        Jacobi sweeps of the conjugate gradients' preconditioner; newton_coarse: at most this
+        many patches per problem in its two-level coarse term (0: none)."""
         kw = dict(history=history, max_iter=max_iter, initial_step=initial_step, stop=stop,
                   h0_threshold=h0_threshold, verbose=verbose, device_loop=device_loop,
                   accept_rule=accept_rule, repair_rounds=repair_rounds)
         if newton_from is not None and stop == "maxrel" and newton_from > tol:
             w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False, **kw)
             w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds, device_loop,
-                                  newton_sweeps)
+                                  newton_sweeps, newton_coarse)
             if all(r["converged"] for r in r2):
                 w3, r3 = w2, None
             else:
@@ -245,7 +246,8 @@ class OTBatch:
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
 
-    def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=0):
+    def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=0,
+                coarse=0):
         """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol; device_loop == 2
         launches the conjugate-gradient kernels directly instead of replaying a graph."""
         w = self._w(w0).copy()
@@ -258,6 +260,7 @@ class OTBatch:
         o.repair_rounds = repair_rounds
         o.device_loop = int(device_loop)
         o.newton_sweeps = int(sweeps)
+        o.newton_coarse = int(coarse)
         reps = (L.cot_report * self.n_problems)()
         rc = self._lib.cot_solve_newton(self._ctx, _ptr(w), ctypes.byref(o), reps)
         L.check(rc, allow=(L.COT_ENOTCONV,))

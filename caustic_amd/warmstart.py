@@ -336,13 +336,14 @@ def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
 
 
 def _solve_level(problems, init, tol, precision, max_iter, tuning=None, verbose=0,
-                 newton_from=None):
+                 newton_from=None, newton_coarse=0):
     from .ot import OTBatch
 
     with OTBatch(list(problems), precision=precision, tuning=tuning) as b:
         w, rep = b.solve(None if init is None else np.concatenate(init), history=10, tol=tol,
                          stop="maxrel", h0_threshold=0.0, max_iter=max_iter,
-                         raise_on_fail=False, verbose=verbose, newton_from=newton_from)
+                         raise_on_fail=False, verbose=verbose, newton_from=newton_from,
+                         newton_coarse=newton_coarse)
         cen, _ = b.centroids(w, weighted=True)
         ws = [x.copy() for x in b.split(w)]
         cs = [c.copy() for c in b.split(cen.reshape(-1, 2))]
@@ -357,14 +358,15 @@ def coarse_problems(problems, levels):
 
 def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
                      coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print,
-                     tunings=None, verbose=0, newton_from=None):
+                     tunings=None, verbose=0, newton_from=None, newton_coarse=0):
     """Solve every problem to max-rel `tol`, one batch per level: coarse grids first (to
     `coarse_tol`), each level's weights carried to the next level's sites through the weight
     field (field_from_solution / weights_from_field).  fields: optional per-problem fields that
     start the first level (a learned start); levels=() with fields starts the final level from
     them.  tunings: optional launch shapes (caustic_amd.autotune) keyed by level side or
     "final".  verbose > 0: the solver prints a progress line every 2 s.  newton_from: every
-    level runs L-BFGS to this max-rel, then damped Newton steps (OTBatch.solve).  Returns
+    level runs L-BFGS to this max-rel, then damped Newton steps (OTBatch.solve), with
+    newton_coarse patches per problem in the preconditioner's coarse term.  Returns
     (weights, centroids, reports) of the final level."""
     tunings = tunings or {}
     probs = list(problems)
@@ -374,7 +376,7 @@ def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="
                                             for U, cp in zip(fields, cps)]
         t0 = time.perf_counter()
         ws, cs, rep = _solve_level(cps, init, coarse_tol, precision, max_iter,
-                                   tunings.get(n_side), verbose, newton_from)
+                                   tunings.get(n_side), verbose, newton_from, newton_coarse)
         fields = [field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                   for cp, wk, ck in zip(cps, ws, cs)]
         log(f"[multiscale] level {n_side}: {len(cps)} problems, "
@@ -385,7 +387,7 @@ def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="
                                         for U, p in zip(fields, probs)]
     t0 = time.perf_counter()
     ws, cs, rep = _solve_level(probs, init, tol, precision, max_iter, tunings.get("final"),
-                               verbose, newton_from)
+                               verbose, newton_from, newton_coarse)
     log(f"[multiscale] final: {len(probs)} problems, "
         f"iters max {max(r['iterations'] for r in rep)} "
         f"mean {np.mean([r['iterations'] for r in rep]):.0f}, converged "

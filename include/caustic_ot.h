@@ -86,6 +86,11 @@ typedef struct {
                                gradients' preconditioner (default 0: Jacobi.  More sweeps
                                cut the CG iterations ~1.7x at 4 sweeps but cost more per
                                iteration than they save at N >= 65,536 on MI355X) */
+  int newton_coarse;        /* cot_solve_newton: at most this many patches per problem in the
+                               coarse term of the preconditioner, 2..1024 (default 0: none).
+                               Patches are unions of 128-site blocks; the term adds
+                               P (P^T H P)^-1 P^T (inverted on the host each step, ~n^3/2
+                               multiply-adds) and cuts the iterations ~8x at N = 65,536 */
 } cot_solve_opts;
 
 typedef struct {
@@ -148,11 +153,11 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * has brought every cell to non-empty.  Each step builds the Hessian -- the cell-adjacency
  * Laplacian h_ij = int over the common edge of rho ds / (2 |s_i - s_j|) -- from the current
  * cells, solves H d = -g by preconditioned conjugate gradients (opt->newton_sweeps Jacobi sweeps
- * on each 128-site block's submatrix of H), and halves the step until
- * every cell keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).
- * The conjugate gradients run to |r| <= eta |rhs|, eta = 0.1 max-rel clamped to [1e-4, 0.1],
- * replayed from a captured graph in chunks of 32 iterations (launched directly when
- * opt->device_loop == 2).  A problem with an empty or unverified cell, or whose step falls below
+ * on each 128-site block's submatrix of H, plus with opt->newton_coarse the two-level patch
+ * term), and halves the step until every cell keeps mass and |g|_2 falls by (1 - t/2)
+ * (Kitagawa, Merigot, Thibert 2019).  The conjugate gradients run to |r| <= eta |rhs|,
+ * eta = 0.1 max-rel clamped to [1e-4, 0.1], replayed from a captured graph in chunks of 32
+ * iterations (16 with the patch term; launched directly when opt->device_loop == 2).  A problem with an empty or unverified cell, or whose step falls below
  * 1/4096, stops with status COT_ENOTCONV at its last accepted weights. */
 int cot_solve_newton(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports);
 
@@ -196,6 +201,9 @@ typedef struct {
   long long repair_rounds;    /* repair rounds run by host-driven evaluations */
   double integ_seconds;       /* full passes of the integration kernel */
   long long integ_launches;
+  long long newton_steps;     /* cot_solve_newton: Hessians built (one per Newton step) */
+  long long cg_iterations;    /* cot_solve_newton: conjugate-gradient iterations */
+  double coarse_seconds;      /* cot_solve_newton: host time of the coarse-matrix inverses */
 } cot_stats;
 
 int cot_get_stats(cot_context* ctx, cot_stats* out);

@@ -301,4 +301,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   return 0;
 }
 
+// The host inverse of the Newton preconditioner's coarse matrix (cot_layout.h), in place.
+void emu_coarse_inverse(double* a, int n) { coarse_inverse(a, n); }
+
 }  // extern "C"

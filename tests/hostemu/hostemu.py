@@ -26,6 +26,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         _lib.emu_destroy.argtypes = [ctypes.c_void_p]
         _lib.emu_state.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
+        _lib.emu_coarse_inverse.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib.emu_eval.restype = ctypes.c_int
         _lib.emu_eval.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p,
                                   ctypes.c_void_p]
@@ -96,3 +97,10 @@ def evaluate(p, w, fp32=False, ring_init=2, max_repairs=12, verify_tol=1e-4):
         return s.evaluate(w)
     finally:
         s.close()
+
+
+def coarse_inverse(a):
+    """The library's coarse-matrix inverse (cot_layout.h) on a copy of the n x n array a."""
+    m = np.ascontiguousarray(a, dtype=np.float64).copy()
+    lib().emu_coarse_inverse(m.ctypes.data, m.shape[0])
+    return m

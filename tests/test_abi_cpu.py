@@ -32,10 +32,11 @@ def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "caustic_ot.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
         "sizeof(cot_options), sizeof(cot_solve_opts), sizeof(cot_report), sizeof(cot_stats), "
         "offsetof(cot_solve_opts, newton_sweeps), sizeof(cot_reflection), "
-        "offsetof(cot_reflection, if_aa_op_initial), offsetof(cot_options, full_pass_ring));"
+        "offsetof(cot_reflection, if_aa_op_initial), offsetof(cot_options, full_pass_ring), "
+        "offsetof(cot_solve_opts, newton_coarse));"
         "return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
@@ -45,7 +46,7 @@ def test_struct_layouts_match_the_header(tmp_path):
             ctypes.sizeof(L.cot_solve_opts), ctypes.sizeof(L.cot_report),
             ctypes.sizeof(L.cot_stats), L.cot_solve_opts.newton_sweeps.offset,
             ctypes.sizeof(L.cot_reflection), L.cot_reflection.if_aa_op_initial.offset,
-            L.cot_options.full_pass_ring.offset]
+            L.cot_options.full_pass_ring.offset, L.cot_solve_opts.newton_coarse.offset]
     assert got == want
 
 

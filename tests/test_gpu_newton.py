@@ -97,3 +97,44 @@ def test_mixed_precision_weights_match_the_exact_solution(gpu_available, c1):
     assert rep[0]["converged"], rep
     d = (w_mix - w_mix.mean()) - (w_star - w_star.mean())
     assert np.abs(d).max() < 1e-7, np.abs(d).max()
+
+
+def test_newton_coarse_preconditioner_c1(gpu_available, c1):
+    """The two-level preconditioner (16 patches of 256 sites): the Newton solve reaches the same
+    FP64 tolerance, confirmed by the oracle, in at most the Jacobi solve's Newton steps and with
+    at least 1.5x fewer conjugate-gradient iterations per step."""
+    print("[case] Newton solve c1 fp64, coarse term 16 patches vs Jacobi", flush=True)
+    per_step = {}
+    for coarse in (0, 16):
+        with _batch([c1], precision="fp64") as b:
+            w1, _ = b.solve(tol=1e-2, h0_threshold=0.0, raise_on_fail=False)
+            b.reset_stats()
+            w, rep = b._newton(w1, 1e-8, 50, 1, 3, True, 0, coarse)
+            st = b.stats()
+        assert rep[0]["converged"], (coarse, rep)
+        orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+        assert O.max_rel_error(orc.mass, c1.nu) < 1e-8 + 1e-9
+        assert st["newton_steps"] >= 1 and st["cg_iterations"] >= st["newton_steps"]
+        per_step[coarse] = (st["cg_iterations"] / st["newton_steps"], rep[0]["iterations"])
+        print(f"[case] coarse {coarse}: {st['newton_steps']} steps, {st['cg_iterations']} CG "
+              f"iterations, {st['coarse_seconds'] * 1e3:.2f} ms host inverse", flush=True)
+    assert per_step[16][1] <= per_step[0][1] + 1, per_step
+    assert per_step[16][0] * 1.5 <= per_step[0][0], per_step
+
+
+def test_newton_coarse_batch_of_uneven_problems(gpu_available):
+    """Per-problem coarse blocks (different patch counts, one problem with a single patch and so
+    no coarse term) in one CG system; problems that converge early drop out while the others
+    keep their coarse term.  The oracle confirms every problem."""
+    print("[case] Newton batched solve, coarse term 8 patches, 4 uneven problems", flush=True)
+    probs = [W.plain_fp64(seed=31, n_side=24, res=96), W.plain_fp64(seed=32, n_side=48, res=192),
+             W.plain_fp64(seed=33, n_side=10, res=64), W.plain_fp64(seed=34, n_side=40, res=160)]
+    with _batch(probs, precision="fp64") as b:
+        w, rep = b.solve(tol=1e-7, h0_threshold=0.0, newton_from=1e-2, newton_coarse=8)
+        ws = b.split(w)
+        st = b.stats()
+    assert st["cg_iterations"] > 0
+    for p, wk, r in zip(probs, ws, rep):
+        assert r["converged"] and r["newton_iterations"] >= 1, r
+        orc = O.evaluate(p.sites, wk, p.nu, p.rho, p.proj_domain)
+        assert O.max_rel_error(orc.mass, p.nu) < 1e-7 + 1e-9

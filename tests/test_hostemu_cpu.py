@@ -104,3 +104,47 @@ def test_hostemu_lbfgs_converges(fp32, accept):
     assert r.converged
     orc = O.evaluate(p.sites, r.w, p.nu, p.rho, p.proj_domain)
     assert O.max_rel_error(orc.mass, p.nu) < 1e-5 + 1e-6
+
+
+def _patch_laplacian(n, seed):
+    """Off-diagonal couplings of a random patch graph (a jittered grid's neighbours), as the
+    device accumulates them: each row's entries carry their own rounding (a[i][j] != a[j][i])."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    a = np.zeros((n, n))
+    for i in range(n):
+        for j in (i + 1, i + side):
+            if j < n and (j != i + 1 or (i + 1) % side):
+                v = rng.uniform(0.1, 3.0)
+                a[i, j] = -v * (1 + 1e-13 * rng.normal())
+                a[j, i] = -v * (1 + 1e-13 * rng.normal())
+    return a
+
+
+@pytest.mark.parametrize("n,seed", [(2, 1), (7, 2), (64, 3), (200, 4)])
+def test_coarse_inverse_matches_numpy(n, seed):
+    """The Newton preconditioner's coarse inverse: (A_c + s 11^T)^-1 with A_c the symmetrised
+    Laplacian (diagonal = minus the off-diagonal row sums), s = trace / n^2 -- against
+    numpy.linalg.inv of the same matrix built independently."""
+    a = _patch_laplacian(n, seed)
+    got = E.coarse_inverse(a)
+    sym = 0.5 * (a + a.T)
+    np.fill_diagonal(sym, 0.0)
+    lap = sym - np.diag(sym.sum(axis=1))
+    want = np.linalg.inv(lap + np.trace(lap) / n ** 2)
+    assert np.allclose(got, got.T, rtol=0, atol=1e-14 * np.abs(want).max())
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    # the constant vector: eigenvalue 1 / (mean diagonal) of the inverse
+    one = np.ones(n)
+    assert np.allclose(got @ one, one / (np.trace(lap) / n), rtol=1e-10)
+
+
+def test_coarse_inverse_of_zero_and_disconnected():
+    """A zero matrix (an inactive problem) and a matrix that is not positive definite after the
+    rank-one term (two disconnected groups: a second null vector) leave zeros -- no coarse
+    correction rather than a wrong one."""
+    assert not E.coarse_inverse(np.zeros((5, 5))).any()
+    a = np.zeros((4, 4))
+    a[0, 1] = a[1, 0] = -1.0
+    a[2, 3] = a[3, 2] = -1.0
+    assert not E.coarse_inverse(a).any()
