@@ -110,14 +110,16 @@ def main():
     ap.add_argument("--levels", default=None,
                     help="comma-separated coarse grid sides of the warm start (default: the "
                          "workload's)")
-    ap.add_argument("--coarse-tol", type=float, default=1e-3,
+    ap.add_argument("--coarse-tol", type=float, default=1e-2,
                     help="max-rel to which each coarse level of the warm start is solved")
     ap.add_argument("--newton-from", type=float, default=1.0,
                     help="max-rel at which each level hands over from L-BFGS to damped Newton "
                          "steps (0: L-BFGS to the end)")
-    ap.add_argument("--newton-sweeps", type=int, default=0,
+    ap.add_argument("--coarse-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="geometry precision of the warm start's coarse levels")
+    ap.add_argument("--newton-sweeps", type=int, default=2,
                     help="block-local Jacobi sweeps of the Newton steps' preconditioner")
-    ap.add_argument("--newton-coarse", type=int, default=0,
+    ap.add_argument("--newton-coarse", type=int, default=256,
                     help="patches per problem in the coarse term of the Newton steps' "
                          "preconditioner (0: Jacobi only)")
     args = ap.parse_args()
@@ -165,14 +167,17 @@ def main():
             if record is not None:
                 record.append(init)
             tl = time.perf_counter()
-            with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
+            with OTBatch(cps, precision=args.coarse_precision,
+                         tuning=tuning["levels"][li]) as cb:
                 wc, rc = cb.solve(init, **dict(kw, tol=args.coarse_tol))
                 cen, _ = cb.centroids(wc)
                 wcs, ccs = cb.split(wc), cb.split(cen.reshape(-1, 2))
             evals += sum(r["evaluations"] for r in rc)
             log(f"[level {n_side}] {1e3 * (time.perf_counter() - tl):.1f} ms, iterations "
                 f"{max(r['iterations'] for r in rc)} (Newton "
-                f"{max(r.get('newton_iterations', 0) for r in rc)})")
+                f"{max(r.get('newton_iterations', 0) for r in rc)}, L-BFGS "
+                f"{1e3 * max(r.get('lbfgs_seconds', 0.0) for r in rc):.1f} ms, Newton "
+                f"{1e3 * max(r.get('newton_seconds', 0.0) for r in rc):.1f} ms)")
             fields = [WS.field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                       for cp, wk, ck in zip(cps, wcs, ccs)]
         return np.concatenate([WS.weights_from_field(U, p.sites, p.frame)
@@ -185,7 +190,9 @@ def main():
         w, rep = batch.solve(w0, **kw)
         log(f"[level final] {1e3 * (time.perf_counter() - tl):.1f} ms, iterations "
             f"{max(r['iterations'] for r in rep)} (Newton "
-            f"{max(r.get('newton_iterations', 0) for r in rep)})")
+            f"{max(r.get('newton_iterations', 0) for r in rep)}, L-BFGS "
+            f"{1e3 * max(r.get('lbfgs_seconds', 0.0) for r in rep):.1f} ms, Newton "
+            f"{1e3 * max(r.get('newton_seconds', 0.0) for r in rep):.1f} ms)")
         return rep, evals + sum(r["evaluations"] for r in rep)
 
     tune_report = None
@@ -198,7 +205,8 @@ def main():
         inits = []
         w_probe, _ = start_weights(record=inits)
         for li, cps in enumerate(coarse):
-            tuning["levels"][li], r = AT.tune(cps, "mixed", w0=inits[li], iters=40, log=log)
+            tuning["levels"][li], r = AT.tune(cps, args.coarse_precision, w0=inits[li], iters=40,
+                                              log=log)
             tune_report[f"level{levels[li]}"] = r
         tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, iters=40, log=log)
         tune_report["final"] = r
@@ -266,6 +274,7 @@ def main():
             "newton_from": args.newton_from or None,
             "newton_coarse": args.newton_coarse,
             "newton_sweeps": args.newton_sweeps,
+            "coarse_precision": args.coarse_precision,
             "final_level_newton_per_solve": {
                 "steps": st["newton_steps"] / args.steps,
                 "cg_iterations": st["cg_iterations"] / args.steps,

@@ -21,8 +21,11 @@ Knobs (fields of cot_options):
 
 Objective: device time per full evaluation -- construction, integration and repair kernels, read
 from the in-kernel timers (cot_get_stats) -- over a short stretch of the real solve: `iters`
-L-BFGS iterations from the caller's weights, replayed for every candidate.  Coordinate descent
-over the knobs (a change must win by 2%), at most two sweeps.  Results are cached per device,
+L-BFGS iterations from the caller's weights, replayed for every candidate, the smaller of
+`repeats` measurements (the repair passes' work lists fill in a different order from run to run,
+which moves sites between blocks; the resulting spread is one-sided, up to ~1.8x slower in rare
+runs, and one such outlier is enough to make the descent reject the right shape).  Coordinate
+descent over the knobs (a change must win by 2%), at most two sweeps.  Results are cached per device,
 precision and problem shape in a JSON file ($CAUSTIC_TUNE_CACHE, default
 ~/.cache/caustic_amd/tuning.json).
 """
@@ -111,7 +114,7 @@ def measure(problems, precision: str, cfg: dict, w0, iters: int, solve_kw: dict)
 
 
 def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw: dict = None,
-         use_cache: bool = True, log=None):
+         use_cache: bool = True, log=None, repeats: int = 2):
     """Launch shapes for `problems` (a list of workloads.Problem).  w0: weights to start the
     measured iterations from (zeros by default; a warm start is more representative of a real
     solve).  Returns (config dict for OTBatch(tuning=...), report dict)."""
@@ -132,9 +135,12 @@ def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw:
     def run(cfg):
         k = tuple(sorted(cfg.items()))
         if k not in seen:
-            seen[k] = measure(problems, precision, cfg, w0, iters, solve_kw)
+            ms = [measure(problems, precision, cfg, w0, iters, solve_kw)
+                  for _ in range(max(int(repeats), 1))]
+            seen[k] = min(ms, key=lambda m: m["per_eval"])
             if log:
-                log(f"[autotune] {cfg} -> {seen[k]['per_eval'] * 1e6:.1f} us/eval")
+                log(f"[autotune] {cfg} -> "
+                    + " / ".join(f"{m['per_eval'] * 1e6:.1f}" for m in ms) + " us/eval")
         return seen[k]["per_eval"]
 
     t_start = time.perf_counter()

@@ -212,6 +212,7 @@ struct cot_context {
   double *d_hh = nullptr, *d_hdiag = nullptr, *d_nx = nullptr, *d_nr = nullptr, *d_nz = nullptr;
   double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
   double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
+  double* d_nrz = nullptr;  // r.z of the current and the previous CG iteration
   double* d_nshift = nullptr;
   // coarse space of the CG preconditioner, built for coarse_max patches per problem
   std::vector<int2> blk_cell;  // bucket (x, y) of each block's first site
@@ -826,8 +827,8 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->device_loop = 1;
   o->accept_rule = 0;
   o->repair_rounds = 3;
-  o->newton_sweeps = 0;
-  o->newton_coarse = 0;
+  o->newton_sweeps = 2;
+  o->newton_coarse = 256;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -1297,6 +1298,7 @@ int ensure_newton(cot_context* c) {
   CHK(dalloc(c, &c->d_np2, N));
   CHK(dalloc(c, &c->d_nq, N));
   CHK(dalloc(c, &c->d_prz, 2 * nb));
+  CHK(dalloc(c, &c->d_nrz, 2));
   CHK(dalloc(c, &c->d_ppq, nb));
   CHK(dalloc(c, &c->d_prr, nb));
   CHK(dalloc(c, &c->d_nshift, c->np));
@@ -1510,6 +1512,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   ca.part_pq = c->d_ppq;
   ca.part_rr = c->d_prr;
   ca.part_rzc = c->d_prc;
+  ca.rz = c->d_nrz;
   ca.tasks = c->d_ctask;
   ca.cdesc = c->d_cdesc;
   ca.agg_start = c->d_agg_start;

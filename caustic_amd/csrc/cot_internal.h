@@ -93,6 +93,7 @@ struct CgArgs {
   const double* diag;
   double *x, *r, *z, *q;
   double *part_rz, *part_pq, *part_rr, *part_rzc;
+  double* rz;             // two slots: r.z of the current and of the previous iteration
   const int4* tasks;
   const CoarseDesc* cdesc;
   const int* agg_start;   // patch -> its blocks (CSR over agg_blk)

@@ -43,6 +43,17 @@ __device__ __forceinline__ double sum_partials(const double* part, int n, double
   return block_sum(v, red);
 }
 
+// Sum of na partials a and nb partials b in one pass (all loads in flight before the one
+// reduction), the same value in every block.
+__device__ __forceinline__ double sum_partials2(const double* a, int na, const double* b, int nb,
+                                                double* red) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < na; k += blockDim.x) v += a[k];
+  double u = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) u += b[k];
+  return block_sum(v + u, red);
+}
+
 // Sum over pieces of (parameter length) * rho(pixel) along the segment a -> b given in pixel
 // units of the problem's lattice (pm x pn pixels).
 __device__ double edge_rho(double ua, double va, double ub, double vb, const double4* pre,
@@ -247,23 +258,23 @@ __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
   }
 }
 
-// One CG direction and product: beta = (r.z) / (r.z of the previous iteration) (0 on the first),
-// p_new = z + beta p_old for the block's sites, q = H p_new with the neighbours' p_new formed on
-// the fly from z and p_old (which no block writes here), partial p.q.  With the coarse term
-// (sagg != nullptr) the preconditioned residual is z_i + cvec[sagg_i] (sagg_i < 0: no patch) and
-// r.z adds the coarse partials.
+// One CG direction and product: r.z from the partials of the update (and coarse) kernels,
+// stored by block 0 in rz_out for the update kernel and the next iteration; beta = r.z / rz_prev
+// (0 on the first iteration), p_new = z + beta p_old for the block's sites, q = H p_new with the
+// neighbours' p_new formed on the fly from z and p_old (which no block writes here), partial p.q.
+// With the coarse term (sagg != nullptr) the preconditioned residual is z_i + cvec[sagg_i]
+// (sagg_i < 0: no patch) and r.z adds the coarse partials.
 __global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
-    const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rz_prev,
-    const double* part_rzc, const double* part_rzc_prev, int ntask, int first,
-    const uint8_t* nedge, const int* elab, const double* h, const double* diag, const int* sagg,
-    const double* cvec, const double* z, const double* p_old, double* p_new, double* q,
-    double* part_pq) {
+    const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rzc, int ntask,
+    const double* rz_prev, double* rz_out, int first, const uint8_t* nedge, const int* elab,
+    const double* h, const double* diag, const int* sagg, const double* cvec, const double* z,
+    const double* p_old, double* p_new, double* q, double* part_pq) {
   __shared__ double red[kCellBlock / 64];
+  const double rz = sum_partials2(part_rz, nblk, part_rzc, ntask, red);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *rz_out = rz;
   double beta = 0.0;
   if (!first) {
-    const double rz = sum_partials(part_rz, nblk, red) + sum_partials(part_rzc, ntask, red);
-    const double rz0 =
-        sum_partials(part_rz_prev, nblk, red) + sum_partials(part_rzc_prev, ntask, red);
+    const double rz0 = *rz_prev;
     beta = rz0 != 0.0 ? rz / rz0 : 0.0;
   }
   auto zc = [&](int j) {
@@ -319,16 +330,16 @@ __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
   if (threadIdx.x == 0) part_pq[blockIdx.x] = pq;
 }
 
-// alpha = (r.z) / (p.q); x += alpha p; r -= alpha q; z = r / diag; partial r.z (new), r.r;
-// with the coarse term, r.z adds the coarse partials and rsum receives the block sums of r.
+// alpha = (r.z) / (p.q) with r.z as the direction kernel stored it; x += alpha p; r -= alpha q;
+// z = r / diag; partial r.z (new), r.r; with the coarse term rsum receives the block sums of r.
 __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
-    const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rzc, int ntask,
-    const double* part_pq, const double* diag, const uint8_t* nedge, const int* elab,
-    const double* h, int sweeps, const double* p, const double* q, double* x, double* r,
-    double* z, double* part_rz_new, double* part_rr, double* rsum) {
+    const BlockDesc* blocks, int nblk, const double* rz_cur, const double* part_pq,
+    const double* diag, const uint8_t* nedge, const int* elab, const double* h, int sweeps,
+    const double* p, const double* q, double* x, double* r, double* z, double* part_rz_new,
+    double* part_rr, double* rsum) {
   __shared__ double red[kCellBlock / 64];
   __shared__ double zl[kCellBlock];
-  const double rz = sum_partials(part_rz, nblk, red) + sum_partials(part_rzc, ntask, red);
+  const double rz = *rz_cur;
   const double pq = sum_partials(part_pq, nblk, red);
   const double alpha = pq > 0.0 ? rz / pq : 0.0;
   const BlockDesc bd = blocks[blockIdx.x];
@@ -409,8 +420,9 @@ hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const 
 
 // One CG iteration: direction and H p (cg_step), the update, and with the coarse term
 // (cg->ntask > 0) the coarse kernel.  The r.z partials of the current residual are in slot `cur`
-// of part_rz / part_rzc, the previous ones in the other slot, which the update and the coarse
-// kernel overwrite with the new ones.  p_old is the previous direction, p_new receives this one.
+// of part_rz / part_rzc, and the update and the coarse kernel write the new ones into the other
+// slot; the direction kernel stores the summed r.z in rz[cur] (the previous iteration's is in
+// rz[other]).  p_old is the previous direction, p_new receives this one.
 hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_new, int cur,
                                int first, hipStream_t st) {
   const int oth = cur ^ 1;
@@ -418,13 +430,13 @@ hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_n
   double* rzc_oth = a.part_rzc + (size_t)oth * a.ntask;
   const bool coarse = a.ntask > 0;
   hipLaunchKernelGGL(cg_step_kernel, dim3(a.nblk), dim3(kCellBlock), 0, st, a.blocks, a.nblk,
-                     a.part_rz + (size_t)cur * a.nblk, a.part_rz + (size_t)oth * a.nblk, rzc_cur,
-                     rzc_oth, a.ntask, first, a.nedge, a.elab, a.h, a.diag,
-                     coarse ? a.sagg : nullptr, a.cvec, a.z, p_old, p_new, a.q, a.part_pq);
+                     a.part_rz + (size_t)cur * a.nblk, rzc_cur, a.ntask, a.rz + oth, a.rz + cur,
+                     first, a.nedge, a.elab, a.h, a.diag, coarse ? a.sagg : nullptr, a.cvec, a.z,
+                     p_old, p_new, a.q, a.part_pq);
   hipLaunchKernelGGL(cg_update_kernel, dim3(a.nblk), dim3(kCellBlock), 0, st, a.blocks, a.nblk,
-                     a.part_rz + (size_t)cur * a.nblk, rzc_cur, a.ntask, a.part_pq, a.diag,
-                     a.nedge, a.elab, a.h, a.sweeps, p_new, a.q, a.x, a.r, a.z,
-                     a.part_rz + (size_t)oth * a.nblk, a.part_rr, coarse ? a.rsum : nullptr);
+                     a.rz + cur, a.part_pq, a.diag, a.nedge, a.elab, a.h, a.sweeps, p_new, a.q,
+                     a.x, a.r, a.z, a.part_rz + (size_t)oth * a.nblk, a.part_rr,
+                     coarse ? a.rsum : nullptr);
   if (coarse)
     hipLaunchKernelGGL(cg_coarse_kernel, dim3(a.ntask), dim3(256), 0, st, a.tasks, a.cdesc,
                        a.agg_start, a.agg_blk, a.rsum, a.ainv, a.cvec, rzc_oth);

@@ -185,7 +185,7 @@ class OTBatch:
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop=True,
               accept_rule: str = "auto", repair_rounds: int = 3, newton_from: float = None,
-              newton_max_iter: int = 50, newton_sweeps: int = 0, newton_coarse: int = 0):
+              newton_max_iter: int = 50, newton_sweeps: int = 2, newton_coarse: int = 256):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
@@ -194,8 +194,9 @@ class OTBatch:
         newton_from (stop="maxrel" only): L-BFGS until max-rel newton_from, then damped Newton
         steps (cot_solve_newton) to tol; problems the Newton steps leave unconverged finish
         with L-BFGS.  Reports then add "newton_iterations".  newton_sweeps: block-local
-        Jacobi sweeps of the conjugate gradients' preconditioner; newton_coarse: at most this
-        many patches per problem in its two-level coarse term (0: none)."""
+        Jacobi sweeps of the conjugate gradients' preconditioner (0: plain Jacobi);
+        newton_coarse: at most this many patches per problem in its two-level coarse term
+        (0: none)."""
         kw = dict(history=history, max_iter=max_iter, initial_step=initial_step, stop=stop,
                   h0_threshold=h0_threshold, verbose=verbose, device_loop=device_loop,
                   accept_rule=accept_rule, repair_rounds=repair_rounds)
@@ -216,6 +217,8 @@ class OTBatch:
                 rep["evaluations"] = r1[k]["evaluations"] + r2[k]["evaluations"] + (
                     r3[k]["evaluations"] if r3 is not None else 0)
                 rep["newton_iterations"] = r2[k]["iterations"]
+                rep["lbfgs_seconds"] = r1[k]["seconds"]
+                rep["newton_seconds"] = r2[k]["seconds"]
                 rep["seconds"] = r1[k]["seconds"] + r2[k]["seconds"] + (
                     r3[k]["seconds"] if r3 is not None else 0.0)
                 out.append(rep)
@@ -246,8 +249,8 @@ class OTBatch:
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
 
-    def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=0,
-                coarse=0):
+    def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=2,
+                coarse=256):
         """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol; device_loop == 2
         launches the conjugate-gradient kernels directly instead of replaying a graph."""
         w = self._w(w0).copy()

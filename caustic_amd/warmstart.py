@@ -336,7 +336,7 @@ def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
 
 
 def _solve_level(problems, init, tol, precision, max_iter, tuning=None, verbose=0,
-                 newton_from=None, newton_coarse=0):
+                 newton_from=None, newton_coarse=256):
     from .ot import OTBatch
 
     with OTBatch(list(problems), precision=precision, tuning=tuning) as b:
@@ -358,7 +358,7 @@ def coarse_problems(problems, levels):
 
 def multiscale_solve(problems: Sequence[W.Problem], levels=(32, 64), precision="mixed",
                      coarse_tol=1e-3, tol=1e-5, max_iter=20000, fields=None, log=print,
-                     tunings=None, verbose=0, newton_from=None, newton_coarse=0):
+                     tunings=None, verbose=0, newton_from=None, newton_coarse=256):
     """Solve every problem to max-rel `tol`, one batch per level: coarse grids first (to
     `coarse_tol`), each level's weights carried to the next level's sites through the weight
     field (field_from_solution / weights_from_field).  fields: optional per-problem fields that

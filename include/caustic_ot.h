@@ -83,14 +83,15 @@ typedef struct {
                                   whose f is too coarse to resolve the last decades */
   int repair_rounds;        /* conformity repair rounds of a line-search trial (default 3) */
   int newton_sweeps;        /* cot_solve_newton: block-local Jacobi sweeps of the conjugate
-                               gradients' preconditioner (default 0: Jacobi.  More sweeps
-                               cut the CG iterations ~1.7x at 4 sweeps but cost more per
-                               iteration than they save at N >= 65,536 on MI355X) */
+                               gradients' preconditioner (default 2; 0: plain Jacobi).  With
+                               the coarse term, 2 sweeps cut the iterations ~20% at
+                               N = 65,536 for ~2 us more per iteration */
   int newton_coarse;        /* cot_solve_newton: at most this many patches per problem in the
-                               coarse term of the preconditioner, 2..1024 (default 0: none).
-                               Patches are unions of 128-site blocks; the term adds
+                               coarse term of the preconditioner, 0 or 2..1024 (default 256;
+                               0: none).  Patches are unions of 128-site blocks; the term adds
                                P (P^T H P)^-1 P^T (inverted on the host each step, ~n^3/2
-                               multiply-adds) and cuts the iterations ~8x at N = 65,536 */
+                               multiply-adds: 0.3-1.7 ms at 256) and cuts the iterations ~4x
+                               at N = 65,536 */
 } cot_solve_opts;
 
 typedef struct {

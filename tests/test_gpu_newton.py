@@ -72,10 +72,12 @@ def test_newton_batch_of_uneven_problems(gpu_available):
 
 
 def test_newton_with_block_sweeps_converges(gpu_available, c1):
-    """The block-local Jacobi-sweep preconditioner (4 sweeps) gives a convergent Newton solve."""
-    print("[case] Newton solve c1 fp64, 4 preconditioner sweeps", flush=True)
+    """The block-local Jacobi-sweep preconditioner alone (4 sweeps, no coarse term) gives a
+    convergent Newton solve."""
+    print("[case] Newton solve c1 fp64, 4 preconditioner sweeps, no coarse term", flush=True)
     with _batch([c1], precision="fp64") as b:
-        w, rep = b.solve(tol=1e-8, h0_threshold=0.0, newton_from=1e-2, newton_sweeps=4)
+        w, rep = b.solve(tol=1e-8, h0_threshold=0.0, newton_from=1e-2, newton_sweeps=4,
+                         newton_coarse=0)
     assert rep[0]["converged"] and rep[0]["newton_iterations"] >= 1, rep
     orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
     assert O.max_rel_error(orc.mass, c1.nu) < 1e-8 + 1e-9
