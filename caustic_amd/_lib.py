@@ -97,7 +97,7 @@ class cot_stats(ctypes.Structure):
                 ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong),
                 ("integ_seconds", ctypes.c_double), ("integ_launches", ctypes.c_longlong),
                 ("newton_steps", ctypes.c_longlong), ("cg_iterations", ctypes.c_longlong),
-                ("coarse_seconds", ctypes.c_double)]
+                ("coarse_seconds", ctypes.c_double), ("cg_restarts", ctypes.c_longlong)]
 
 _lib = None
 

@@ -87,10 +87,9 @@ hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const 
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
                           const int* elab, const double4* pre, const int* bagg, double* h,
                           double* diag, int* sagg, int nblk, hipStream_t st);
-hipError_t launch_coarse_assemble(const BlockDesc* blocks, const CoarseDesc* cdesc,
-                                  const int* sagg, const uint8_t* nedge, const int* elab,
-                                  const double* h, double* ac, size_t ac_len, int nblk,
-                                  hipStream_t st);
+hipError_t launch_coarse_rows(const BlockDesc* blocks, const int* bagg, const int* sagg,
+                              const uint8_t* nedge, const int* elab, const double* h,
+                              CoarseEntry* out, int nblk, hipStream_t st);
 hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdesc,
                             const int* agg_start, const int* agg_blk, const double* rsum,
                             const double* ainv, double* cvec, double* part_rzc, hipStream_t st);
@@ -219,14 +218,17 @@ struct cot_context {
   int coarse_max = 0, coarse_total = 0, coarse_ntask = 0;
   size_t coarse_mat = 0;  // doubles of all problems' dense patch matrices
   std::vector<CoarseDesc> cdesc;
-  int* d_bagg = nullptr;   // block -> global patch (-1: none)
+  std::vector<int> bagg;   // block -> global patch (-1: none)
+  int* d_bagg = nullptr;
   int* d_sagg = nullptr;   // site -> global patch, rewritten with every Hessian
   int *d_agg_start = nullptr, *d_agg_blk = nullptr;
   CoarseDesc* d_cdesc = nullptr;
   int4* d_ctask = nullptr;
-  double *d_ac = nullptr, *d_ainv = nullptr, *d_cvec = nullptr, *d_prc = nullptr;
+  double *d_ainv = nullptr, *d_cvec = nullptr, *d_prc = nullptr;
   double* d_rsum = nullptr;
-  double* h_ac = nullptr;  // pinned: A_c down, its inverse up
+  CoarseEntry* d_crows = nullptr;  // per-block couplings to other patches
+  CoarseEntry* h_crows = nullptr;  // pinned copy
+  double* h_ac = nullptr;          // pinned: the patch matrices, then their inverses (up)
   std::vector<double> nu_sum;  // per-problem sum of the source masses
   size_t img_total = 0;        // pixels of all targets
   double* d_img = nullptr;     // diagnostic render (allocated on first use)
@@ -257,6 +259,7 @@ struct cot_context {
     if (h_counts) hipHostFree(h_counts);
     if (h_vec) hipHostFree(h_vec);
     if (h_ac) hipHostFree(h_ac);
+    if (h_crows) hipHostFree(h_crows);
     if (h_small) hipHostFree(h_small);
     if (h_coef) hipHostFree(h_coef);
     if (h_ismall) hipHostFree(h_ismall);
@@ -1367,6 +1370,7 @@ int build_coarse(cot_context* c, int maxp) {
   c->coarse_ntask = (int)tasks.size();
   c->coarse_mat = moff;
   c->cdesc = cd;
+  c->bagg = bagg;
   if (tasks.empty()) return COT_OK;
   CHK(dalloc(c, &c->d_bagg, nb));
   if (!c->d_sagg) CHK(dalloc(c, &c->d_sagg, c->ntot));
@@ -1375,13 +1379,17 @@ int build_coarse(cot_context* c, int maxp) {
   CHK(dalloc(c, &c->d_agg_blk, blk.size()));
   CHK(dalloc(c, &c->d_cdesc, np));
   CHK(dalloc(c, &c->d_ctask, tasks.size()));
-  CHK(dalloc(c, &c->d_ac, moff));
+  if (!c->d_crows) CHK(dalloc(c, &c->d_crows, (size_t)nb * kBlockNbr));
   CHK(dalloc(c, &c->d_ainv, moff));
   CHK(dalloc(c, &c->d_cvec, total));
   CHK(dalloc(c, &c->d_prc, 2 * tasks.size()));
   if (c->h_ac) (void)hipHostFree(c->h_ac);
   c->h_ac = nullptr;
   if (hipHostMalloc((void**)&c->h_ac, sizeof(double) * moff) != hipSuccess)
+    return fail(COT_ENOMEM, "hipHostMalloc failed");
+  if (!c->h_crows &&
+      hipHostMalloc((void**)&c->h_crows, sizeof(CoarseEntry) * (size_t)nb * kBlockNbr) !=
+          hipSuccess)
     return fail(COT_ENOMEM, "hipHostMalloc failed");
   HIPCHK(hipMemcpy(c->d_bagg, bagg.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_agg_start, start.data(), sizeof(int) * start.size(),
@@ -1521,23 +1529,30 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   ca.rsum = c->d_rsum;
   ca.ainv = c->d_ainv;
   ca.cvec = c->d_cvec;
-  hipGraph_t cg_graph = nullptr;
-  hipGraphExec_t cg_exec = nullptr;
+  // the Jacobi-only arguments: the fallback when the full preconditioner's CG diverges
+  CgArgs cj = ca;
+  cj.ntask = 0;
+  cj.sweeps = 0;
+  // captured chunk graphs: [0] the full preconditioner, [1] Jacobi only
+  hipGraph_t cg_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
   struct CgGraphGuard {
     hipGraph_t* g;
     hipGraphExec_t* e;
     ~CgGraphGuard() {
-      if (*e) (void)hipGraphExecDestroy(*e);
-      if (*g) (void)hipGraphDestroy(*g);
+      for (int k = 0; k < 2; ++k) {
+        if (e[k]) (void)hipGraphExecDestroy(e[k]);
+        if (g[k]) (void)hipGraphDestroy(g[k]);
+      }
     }
-  } cg_guard{&cg_graph, &cg_exec};
+  } cg_guard{cg_graph, cg_exec};
   // iteration k: r.z partials in slot k & 1 (current) and (k + 1) & 1 (previous), the
   // direction read from pbuf[k & 1] and written to pbuf[(k + 1) & 1]; a chunk runs iterations
   // k0 .. k0 + check - 1 with k0 odd, so the buffers realign from chunk to chunk
   double* pbuf[2] = {c->d_np, c->d_np2};
-  auto cg_chunk = [&]() -> int {
+  auto cg_chunk = [&](const CgArgs& a) -> int {
     for (int k = 1; k <= check; ++k)
-      HIPCHK(launch_cg_iteration(ca, pbuf[k & 1], pbuf[(k + 1) & 1], k & 1, 0, st));
+      HIPCHK(launch_cg_iteration(a, pbuf[k & 1], pbuf[(k + 1) & 1], k & 1, 0, st));
     return COT_OK;
   };
   auto sum_rr = [&](double& rr) -> int {
@@ -1573,10 +1588,11 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
                           c->d_elab, c->d_pre, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hdiag,
                           coarse ? c->d_sagg : nullptr, nb, st));
     if (coarse) {
-      // patches' Laplacian down, its inverse (host FP64) up
-      HIPCHK(launch_coarse_assemble(c->d_blocks, c->d_cdesc, c->d_sagg, c->d_nedge, c->d_elab,
-                                    c->d_hh, c->d_ac, c->coarse_mat, nb, st));
-      HIPCHK(hipMemcpyAsync(c->h_ac, c->d_ac, sizeof(double) * c->coarse_mat,
+      // the blocks' couplings down; the patches' Laplacian summed over each patch's blocks in
+      // block order and inverted (host FP64); the inverses up
+      HIPCHK(launch_coarse_rows(c->d_blocks, c->d_bagg, c->d_sagg, c->d_nedge, c->d_elab,
+                                c->d_hh, c->d_crows, nb, st));
+      HIPCHK(hipMemcpyAsync(c->h_crows, c->d_crows, sizeof(CoarseEntry) * (size_t)nb * kBlockNbr,
                             hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       const auto t0 = std::chrono::steady_clock::now();
@@ -1584,48 +1600,76 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
         const CoarseDesc& cd = c->cdesc[p];
         if (cd.na == 0) continue;
         double* m = c->h_ac + cd.moff;
-        if (act[p]) {
-          coarse_inverse(m, cd.na);
-        } else {
-          std::fill(m, m + (size_t)cd.na * cd.na, 0.0);
+        std::fill(m, m + (size_t)cd.na * cd.na, 0.0);
+        if (!act[p]) continue;
+        const ProbDesc& pd = c->probs[p];
+        for (int b = pd.blk_off; b < pd.blk_off + pd.nblk; ++b) {
+          const int a = c->bagg[b] - cd.a0;
+          if (a < 0) continue;
+          const CoarseEntry* row = c->h_crows + (size_t)b * kBlockNbr;
+          for (int k = 0; k < kBlockNbr && row[k].b >= 0; ++k)
+            m[(size_t)a * cd.na + (row[k].b - cd.a0)] -= row[k].v;
         }
+        coarse_inverse(m, cd.na);
       }
       c->stats.coarse_seconds +=
           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       HIPCHK(hipMemcpyAsync(c->d_ainv, c->h_ac, sizeof(double) * c->coarse_mat,
                             hipMemcpyHostToDevice, st));
     }
-    HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag, c->d_nedge,
-                          c->d_elab, c->d_hh, sweeps, c->d_nx, c->d_nr, c->d_nz, c->d_prz,
-                          c->d_prr, coarse ? c->d_rsum : nullptr, nb, st));
-    if (coarse)
-      HIPCHK(launch_cg_coarse(c->d_ctask, c->coarse_ntask, c->d_cdesc, c->d_agg_start,
-                              c->d_agg_blk, c->d_rsum, c->d_ainv, c->d_cvec, c->d_prc, st));
     double rr0 = 0.0, rr = 0.0, worst = 0.0;
     for (int p = 0; p < np; ++p)
       if (act[p]) worst = std::max(worst, cur[p].maxrel);
     const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
-    CHK(sum_rr(rr0));
-    HIPCHK(launch_cg_iteration(ca, pbuf[0], pbuf[1], 0, 1, st));
-    int cg = 1;
-    while (cg < kMaxCG) {
-      if (eager) {
-        CHK(cg_chunk());
-      } else {
-        if (!cg_exec) {
-          HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-          const int rc = cg_chunk();
-          const hipError_t ce = hipStreamEndCapture(st, &cg_graph);
-          if (rc != COT_OK) return rc;
-          HIPCHK(ce);
-          HIPCHK(hipGraphInstantiate(&cg_exec, cg_graph, nullptr, nullptr, 0));
+    // CG to |r| <= eta |r0| with the arguments a (graph slot gs); returns the iterations run, or
+    // -1 when the residual grew past 1e4 |r0| (or is not a number)
+    auto run_cg = [&](const CgArgs& a, int gs, int& iters) -> int {
+      const bool co = a.ntask > 0;
+      HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag,
+                            c->d_nedge, c->d_elab, c->d_hh, a.sweeps, c->d_nx, c->d_nr, c->d_nz,
+                            c->d_prz, c->d_prr, co ? c->d_rsum : nullptr, nb, st));
+      if (co)
+        HIPCHK(launch_cg_coarse(c->d_ctask, c->coarse_ntask, c->d_cdesc, c->d_agg_start,
+                                c->d_agg_blk, c->d_rsum, c->d_ainv, c->d_cvec, c->d_prc, st));
+      CHK(sum_rr(rr0));
+      HIPCHK(launch_cg_iteration(a, pbuf[0], pbuf[1], 0, 1, st));
+      iters = 1;
+      while (iters < kMaxCG) {
+        if (eager) {
+          CHK(cg_chunk(a));
+        } else {
+          if (!cg_exec[gs]) {
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = cg_chunk(a);
+            const hipError_t ce = hipStreamEndCapture(st, &cg_graph[gs]);
+            if (rc != COT_OK) return rc;
+            HIPCHK(ce);
+            HIPCHK(hipGraphInstantiate(&cg_exec[gs], cg_graph[gs], nullptr, nullptr, 0));
+          }
+          HIPCHK(hipGraphLaunch(cg_exec[gs], st));
         }
-        HIPCHK(hipGraphLaunch(cg_exec, st));
+        iters += check;
+        CHK(sum_rr(rr));
+        if (!(rr <= 1e8 * rr0)) {
+          iters = -iters;
+          return COT_OK;
+        }
+        if (!(rr > eta * eta * rr0)) break;
       }
-      cg += check;
-      CHK(sum_rr(rr));
-      if (!(rr > eta * eta * rr0)) break;
+      return COT_OK;
+    };
+    int cg = 0;
+    CHK(run_cg(ca, 0, cg));
+    if (cg < 0 && (coarse || sweeps > 0)) {
+      // the preconditioned iteration diverged (rounding along the Hessian's null space at
+      // residuals near the FP64 floor): redo the step's CG with Jacobi alone
+      if (o.verbose > 0)
+        std::printf("[cot] newton CG diverged after %d iterations; Jacobi restart\n", -cg);
+      c->stats.cg_iterations += -cg;
+      c->stats.cg_restarts += 1;
+      CHK(run_cg(cj, 1, cg));
     }
+    if (cg < 0) cg = -cg;  // a diverged Jacobi CG: the line search rejects its direction
     c->stats.newton_steps += 1;
     c->stats.cg_iterations += cg;
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)

@@ -40,8 +40,10 @@ enum : int { kCandBucket = 0, kCandAdj1 = 1, kCandAdj2 = 2 };
 constexpr double kSliverRel = 1e-3;
 constexpr double kMarginRel = 1e-3;
 // Candidates the conformity check hands to a repair: when site i has an edge with site j that
-// j's cell lacks, i is queued for j (at most kMaxExtra per site and round).
-constexpr int kMaxExtra = 4;
+// j's cell lacks, i is queued for j (at most kMaxExtra per site and round; the repair takes them
+// in increasing site order, so the result does not depend on the order they were queued in
+// unless more than kMaxExtra arrive).
+constexpr int kMaxExtra = 24;
 // Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
 // folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
 // the next kernel on the stream closes the launch: [2 * kTimerSlots] accumulates the ticks from
@@ -79,6 +81,13 @@ constexpr int kMaxCoarse = 1024;
 constexpr int kCoarseRows = 16;
 struct CoarseDesc {
   int a0, na, moff, pad;
+};
+// One coupling of a block's sites to another patch b (sum of the edge weights h); at most
+// kBlockNbr per block, patch -1 ends a short list.
+constexpr int kBlockNbr = 16;
+struct CoarseEntry {
+  int b, pad;
+  double v;
 };
 
 // Buffers of one conjugate-gradient iteration (cot_newton.hip).  part_rz holds two slots of nblk

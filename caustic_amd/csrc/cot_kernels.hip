@@ -508,10 +508,23 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   const int nadj = a.cand != kCandBucket ? (int)a.adjn[i] : 0;
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
-  // candidates queued by the conformity check
+  // candidates queued by the conformity check, taken in increasing site order (the check appends
+  // them in whatever order its threads run; sorting makes the clipping order, and so the FP64
+  // vertices, independent of it)
   if (a.xcnt) {
     const int nx = min(a.xcnt[i], kMaxExtra);
-    for (int k = 0; k < nx && nv > 0; ++k) consider(a.xcand[(size_t)i * kMaxExtra + k], 0, true);
+    const int* xc = a.xcand + (size_t)i * kMaxExtra;
+    int last = -1;
+    for (int t = 0; t < nx && nv > 0; ++t) {
+      int m = INT32_MAX;
+      for (int k = 0; k < nx; ++k) {
+        const int j = xc[k];
+        if (j > last && j < m) m = j;
+      }
+      if (m == INT32_MAX) break;
+      consider(m, 0, true);
+      last = m;
+    }
     a.xcnt[i] = 0;
   }
   if (nadj > 0) {

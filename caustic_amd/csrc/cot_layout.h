@@ -85,11 +85,11 @@ inline void build_prefix(const double* rho, int pm, int pn, bool fp32, double4* 
 
 // In place: a (n x n, row-major) holds the off-diagonal entries of the patches' Laplacian as the
 // device accumulated them (each side of an edge adds to its own row, so a[i][j] and a[j][i] differ
-// by rounding); on return it holds (A_c + s 11^T)^-1, A_c symmetrised with its diagonal set to
-// minus the off-diagonal row sums, s = trace / n^2 (the constant vector, A_c's null space, gets
-// the mean diagonal as its eigenvalue).  Cholesky factor L, then L^-1 row by row, then
-// L^-T L^-1; about n^3 / 2 multiply-adds.  A zero or numerically indefinite matrix leaves zeros
-// (no coarse correction).
+// by rounding); on return it holds the pseudo-inverse of A_c, A_c symmetrised with its diagonal
+// set to minus the off-diagonal row sums: J (A_c + s 11^T)^-1 J with J = I - 11^T / n and
+// s = trace / n^2 (the rank-one term makes the constant vector, A_c's null space, invertible;
+// J removes it again).  Cholesky factor L, then L^-1 row by row, then L^-T L^-1; about n^3 / 2
+// multiply-adds.  A zero or numerically indefinite matrix leaves zeros (no coarse correction).
 inline void coarse_inverse(double* a, int n) {
   for (int i = 0; i < n; ++i)
     for (int j = i + 1; j < n; ++j) {
@@ -156,6 +156,19 @@ inline void coarse_inverse(double* a, int n) {
   }
   for (int p = 0; p < n; ++p)
     for (int q = p + 1; q < n; ++q) a[(size_t)p * n + q] = a[(size_t)q * n + p];
+  // project out the constant vector (J a J, J = I - 11^T / n): the pseudo-inverse of A_c, so the
+  // coarse term adds nothing along the Hessian's null space
+  std::vector<double> mean(n, 0.0);
+  double all = 0.0;
+  for (int p = 0; p < n; ++p) {
+    double r = 0.0;
+    for (int q = 0; q < n; ++q) r += a[(size_t)p * n + q];
+    mean[p] = r / n;
+    all += r;
+  }
+  all /= (double)n * n;
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) a[(size_t)p * n + q] += all - mean[p] - mean[q];
 }
 
 }  // namespace cot

@@ -12,8 +12,9 @@
 //
 // Preconditioner: M = D^-1 (or block-local Jacobi sweeps in LDS) + P A_c^-1 P^T, an additive
 // two-level operator.  P is the aggregation of sites into patches (unions of whole 128-site blocks,
-// at most a few hundred per problem), A_c = P^T H P the patches' Laplacian, assembled here with
-// atomics and inverted on the host (plus a rank-one term for its constant null space).  The
+// at most a few hundred per problem), A_c = P^T H P the patches' Laplacian: each block sums its
+// couplings to the neighbouring patches here (deterministically), the host adds the blocks of a
+// patch and inverts (plus a rank-one term for its constant null space).  The
 // coarse term carries the smooth part of the residual across the domain in one step, which
 // Jacobi alone needs O(sqrt N) iterations for: at N = 65,536 the iterations to 1e-4 fall from
 // ~700 to ~90.  One iteration is three kernels -- cg_update (x, r, the Jacobi part, per-block
@@ -157,33 +158,67 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
   }
 }
 
-// Off-diagonal entries of the patches' Laplacian A_c = P^T H P: every edge between sites of two
-// different patches a != b of a problem adds -h to A_c[a][b] (row-major, the problem's dense
-// block at cd.moff; a, b local to the problem).  The host symmetrises and sets the diagonal to
-// minus the row sums.  ac must be zero on entry.
-__global__ __launch_bounds__(kCellBlock) void coarse_assemble_kernel(
-    const BlockDesc* blocks, const CoarseDesc* cdesc, const int* sagg, const uint8_t* nedge,
-    const int* elab, const double* h, double* ac) {
+// Couplings of each block's sites to the other patches, the block's row of A_c = P^T H P before
+// the sum over the patch's blocks: out[blk * kBlockNbr + k] = (patch b, sum of h over the edges
+// from the block's sites to sites of patch b), for the block's distinct neighbouring patches in
+// increasing order (patch -1 ends a short list).  Deterministic: the patches are collected,
+// sorted, then each is summed with one block reduction in a fixed order.  A block with more than
+// kBlockNbr neighbouring patches keeps the kBlockNbr smallest -- the couplings dropped leave a
+// weaker but still valid (Laplacian, so SPD after the rank-one term) coarse matrix.
+__global__ __launch_bounds__(kCellBlock) void coarse_rows_kernel(
+    const BlockDesc* blocks, const int* bagg, const int* sagg, const uint8_t* nedge,
+    const int* elab, const double* h, CoarseEntry* out) {
+  __shared__ int ids[kBlockNbr];
+  __shared__ int wmin[kCellBlock / 64];
+  __shared__ double red[kCellBlock / 64];
   const BlockDesc bd = blocks[blockIdx.x];
-  if ((int)threadIdx.x >= bd.count) return;
-  const int i = bd.first + threadIdx.x;
-  const int a = sagg[i];
-  if (a < 0) return;
-  const CoarseDesc cd = cdesc[bd.prob];
-  const int nv = nedge[i];
-  for (int k = 0; k < nv; ++k) {
-    const int j = elab[(size_t)i * kMaxVerts + k];
-    if (j < 0) continue;
-    const int b = sagg[j];
-    const double hv = h[(size_t)i * kMaxVerts + k];
-    if (b >= 0 && b != a && hv != 0.0)
-      atomicAdd(&ac[(size_t)cd.moff + (size_t)(a - cd.a0) * cd.na + (b - cd.a0)], -hv);
+  const int a = bagg[blockIdx.x];
+  CoarseEntry* row = out + (size_t)blockIdx.x * kBlockNbr;
+  if (a < 0) {
+    if (threadIdx.x < kBlockNbr) row[threadIdx.x] = CoarseEntry{-1, 0, 0.0};
+    return;
   }
-}
-
-__global__ void zero_f64_kernel(double* p, size_t n) {
-  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0.0;
+  const bool own = (int)threadIdx.x < bd.count;
+  const int i = bd.first + threadIdx.x;
+  const int nv = own ? nedge[i] : 0;
+  const bool live = own && sagg[i] >= 0;
+  // round k takes the smallest neighbouring patch above the one taken in round k - 1 (a block
+  // minimum), so the list comes out sorted and free of repeats; n is the same in every thread
+  int n = 0, last = -1;
+  for (int k = 0; k < kBlockNbr; ++k) {
+    int m = INT32_MAX;
+    if (live)
+      for (int e = 0; e < nv; ++e) {
+        const int j = elab[(size_t)i * kMaxVerts + e];
+        if (j < 0) continue;
+        const int b = sagg[j];
+        if (b >= 0 && b != a && b > last && b < m) m = b;
+      }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+    __syncthreads();
+    int bm = INT32_MAX;
+    for (int w = 0; w < kCellBlock / 64; ++w) bm = min(bm, wmin[w]);
+    __syncthreads();
+    if (bm == INT32_MAX) break;
+    if (threadIdx.x == 0) ids[k] = bm;
+    n = k + 1;
+    last = bm;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const int b = ids[k];
+    double v = 0.0;
+    if (live)
+      for (int e = 0; e < nv; ++e) {
+        const int j = elab[(size_t)i * kMaxVerts + e];
+        if (j >= 0 && sagg[j] == b) v += h[(size_t)i * kMaxVerts + e];
+      }
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) row[k] = CoarseEntry{b, 0, v};
+  }
+  if ((int)threadIdx.x >= n && threadIdx.x < kBlockNbr) row[threadIdx.x] = CoarseEntry{-1, 0, 0.0};
 }
 
 // Coarse part of the preconditioner for kCoarseRows patches of one problem (a task): R = P^T r
@@ -389,14 +424,11 @@ hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const 
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_assemble(const BlockDesc* blocks, const CoarseDesc* cdesc,
-                                  const int* sagg, const uint8_t* nedge, const int* elab,
-                                  const double* h, double* ac, size_t ac_len, int nblk,
-                                  hipStream_t st) {
-  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)((ac_len + 255) / 256)), dim3(256), 0, st, ac,
-                     ac_len);
-  hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, cdesc,
-                     sagg, nedge, elab, h, ac);
+hipError_t launch_coarse_rows(const BlockDesc* blocks, const int* bagg, const int* sagg,
+                              const uint8_t* nedge, const int* elab, const double* h,
+                              CoarseEntry* out, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(coarse_rows_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, bagg, sagg,
+                     nedge, elab, h, out);
   return hipGetLastError();
 }
 

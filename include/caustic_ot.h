@@ -205,6 +205,8 @@ typedef struct {
   long long newton_steps;     /* cot_solve_newton: Hessians built (one per Newton step) */
   long long cg_iterations;    /* cot_solve_newton: conjugate-gradient iterations */
   double coarse_seconds;      /* cot_solve_newton: host time of the coarse-matrix inverses */
+  long long cg_restarts;      /* cot_solve_newton: steps whose CG diverged with the full
+                                 preconditioner and was redone with Jacobi alone */
 } cot_stats;
 
 int cot_get_stats(cot_context* ctx, cot_stats* out);

@@ -140,3 +140,17 @@ def test_newton_coarse_batch_of_uneven_problems(gpu_available):
         assert r["converged"] and r["newton_iterations"] >= 1, r
         orc = O.evaluate(p.sites, wk, p.nu, p.rho, p.proj_domain)
         assert O.max_rel_error(orc.mass, p.nu) < 1e-7 + 1e-9
+
+
+def test_newton_solve_is_deterministic(gpu_available, c1):
+    """Two mixed-precision solves (L-BFGS, then Newton steps with the coarse term) in fresh
+    contexts from the same start give bit-identical weights: the coarse matrix is summed in a
+    fixed order, and no other reduction of the path depends on scheduling."""
+    print("[case] Newton solve c1 mixed twice, bitwise comparison", flush=True)
+    out = []
+    for _ in range(2):
+        with _batch([c1], precision="mixed") as b:
+            w, rep = b.solve(tol=1e-6, h0_threshold=0.0, newton_from=1.0)
+        assert rep[0]["converged"] and rep[0]["newton_iterations"] >= 1, rep
+        out.append(w)
+    assert np.array_equal(out[0], out[1])

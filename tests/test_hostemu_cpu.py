@@ -123,20 +123,23 @@ def _patch_laplacian(n, seed):
 
 @pytest.mark.parametrize("n,seed", [(2, 1), (7, 2), (64, 3), (200, 4)])
 def test_coarse_inverse_matches_numpy(n, seed):
-    """The Newton preconditioner's coarse inverse: (A_c + s 11^T)^-1 with A_c the symmetrised
-    Laplacian (diagonal = minus the off-diagonal row sums), s = trace / n^2 -- against
-    numpy.linalg.inv of the same matrix built independently."""
+    """The Newton preconditioner's coarse inverse: the pseudo-inverse of A_c, the symmetrised
+    Laplacian (diagonal = minus the off-diagonal row sums) -- against numpy.linalg.pinv of the
+    same matrix built independently, and through its defining properties."""
     a = _patch_laplacian(n, seed)
     got = E.coarse_inverse(a)
     sym = 0.5 * (a + a.T)
     np.fill_diagonal(sym, 0.0)
     lap = sym - np.diag(sym.sum(axis=1))
-    want = np.linalg.inv(lap + np.trace(lap) / n ** 2)
+    want = np.linalg.pinv(lap)  # the graph is connected: null space = the constants
     assert np.allclose(got, got.T, rtol=0, atol=1e-14 * np.abs(want).max())
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
-    # the constant vector: eigenvalue 1 / (mean diagonal) of the inverse
+    # nothing along the constant vector; an inverse on its complement
     one = np.ones(n)
-    assert np.allclose(got @ one, one / (np.trace(lap) / n), rtol=1e-10)
+    assert np.abs(got @ one).max() <= 1e-12 * np.abs(want).max() * n
+    x = np.random.default_rng(seed).normal(size=n)
+    x -= x.mean()
+    assert np.allclose(got @ (lap @ x), x, rtol=0, atol=1e-9 * np.abs(x).max())
 
 
 def test_coarse_inverse_of_zero_and_disconnected():
