@@ -203,12 +203,16 @@ def main():
 
         tune_report = {}
         inits = []
+        # the measured stretch: each level's L-BFGS phase up to the Newton hand-off (or 40
+        # iterations of a pure L-BFGS solve)
+        stretch = (dict(iters=400, stop_tol=args.newton_from) if args.newton_from
+                   else dict(iters=40))
         w_probe, _ = start_weights(record=inits)
         for li, cps in enumerate(coarse):
-            tuning["levels"][li], r = AT.tune(cps, args.coarse_precision, w0=inits[li], iters=40,
-                                              log=log)
+            tuning["levels"][li], r = AT.tune(cps, args.coarse_precision, w0=inits[li],
+                                              log=log, **stretch)
             tune_report[f"level{levels[li]}"] = r
-        tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, iters=40, log=log)
+        tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, log=log, **stretch)
         tune_report["final"] = r
         log(f"[autotune] {json.dumps(tune_report)}")
 

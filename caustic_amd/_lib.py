@@ -44,7 +44,7 @@ class cot_solve_opts(ctypes.Structure):
                 ("h0_threshold", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("device_loop", ctypes.c_int), ("accept_rule", ctypes.c_int),
                 ("repair_rounds", ctypes.c_int), ("newton_sweeps", ctypes.c_int),
-                ("newton_coarse", ctypes.c_int)]
+                ("newton_coarse", ctypes.c_int), ("small_cg", ctypes.c_int)]
 
 
 class cot_report(ctypes.Structure):

@@ -20,8 +20,11 @@ Knobs (fields of cot_options):
                     evaluations.
 
 Objective: device time per full evaluation -- construction, integration and repair kernels, read
-from the in-kernel timers (cot_get_stats) -- over a short stretch of the real solve: `iters`
-L-BFGS iterations from the caller's weights, replayed for every candidate, the smaller of
+from the in-kernel timers (cot_get_stats) -- over a stretch of the real solve: L-BFGS from the
+caller's weights for `iters` iterations, or with `stop_tol` until the max-rel error reaches it
+(the whole L-BFGS phase before a Newton hand-off: a fixed 40-iteration stretch from the start
+ranked two bucket densities the other way round from the full phase), replayed for every
+candidate, the smaller of
 `repeats` measurements (the repair passes' work lists fill in a different order from run to run,
 which moves sites between blocks; the resulting spread is one-sided, up to ~1.8x slower in rare
 runs, and one such outlier is enough to make the descent reject the right shape).  Coordinate
@@ -114,18 +117,20 @@ def measure(problems, precision: str, cfg: dict, w0, iters: int, solve_kw: dict)
 
 
 def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw: dict = None,
-         use_cache: bool = True, log=None, repeats: int = 2):
+         use_cache: bool = True, log=None, repeats: int = 2, stop_tol: float = None):
     """Launch shapes for `problems` (a list of workloads.Problem).  w0: weights to start the
     measured iterations from (zeros by default; a warm start is more representative of a real
-    solve).  Returns (config dict for OTBatch(tuning=...), report dict)."""
-    key = shape_key(problems, precision)
+    solve).  stop_tol: measure L-BFGS from w0 until max-rel stop_tol (at most `iters`
+    iterations) instead of exactly `iters` iterations.  Returns (config dict for
+    OTBatch(tuning=...), report dict)."""
+    key = shape_key(problems, precision) + (f"|to={stop_tol:g}" if stop_tol else f"|it={iters}")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
             return dict(hit["config"]), dict(hit, cached=True)
     solve_kw = dict(solve_kw or {})
     solve_kw.setdefault("history", 10)
-    solve_kw.setdefault("tol", 1e-12)  # run the full stretch
+    solve_kw.setdefault("tol", stop_tol if stop_tol else 1e-12)  # 1e-12: the full stretch
     solve_kw.setdefault("stop", "maxrel")
     solve_kw.setdefault("h0_threshold", 0.0)
     n = sum(int(p.n) for p in problems)

@@ -100,6 +100,7 @@ hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const 
                           hipStream_t st);
 hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_new, int cur,
                                int first, hipStream_t st);
+hipError_t launch_cg_small(const SmallCgArgs& a, hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
                             const double* d, double* w, int nblk, hipStream_t st);
 hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
@@ -222,6 +223,8 @@ struct cot_context {
   int* d_bagg = nullptr;
   int* d_sagg = nullptr;   // site -> global patch, rewritten with every Hessian
   int *d_agg_start = nullptr, *d_agg_blk = nullptr;
+  int* d_agg_prob = nullptr;  // patch -> problem
+  double* d_cgout = nullptr;  // the small-system CG's results (4 doubles)
   CoarseDesc* d_cdesc = nullptr;
   int4* d_ctask = nullptr;
   double *d_ainv = nullptr, *d_cvec = nullptr, *d_prc = nullptr;
@@ -832,6 +835,7 @@ void cot_default_solve_opts(cot_solve_opts* o) {
   o->repair_rounds = 3;
   o->newton_sweeps = 2;
   o->newton_coarse = 256;
+  o->small_cg = 1;
 }
 
 const char* cot_last_error(void) { return g_err.c_str(); }
@@ -1305,6 +1309,7 @@ int ensure_newton(cot_context* c) {
   CHK(dalloc(c, &c->d_ppq, nb));
   CHK(dalloc(c, &c->d_prr, nb));
   CHK(dalloc(c, &c->d_nshift, c->np));
+  CHK(dalloc(c, &c->d_cgout, 4));
   return COT_OK;
 }
 
@@ -1317,7 +1322,7 @@ int build_coarse(cot_context* c, int maxp) {
   maxp = std::min(std::max(maxp, 0), kMaxCoarse);
   if (maxp == c->coarse_max) return COT_OK;
   const int np = c->np, nb = c->nblk;
-  std::vector<int> bagg(nb, -1), start, blk;
+  std::vector<int> bagg(nb, -1), start, blk, aprob;
   std::vector<CoarseDesc> cd(np, CoarseDesc{0, 0, 0, 0});
   std::vector<int4> tasks;
   int total = 0;
@@ -1354,6 +1359,7 @@ int build_coarse(cot_context* c, int maxp) {
     cd[p] = CoarseDesc{total, na, (int)moff, 0};
     for (int a = 0; a < na; ++a) {
       start.push_back((int)blk.size());
+      aprob.push_back(p);
       for (int b : members[a]) {
         blk.push_back(b);
         bagg[b] = total + a;
@@ -1377,6 +1383,7 @@ int build_coarse(cot_context* c, int maxp) {
   if (!c->d_rsum) CHK(dalloc(c, &c->d_rsum, nb));
   CHK(dalloc(c, &c->d_agg_start, start.size()));
   CHK(dalloc(c, &c->d_agg_blk, blk.size()));
+  CHK(dalloc(c, &c->d_agg_prob, aprob.size()));
   CHK(dalloc(c, &c->d_cdesc, np));
   CHK(dalloc(c, &c->d_ctask, tasks.size()));
   if (!c->d_crows) CHK(dalloc(c, &c->d_crows, (size_t)nb * kBlockNbr));
@@ -1395,6 +1402,8 @@ int build_coarse(cot_context* c, int maxp) {
   HIPCHK(hipMemcpy(c->d_agg_start, start.data(), sizeof(int) * start.size(),
                    hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_agg_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_agg_prob, aprob.data(), sizeof(int) * aprob.size(),
+                   hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_cdesc, cd.data(), sizeof(CoarseDesc) * np, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_ctask, tasks.data(), sizeof(int4) * tasks.size(),
                    hipMemcpyHostToDevice));
@@ -1658,8 +1667,49 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       }
       return COT_OK;
     };
+    // a small system: the whole CG in one workgroup, one launch
+    auto run_cg_small = [&](const CgArgs& a, int& iters) -> int {
+      SmallCgArgs sa{};
+      sa.blocks = c->d_blocks;
+      sa.nblk = nb;
+      sa.sweeps = a.sweeps;
+      sa.npatch = a.ntask > 0 ? c->coarse_total : 0;
+      sa.max_iter = kMaxCG;
+      sa.active = c->d_active;
+      sa.g = c->d_g;
+      sa.shift = c->d_nshift;
+      sa.diag = c->d_hdiag;
+      sa.nedge = c->d_nedge;
+      sa.elab = c->d_elab;
+      sa.h = c->d_hh;
+      sa.sagg = a.ntask > 0 ? c->d_sagg : nullptr;
+      sa.agg_start = c->d_agg_start;
+      sa.agg_blk = c->d_agg_blk;
+      sa.agg_prob = c->d_agg_prob;
+      sa.cdesc = c->d_cdesc;
+      sa.ainv = c->d_ainv;
+      sa.x = c->d_nx;
+      sa.z = c->d_nz;
+      sa.p = c->d_np;
+      sa.eta2 = eta * eta;
+      sa.out = c->d_cgout;
+      HIPCHK(launch_cg_small(sa, st));
+      double res[4];
+      HIPCHK(hipMemcpyAsync(res, c->d_cgout, sizeof(res), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      iters = (int)res[0];
+      rr0 = res[1];
+      rr = res[2];
+      if (res[3] != 0.0) iters = -iters;
+      return COT_OK;
+    };
+    const bool small = nb <= kSmallBlocks && o.small_cg;
     int cg = 0;
-    CHK(run_cg(ca, 0, cg));
+    if (small) {
+      CHK(run_cg_small(ca, cg));
+    } else {
+      CHK(run_cg(ca, 0, cg));
+    }
     if (cg < 0 && (coarse || sweeps > 0)) {
       // the preconditioned iteration diverged (rounding along the Hessian's null space at
       // residuals near the FP64 floor): redo the step's CG with Jacobi alone
@@ -1667,7 +1717,11 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
         std::printf("[cot] newton CG diverged after %d iterations; Jacobi restart\n", -cg);
       c->stats.cg_iterations += -cg;
       c->stats.cg_restarts += 1;
-      CHK(run_cg(cj, 1, cg));
+      if (small) {
+        CHK(run_cg_small(cj, cg));
+      } else {
+        CHK(run_cg(cj, 1, cg));
+      }
     }
     if (cg < 0) cg = -cg;  // a diverged Jacobi CG: the line search rejects its direction
     c->stats.newton_steps += 1;

@@ -90,6 +90,33 @@ struct CoarseEntry {
   double v;
 };
 
+// The whole conjugate-gradient solve of a small system in one workgroup (cot_newton.hip): at
+// most kSmallBlocks blocks, one site per thread (measured on MI355X: 8 blocks, 1,024 sites, gain
+// ~10% per Newton step over the multi-kernel iteration; at 32 blocks, four sites per thread on
+// one compute unit lose to 32 blocks on 32 units).
+constexpr int kSmallBlocks = 8;
+constexpr int kSmallThreads = 1024;
+struct SmallCgArgs {
+  const BlockDesc* blocks;
+  int nblk, sweeps, npatch, max_iter;
+  const uint8_t* active;
+  const double* g;
+  const double* shift;
+  const double* diag;
+  const uint8_t* nedge;
+  const int* elab;
+  const double* h;
+  const int* sagg;        // nullptr: no coarse term
+  const int* agg_start;
+  const int* agg_blk;
+  const int* agg_prob;    // patch -> problem
+  const CoarseDesc* cdesc;
+  const double* ainv;
+  double *x, *z, *p;      // N each (z, p: the neighbours' values between barriers)
+  double eta2;            // stop at r.r <= eta2 r0.r0
+  double* out;            // [0] iterations, [1] r0.r0, [2] r.r, [3] 1 when the residual diverged
+};
+
 // Buffers of one conjugate-gradient iteration (cot_newton.hip).  part_rz holds two slots of nblk
 // r.z partials, part_rzc two slots of ntask coarse partials; ntask == 0 runs without the coarse
 // term (tasks, cdesc, agg_*, rsum, ainv, cvec, sagg unused).

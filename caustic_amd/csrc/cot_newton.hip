@@ -404,6 +404,194 @@ __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
   }
 }
 
+// The conjugate-gradient solve of H x = shift - g (the right-hand side of cg_init_kernel) for a
+// small system, start to finish in one workgroup: the same preconditioner as the multi-kernel
+// iteration (Jacobi, `sweeps` block-local sweeps, the coarse patch term), the same stopping rule,
+// workgroup barriers in place of kernel boundaries and no host round trip per chunk.  Thread t
+// serves lane t % 128 of blocks t / 128, t / 128 + 8, ... (at most kSmallBlocks blocks); the
+// residual and q stay in registers, p and z go through memory for the neighbours, the solution
+// accumulates in memory.
+// Every branch that contains a barrier depends only on workgroup-wide values.
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kSmallThreads / 64; ++k) t += red[k];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) {
+  constexpr int kRounds = kSmallBlocks / (kSmallThreads / kCellBlock);
+  __shared__ double red[kSmallThreads / 64];
+  __shared__ double wsum[2 * kSmallBlocks];
+  __shared__ double Rp[kSmallBlocks];
+  __shared__ double cp[kSmallBlocks];
+  const int t = threadIdx.x, l = t & (kCellBlock - 1), b0 = t / kCellBlock;
+  const int half = (t >> 6) & 1, lane = t & 63;
+  // per round: the site (-1: none), its block's size, diagonal, residual, p and z; the solution
+  // accumulates in memory (a.x)
+  int site[kRounds], bcount[kRounds];
+  double dg[kRounds], r[kRounds], pl[kRounds], zl[kRounds];
+#pragma unroll
+  for (int m = 0; m < kRounds; ++m) {
+    site[m] = -1;
+    bcount[m] = 0;
+    dg[m] = r[m] = pl[m] = zl[m] = 0.0;
+    const int b = b0 + m * (kSmallThreads / kCellBlock);
+    if (b < a.nblk) {
+      const BlockDesc bd = a.blocks[b];
+      bcount[m] = bd.count;
+      if (l < bd.count) {
+        const int i = bd.first + l;
+        site[m] = i;
+        dg[m] = a.diag[i];
+        if (a.active[bd.prob] && dg[m] > 0.0) r[m] = a.shift[bd.prob] - a.g[i];
+        a.x[i] = 0.0;
+      }
+    }
+  }
+  // z = M r, returns r.z
+  auto apply_m = [&]() -> double {
+#pragma unroll
+    for (int m = 0; m < kRounds; ++m) {
+      zl[m] = dg[m] > 0.0 ? r[m] / dg[m] : 0.0;
+      if (site[m] >= 0) a.z[site[m]] = zl[m];
+    }
+    for (int sw = 0; sw < a.sweeps; ++sw) {
+      __syncthreads();
+      double zn[kRounds];
+#pragma unroll
+      for (int m = 0; m < kRounds; ++m) {
+        zn[m] = zl[m];
+        const int i = site[m];
+        if (i >= 0 && dg[m] > 0.0) {
+          double acc = r[m];
+          const int nv = a.nedge[i];
+          for (int k = 0; k < nv; ++k) {
+            const int bf = i - l;  // the block's first site
+            const int j = a.elab[(size_t)i * kMaxVerts + k] - bf;
+            if (j >= 0 && j < bcount[m]) acc += a.h[(size_t)i * kMaxVerts + k] * a.z[bf + j];
+          }
+          zn[m] = acc / dg[m];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kRounds; ++m) {
+        zl[m] = zn[m];
+        if (site[m] >= 0) a.z[site[m]] = zl[m];
+      }
+    }
+    if (a.sagg) {
+      // patch sums of r: per block the two waves' sums, then the patch's blocks in order
+#pragma unroll
+      for (int m = 0; m < kRounds; ++m) {
+        const int b = b0 + m * (kSmallThreads / kCellBlock);
+        if (b < a.nblk) {  // uniform across the wave
+          double v = site[m] >= 0 ? r[m] : 0.0;
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+          if (lane == 0) wsum[2 * b + half] = v;
+        }
+      }
+      __syncthreads();
+      for (int q = t; q < a.npatch; q += kSmallThreads) {
+        double v = 0.0;
+        for (int k = a.agg_start[q]; k < a.agg_start[q + 1]; ++k) {
+          const int b = a.agg_blk[k];
+          v += wsum[2 * b] + wsum[2 * b + 1];
+        }
+        Rp[q] = v;
+      }
+      __syncthreads();
+      for (int q = t; q < a.npatch; q += kSmallThreads) {
+        const CoarseDesc cd = a.cdesc[a.agg_prob[q]];
+        const double* row = a.ainv + (size_t)cd.moff + (size_t)(q - cd.a0) * cd.na;
+        double v = 0.0;
+        for (int k = 0; k < cd.na; ++k) v += row[k] * Rp[cd.a0 + k];
+        cp[q] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kRounds; ++m)
+        if (site[m] >= 0) {
+          const int q = a.sagg[site[m]];
+          if (q >= 0) zl[m] += cp[q];
+        }
+    }
+    double rz = 0.0;
+#pragma unroll
+    for (int m = 0; m < kRounds; ++m) rz += r[m] * zl[m];
+    return wg_sum(rz, red);
+  };
+
+  double rr0 = 0.0;
+#pragma unroll
+  for (int m = 0; m < kRounds; ++m) rr0 += r[m] * r[m];
+  rr0 = wg_sum(rr0, red);
+  double rz = apply_m();
+#pragma unroll
+  for (int m = 0; m < kRounds; ++m) {
+    pl[m] = zl[m];
+    if (site[m] >= 0) a.p[site[m]] = pl[m];
+  }
+  int it = 0;
+  double rr = rr0;
+  int diverged = 0;
+  while (it < a.max_iter) {
+    __syncthreads();  // p of every site written
+    double q[kRounds], pq = 0.0;
+#pragma unroll
+    for (int m = 0; m < kRounds; ++m) {
+      q[m] = 0.0;
+      const int i = site[m];
+      if (i < 0) continue;
+      double sacc = dg[m] * pl[m];
+      const int nv = a.nedge[i];
+      for (int k = 0; k < nv; ++k) {
+        const int j = a.elab[(size_t)i * kMaxVerts + k];
+        if (j >= 0) sacc -= a.h[(size_t)i * kMaxVerts + k] * a.p[j];
+      }
+      q[m] = sacc;
+      pq += pl[m] * sacc;
+    }
+    pq = wg_sum(pq, red);
+    const double alpha = pq > 0.0 ? rz / pq : 0.0;
+    double rrl = 0.0;
+#pragma unroll
+    for (int m = 0; m < kRounds; ++m) {
+      if (site[m] >= 0) a.x[site[m]] += alpha * pl[m];
+      r[m] -= alpha * q[m];
+      rrl += r[m] * r[m];
+    }
+    rr = wg_sum(rrl, red);
+    ++it;
+    if (!(rr <= 1e8 * rr0)) {
+      diverged = 1;
+      break;
+    }
+    if (!(rr > a.eta2 * rr0)) break;
+    const double rzn = apply_m();
+    const double beta = rz != 0.0 ? rzn / rz : 0.0;
+    rz = rzn;
+    __syncthreads();  // every thread's reads of p (H p above) are done
+#pragma unroll
+    for (int m = 0; m < kRounds; ++m) {
+      pl[m] = zl[m] + beta * pl[m];
+      if (site[m] >= 0) a.p[site[m]] = pl[m];
+    }
+  }
+  if (t == 0) {
+    a.out[0] = it;
+    a.out[1] = rr0;
+    a.out[2] = rr;
+    a.out[3] = diverged;
+  }
+}
+
 // w += step[prob] * d for the problems with active[prob] != 0.
 __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* blocks,
                                                                const uint8_t* active,
@@ -472,6 +660,11 @@ hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_n
   if (coarse)
     hipLaunchKernelGGL(cg_coarse_kernel, dim3(a.ntask), dim3(256), 0, st, a.tasks, a.cdesc,
                        a.agg_start, a.agg_blk, a.rsum, a.ainv, a.cvec, rzc_oth);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_small(const SmallCgArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(cg_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, a);
   return hipGetLastError();
 }
 

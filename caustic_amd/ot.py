@@ -185,7 +185,8 @@ class OTBatch:
               initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
               verbose: int = 0, raise_on_fail: bool = True, device_loop=True,
               accept_rule: str = "auto", repair_rounds: int = 3, newton_from: float = None,
-              newton_max_iter: int = 50, newton_sweeps: int = 2, newton_coarse: int = 256):
+              newton_max_iter: int = 50, newton_sweeps: int = 2, newton_coarse: int = 256,
+              newton_small_cg: bool = True):
         """L-BFGS dual solve from w0 (zeros by default).  Returns (w, reports).
 
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
@@ -196,14 +197,14 @@ class OTBatch:
         with L-BFGS.  Reports then add "newton_iterations".  newton_sweeps: block-local
         Jacobi sweeps of the conjugate gradients' preconditioner (0: plain Jacobi);
         newton_coarse: at most this many patches per problem in its two-level coarse term
-        (0: none)."""
+        (0: none); newton_small_cg: solve a system of at most 8 blocks in one workgroup."""
         kw = dict(history=history, max_iter=max_iter, initial_step=initial_step, stop=stop,
                   h0_threshold=h0_threshold, verbose=verbose, device_loop=device_loop,
                   accept_rule=accept_rule, repair_rounds=repair_rounds)
         if newton_from is not None and stop == "maxrel" and newton_from > tol:
             w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False, **kw)
             w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds, device_loop,
-                                  newton_sweeps, newton_coarse)
+                                  newton_sweeps, newton_coarse, newton_small_cg)
             if all(r["converged"] for r in r2):
                 w3, r3 = w2, None
             else:
@@ -250,7 +251,7 @@ class OTBatch:
         return w, out
 
     def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=2,
-                coarse=256):
+                coarse=256, small_cg=True):
         """Damped Newton steps (cot_solve_newton) from w0 to max-rel tol; device_loop == 2
         launches the conjugate-gradient kernels directly instead of replaying a graph."""
         w = self._w(w0).copy()
@@ -264,6 +265,7 @@ class OTBatch:
         o.device_loop = int(device_loop)
         o.newton_sweeps = int(sweeps)
         o.newton_coarse = int(coarse)
+        o.small_cg = int(bool(small_cg))
         reps = (L.cot_report * self.n_problems)()
         rc = self._lib.cot_solve_newton(self._ctx, _ptr(w), ctypes.byref(o), reps)
         L.check(rc, allow=(L.COT_ENOTCONV,))

@@ -92,6 +92,9 @@ typedef struct {
                                P (P^T H P)^-1 P^T (inverted on the host each step, ~n^3/2
                                multiply-adds: 0.3-1.7 ms at 256) and cuts the iterations ~4x
                                at N = 65,536 */
+  int small_cg;             /* cot_solve_newton: run the whole CG of a system of at most 8
+                               blocks of 128 sites in one workgroup, one launch per Newton
+                               step (default 1; 0: the multi-kernel iteration at every size) */
 } cot_solve_opts;
 
 typedef struct {

@@ -154,3 +154,31 @@ def test_newton_solve_is_deterministic(gpu_available, c1):
         assert rep[0]["converged"] and rep[0]["newton_iterations"] >= 1, rep
         out.append(w)
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("coarse,sweeps", [(0, 0), (4, 2)])
+def test_small_system_cg_matches_the_multi_kernel_cg(gpu_available, coarse, sweeps):
+    """A 1,024-site problem (8 blocks) fits the one-workgroup CG: from the same start both CG
+    paths give Newton solves that the oracle confirms, with the same Newton step count and CG
+    iteration counts within 10% (the paths sum in different orders; the multi-kernel one checks
+    every 16-32 iterations)."""
+    c1 = W.plain_fp64(seed=41, n_side=32, res=128)
+    print(f"[case] one-workgroup vs multi-kernel CG, 32x32 fp64, coarse {coarse} sweeps {sweeps}",
+          flush=True)
+    with _batch([c1], precision="fp64") as b:
+        w1, _ = b.solve(tol=1e-2, h0_threshold=0.0, raise_on_fail=False)
+    got = {}
+    for small in (True, False):
+        with _batch([c1], precision="fp64") as b:
+            b.reset_stats()
+            w, rep = b._newton(w1, 1e-8, 50, 1, 3, True, sweeps, coarse, small)
+            st = b.stats()
+        assert rep[0]["converged"], (small, rep)
+        orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+        assert O.max_rel_error(orc.mass, c1.nu) < 1e-8 + 1e-9
+        got[small] = (rep[0]["iterations"], st["cg_iterations"], st["cg_restarts"])
+        print(f"[case] small {small}: {got[small]}", flush=True)
+    assert got[True][0] == got[False][0], got
+    assert got[True][2] == got[False][2] == 0, got
+    # the multi-kernel path rounds its count up to a whole chunk per step
+    assert got[True][1] <= got[False][1] <= 1.1 * got[True][1] + 32 * got[True][0], got
