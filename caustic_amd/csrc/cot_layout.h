@@ -86,10 +86,11 @@ inline void build_prefix(const double* rho, int pm, int pn, bool fp32, double4* 
 // In place: a (n x n, row-major) holds the off-diagonal entries of the patches' Laplacian as the
 // device accumulated them (each side of an edge adds to its own row, so a[i][j] and a[j][i] differ
 // by rounding); on return it holds the pseudo-inverse of A_c, A_c symmetrised with its diagonal
-// set to minus the off-diagonal row sums: J (A_c + s 11^T)^-1 J with J = I - 11^T / n and
-// s = trace / n^2 (the rank-one term makes the constant vector, A_c's null space, invertible;
-// J removes it again).  Cholesky factor L, then L^-1 row by row, then L^-T L^-1; about n^3 / 2
-// multiply-adds.  A zero or numerically indefinite matrix leaves zeros (no coarse correction).
+// set to minus the off-diagonal row sums.  A narrow band (bandwidth <= n / 4, the usual case)
+// takes the grounded banded route below (~1.5 bw n^2 multiply-adds); otherwise
+// J (A_c + s 11^T)^-1 J with J = I - 11^T / n by a dense Cholesky factor, L^-1 and L^-T L^-1
+// (~n^3 / 2).  A zero, disconnected or numerically indefinite matrix leaves zeros (no coarse
+// correction).
 inline void coarse_inverse(double* a, int n) {
   for (int i = 0; i < n; ++i)
     for (int j = i + 1; j < n; ++j) {
@@ -110,52 +111,115 @@ inline void coarse_inverse(double* a, int n) {
     std::fill(a, a + nn, 0.0);
     return;
   }
-  const double sig = tr / ((double)n * n);
-  for (size_t k = 0; k < nn; ++k) a[k] += sig;
-  // Cholesky, lower triangle in place
-  for (int j = 0; j < n; ++j) {
-    double* rj = a + (size_t)j * n;
-    double d = rj[j];
-    for (int k = 0; k < j; ++k) d -= rj[k] * rj[k];
-    if (!(d > 1e-14 * tr / n)) {
-      std::fill(a, a + nn, 0.0);
-      return;
+  // bandwidth of the patch graph (patches are numbered row by row over the problem's grid of
+  // rectangles, so a grid of k columns has bandwidth ~k + 1)
+  int bw = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i - bw; ++j)
+      if (a[(size_t)i * n + j] != 0.0) {
+        bw = i - j;
+        break;
+      }
+  if (n > 8 && 4 * bw <= n) {
+    // Banded path.  Ground the last patch: A_red (the first m = n - 1 rows and columns) is
+    // SPD for a connected patch graph and keeps the band.  Its banded Cholesky factor, then
+    // G = A_red^-1 by two banded triangular solves with all right-hand sides (~1.5 bw n^2
+    // multiply-adds), then the pseudo-inverse J G J (G padded with a zero last row and
+    // column): for a connected graph Laplacian the centred grounded inverse is the
+    // pseudo-inverse.
+    const int m = n - 1, w = bw + 1;
+    std::vector<double> L((size_t)m * w, 0.0);  // L[i][i - k] at L[i * w + k], k = 0..bw
+    auto el = [&](int i, int j) -> double& { return L[(size_t)i * w + (i - j)]; };
+    for (int j = 0; j < m; ++j) {
+      double d = a[(size_t)j * n + j];
+      for (int k = std::max(0, j - bw); k < j; ++k) d -= el(j, k) * el(j, k);
+      if (!(d > 1e-14 * tr / n)) {
+        std::fill(a, a + nn, 0.0);
+        return;
+      }
+      el(j, j) = std::sqrt(d);
+      const double inv = 1.0 / el(j, j);
+      for (int i = j + 1; i < std::min(m, j + bw + 1); ++i) {
+        double v = a[(size_t)i * n + j];
+        for (int k = std::max(0, i - bw); k < j; ++k) v -= el(i, k) * el(j, k);
+        el(i, j) = v * inv;
+      }
     }
-    rj[j] = std::sqrt(d);
-    const double inv = 1.0 / rj[j];
-    for (int i = j + 1; i < n; ++i) {
-      double* ri = a + (size_t)i * n;
-      double v = ri[j];
-      for (int k = 0; k < j; ++k) v -= ri[k] * rj[k];
-      ri[j] = v * inv;
+    // all right-hand sides at once, row by row (independent axpys over rows, which vectorise,
+    // instead of one dependent reduction per column): rows of Y = L^-1 (row k is zero right of
+    // column k), then rows of X = L^-T Y from the bottom up
+    std::fill(a, a + nn, 0.0);
+    for (int k = 0; k < m; ++k) {
+      double* rk = a + (size_t)k * n;
+      rk[k] = 1.0;
+      for (int l = std::max(0, k - bw); l < k; ++l) {
+        const double f = el(k, l);
+        const double* rl = a + (size_t)l * n;
+        for (int j = 0; j <= l; ++j) rk[j] -= f * rl[j];
+      }
+      const double inv = 1.0 / el(k, k);
+      for (int j = 0; j <= k; ++j) rk[j] *= inv;
     }
-  }
-  // M = L^-1 (lower): row i = (e_i - sum_{k<i} L[i][k] row k) / L[i][i]
-  std::vector<double> m(nn, 0.0);
-  for (int i = 0; i < n; ++i) {
-    double* mi = m.data() + (size_t)i * n;
-    const double* li = a + (size_t)i * n;
-    for (int k = 0; k < i; ++k) {
-      const double f = li[k];
+    for (int k = m - 1; k >= 0; --k) {
+      double* rk = a + (size_t)k * n;
+      for (int l = k + 1; l < std::min(m, k + bw + 1); ++l) {
+        const double f = el(l, k);
+        const double* rl = a + (size_t)l * n;
+        for (int j = 0; j < m; ++j) rk[j] -= f * rl[j];
+      }
+      const double inv = 1.0 / el(k, k);
+      for (int j = 0; j < m; ++j) rk[j] *= inv;
+    }
+  } else {
+    // Dense path: (A_c + s 11^T)^-1, s = trace / n^2 (the rank-one term makes the constant
+    // vector invertible; the projection below removes it again).  Cholesky, lower triangle in
+    // place
+    const double sig = tr / ((double)n * n);
+    for (size_t k = 0; k < nn; ++k) a[k] += sig;
+    for (int j = 0; j < n; ++j) {
+      double* rj = a + (size_t)j * n;
+      double d = rj[j];
+      for (int k = 0; k < j; ++k) d -= rj[k] * rj[k];
+      if (!(d > 1e-14 * tr / n)) {
+        std::fill(a, a + nn, 0.0);
+        return;
+      }
+      rj[j] = std::sqrt(d);
+      const double inv = 1.0 / rj[j];
+      for (int i = j + 1; i < n; ++i) {
+        double* ri = a + (size_t)i * n;
+        double v = ri[j];
+        for (int k = 0; k < j; ++k) v -= ri[k] * rj[k];
+        ri[j] = v * inv;
+      }
+    }
+    // M = L^-1 (lower): row i = (e_i - sum_{k<i} L[i][k] row k) / L[i][i]
+    std::vector<double> m(nn, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double* mi = m.data() + (size_t)i * n;
+      const double* li = a + (size_t)i * n;
+      for (int k = 0; k < i; ++k) {
+        const double f = li[k];
+        const double* mk = m.data() + (size_t)k * n;
+        for (int j = 0; j <= k; ++j) mi[j] -= f * mk[j];
+      }
+      mi[i] = 1.0;
+      const double inv = 1.0 / li[i];
+      for (int j = 0; j <= i; ++j) mi[j] *= inv;
+    }
+    // a = M^T M: a[p][q] = sum_{k >= max(p, q)} M[k][p] M[k][q]
+    std::fill(a, a + nn, 0.0);
+    for (int k = 0; k < n; ++k) {
       const double* mk = m.data() + (size_t)k * n;
-      for (int j = 0; j <= k; ++j) mi[j] -= f * mk[j];
+      for (int p = 0; p <= k; ++p) {
+        const double f = mk[p];
+        double* ap = a + (size_t)p * n;
+        for (int q = 0; q <= p; ++q) ap[q] += f * mk[q];
+      }
     }
-    mi[i] = 1.0;
-    const double inv = 1.0 / li[i];
-    for (int j = 0; j <= i; ++j) mi[j] *= inv;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) a[(size_t)p * n + q] = a[(size_t)q * n + p];
   }
-  // a = M^T M: a[p][q] = sum_{k >= max(p, q)} M[k][p] M[k][q]
-  std::fill(a, a + nn, 0.0);
-  for (int k = 0; k < n; ++k) {
-    const double* mk = m.data() + (size_t)k * n;
-    for (int p = 0; p <= k; ++p) {
-      const double f = mk[p];
-      double* ap = a + (size_t)p * n;
-      for (int q = 0; q <= p; ++q) ap[q] += f * mk[q];
-    }
-  }
-  for (int p = 0; p < n; ++p)
-    for (int q = p + 1; q < n; ++q) a[(size_t)p * n + q] = a[(size_t)q * n + p];
   // project out the constant vector (J a J, J = I - 11^T / n): the pseudo-inverse of A_c, so the
   // coarse term adds nothing along the Hessian's null space
   std::vector<double> mean(n, 0.0);

@@ -151,3 +151,22 @@ def test_coarse_inverse_of_zero_and_disconnected():
     a[0, 1] = a[1, 0] = -1.0
     a[2, 3] = a[3, 2] = -1.0
     assert not E.coarse_inverse(a).any()
+
+
+def test_coarse_inverse_wide_band_and_disconnected_banded():
+    """A randomly renumbered patch graph (bandwidth ~n: the dense route) still gives the
+    pseudo-inverse; a banded graph with two components (the grounded matrix is singular) gives
+    zeros."""
+    n = 100
+    a = _patch_laplacian(n, 5)
+    perm = np.random.default_rng(6).permutation(n)
+    ap = a[np.ix_(perm, perm)]
+    got = E.coarse_inverse(ap)
+    sym = 0.5 * (ap + ap.T)
+    np.fill_diagonal(sym, 0.0)
+    want = np.linalg.pinv(sym - np.diag(sym.sum(axis=1)))
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    b = _patch_laplacian(n, 7)
+    b[:50, 50:] = 0.0
+    b[50:, :50] = 0.0
+    assert not E.coarse_inverse(b).any()
