@@ -154,9 +154,10 @@ def main():
     coarse = WS.coarse_problems(probs, levels)
     tuning = {"final": None, "levels": [None] * len(levels)}
 
-    def start_weights(record=None):
+    def start_weights(before_level=None):
         """The warm start's weights for the full-resolution problems and its evaluations;
-        record: a list that receives each coarse level's starting weights (None: zeros)."""
+        before_level(li, problems, init): called with each coarse level's starting weights
+        (None: zeros) before that level is solved."""
         if args.start == "zero":
             return None, 0
         fields = None if args.start == "multiscale" else WS.predict_fields(net, probs)
@@ -164,8 +165,8 @@ def main():
         for li, (n_side, cps) in enumerate(zip(levels, coarse)):
             init = None if fields is None else np.concatenate(
                 [WS.weights_from_field(U, cp.sites, cp.frame) for U, cp in zip(fields, cps)])
-            if record is not None:
-                record.append(init)
+            if before_level is not None:
+                before_level(li, cps, init)
             tl = time.perf_counter()
             with OTBatch(cps, precision=args.coarse_precision,
                          tuning=tuning["levels"][li]) as cb:
@@ -197,21 +198,24 @@ def main():
 
     tune_report = None
     if not args.no_tune:
-        # launch shapes per level, each measured from the weights the warm start hands that
-        # level (a pass with the default shapes), outside the timed region
+        # launch shapes per level, outside the timed region, level by level: each level is
+        # tuned from the weights the already tuned levels below hand it, then solved with its
+        # shapes -- the solves are deterministic, so the tuner measures exactly the trajectory
+        # the timed solves will follow
         from caustic_amd import autotune as AT
 
         tune_report = {}
-        inits = []
         # the measured stretch: each level's L-BFGS phase up to the Newton hand-off (or 40
         # iterations of a pure L-BFGS solve)
         stretch = (dict(iters=400, stop_tol=args.newton_from) if args.newton_from
                    else dict(iters=40))
-        w_probe, _ = start_weights(record=inits)
-        for li, cps in enumerate(coarse):
-            tuning["levels"][li], r = AT.tune(cps, args.coarse_precision, w0=inits[li],
-                                              log=log, **stretch)
+
+        def tune_level(li, cps, init):
+            tuning["levels"][li], r = AT.tune(cps, args.coarse_precision, w0=init, log=log,
+                                              **stretch)
             tune_report[f"level{levels[li]}"] = r
+
+        w_probe, _ = start_weights(before_level=tune_level)
         tuning["final"], r = AT.tune(probs, "mixed", w0=w_probe, log=log, **stretch)
         tune_report["final"] = r
         log(f"[autotune] {json.dumps(tune_report)}")

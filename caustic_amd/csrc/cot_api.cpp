@@ -692,7 +692,8 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
     s.coef[0] = 1.0;  // first direction: -g
     s.phase = kPhaseDir;
     if (cur[p].fails) s.status = COT_EUNVERIFIED;
-    if (o.stop_rule == COT_STOP_MAXREL && cur[p].maxrel < o.tol) {
+    if ((o.stop_rule == COT_STOP_MAXREL && cur[p].maxrel < o.tol) ||
+        (o.stop_rule == COT_STOP_HANDOFF && cur[p].maxrel < o.tol && cur[p].zeros == 0.0)) {
       s.conv = 1;
       s.phase = kPhaseDone;
     }
@@ -706,7 +707,7 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
   SolveParams sp{};
   sp.m = o.history;
   sp.max_iter = o.max_iter;
-  sp.stop_rule = o.stop_rule == COT_STOP_MAXREL ? 1 : 0;
+  sp.stop_rule = o.stop_rule == COT_STOP_MAXREL ? 1 : (o.stop_rule == COT_STOP_HANDOFF ? 2 : 0);
   sp.mixed_ls = o.accept_rule;
   sp.ring_max = c->opt.ring_max;
   sp.max_unverified = 4;
@@ -1854,7 +1855,8 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
     HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, c->nblk, c->st));
     for (int p = 0; p < np; ++p) {
       if (cur[p].fails) status[p] = COT_EUNVERIFIED;
-      if (o.stop_rule == COT_STOP_MAXREL && cur[p].maxrel < o.tol) {
+      if ((o.stop_rule == COT_STOP_MAXREL && cur[p].maxrel < o.tol) ||
+          (o.stop_rule == COT_STOP_HANDOFF && cur[p].maxrel < o.tol && cur[p].zeros == 0.0)) {
         conv[p] = 1;
         ph[p] = kDone;
       }
@@ -2017,12 +2019,15 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
         bool done = false;
         if (o.stop_rule == COT_STOP_MAXREL) {
           done = cur[p].maxrel < o.tol;
+        } else if (o.stop_rule == COT_STOP_HANDOFF) {
+          done = cur[p].maxrel < o.tol && cur[p].zeros == 0.0;
         } else {
           done = snap_g[p] < o.tol && snap_zero[p] == 0.0;
         }
         if (o.verbose > 0 && (iters[p] % o.verbose == 0 || done))
-          std::printf("[cot] prob %d iter %d f %.12e |g| %.6e maxrel %.3e step %g\n", p,
-                      iters[p], cur[p].f, std::sqrt(cur[p].g2), cur[p].maxrel, t[p]);
+          std::printf("[cot] prob %d iter %d f %.12e |g| %.6e maxrel %.3e empty %g step %g\n", p,
+                      iters[p], cur[p].f, std::sqrt(cur[p].g2), cur[p].maxrel, cur[p].zeros,
+                      t[p]);
         if (done) {
           conv[p] = 1;
           ph[p] = kDone;

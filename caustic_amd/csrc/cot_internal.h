@@ -203,7 +203,8 @@ enum : int { kPhaseDir = 0, kPhaseTrial = 1, kPhaseDone = 2 };
 struct SolveParams {
   int m;             // history size
   int max_iter;      // iteration limit
-  int stop_rule;     // 0: reference (lagged |g|_2 < tol, no empty cell), 1: max relative error
+  int stop_rule;     // 0: reference (lagged |g|_2 < tol, no empty cell), 1: max relative error,
+                     // 2: max relative error and no empty cell (the hand-off to Newton steps)
   int mixed_ls;      // accept a trial on f decrease OR non-positive directional derivative
   int ring_max;
   int max_unverified;  // consecutive trials left unverified before giving up

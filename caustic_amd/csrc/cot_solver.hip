@@ -407,6 +407,8 @@ __global__ __launch_bounds__(kFinishBlock) void dev_accept_finish_kernel(
   bool done;
   if (sp.stop_rule == 1) {
     done = s.maxrel_cur < sp.tol;
+  } else if (sp.stop_rule == 2) {
+    done = s.maxrel_cur < sp.tol && s.zeros_cur == 0.0;
   } else {
     done = s.snap_g < sp.tol && s.snap_zero == 0.0;
   }

@@ -192,8 +192,8 @@ class OTBatch:
         accept_rule: "decrease" (f_trial < f, the reference rule), "derivative" (also accept a
         trial whose directional derivative is <= 0 -- for FP32 geometry, whose f is too coarse
         for the last decades), or "auto" (derivative for mixed precision, decrease for fp64).
-        newton_from (stop="maxrel" only): L-BFGS until max-rel newton_from, then damped Newton
-        steps (cot_solve_newton) to tol; problems the Newton steps leave unconverged finish
+        newton_from (stop="maxrel" only): L-BFGS until max-rel newton_from with no empty cell,
+        then damped Newton steps (cot_solve_newton) to tol; problems the Newton steps leave unconverged finish
         with L-BFGS.  Reports then add "newton_iterations".  newton_sweeps: block-local
         Jacobi sweeps of the conjugate gradients' preconditioner (0: plain Jacobi);
         newton_coarse: at most this many patches per problem in its two-level coarse term
@@ -202,7 +202,9 @@ class OTBatch:
                   h0_threshold=h0_threshold, verbose=verbose, device_loop=device_loop,
                   accept_rule=accept_rule, repair_rounds=repair_rounds)
         if newton_from is not None and stop == "maxrel" and newton_from > tol:
-            w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False, **kw)
+            # L-BFGS until max-rel newton_from with no empty cell (Newton's starting condition)
+            w1, r1 = self.solve(w0, tol=newton_from, raise_on_fail=False,
+                                **dict(kw, stop="handoff"))
             w2, r2 = self._newton(w1, tol, newton_max_iter, verbose, repair_rounds, device_loop,
                                   newton_sweeps, newton_coarse, newton_small_cg)
             if all(r["converged"] for r in r2):
@@ -235,7 +237,8 @@ class OTBatch:
         o.max_iter = max_iter
         o.tol = tol
         o.initial_step = initial_step
-        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL}[stop]
+        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL,
+                       "handoff": L.COT_STOP_HANDOFF}[stop]
         o.h0_threshold = h0_threshold
         o.verbose = verbose
         # True/1: graph-replayed device rounds; 2: the same rounds launched directly; False/0:

@@ -34,7 +34,7 @@ enum {
 };
 
 enum { COT_PREC_FP64 = 0, COT_PREC_MIXED = 1 };
-enum { COT_STOP_REFERENCE = 0, COT_STOP_MAXREL = 1 };
+enum { COT_STOP_REFERENCE = 0, COT_STOP_MAXREL = 1, COT_STOP_HANDOFF = 2 };
 
 typedef struct cot_context cot_context;
 
@@ -70,7 +70,9 @@ typedef struct {
   double tol;               /* convergence_tolerance */
   double initial_step;      /* initial_step */
   int stop_rule;            /* COT_STOP_REFERENCE: |g|_2 < tol with no empty cell (ot_op.cuh:510);
-                               COT_STOP_MAXREL: max_i |mass_i - nu_i| / nu_i < tol */
+                               COT_STOP_MAXREL: max_i |mass_i - nu_i| / nu_i < tol;
+                               COT_STOP_HANDOFF: that and no empty cell (the starting condition
+                               of cot_solve_newton; the same as MAXREL for tol <= 1) */
   double h0_threshold;      /* H0 = y.s / y.y when y.y > h0_threshold, else 1 (1e-5 reference) */
   int verbose;              /* host loop: a line every `verbose` iterations; device loop: a
                                progress line every 2 s (0: silent) */

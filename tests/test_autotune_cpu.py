@@ -63,3 +63,22 @@ def test_a_slow_outlier_does_not_reject_the_best_shape(tmp_path, monkeypatch):
     cfg, rep = AT.tune([W.plain_fp64(n_side=8, res=16)], "mixed", use_cache=False)
     assert cfg["full_pass_ring"] == 2, cfg
     assert abs(rep["per_eval_us"] - 1e6) < 1e-6
+
+
+
+def test_phase_stretch_runs_to_the_handoff(tmp_path, monkeypatch):
+    """With stop_tol the measured stretch is the L-BFGS phase up to the Newton hand-off (stop
+    rule "handoff", tol = stop_tol, at most `iters` iterations), ranked per evaluation."""
+    monkeypatch.setenv("CAUSTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
+
+    def phase(problems, precision, cfg, w0, iters, solve_kw):
+        assert solve_kw["stop"] == "handoff" and solve_kw["tol"] == 1.0 and iters == 400
+        two = cfg["full_pass_ring"] == 2
+        per, evals = (0.8, 80) if two else (1.0, 50)  # cheaper per evaluation, more of them
+        return {"per_eval": per, "total": per * evals, "evaluations": evals, "wall": 0.0,
+                "cell": 0.0, "integrate": 0.0, "repair": 0.0}
+
+    monkeypatch.setattr(AT, "measure", phase)
+    cfg, rep = AT.tune([W.plain_fp64(n_side=8, res=16)], "mixed", use_cache=False,
+                       stop_tol=1.0, iters=400)
+    assert cfg["full_pass_ring"] == 2 and rep["objective"] == "per_eval", (cfg, rep)
