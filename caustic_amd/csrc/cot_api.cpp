@@ -1690,8 +1690,6 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       sa.cdesc = c->d_cdesc;
       sa.ainv = c->d_ainv;
       sa.x = c->d_nx;
-      sa.z = c->d_nz;
-      sa.p = c->d_np;
       sa.eta2 = eta * eta;
       sa.out = c->d_cgout;
       HIPCHK(launch_cg_small(sa, st));
@@ -1704,7 +1702,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       if (res[3] != 0.0) iters = -iters;
       return COT_OK;
     };
-    const bool small = nb <= kSmallBlocks && o.small_cg;
+    const bool small = nb <= kSmallBlocks && c->ntot <= kSmallBlocks * kCellBlock && o.small_cg;
     int cg = 0;
     if (small) {
       CHK(run_cg_small(ca, cg));

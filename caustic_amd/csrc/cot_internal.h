@@ -112,7 +112,7 @@ struct SmallCgArgs {
   const int* agg_prob;    // patch -> problem
   const CoarseDesc* cdesc;
   const double* ainv;
-  double *x, *z, *p;      // N each (z, p: the neighbours' values between barriers)
+  double* x;              // N: the solution
   double eta2;            // stop at r.r <= eta2 r0.r0
   double* out;            // [0] iterations, [1] r0.r0, [2] r.r, [3] 1 when the residual diverged
 };

@@ -409,8 +409,8 @@ __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
 // iteration (Jacobi, `sweeps` block-local sweeps, the coarse patch term), the same stopping rule,
 // workgroup barriers in place of kernel boundaries and no host round trip per chunk.  Thread t
 // serves lane t % 128 of blocks t / 128, t / 128 + 8, ... (at most kSmallBlocks blocks); the
-// residual and q stay in registers, p and z go through memory for the neighbours, the solution
-// accumulates in memory.
+// residual, the solution and q stay in registers, p and z go through LDS for the neighbours (the
+// system's sites are 0 .. N - 1, N <= kSmallBlocks * 128).
 // Every branch that contains a barrier depends only on workgroup-wide values.
 __device__ __forceinline__ double wg_sum(double v, double* red) {
 #pragma unroll
@@ -425,21 +425,25 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
 
 __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) {
   constexpr int kRounds = kSmallBlocks / (kSmallThreads / kCellBlock);
+  constexpr int kSites = kSmallBlocks * kCellBlock;  // sites 0 .. N - 1 with N <= kSites
   __shared__ double red[kSmallThreads / 64];
   __shared__ double wsum[2 * kSmallBlocks];
   __shared__ double Rp[kSmallBlocks];
   __shared__ double cp[kSmallBlocks];
+  // z and p of every site, for the neighbours; each site's owner writes both before any thread
+  // reads them (apply_m, then the p update, each followed by a barrier)
+  __shared__ double zs[kSites];
+  __shared__ double ps[kSites];
   const int t = threadIdx.x, l = t & (kCellBlock - 1), b0 = t / kCellBlock;
   const int half = (t >> 6) & 1, lane = t & 63;
-  // per round: the site (-1: none), its block's size, diagonal, residual, p and z; the solution
-  // accumulates in memory (a.x)
+  // per round: the site (-1: none), its block's size, diagonal, residual, solution, p and z
   int site[kRounds], bcount[kRounds];
-  double dg[kRounds], r[kRounds], pl[kRounds], zl[kRounds];
+  double dg[kRounds], r[kRounds], x[kRounds], pl[kRounds], zl[kRounds];
 #pragma unroll
   for (int m = 0; m < kRounds; ++m) {
     site[m] = -1;
     bcount[m] = 0;
-    dg[m] = r[m] = pl[m] = zl[m] = 0.0;
+    dg[m] = r[m] = x[m] = pl[m] = zl[m] = 0.0;
     const int b = b0 + m * (kSmallThreads / kCellBlock);
     if (b < a.nblk) {
       const BlockDesc bd = a.blocks[b];
@@ -449,7 +453,6 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
         site[m] = i;
         dg[m] = a.diag[i];
         if (a.active[bd.prob] && dg[m] > 0.0) r[m] = a.shift[bd.prob] - a.g[i];
-        a.x[i] = 0.0;
       }
     }
   }
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 #pragma unroll
     for (int m = 0; m < kRounds; ++m) {
       zl[m] = dg[m] > 0.0 ? r[m] / dg[m] : 0.0;
-      if (site[m] >= 0) a.z[site[m]] = zl[m];
+      if (site[m] >= 0) zs[site[m]] = zl[m];
     }
     for (int sw = 0; sw < a.sweeps; ++sw) {
       __syncthreads();
@@ -470,10 +473,10 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
         if (i >= 0 && dg[m] > 0.0) {
           double acc = r[m];
           const int nv = a.nedge[i];
+          const int bf = i - l;  // the block's first site
           for (int k = 0; k < nv; ++k) {
-            const int bf = i - l;  // the block's first site
             const int j = a.elab[(size_t)i * kMaxVerts + k] - bf;
-            if (j >= 0 && j < bcount[m]) acc += a.h[(size_t)i * kMaxVerts + k] * a.z[bf + j];
+            if (j >= 0 && j < bcount[m]) acc += a.h[(size_t)i * kMaxVerts + k] * zs[bf + j];
           }
           zn[m] = acc / dg[m];
         }
@@ -482,7 +485,7 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 #pragma unroll
       for (int m = 0; m < kRounds; ++m) {
         zl[m] = zn[m];
-        if (site[m] >= 0) a.z[site[m]] = zl[m];
+        if (site[m] >= 0) zs[site[m]] = zl[m];
       }
     }
     if (a.sagg) {
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 #pragma unroll
   for (int m = 0; m < kRounds; ++m) {
     pl[m] = zl[m];
-    if (site[m] >= 0) a.p[site[m]] = pl[m];
+    if (site[m] >= 0) ps[site[m]] = pl[m];
   }
   int it = 0;
   double rr = rr0;
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
       const int nv = a.nedge[i];
       for (int k = 0; k < nv; ++k) {
         const int j = a.elab[(size_t)i * kMaxVerts + k];
-        if (j >= 0) sacc -= a.h[(size_t)i * kMaxVerts + k] * a.p[j];
+        if (j >= 0) sacc -= a.h[(size_t)i * kMaxVerts + k] * ps[j];
       }
       q[m] = sacc;
       pq += pl[m] * sacc;
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
     double rrl = 0.0;
 #pragma unroll
     for (int m = 0; m < kRounds; ++m) {
-      if (site[m] >= 0) a.x[site[m]] += alpha * pl[m];
+      x[m] += alpha * pl[m];
       r[m] -= alpha * q[m];
       rrl += r[m] * r[m];
     }
@@ -581,9 +584,12 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 #pragma unroll
     for (int m = 0; m < kRounds; ++m) {
       pl[m] = zl[m] + beta * pl[m];
-      if (site[m] >= 0) a.p[site[m]] = pl[m];
+      if (site[m] >= 0) ps[site[m]] = pl[m];
     }
   }
+#pragma unroll
+  for (int m = 0; m < kRounds; ++m)
+    if (site[m] >= 0) a.x[site[m]] = x[m];
   if (t == 0) {
     a.out[0] = it;
     a.out[1] = rr0;
