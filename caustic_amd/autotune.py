@@ -30,7 +30,7 @@ phase 3x longer).  Each candidate is replayed, the smaller of
 `repeats` measurements (the repair passes' work lists fill in a different order from run to run,
 which moves sites between blocks; the resulting spread is one-sided, up to ~1.8x slower in rare
 runs, and one such outlier is enough to make the descent reject the right shape).  Coordinate
-descent over the knobs (a change must win by 2%), at most two sweeps.  Results are cached per device,
+descent over the knobs (a change must win by 5%), at most two sweeps.  Results are cached per device,
 precision and problem shape in a JSON file ($CAUSTIC_TUNE_CACHE, default
 ~/.cache/caustic_amd/tuning.json).
 """
@@ -49,7 +49,10 @@ KNOBS = {
     "full_pass_ring": (1, 2),
 }
 DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 1}
-MIN_GAIN = 0.02
+# a change must win by 5%: shapes within a few percent of each other (8 or 16 integration lanes,
+# 256 or 512 repair blocks) trade places from run to run, and each flip sends the solve down a
+# different (equally fast on average) trajectory, so near-ties keep the incumbent
+MIN_GAIN = 0.05
 
 
 def cache_path() -> str:
