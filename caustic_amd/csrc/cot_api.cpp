@@ -28,8 +28,8 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
-                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
-                                 hipStream_t st);
+                                 const uint8_t* wcnt, const int* elab, uint8_t* adjn, int* adj,
+                                 int nblocks, hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
                              uint8_t* ring, int rmax, int repair_blocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
@@ -186,6 +186,7 @@ struct cot_context {
   double *d_fcell = nullptr, *d_area = nullptr, *d_g = nullptr;
   double2 *d_cen = nullptr, *d_geo = nullptr;
   uint8_t *d_exist = nullptr, *d_nedge = nullptr, *d_ring = nullptr, *d_used = nullptr;
+  uint8_t* d_wcnt = nullptr;     // emptiness witnesses' label counts (empty cells)
   uint8_t* d_active = nullptr;
   int* d_redo = nullptr;         // repair flags (see CellArgs)
   int* d_list = nullptr;         // two work lists of flagged sites, N slots each
@@ -356,6 +357,7 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.elab = c->d_elab;
   ca.espan = c->d_espan;
   ca.ring_used = c->d_used;
+  ca.wcnt = c->d_wcnt;
   ca.overflow = c->d_overflow;
   ca.org = c->d_org;
   ca.ring_max = c->opt.ring_max;
@@ -478,7 +480,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
+      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn,
                                    c->d_adj, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
@@ -494,7 +496,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   c->stats.repair_rounds += rounds;
   sum_partials(c, sum);
   // the cells just built are the next evaluation's candidates
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
                                c->nblk, st));
   c->have_adj = true;
   // after a clean check the next bucket search starts from the outermost used ring plus one;
@@ -642,14 +644,14 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
                                    c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
     HIPCHK(launch_verify(va, c->nblk, st));
   }
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
                                c->nblk, st));
   const int* final_fails = c->d_rfails + (size_t)R * np;
   HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
@@ -967,6 +969,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
       (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
+      (rc = dalloc(c, &c->d_wcnt, N)) ||
       (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
       (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
@@ -1015,6 +1018,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
       launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
+      launch_fill_u8(c->d_wcnt, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
