@@ -1505,7 +1505,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   std::vector<int> iters(np, 0), evals(np, 1);
   CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
   HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
-  std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
+  std::vector<double> shift(np, 0.0), t(np, 1.0), t_last(np, 1.0), rr_part(nb);
   // CG stops at |r| <= eta |rhs| -- an inexact Newton step, eta = 0.1 max-rel clamped to
   // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
   // is launched directly; the rest run in chunks of `check` iterations (a captured graph unless
@@ -1731,7 +1731,9 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     c->stats.cg_iterations += cg;
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
     std::vector<uint8_t> tr = act, acc(np, 0);
-    std::fill(t.begin(), t.end(), 1.0);
+    // backtracking starts at twice the last accepted step (at most 1): far from the solution the
+    // accepted steps stay small for several iterations, and each rejected trial is an evaluation
+    for (int p = 0; p < np; ++p) t[p] = std::min(1.0, 2.0 * t_last[p]);
     for (;;) {
       bool any_trial = false;
       for (int p = 0; p < np; ++p) {
@@ -1756,6 +1758,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
           acc[p] = 1;
           tr[p] = 0;
           cur[p] = trial[p];
+          t_last[p] = t[p];
         } else {
           t[p] *= 0.5;
           if (t[p] < 1.0 / 4096.0) {
