@@ -28,8 +28,8 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
-                                 const uint8_t* wcnt, const int* elab, uint8_t* adjn, int* adj,
-                                 int nblocks, hipStream_t st);
+                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
+                                 hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
                              uint8_t* ring, int rmax, int repair_blocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
@@ -186,7 +186,6 @@ struct cot_context {
   double *d_fcell = nullptr, *d_area = nullptr, *d_g = nullptr;
   double2 *d_cen = nullptr, *d_geo = nullptr;
   uint8_t *d_exist = nullptr, *d_nedge = nullptr, *d_ring = nullptr, *d_used = nullptr;
-  uint8_t* d_wcnt = nullptr;     // emptiness witnesses' label counts (empty cells)
   uint8_t* d_active = nullptr;
   int* d_redo = nullptr;         // repair flags (see CellArgs)
   int* d_list = nullptr;         // two work lists of flagged sites, N slots each
@@ -357,7 +356,6 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.elab = c->d_elab;
   ca.espan = c->d_espan;
   ca.ring_used = c->d_used;
-  ca.wcnt = c->d_wcnt;
   ca.overflow = c->d_overflow;
   ca.org = c->d_org;
   ca.ring_max = c->opt.ring_max;
@@ -480,7 +478,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn,
+      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
                                    c->d_adj, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
@@ -496,7 +494,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   c->stats.repair_rounds += rounds;
   sum_partials(c, sum);
   // the cells just built are the next evaluation's candidates
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                c->nblk, st));
   c->have_adj = true;
   // after a clean check the next bucket search starts from the outermost used ring plus one;
@@ -644,14 +642,14 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
+      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                    c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
     HIPCHK(launch_verify(va, c->nblk, st));
   }
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_wcnt, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                c->nblk, st));
   const int* final_fails = c->d_rfails + (size_t)R * np;
   HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
@@ -969,7 +967,6 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
       (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
-      (rc = dalloc(c, &c->d_wcnt, N)) ||
       (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
       (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
@@ -1018,7 +1015,6 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
       launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
-      launch_fill_u8(c->d_wcnt, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
@@ -1505,7 +1501,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   std::vector<int> iters(np, 0), evals(np, 1);
   CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
   HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
-  std::vector<double> shift(np, 0.0), t(np, 1.0), t_last(np, 1.0), rr_part(nb);
+  std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
   // CG stops at |r| <= eta |rhs| -- an inexact Newton step, eta = 0.1 max-rel clamped to
   // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
   // is launched directly; the rest run in chunks of `check` iterations (a captured graph unless
@@ -1731,9 +1727,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     c->stats.cg_iterations += cg;
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
     std::vector<uint8_t> tr = act, acc(np, 0);
-    // backtracking starts at twice the last accepted step (at most 1): far from the solution the
-    // accepted steps stay small for several iterations, and each rejected trial is an evaluation
-    for (int p = 0; p < np; ++p) t[p] = std::min(1.0, 2.0 * t_last[p]);
+    std::fill(t.begin(), t.end(), 1.0);
     for (;;) {
       bool any_trial = false;
       for (int p = 0; p < np; ++p) {
@@ -1758,7 +1752,6 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
           acc[p] = 1;
           tr[p] = 0;
           cur[p] = trial[p];
-          t_last[p] = t[p];
         } else {
           t[p] *= 0.5;
           if (t[p] < 1.0 / 4096.0) {

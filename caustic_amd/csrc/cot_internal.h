@@ -176,8 +176,6 @@ struct CellArgs {
   int* elab;                // [site * kMaxVerts + k]: label of edge k (site or side)
   float2* espan;            // [site * kMaxVerts + k]: edge span along the canonical bisector axis
   uint8_t* ring_used;       // outermost ring that cut the cell
-  uint8_t* wcnt;            // per site: labels of an emptiness witness in its elab row (empty
-                            // cells only; 0: none) -- the next evaluation's candidates
   int* overflow;            // per-problem count of cells that exceeded kMaxVerts
   int* coarse;              // per-problem count of FP32 cells handed to the FP64 repair (the
                             // conformity-failure counter of the check that follows)

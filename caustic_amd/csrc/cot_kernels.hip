@@ -476,7 +476,6 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   double bcx, bcy, br2;
   bound(bcx, bcy, br2);
   int used = 0;
-  int wit_n = 0, wit_last = -1;  // emptiness witness: the last polygon's size and the emptying site
   // clip by the bisector of site j (candidate from bucket ring r); `dedupe`: j may already bound
   // an edge (a candidate list with repeats), then clipping by the same line again is skipped
   auto consider = [&](int j, int r, bool dedupe) {
@@ -495,21 +494,13 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     }
     bool changed;
     T sdmin;
-    const int nprev = nv;
     nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed, sdmin);
     if (changed) {
       used = max(used, r);
       if (nv > 0) {
         bound(bcx, bcy, br2);
-      } else {
-        // emptied: when every vertex was cut (the clip then writes nothing) the polygon before
-        // this clip is still in LDS, and its edges' constraints with j's prove the cell empty
-        if ((double)sdmin > 0.0) {
-          wit_n = nprev;
-          wit_last = j;
-        }
-        if (kF32 && (double)sdmin < kMarginRel * pd.bsize * sqrt(ddx * ddx + ddy * ddy))
-          coarse = true;  // emptied by a margin FP32 cannot resolve: may be a sliver
+      } else if (kF32 && (double)sdmin < kMarginRel * pd.bsize * sqrt(ddx * ddx + ddy * ddy)) {
+        coarse = true;  // emptied by a margin FP32 cannot resolve: may be a sliver
       }
     }
   };
@@ -587,21 +578,6 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   }
   a.ring_used[i] = (uint8_t)min(used, 255);
   a.nedge[i] = (uint8_t)nv;
-  if (a.wcnt) {
-    // an empty cell keeps its witness (the last polygon's edge labels and the emptying site) as
-    // the next evaluation's candidates, instead of searching bucket rings again; a witness that
-    // no longer empties the cell leaves a polygon whose edges the conformity check rejects
-    int nw = 0;
-    if (nv == 0 && wit_n > 0) {
-      const int keep = min(wit_n, kMaxVerts - 1);
-      for (int k = 0; k < keep; ++k) {
-        const int lab = P.lab[k][ln];
-        if (lab >= 0 && lab != i) a.elab[(size_t)i * kMaxVerts + nw++] = lab;
-      }
-      a.elab[(size_t)i * kMaxVerts + nw++] = wit_last;
-    }
-    a.wcnt[i] = (uint8_t)nw;
-  }
 
   double ar = 0.0;
   double2 geo = make_double2(0.0, 0.0);
@@ -1033,15 +1009,14 @@ __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
 // Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
 // repair).  gate: skip problems whose counter is zero (nullptr: none).
 __global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
-                                      const uint8_t* nedge, const uint8_t* wcnt, const int* elab,
-                                      uint8_t* adjn, int* adj) {
+                                      const uint8_t* nedge, const int* elab, uint8_t* adjn,
+                                      int* adj) {
   const BlockDesc bd = blocks[blockIdx.x];
   if (gate && gate[bd.prob] == 0) return;
   const int ln = threadIdx.x;
   if (ln >= bd.count) return;
   const int i = bd.first + ln;
-  // an empty cell's row holds its emptiness witness (wcnt labels)
-  const int ne = nedge[i] ? nedge[i] : (wcnt ? wcnt[i] : 0);
+  const int ne = nedge[i];
   adjn[i] = (uint8_t)ne;
   const int* src = elab + (size_t)i * kMaxVerts;
   int* dst = adj + (size_t)i * kMaxVerts;
@@ -1249,10 +1224,10 @@ hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
 }
 
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
-                                 const uint8_t* wcnt, const int* elab, uint8_t* adjn, int* adj,
-                                 int nblocks, hipStream_t st) {
+                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
+                                 hipStream_t st) {
   hipLaunchKernelGGL(save_adjacency_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, gate,
-                     nedge, wcnt, elab, adjn, adj);
+                     nedge, elab, adjn, adj);
   return hipGetLastError();
 }
 

@@ -160,8 +160,8 @@ int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
  * Laplacian h_ij = int over the common edge of rho ds / (2 |s_i - s_j|) -- from the current
  * cells, solves H d = -g by preconditioned conjugate gradients (opt->newton_sweeps Jacobi sweeps
  * on each 128-site block's submatrix of H, plus with opt->newton_coarse the two-level patch
- * term), and halves the step (from twice the last accepted step, at most 1) until every cell
- * keeps mass and |g|_2 falls by (1 - t/2) (Kitagawa, Merigot, Thibert 2019).  The conjugate gradients run to |r| <= eta |rhs|,
+ * term), and halves the step until every cell keeps mass and |g|_2 falls by (1 - t/2)
+ * (Kitagawa, Merigot, Thibert 2019).  The conjugate gradients run to |r| <= eta |rhs|,
  * eta = 0.1 max-rel clamped to [1e-4, 0.1], replayed from a captured graph in chunks of 32
  * iterations (16 with the patch term; launched directly when opt->device_loop == 2).  A problem with an empty or unverified cell, or whose step falls below
  * 1/4096, stops with status COT_ENOTCONV at its last accepted weights. */

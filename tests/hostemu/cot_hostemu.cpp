@@ -26,7 +26,7 @@ struct Session {
   std::vector<double> nu, w, mass, fcell, area;
   std::vector<double4> pre, tot;
   std::vector<double2> cen, geo;
-  std::vector<uint8_t> exist, ne, ring, used, adjn, wcnt;
+  std::vector<uint8_t> exist, ne, ring, used, adjn;
   std::vector<int> redo;
   std::vector<int> elab, adj, xcnt, xcand;
   std::vector<double2> vtx;
@@ -102,7 +102,6 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
   S->used.assign(n, 0);
   S->redo.assign(n, 0);
   S->adjn.assign(n, 0);
-  S->wcnt.assign(n, 0);
   S->elab.assign((size_t)n * kMaxVerts, 0);
   S->adj.assign((size_t)n * kMaxVerts, 0);
   S->xcnt.assign(n, 0);
@@ -172,7 +171,6 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   a.elab = S.elab.data();
   a.espan = S.espan.data();
   a.ring_used = S.used.data();
-  a.wcnt = S.wcnt.data();
   a.overflow = &overflow;
   a.org = S.org.data();
   a.ring_max = S.ring_max;
@@ -246,10 +244,8 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   };
   auto save_adjacency = [&]() {
     for (int r = 0; r < n; ++r) {
-      // an empty cell's row holds its emptiness witness (as save_adjacency_kernel)
-      const int ne = S.ne[r] ? S.ne[r] : S.wcnt[r];
-      S.adjn[r] = (uint8_t)ne;
-      for (int k = 0; k < ne; ++k)
+      S.adjn[r] = S.ne[r];
+      for (int k = 0; k < S.ne[r]; ++k)
         S.adj[(size_t)r * kMaxVerts + k] = S.elab[(size_t)r * kMaxVerts + k];
     }
   };
