@@ -182,3 +182,21 @@ def test_small_system_cg_matches_the_multi_kernel_cg(gpu_available, coarse, swee
     assert got[True][2] == got[False][2] == 0, got
     # the multi-kernel path rounds its count up to a whole chunk per step
     assert got[True][1] <= got[False][1] <= 1.1 * got[True][1] + 32 * got[True][0], got
+
+
+@pytest.mark.parametrize("device_loop", [True, False])
+def test_handoff_stop_rule(gpu_available, c1, device_loop):
+    """stop="handoff" (COT_STOP_HANDOFF) at tol 2: L-BFGS stops at the first iterate with max-rel
+    below 2 and no empty cell -- checked by the FP64 oracle's masses at the returned weights --
+    and never later than the plain max-rel rule at tol 1, which implies both conditions."""
+    print(f"[case] hand-off stop rule, c1 fp64, device loop {device_loop}", flush=True)
+    # fresh contexts from the same start follow the same (deterministic) iterates
+    with _batch([c1], precision="fp64") as b:
+        w, rep = b.solve(tol=2.0, stop="handoff", h0_threshold=0.0, device_loop=device_loop)
+    with _batch([c1], precision="fp64") as b:
+        _, rep1 = b.solve(tol=1.0, stop="maxrel", h0_threshold=0.0, device_loop=device_loop)
+    assert rep[0]["converged"], rep
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    assert O.max_rel_error(orc.mass, c1.nu) < 2.0 + 1e-9
+    assert orc.exist.all() and (orc.mass > 0).all()
+    assert rep[0]["iterations"] <= rep1[0]["iterations"], (rep, rep1)
