@@ -59,7 +59,7 @@ class cot_reflection(ctypes.Structure):
                 ("z_of_len", ctypes.c_double), ("z_of_receiver", ctypes.c_double),
                 ("refraction_index", ctypes.c_double), ("area_scalar", ctypes.c_double),
                 ("aa_history_size", ctypes.c_int), ("aa_fixed_para", ctypes.c_double),
-                ("if_aa_op_initial", ctypes.c_int)]
+                ("if_aa_op_initial", ctypes.c_int), ("incident_dir", _dp)]
 
 
 # name -> (restype, argtypes); every symbol include/caustic_ot.h declares
@@ -85,6 +85,7 @@ SIGNATURES = {
                                          _dp]),
     "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
     "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cot_cell_solid_angles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }

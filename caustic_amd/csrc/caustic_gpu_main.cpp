@@ -287,6 +287,32 @@ struct CtxGuard {
   }
 };
 
+// Solid angle under which the point light sees the sample domain [x0, x1, y0, y1] in the plane
+// z: the rectangle (x0,y0) (x0,y1) (x1,y1) (x1,y0) fanned from its first corner, each spherical
+// triangle 2 atan(a.(b x c) / (1 + a.b + b.c + c.a)) of the unit vectors to its corners, a
+// triangle with |denominator| < 1e-6 skipped, magnitude of the sum (len_solid_angle and
+// compute_solid_angle, basic_function.cuh:653-658, 558-588).  One polygon: host arithmetic.
+double domain_solid_angle(const std::vector<double>& d, const std::vector<double>& light,
+                          double z) {
+  const double cx[4] = {d[0], d[0], d[1], d[1]}, cy[4] = {d[2], d[3], d[3], d[2]};
+  double u[4][3];
+  for (int k = 0; k < 4; ++k) {
+    const double e[3] = {cx[k] - light[0], cy[k] - light[1], z - light[2]};
+    const double r = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    for (int j = 0; j < 3; ++j) u[k][j] = e[j] / r;
+  }
+  auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  double sum = 0.0;
+  for (int k = 1; k + 1 < 4; ++k) {
+    const double *a = u[0], *b = u[k], *c = u[k + 1];
+    const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2],
+                          b[0] * c[1] - b[1] * c[0]};
+    const double den = 1.0 + dot(a, b) + dot(b, c) + dot(a, c);
+    if (std::fabs(den) >= 1e-6) sum += 2.0 * std::atan(dot(a, bc) / den);
+  }
+  return std::fabs(sum);
+}
+
 int die(const std::string& msg, int code = 1) {
   std::cerr << "caustic_gpu: " << msg << std::endl;
   return code;
@@ -306,7 +332,7 @@ int main(int argc, char** argv) {
   std::string picture_name, light_type;
   int total_iter = 0, history_size = 0;
   double tol = 0, initial_step = 1.0;
-  std::vector<double> sample_domain, proj_domain;
+  std::vector<double> sample_domain, proj_domain, point_light;
   bool full_precision = false;
   bool diagnostics = true;
   int aa_history_size = 7;
@@ -332,7 +358,7 @@ int main(int argc, char** argv) {
     proj_domain = lp.at("proj_domain").as_nums();
     const JVal& li = J.at("light_para");
     light_type = li.at("light_type").as_str();
-    (void)li.at("point_light_coordinate").as_nums();
+    point_light = li.at("point_light_coordinate").as_nums();
     picture_name = J.at("picture_para").at("name").as_str();
     if (const JVal* ext = J.find("mi355x"))
       if (const JVal* fp = ext->find("full_precision_weights")) full_precision = fp->as_bool();
@@ -363,8 +389,11 @@ int main(int argc, char** argv) {
     return die("cannot read " + seed + ot_time + "/height.txt");
   const int N = (int)(points.size() / 2);
   if (N == 0 || proj_xy.size() != points.size()) return die("points/proj_xy size mismatch");
-  if (light_type != "parallel_light")
-    return die("light_type '" + light_type + "' is not supported by this build", 3);
+  const bool is_point = light_type == "point_light";
+  if (!is_point && light_type != "parallel_light")
+    return die("unknown light_type '" + light_type + "'", 3);
+  if (is_point && (point_light.size() != 3 || point_light[2] == z_of_len))
+    return die("settings: point_light_coordinate needs 3 numbers off the lens plane", 2);
   if (inside.size() < 3 * (size_t)(points.size() / 2) || height.size() < points.size() / 2)
     return die("inside_light_direct.txt / height.txt hold fewer rows than points.txt");
   std::vector<double> w_start(points.size() / 2, 0.0);
@@ -387,9 +416,10 @@ int main(int argc, char** argv) {
   const double xs = sample_domain[1] - sample_domain[0], ys = sample_domain[3] - sample_domain[2];
   const double area_scalar = xs * ys;
 
-  // source masses: Voronoi cells of the sample points inside the sample domain
-  // (ot_op.cuh:90-92,158,199-206)
-  std::vector<double> nu(N, 1.0), area(N, 0.0), w0(N, 0.0);
+  // source masses: Voronoi cells of the sample points inside the sample domain, weighted by
+  // their area (parallel light) or by the solid angle the light sees them under (point light,
+  // each divided by the solid angle of the whole sample domain) (ot_op.cuh:90-92,154-206)
+  std::vector<double> nu(N, 1.0), area(N, 0.0), w0(N, 0.0), incident;
   {
     std::vector<double> one(1, 1.0);
     cot_problem pr{};
@@ -403,8 +433,25 @@ int main(int argc, char** argv) {
     CtxGuard g;
     if (cot_create(&pr, 1, nullptr, &g.c) != COT_OK)
       return die(std::string("source cells: ") + cot_last_error(), 4);
-    if (cot_cell_areas(g.c, w0.data(), area.data()) != COT_OK)
-      return die(std::string("source cells: ") + cot_last_error(), 4);
+    if (!is_point) {
+      if (cot_cell_areas(g.c, w0.data(), area.data()) != COT_OK)
+        return die(std::string("source cells: ") + cot_last_error(), 4);
+    } else {
+      if (cot_cell_solid_angles(g.c, w0.data(), point_light.data(), z_of_len, area.data()) !=
+          COT_OK)
+        return die(std::string("source cells: ") + cot_last_error(), 4);
+      const double whole = domain_solid_angle(sample_domain, point_light, z_of_len);
+      if (!(whole > 0.0)) return die("the light sees the sample domain under no solid angle", 4);
+      for (double& a : area) a = a / whole * area_scalar;
+      // direction of the light at each point of the lens plane (main.cu:218-231)
+      incident.resize(3 * (size_t)N);
+      for (int i = 0; i < N; ++i) {
+        const double d[3] = {points[2 * i] - point_light[0], points[2 * i + 1] - point_light[1],
+                             z_of_len - point_light[2]};
+        const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int k = 0; k < 3; ++k) incident[3 * (size_t)i + k] = d[k] / r;
+      }
+    }
     double tot = 0.0;
     for (double a : area) tot += a;
     for (int i = 0; i < N; ++i) nu[i] = area[i] / tot * area_scalar;
@@ -474,6 +521,7 @@ int main(int argc, char** argv) {
     rf.aa_history_size = aa_history_size;
     rf.aa_fixed_para = aa_fixed_para;
     rf.if_aa_op_initial = if_aa_op_initial ? 1 : 0;
+    rf.incident_dir = is_point ? incident.data() : nullptr;
     rc = cot_solve_reflection(g.c, w.data(), &so, &rf, &rep);
   }
   if (rc != COT_OK && rc != COT_ENOTCONV) return die(std::string("solve: ") + cot_last_error(), 5);

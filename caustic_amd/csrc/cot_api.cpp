@@ -71,10 +71,13 @@ hipError_t launch_dev_accept_finish(const ProbDesc* probs, DevState* st, const S
                                     hipStream_t stream);
 
 hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points, const double* ins,
-                                 const double* height, const double* clip_area,
-                                 const double2* centroid, double z_len, double z_recv,
-                                 double n_idx, double* weight, double* part, int nblk,
-                                 hipStream_t st);
+                                 const double* wdir, const double* height,
+                                 const double* clip_area, const double2* centroid, double z_len,
+                                 double z_recv, double n_idx, double* weight, double* part,
+                                 int nblk, hipStream_t st);
+hipError_t launch_solid_angle(const BlockDesc* blocks, const double2* vtx, const uint8_t* nedge,
+                              const double light[3], double z, double* omega, int nblk,
+                              hipStream_t st);
 hipError_t launch_scale(const BlockDesc* blocks, const double* weight, double scale, double* nu,
                         int nblk, hipStream_t st);
 hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const double* g, double beta,
@@ -1069,6 +1072,24 @@ int cot_cell_areas(cot_context* c, const double* w, double* area) {
   return COT_OK;
 }
 
+int cot_cell_solid_angles(cot_context* c, const double* w, const double* light, double z,
+                          double* omega) {
+  if (!c || !w || !light || !omega) return fail(COT_EINVAL, "null argument");
+  if (!(z != light[2])) return fail(COT_EINVAL, "the light lies in the plane of the cells");
+  double* d_om = nullptr;
+  CallBuffers buf(c->st);
+  CHK(buf.get(&d_om, c->ntot));
+  std::vector<uint8_t> act(c->np, 1);
+  std::vector<EvalSum> cur;
+  CHK(upload_sorted(c, w, c->d_w));
+  CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
+  for (int p = 0; p < c->np; ++p)
+    if (cur[p].fails) return fail(COT_EUNVERIFIED, "cells failed the conformity check");
+  HIPCHK(launch_solid_angle(c->d_blocks, c->d_vtx, c->d_nedge, light, z, d_om, c->nblk, c->st));
+  CHK(download_sorted(c, d_om, omega));
+  return COT_OK;
+}
+
 namespace {
 
 // Solve (A + 1e-10 I) a = 1 for a small symmetric positive semi-definite A (Cholesky), then
@@ -1131,10 +1152,11 @@ int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
   // per-site lens data in bucket order
   double2* d_pts = nullptr;
   double *d_ins = nullptr, *d_h = nullptr, *d_area = nullptr, *d_wt = nullptr, *d_G = nullptr;
-  double *d_R = nullptr, *d_alpha = nullptr, *d_pp = nullptr;
+  double *d_R = nullptr, *d_alpha = nullptr, *d_pp = nullptr, *d_wi = nullptr;
   CallBuffers buf(st);
   CHK(buf.get(&d_pts, N));
   CHK(buf.get(&d_ins, 3 * (size_t)N));
+  if (r->incident_dir) CHK(buf.get(&d_wi, 3 * (size_t)N));
   CHK(buf.get(&d_h, N));
   CHK(buf.get(&d_area, N));
   CHK(buf.get(&d_wt, N));
@@ -1144,23 +1166,26 @@ int cot_solve_reflection(cot_context* c, double* w, const cot_solve_opts* so,
   CHK(buf.get(&d_pp, (size_t)std::max(mA, 1) * nb));
   {
     std::vector<double2> pts(N);
-    std::vector<double> ins(3 * (size_t)N), h(N), ar(N);
+    std::vector<double> ins(3 * (size_t)N), wi(r->incident_dir ? 3 * (size_t)N : 0), h(N), ar(N);
     for (int q = 0; q < N; ++q) {
       const int o2 = c->perm[q];
       pts[q] = make_double2(r->points[2 * o2], r->points[2 * o2 + 1]);
       for (int k = 0; k < 3; ++k) ins[3 * q + k] = r->inside_dir[3 * o2 + k];
+      if (r->incident_dir)
+        for (int k = 0; k < 3; ++k) wi[3 * q + k] = r->incident_dir[3 * o2 + k];
       h[q] = r->height[o2];
       ar[q] = r->clip_area[o2];
     }
     HIPCHK(hipMemcpy(d_pts, pts.data(), sizeof(double2) * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_ins, ins.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    if (d_wi) HIPCHK(hipMemcpy(d_wi, wi.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_h, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_area, ar.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   }
   std::vector<double> hp(std::max(mA, 1) * (size_t)nb);
   // nu from the Fresnel weights at the centroids the last evaluation left in d_geo
   auto update_nu = [&]() -> int {
-    HIPCHK(launch_fresnel_weight(c->d_blocks, d_pts, d_ins, d_h, d_area, c->d_geo, r->z_of_len,
+    HIPCHK(launch_fresnel_weight(c->d_blocks, d_pts, d_ins, d_wi, d_h, d_area, c->d_geo, r->z_of_len,
                                  r->z_of_receiver, r->refraction_index, d_wt, d_pp, nb, st));
     HIPCHK(hipMemcpyAsync(hp.data(), d_pp, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

@@ -38,10 +38,11 @@ __device__ double wave_sum_d(double v) {
 }  // namespace
 
 // weight_i = clip_area_i * T1_i * T2_i with the receiver point at the cell's geometric centroid
-// (compute_energy_remain_rate, basic_function.cuh:437-494; parallel light along +z).  Per-block
-// partial sums of the weights go to part[block].
+// (compute_energy_remain_rate, basic_function.cuh:437-494).  The light reaching point i travels
+// along wdir[3i..3i+2] (a point light, main.cu:218-231), or along +z when wdir is null (parallel
+// light, main.cu:211-217).  Per-block partial sums of the weights go to part[block].
 __global__ void fresnel_weight_kernel(const BlockDesc* blocks, const double2* points,
-                                      const double* ins, const double* height,
+                                      const double* ins, const double* wdir, const double* height,
                                       const double* clip_area, const double2* centroid,
                                       double z_len, double z_recv, double n_idx, double* weight,
                                       double* part) {
@@ -58,7 +59,8 @@ __global__ void fresnel_weight_kernel(const BlockDesc* blocks, const double2* po
     const double2 c = centroid[i];
     const V3 wt = unit(V3{c.x - outp.x, c.y - outp.y, z_recv - outp.z});
     const V3 n2 = unit(V3{n_idx * d.x - wt.x, n_idx * d.y - wt.y, n_idx * d.z - wt.z});
-    const V3 wi{0.0, 0.0, 1.0}, n1{0.0, 0.0, 1.0};
+    const V3 wi = wdir ? V3{wdir[3 * i], wdir[3 * i + 1], wdir[3 * i + 2]} : V3{0.0, 0.0, 1.0};
+    const V3 n1{0.0, 0.0, 1.0};
     const double rate = transmittance(wi, d, n1, 1.0, n_idx) * transmittance(d, wt, n2, n_idx, 1.0);
     wv = clip_area[i] * rate;
     weight[i] = wv;
@@ -135,12 +137,12 @@ __global__ void axpy_kernel(const BlockDesc* blocks, double t, const double* d, 
 }
 
 hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points, const double* ins,
-                                 const double* height, const double* clip_area,
-                                 const double2* centroid, double z_len, double z_recv,
-                                 double n_idx, double* weight, double* part, int nblk,
-                                 hipStream_t st) {
+                                 const double* wdir, const double* height,
+                                 const double* clip_area, const double2* centroid, double z_len,
+                                 double z_recv, double n_idx, double* weight, double* part,
+                                 int nblk, hipStream_t st) {
   hipLaunchKernelGGL(fresnel_weight_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, points,
-                     ins, height, clip_area, centroid, z_len, z_recv, n_idx, weight, part);
+                     ins, wdir, height, clip_area, centroid, z_len, z_recv, n_idx, weight, part);
   return hipGetLastError();
 }
 

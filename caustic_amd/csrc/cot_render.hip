@@ -109,6 +109,58 @@ __global__ __launch_bounds__(kCellBlock) void render_kernel(const ProbDesc* prob
   }
 }
 
+// Solid angle under which a point light at (lx, ly, lz) sees each cell placed in the plane z:
+// the cell fanned from its vertex 0 into triangles, each a spherical triangle of the unit
+// vectors a, b, c towards its corners, Omega = 2 atan(a.(b x c) / (1 + a.b + b.c + c.a))
+// (Van Oosterom and Strackee 1983), a triangle with |denominator| < 1e-6 skipped, and the
+// magnitude of the sum taken (compute_solid_angle, basic_function.cuh:558-588).  Cells with
+// fewer than 3 vertices get 0.  One site per lane.
+__global__ __launch_bounds__(kCellBlock) void solid_angle_kernel(const BlockDesc* blocks,
+                                                                 const double2* vtx,
+                                                                 const uint8_t* nedge, double lx,
+                                                                 double ly, double lz, double z,
+                                                                 double* omega) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if ((int)threadIdx.x >= bd.count) return;
+  const int i = bd.first + threadIdx.x;
+  const int nv = nedge[i];
+  double sum = 0.0;
+  if (nv >= 3) {
+    const double2* v = vtx + (size_t)i * kMaxVerts;
+    const double dz = z - lz;
+    auto dir = [&](double2 p, double& x, double& y, double& zz) {
+      const double ex = p.x - lx, ey = p.y - ly;
+      const double r = sqrt(ex * ex + ey * ey + dz * dz);
+      x = ex / r;
+      y = ey / r;
+      zz = dz / r;
+    };
+    double ax, ay, az, bx, by, bz, cx, cy, cz;
+    dir(v[0], ax, ay, az);
+    dir(v[1], bx, by, bz);
+    for (int k = 2; k < nv; ++k) {
+      dir(v[k], cx, cy, cz);
+      const double num = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) +
+                         az * (bx * cy - by * cx);
+      const double den = 1.0 + (ax * bx + ay * by + az * bz) + (bx * cx + by * cy + bz * cz) +
+                         (ax * cx + ay * cy + az * cz);
+      if (fabs(den) >= 1e-6) sum += 2.0 * atan(num / den);
+      bx = cx;
+      by = cy;
+      bz = cz;
+    }
+  }
+  omega[i] = fabs(sum);
+}
+
+hipError_t launch_solid_angle(const BlockDesc* blocks, const double2* vtx, const uint8_t* nedge,
+                              const double light[3], double z, double* omega, int nblk,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(solid_angle_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, vtx, nedge,
+                     light[0], light[1], light[2], z, omega);
+  return hipGetLastError();
+}
+
 __global__ void fill_f64_kernel(double* p, size_t n, double v) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;

@@ -172,6 +172,16 @@ class OTBatch:
         L.check(self._lib.cot_cell_areas(self._ctx, _ptr(w), _ptr(a)))
         return a
 
+    def cell_solid_angles(self, light, z: float, w=None):
+        """Solid angle under which a point light at light (x, y, z) sees each cell placed in
+        the plane z (cot_cell_solid_angles; solid_angle, basic_function.cuh:635-651)."""
+        w = self._w(w)
+        lt = np.ascontiguousarray(light, dtype=np.float64).reshape(3)
+        om = np.zeros(self.n_total)
+        L.check(self._lib.cot_cell_solid_angles(self._ctx, _ptr(w), _ptr(lt), float(z),
+                                                _ptr(om)))
+        return om
+
     def stats(self):
         """Kernel time and counts accumulated since the last reset_stats()."""
         s = L.cot_stats()
@@ -284,8 +294,9 @@ class OTBatch:
         L-BFGS with those masses frozen (if_aa_op_initial), then Anderson acceleration of
         w + aa_fixed_para * (mass - nu) with the masses recomputed at every trial, until
         |mass - nu|_2 < tol.  lens: dict with points (N, 2), ins (N, 3), height (N,),
-        clip_area (N,), z_of_len, z_of_receiver, refraction_index, area_scalar.
-        Returns (w, report)."""
+        clip_area (N,), z_of_len, z_of_receiver, refraction_index, area_scalar, and for a
+        point light incident (N, 3), the unit directions of the light at the points
+        (main.cu:218-231; parallel light along +z without it).  Returns (w, report)."""
         if self.n_problems != 1:
             raise ValueError("the reflection solve takes one-problem batches")
         w = self._w(w0).copy()
@@ -303,6 +314,9 @@ class OTBatch:
         r.aa_history_size = int(aa_history_size)
         r.aa_fixed_para = float(aa_fixed_para)
         r.if_aa_op_initial = int(bool(if_aa_op_initial))
+        if lens.get("incident") is not None:
+            inc = np.ascontiguousarray(lens["incident"], dtype=np.float64).reshape(n, 3)
+            r.incident_dir = _ptr(inc)
         o = L.cot_solve_opts()
         self._lib.cot_default_solve_opts(ctypes.byref(o))
         o.history = history

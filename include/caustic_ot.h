@@ -15,6 +15,8 @@
  *   cot_centroids              compute_center, weighted or geometric, with if_exist
  *                              (basic_function.cuh:1056-1191; ot_op.cuh:539-546)
  *   cot_cell_areas             compute_clip_area, the source-mass step (basic_function.cuh:610-633)
+ *   cot_cell_solid_angles      solid_angle + compute_solid_angle, the point-light source-mass
+ *                              step (basic_function.cuh:558-588, 635-651; ot_op.cuh:160-184)
  */
 #ifndef CAUSTIC_OT_H_
 #define CAUSTIC_OT_H_
@@ -148,6 +150,9 @@ typedef struct {
   int aa_history_size;
   double aa_fixed_para;
   int if_aa_op_initial;
+  /* 3n unit directions of the light arriving at the points: normalize((x, y, z_of_len) -
+   * light) for a point light (main.cu:218-231); NULL for parallel light along +z (main.cu:211-217) */
+  const double* incident_dir;
 } cot_reflection;
 
 int cot_solve_reflection(cot_context* ctx, double* w, const cot_solve_opts* opt,
@@ -190,6 +195,16 @@ int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centr
 
 /* Area of each cell (inside proj_domain) at weights w. */
 int cot_cell_areas(cot_context* ctx, const double* w, double* area);
+
+/* Solid angle under which a point light at light[0..2] sees each cell (inside proj_domain, at
+ * weights w) placed in the plane z (solid_angle, basic_function.cuh:635-651, with
+ * compute_solid_angle, :558-588): the cell fanned from its first vertex into spherical
+ * triangles, each 2 atan(a.(b x c) / (1 + a.b + b.c + c.a)) for the unit vectors towards its
+ * corners, triangles with |denominator| < 1e-6 skipped, the magnitude of the sum returned; 0
+ * for an empty cell.  For a point light the reference's source masses are
+ * nu_i = omega_i / sum_j omega_j * area_scalar (ot_op.cuh:160-206). */
+int cot_cell_solid_angles(cot_context* ctx, const double* w, const double* light, double z,
+                          double* omega);
 
 /* Kernel time accumulated since the last reset.  Each launch of the cell-construction kernel,
  * of the integration kernel and of their repair passes (cells that failed the conformity check)

@@ -16,6 +16,9 @@ Pieces and what they follow:
     ``stop="maxrel"`` swaps only the stopping test for max_i |mass_i - nu_i| / nu_i < tol.
   * ``anderson`` / ``reflection_solve`` -- the t >= 1 path of aarl_direct_ot (ot_op.cuh:591-1323):
     Fresnel-weighted source masses, L-BFGS with frozen masses, then Anderson acceleration.
+  * ``solid_angle`` / ``point_light_source`` -- the point-light source masses (compute_solid_angle,
+    solid_angle, len_solid_angle: basic_function.cuh:558-588, 635-658; ot_op.cuh:160-206) and the
+    light's directions at the points (main.cu:218-231).
   * ``read_pic_txt`` / ``format_weight`` -- text formats (basic_function.cuh:145-204,
     main.cu:295-299).
 """
@@ -210,6 +213,63 @@ def source_masses(points, sample_domain):
     cells = power_cells(points, np.zeros(points.shape[0]), corner)
     a = clip_area(cells, sd)
     return a / a.sum() * (xs * ys)
+
+
+def solid_angle(poly, light, z):
+    """Solid angle under which a point light sees the polygon poly (k, 2) placed in the plane
+    z, as compute_solid_angle does it (basic_function.cuh:558-588): fan from vertex 0, each
+    spherical triangle 2 atan(a.(b x c) / (1 + a.b + b.c + c.a)) of the unit vectors a, b, c
+    from the light to its corners (Van Oosterom and Strackee 1983), triangles with
+    |denominator| < 1e-6 skipped, the magnitude of the sum."""
+    poly = _f64(poly).reshape(-1, 2)
+    if poly.shape[0] < 3:
+        return 0.0
+    lt = _f64(light)
+    v = np.column_stack([poly[:, 0] - lt[0], poly[:, 1] - lt[1],
+                         np.full(poly.shape[0], z - lt[2])])
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    total = 0.0
+    a = v[0]
+    for k in range(1, poly.shape[0] - 1):
+        b, c = v[k], v[k + 1]
+        num = float(np.dot(a, np.cross(b, c)))
+        den = 1.0 + float(np.dot(a, b) + np.dot(b, c) + np.dot(a, c))
+        if abs(den) < 1e-6:
+            continue
+        total += 2.0 * np.arctan(num / den)
+    return abs(total)
+
+
+def domain_solid_angle(sample_domain, light, z):
+    """The sample domain's solid angle, its corners taken in len_solid_angle's order
+    (basic_function.cuh:653-658)."""
+    x0, x1, y0, y1 = _f64(sample_domain)
+    return solid_angle(np.array([[x0, y0], [x0, y1], [x1, y1], [x1, y0]]), light, z)
+
+
+def cell_solid_angles(cells, light, z):
+    start, length, verts = cells
+    return np.array([solid_angle(cell_polygon(start, length, verts, i), light, z)
+                     for i in range(start.shape[0])])
+
+
+def point_light_source(points, sample_domain, light, z_of_len):
+    """Point-light source data (ot_op.cuh:160-206; main.cu:218-231): the Voronoi cells of the
+    points inside the sample domain, each weighted by omega_i / omega_domain * area_scalar
+    (returned as clip_area, which the t >= 1 masses take), nu_i = that weight normalised to sum
+    area_scalar, and the unit directions from the light to the points in the plane z_of_len.
+    The cells are built in the sample domain directly: inside it they equal the widened
+    domain's cells clipped to it."""
+    sd = _f64(sample_domain)
+    area_scalar = (sd[1] - sd[0]) * (sd[3] - sd[2])
+    pts = _f64(points)
+    cells = power_cells(pts, np.zeros(pts.shape[0]), sd)
+    clip = cell_solid_angles(cells, light, z_of_len) / domain_solid_angle(sd, light, z_of_len) \
+        * area_scalar
+    d = np.column_stack([pts[:, 0] - light[0], pts[:, 1] - light[1],
+                         np.full(pts.shape[0], z_of_len - light[2])])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return clip / clip.sum() * area_scalar, clip, d
 
 
 def fresnel_rate(points, in_dir, ins, height, targets, z_len, z_recv, n_idx):
@@ -447,9 +507,11 @@ class ReflectionResult:
 
 def fresnel_masses(ev, lens):
     """nu_i = clip_area_i T1_i T2_i / sum * area_scalar, the transmittances taken at the cells'
-    geometric centroids ev.geo (ot_op.cuh:779-795, parallel light: wi = (0, 0, 1))."""
+    geometric centroids ev.geo (ot_op.cuh:779-795); the light arrives along lens["incident"]
+    (a point light) or along wi = (0, 0, 1) without it (parallel light)."""
     n = lens["points"].shape[0]
-    wi = np.tile([0.0, 0.0, 1.0], (n, 1))
+    wi = lens.get("incident")
+    wi = np.tile([0.0, 0.0, 1.0], (n, 1)) if wi is None else _f64(wi).reshape(n, 3)
     rate = fresnel_rate(lens["points"], wi, lens["ins"], lens["height"], ev.geo,
                         lens["z_of_len"], lens["z_of_receiver"], lens["refraction_index"])
     we = _f64(lens["clip_area"]) * rate

@@ -139,11 +139,19 @@ def test_cli_missing_inputs(tmp_path, drop, msg):
 
 
 @needs_cli
-def test_cli_rejects_unsupported_light(tmp_path):
+def test_cli_rejects_unknown_light(tmp_path):
+    """An unknown light_type is an error (the reference prints it and goes on with
+    uninitialised source masses, main.cu:232-234); a point light on the lens plane is a
+    settings error.  Both are caught before any GPU work."""
     build, xi = _layout(tmp_path)
     cfgp = build.parent / "set_json" / xi / "_py_sp_settings.json"
     cfg = json.loads(cfgp.read_text())
-    cfg["light_para"]["light_type"] = "point_light"
+    cfg["light_para"]["light_type"] = "area_light"
     cfgp.write_text(json.dumps(cfg))
     r = subprocess.run([CLI, "0", xi], cwd=build, capture_output=True, text=True)
-    assert r.returncode == 3 and "not supported" in r.stderr
+    assert r.returncode == 3 and "unknown light_type" in r.stderr
+    cfg["light_para"]["light_type"] = "point_light"
+    cfg["light_para"]["point_light_coordinate"] = [0.5, 0.5, cfg["len_para"]["z_of_len"]]
+    cfgp.write_text(json.dumps(cfg))
+    r = subprocess.run([CLI, "0", xi], cwd=build, capture_output=True, text=True)
+    assert r.returncode == 2 and "point_light_coordinate" in r.stderr

@@ -25,9 +25,10 @@ def write_picture(path, A):
     np.savetxt(path, A.T[::-1], fmt="%.17g")
 
 
-def make_tree(root):
+def make_tree(root, xiushi=XIUSHI, light=None):
     """Sites on a 16x16 jittered grid of the sample domain, projected straight down (proj_xy =
-    points); a 64x48 target of three Gaussians; lens data for t = 0 and t = 1."""
+    points); a 64x48 target of three Gaussians; lens data for t = 0 and t = 1.  light: None for
+    a parallel light, else the point light's coordinates."""
     rng = W.rng_for(5000)
     pts = np.array(SAMPLE[0::2]) + W.jittered_grid(16, rng) * (SAMPLE[1] - SAMPLE[0])
     n = pts.shape[0]
@@ -36,8 +37,8 @@ def make_tree(root):
     ins = np.concatenate([rng.uniform(-0.1, 0.1, (n, 2)), np.ones((n, 1))], 1)
     rl = os.path.join(root, "Optimal_Transport_rl")
     build = os.path.join(rl, "build")
-    seed = os.path.join(rl, "seed_point", XIUSHI)
-    for d in (build, os.path.join(rl, "set_json", XIUSHI), os.path.join(seed, "0"),
+    seed = os.path.join(rl, "seed_point", xiushi)
+    for d in (build, os.path.join(rl, "set_json", xiushi), os.path.join(seed, "0"),
               os.path.join(seed, "1"), os.path.join(root, "data", "pic_data")):
         os.makedirs(d, exist_ok=True)
     settings = {
@@ -49,10 +50,11 @@ def make_tree(root):
                               "aa_history_size": 7, "aa_fixed_para": -0.5},
         "len_para": {"z_of_len": 0, "z_of_receiver": 4.0, "proj_domain": list(PROJ),
                      "refraction_index": 1.5},
-        "light_para": {"light_type": "parallel_light", "point_light_coordinate": [1, 1, -1]},
+        "light_para": {"light_type": "parallel_light" if light is None else "point_light",
+                       "point_light_coordinate": [1, 1, -1] if light is None else list(light)},
         "mi355x": {"full_precision_weights": True},
     }
-    with open(os.path.join(rl, "set_json", XIUSHI, "_py_sp_settings.json"), "w") as f:
+    with open(os.path.join(rl, "set_json", xiushi, "_py_sp_settings.json"), "w") as f:
         json.dump(settings, f, indent=2)
     write_picture(os.path.join(root, "data", "pic_data", PIC + ".txt"), A)
     for name in ("points.txt", "proj_xy.txt"):
@@ -68,9 +70,9 @@ def make_tree(root):
             "settings": settings}
 
 
-def run_cli(build, t):
-    print(f"[case] caustic_gpu {t} {XIUSHI}", flush=True)
-    return subprocess.run([CLI, str(t), XIUSHI], cwd=build, capture_output=True, text=True,
+def run_cli(build, t, xiushi=XIUSHI):
+    print(f"[case] caustic_gpu {t} {xiushi}", flush=True)
+    return subprocess.run([CLI, str(t), xiushi], cwd=build, capture_output=True, text=True,
                           timeout=300)
 
 
@@ -137,3 +139,38 @@ def test_cli_t0_then_t1(gpu_available, tmp_path):
     ex1 = ev1.exist.astype(bool)
     out1 = np.array(read_rows(os.path.join(pdir, "1", "out.txt")), dtype=np.float64)
     np.testing.assert_allclose(out1, ev1.centroid[ex1], rtol=6e-6, atol=1e-9)
+
+
+@pytest.mark.gpu
+def test_cli_point_light_t0_then_t1(gpu_available, tmp_path):
+    """light_type point_light (ot_op.cuh:160-184, 687-709; main.cu:218-231): source masses from
+    the solid angles under which the light sees the source cells, and at t = 1 the Fresnel
+    weights with the light's own direction at each point; both checked by the oracle."""
+    assert os.path.exists(CLI), "caustic_amd/bin/caustic_gpu is not built"
+    xiushi, light = "synthetic_point_light", (0.8, 1.3, -0.9)
+    root = str(tmp_path)
+    T = make_tree(root, xiushi, light)
+    area_scalar = (SAMPLE[1] - SAMPLE[0]) * (SAMPLE[3] - SAMPLE[2])
+    rho = T["A"] / T["A"].sum() * area_scalar
+    pts = T["points"]
+    nu, clip, inc = O.point_light_source(pts, SAMPLE, light, 0.0)
+    par = O.source_masses(pts, SAMPLE)
+    assert np.abs(nu - par).max() > 0.1 * par.max()  # the light is near: masses differ
+    wdir = os.path.join(root, "data", "weights", xiushi)
+    r = run_cli(T["build"], 0, xiushi)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    w0 = np.loadtxt(os.path.join(wdir, "0", "final_proj_point_weights_full.txt"))
+    ev = O.evaluate(pts, w0, nu, rho, PROJ)
+    assert np.linalg.norm(ev.mass - nu) < 2 * TOL
+    r = run_cli(T["build"], 1, xiushi)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    w1 = np.loadtxt(os.path.join(wdir, "1", "final_proj_point_weights_full.txt"))
+    lens = {"points": pts, "ins": T["ins"] / np.linalg.norm(T["ins"], axis=1, keepdims=True),
+            "height": T["heights"][1], "clip_area": clip, "incident": inc, "z_of_len": 0.0,
+            "z_of_receiver": 4.0, "refraction_index": 1.5, "area_scalar": area_scalar}
+    ev1 = O.evaluate(pts, w1, nu, rho, PROJ)
+    nu1 = O.fresnel_masses(ev1, lens)
+    assert np.linalg.norm(ev1.mass - nu1) < TOL * (1 + 1e-6)
+    # the light's slant changes the front-surface transmittance: parallel-light masses differ
+    nu1_par = O.fresnel_masses(ev1, dict(lens, incident=None))
+    assert np.abs(nu1 - nu1_par).max() > 1e-4 * nu1.max()

@@ -237,3 +237,24 @@ def test_render_and_cell_polygons_match_oracle(gpu_available, c1):
         areas[i] = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
     assert (areas[ex] > 0).all()  # counter-clockwise
     np.testing.assert_allclose(areas, orc.area, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cell_solid_angles_match_oracle(gpu_available, c1, precision):
+    """Point-light source weights: the solid angle of each cell seen from a light below the
+    plane (cot_cell_solid_angles) against the oracle's restatement of compute_solid_angle on
+    its own cells; hidden cells get 0 and the cells add up to the domain's solid angle."""
+    print(f"[case] cell solid angles c1 {precision}", flush=True)
+    rng = np.random.default_rng(9)
+    w = rng.normal(0.0, 2e-4, c1.n)
+    light, z = (0.35, 0.6, -0.45), 0.1
+    orc = O.evaluate(c1.sites, w, c1.nu, c1.rho, c1.proj_domain)
+    ref = O.cell_solid_angles(orc.cells, light, z)
+    with _batch([c1], precision=precision) as b:
+        om = b.cell_solid_angles(light, z, w)
+    ex = orc.exist.astype(bool)
+    assert (~ex).any() and np.all(om[~ex] == 0.0)
+    # FP64 vertices in both precisions; vertex order and fan start differ from the oracle's
+    np.testing.assert_allclose(om, ref, rtol=1e-11, atol=1e-16)
+    whole = O.domain_solid_angle(c1.proj_domain, light, z)
+    assert abs(om.sum() - whole) < 1e-11 * whole

@@ -295,3 +295,43 @@ def test_anderson_rejected_trials_restart_and_the_zero_count_rule():
 
     r = O.anderson(tiny, np.zeros(2), np.full(2, 1e-9), 1, 3, -1.0, 10, 1e-6)
     assert r.converged and r.iterations == 2  # iteration 0 sees a zero cell; 1 does not count
+
+
+def test_solid_angle_of_a_rectangle_closed_form():
+    """A rectangle [0, a] x [0, b] seen from the point h below its corner subtends
+    atan(a b / (h sqrt(a^2 + b^2 + h^2))); four such rectangles around the foot of the light
+    add up to the centred rectangle; the fan's first vertex does not matter."""
+    for a, b, h in ((1.0, 1.0, 1.0), (0.3, 2.0, 0.7), (5.0, 0.1, 3.0)):
+        rect = np.array([[0, 0], [a, 0], [a, b], [0, b]], dtype=np.float64)
+        want = np.arctan(a * b / (h * np.sqrt(a * a + b * b + h * h)))
+        for k in range(4):
+            got = O.solid_angle(np.roll(rect, k, axis=0), (0.0, 0.0, -h), 0.0)
+            assert abs(got - want) < 1e-14 * max(want, 1.0), (a, b, h, k, got, want)
+        # clockwise order, light above the plane: same magnitude
+        assert abs(O.solid_angle(rect[::-1], (0.0, 0.0, h), 0.0) - want) < 1e-14
+    a, b, h = 0.8, 0.5, 0.6
+    whole = O.domain_solid_angle((-a, a, -b, b), (0.0, 0.0, -h), 0.0)
+    want = 4.0 * np.arctan(a * b / (h * np.sqrt(a * a + b * b + h * h)))
+    assert abs(whole - want) < 1e-13
+    # the reference's atan (not atan2) and its 1e-6 denominator rule hold while every fan
+    # triangle subtends less than pi: a huge square's two triangles approach pi each, their
+    # denominators vanish and they are skipped (the formula's limit, kept as the reference has it)
+    assert O.domain_solid_angle((-1e7, 1e7, -1e7, 1e7), (0.0, 0.0, -1.0), 0.0) < 1e-6
+
+
+def test_point_light_cells_add_up_and_tend_to_areas():
+    """The cells' solid angles add up to the sample domain's; a light far away weights the
+    cells by area (the parallel-light masses)."""
+    rng = W.rng_for(77)
+    sd = (0.0, 2.0, 0.0, 1.5)
+    pts = np.array([0.0, 0.0]) + W.jittered_grid(12, rng) * [2.0, 1.5]
+    light = (0.7, 0.4, -1.3)
+    nu, clip, d = O.point_light_source(pts, sd, light, 0.25)
+    area_scalar = 3.0
+    assert abs(clip.sum() - area_scalar) < 1e-12 * area_scalar
+    assert abs(nu.sum() - area_scalar) < 1e-12
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, rtol=0, atol=1e-15)
+    par = O.source_masses(pts, sd)
+    assert np.abs(nu - par).max() > 0.05 * par.max()  # a near light reweights the cells
+    far, _, _ = O.point_light_source(pts, sd, (1.0, 0.75, -1e5), 0.0)
+    np.testing.assert_allclose(far, par, rtol=1e-8)
