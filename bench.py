@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--start", default="multiscale", choices=["multiscale", "learned", "zero"])
     ap.add_argument("--no-cold", action="store_true", help="skip the plain zero-start solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--learned-seeds", type=int, default=4,
+                    help="held-out problems for the learned-vs-zero warm-start comparison")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--device-loop", default="graph", choices=["graph", "eager"],
                     help="eager: the same device rounds launched without graph capture")
@@ -160,7 +162,9 @@ def main():
         (None: zeros) before that level is solved."""
         if args.start == "zero":
             return None, 0
-        fields = None if args.start == "multiscale" else WS.predict_fields(net, probs)
+        fields = None if args.start == "multiscale" else [
+            WS.LEARNED_ALPHA * WS.smooth_field(U, WS.LEARNED_SIGMA)
+            for U in WS.predict_fields(net, probs)]
         evals = 0
         for li, (n_side, cps) in enumerate(zip(levels, coarse)):
             init = None if fields is None else np.concatenate(
@@ -312,6 +316,21 @@ def main():
     if not args.no_cpu_baseline and args.workload == "shapes65k":
         n_eval = cold["evaluations"] if cold else out["config"]["evaluations_all_levels"]
         out["cpu_baseline"] = cpu_baseline(probs[0], n_eval, args.cpu_budget)
+    if args.learned_seeds > 0 and os.path.exists(WS.DEFAULT_MODEL):
+        # the metric's second part: the learned warm start against the zero start on held-out
+        # N = 16,384 problems (128 x 128 jittered sites, 512 x 512 shapes targets)
+        rows = WS.evaluate_warm_start(WS.WarmStartNet.load(WS.DEFAULT_MODEL),
+                                      range(900000, 900000 + args.learned_seeds), log=log)
+        ci, wi = sum(r["cold_iters"] for r in rows), sum(r["warm_iters"] for r in rows)
+        cs, ws_ = sum(r["cold_s"] for r in rows), sum(r["warm_s"] for r in rows)
+        out["learned_vs_zero"] = {
+            "workload": "N=16,384 (128x128 jittered grid), 512x512 shapes target, mixed "
+                        "precision, L-BFGS to max-rel 1 then Newton to 1e-5",
+            "held_out_seeds": [r["seed"] for r in rows],
+            "zero_iterations": ci, "learned_iterations": wi, "iteration_ratio": ci / max(wi, 1),
+            "zero_seconds": cs, "learned_seconds": ws_, "speedup": cs / ws_,
+            "learned_used": [r["learned_used"] for r in rows],
+            "converged": all(r["converged"] for r in rows)}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -108,3 +108,29 @@ def test_training_reduces_the_loss(gpu_available):
     st = net.train_stats()
     assert st["samples"] == 15 * n and st["seconds"] > 0
     net.close()
+
+
+def test_learned_start_gains_over_zero_start(gpu_available):
+    """The committed network (caustic_amd/warmstart_unet128.npz: trained on the GPU by this
+    package's kernels on 1,152 converged N = 16,384 solves of seeds 100000 and up) against the
+    zero start on two held-out targets (seeds 900000 and 900001), each solved to max-rel 1e-5
+    (L-BFGS to max-rel 1, then Newton): the learned start is taken (its |mass - nu|_2 is below
+    the zero start's), needs at least 10% fewer iterations, and its solution is confirmed by the
+    FP64 oracle."""
+    import os
+
+    from caustic_amd import warmstart as WS
+    from caustic_amd import workloads as W
+    from oracle import oracle as O
+
+    assert os.path.exists(WS.DEFAULT_MODEL), "the trained network is missing"
+    print("[case] learned vs zero start, N=16384, seeds 900000-900001", flush=True)
+    net = WS.WarmStartNet.load(WS.DEFAULT_MODEL)
+    rows = WS.evaluate_warm_start(net, [900000, 900001], log=print)
+    net.close()
+    for r in rows:
+        assert r["converged"] and r["learned_used"], r
+        assert r["warm_iters"] < 0.9 * r["cold_iters"], (r["warm_iters"], r["cold_iters"])
+        p = W.warm_start_sample(r["seed"])
+        ev = O.evaluate(p.sites, r["w_warm"], p.nu, p.rho, p.proj_domain)
+        assert O.max_rel_error(ev.mass, p.nu) < 1.1e-5

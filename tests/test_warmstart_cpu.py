@@ -111,3 +111,20 @@ def test_field_round_trip_in_a_frame():
     cen = rng.uniform(0, 1, (S * S, 2))
     U = WS.field_from_solution(nodes, w, cen, S, frame)
     np.testing.assert_allclose(WS.weights_from_field(U, nodes, frame), w - w.mean(), atol=1e-12)
+
+
+def test_learned_start_low_pass_is_in_grid_nodes():
+    """smooth_field's width is in grid nodes: a cosine of k cycles over 128 nodes keeps
+    exp(-(2 pi k sigma / 128)^2 / 2) of its amplitude away from the edges; the mean goes."""
+    from caustic_amd import warmstart as WS
+
+    S, k, sigma = 128, 4, 4.0
+    x = np.arange(S)
+    U = np.cos(2 * np.pi * k * x / S)[None, :] * np.ones((S, 1)) + 3.0
+    f = WS.smooth_field(U, sigma)
+    want = np.exp(-0.5 * (2 * np.pi * k * sigma / S) ** 2)
+    # crest minus trough away from the edges: 2 x the kept amplitude (removing the mean shifts
+    # both by the same constant)
+    amp = 0.5 * (f[S // 2, [32, 64]] - f[S // 2, [48, 80]])
+    np.testing.assert_allclose(amp, want, rtol=1e-4)
+    assert abs(f.mean()) < 1e-12
