@@ -38,7 +38,9 @@ hipError_t launch_pack_forward(const float* w, int K, int N, int Kp, int Np, uin
 hipError_t launch_pack_backward(const float* w, int ksize, int Cin, int Cout, int Kp, int Np,
                                 uint16_t* wd, hipStream_t st);
 hipError_t launch_relu_grad(const float* y, const float* dy, float* dz, size_t n, hipStream_t st);
-hipError_t launch_bias_grad(const float* dz, int M, int N, float* db, hipStream_t st);
+hipError_t launch_bias_grad(const float* dz, int M, int N, float* db, float* part,
+                            hipStream_t st);
+constexpr size_t kBiasPartFloats = 1024 * 256;  // kBiasBlocks x 256 (wsn_kernels.hip)
 hipError_t launch_pool(const float* in, int B, int H, int W, int C, float* out, hipStream_t st);
 hipError_t launch_pool_back(const float* dout, int B, int H, int W, int C, float* din, int acc,
                             hipStream_t st);
@@ -126,6 +128,7 @@ struct wsn_net {
   float *gp0 = nullptr, *gp1 = nullptr, *gp2 = nullptr, *gc2 = nullptr, *gc1 = nullptr,
         *gc0 = nullptr;
   float* dz = nullptr;      // pre-activation gradient scratch
+  float* bpart = nullptr;   // per-block partial sums of the bias gradients
   float* tgt = nullptr;     // targets of the batch
   float* loss = nullptr;    // per-sample loss
   // resident training set
@@ -256,7 +259,7 @@ int conv_bwd(wsn_net* n, int li, const float* x, const float* gy, float* gx, int
   chunk = std::min(chunk, rup(M, 32));
   w.chunk = chunk;
   HIPCHK(launch_wgrad(w, n->st));
-  HIPCHK(launch_bias_grad(dz, M, l.s.cout, n->g + l.boff, n->st));
+  HIPCHK(launch_bias_grad(dz, M, l.s.cout, n->g + l.boff, n->bpart, n->st));
   if (gx) {
     ConvArgs a{};
     a.in = dz;
@@ -395,7 +398,7 @@ int wsn_create(int S, int max_batch, wsn_net** out) {
   const size_t B = max_batch, P0 = (size_t)S * S;
   auto A = [&](float** p, size_t count) { return dalloc(n, p, count); };
   if ((rc = A(&n->in, B * P0)) || (rc = A(&n->tgt, B * P0)) || (rc = A(&n->loss, B)) ||
-      (rc = A(&n->dz, B * P0 * 32)) || (rc = A(&n->p0, B * P0 / 4 * 32)) ||
+      (rc = A(&n->dz, B * P0 * 32)) || (rc = A(&n->bpart, kBiasPartFloats)) || (rc = A(&n->p0, B * P0 / 4 * 32)) ||
       (rc = A(&n->p1, B * P0 / 16 * 64)) || (rc = A(&n->p2, B * P0 / 64 * 128)) ||
       (rc = A(&n->c2, B * P0 / 16 * 320)) || (rc = A(&n->c1, B * P0 / 4 * 192)) ||
       (rc = A(&n->c0, B * P0 * 96)) || (rc = A(&n->gp0, B * P0 / 4 * 32)) ||

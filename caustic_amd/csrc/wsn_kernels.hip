@@ -15,9 +15,13 @@
 // The rest (pooling, upsampling, channel copies, ReLU mask, bias gradient, loss, Adam) is
 // elementwise or a short reduction.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 namespace wsn {
+
+// rows of the bias gradient's partial-sum buffer (launch_bias_grad): kBiasBlocks x 256 floats
+constexpr int kBiasBlocks = 1024;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
@@ -222,14 +226,45 @@ __global__ void relu_grad_kernel(const float* y, const float* dy, float* dz, siz
   if (i < n) dz[i] = y[i] > 0.f ? dy[i] : 0.f;
 }
 
-// db[n] += sum over a chunk of rows of dz[m][n]; grid = ceil(M / chunk), block 256 (N <= 256)
-__global__ void bias_grad_kernel(const float* dz, int M, int N, int chunk, float* db) {
-  const int n = threadIdx.x;
-  if (n >= N) return;
-  const int m0 = blockIdx.x * chunk, m1 = min(m0 + chunk, M);
+// Bias gradient db[n] += sum_m dz[m][n] (N <= 256) in two passes with no atomics.  Pass 1:
+// block b sums rows [b * chunk, (b + 1) * chunk); its 256 threads split into R = 256 / N row
+// phases of N columns each (thread t: column t % N, rows from t / N in steps of R), so a
+// narrow layer (N = 1 or 32) still keeps every thread loading; the R phases of a column are
+// added in LDS and the block writes part[b * N + n].
+__global__ __launch_bounds__(256) void bias_part_kernel(const float* dz, int M, int N, int chunk,
+                                                        float* part) {
+  __shared__ float red[256];
+  const int R = 256 / N;
+  const int t = threadIdx.x;
+  const int n = t % N, r = t / N;
   float s = 0.f;
-  for (int m = m0; m < m1; ++m) s += dz[(size_t)m * N + n];
-  atomicAdd(&db[n], s);
+  if (r < R) {
+    const int m0 = blockIdx.x * chunk, m1 = min(m0 + chunk, M);
+    for (int m = m0 + r; m < m1; m += R) s += dz[(size_t)m * N + n];
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < N) {
+    float a = 0.f;
+    for (int q = 0; q < R; ++q) a += red[q * N + t];
+    part[(size_t)blockIdx.x * N + t] = a;
+  }
+}
+
+// Pass 2: one block per column adds the nb partials (tree in LDS) into db[n].
+__global__ __launch_bounds__(256) void bias_sum_kernel(const float* part, int nb, int N,
+                                                       float* db) {
+  __shared__ float red[256];
+  const int n = blockIdx.x, t = threadIdx.x;
+  float a = 0.f;
+  for (int b = t; b < nb; b += 256) a += part[(size_t)b * N + n];
+  red[t] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) db[n] += red[0];
 }
 
 // 2 x 2 average pool: out [B, H/2, W/2, C]
@@ -414,10 +449,16 @@ hipError_t launch_relu_grad(const float* y, const float* dy, float* dz, size_t n
   hipLaunchKernelGGL(relu_grad_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, y, dy, dz, n);
   return hipGetLastError();
 }
-hipError_t launch_bias_grad(const float* dz, int M, int N, float* db, hipStream_t st) {
-  const int chunk = 512;
-  hipLaunchKernelGGL(bias_grad_kernel, dim3((M + chunk - 1) / chunk), dim3(256), 0, st, dz, M, N,
-                     chunk, db);
+hipError_t launch_bias_grad(const float* dz, int M, int N, float* db, float* part,
+                            hipStream_t st) {
+  if (N < 1 || N > 256 || M < 1) return hipErrorInvalidValue;
+  // about 16 loads per thread, at most kBiasBlocks blocks (the partial buffer's rows)
+  long long nb = ((long long)M * N + 4095) / 4096;
+  nb = std::max(1LL, std::min(nb, (long long)kBiasBlocks));
+  const int chunk = (int)((M + nb - 1) / nb);
+  const int blocks = (M + chunk - 1) / chunk;
+  hipLaunchKernelGGL(bias_part_kernel, dim3(blocks), dim3(256), 0, st, dz, M, N, chunk, part);
+  hipLaunchKernelGGL(bias_sum_kernel, dim3(N), dim3(256), 0, st, part, blocks, N, db);
   return hipGetLastError();
 }
 hipError_t launch_pool(const float* in, int B, int H, int W, int C, float* out, hipStream_t st) {

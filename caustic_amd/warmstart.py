@@ -568,8 +568,29 @@ def main(argv=None):
     ap.add_argument("--eval", type=int, default=16, help="held-out problems to evaluate")
     ap.add_argument("--log", default=None, help="JSON summary path")
     ap.add_argument("--eval-only", default=None, help="evaluate this saved model, no training")
+    ap.add_argument("--throughput", type=int, default=0,
+                    help="train on this many synthetic S x S pairs (no solver, nothing saved) and "
+                         "report the training kernels' TFLOP/s")
     args = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
+    if args.throughput:
+        # smooth random inputs and targets of the production shape: the training kernels'
+        # speed does not depend on the values
+        rng = np.random.default_rng(0)
+        c = (np.arange(S_DEFAULT) + 0.5) / S_DEFAULT
+        X, Y = np.meshgrid(c, c, indexing="xy")
+        xs = np.stack([np.sin(2 * np.pi * (a * X + b * Y)) for a, b in
+                       rng.uniform(0.5, 3.0, (args.throughput, 2))]).astype(np.float32)
+        ts = (xs ** 2 - (xs ** 2).mean(axis=(1, 2), keepdims=True)).astype(np.float32)
+        net = WarmStartNet(S=S_DEFAULT, max_batch=args.batch, seed=0)
+        train(net, xs, ts, args.epochs, args.batch, args.lr, augment=False,
+              grad_weight=args.grad_weight, log=log)
+        st = net.train_stats()
+        log(f"[throughput] {st['samples']} samples in {st['seconds']:.2f}s: "
+            f"{st['samples'] * st['flops_per_sample'] / st['seconds'] / 1e12:.1f} TFLOP/s "
+            f"({st['flops_per_sample'] / 1e9:.2f} GFLOP per sample, forward + backward)")
+        net.close()
+        return 0
     if args.eval_only:
         rows = evaluate_warm_start(WarmStartNet.load(args.eval_only),
                                    range(900000, 900000 + args.eval), log=log)
