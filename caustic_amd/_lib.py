@@ -86,6 +86,10 @@ SIGNATURES = {
     "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
     "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "cot_cell_solid_angles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]),
+    "cot_irra_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_float, _dp,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }

@@ -78,6 +78,9 @@ hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points,
 hipError_t launch_solid_angle(const BlockDesc* blocks, const double2* vtx, const uint8_t* nedge,
                               const double light[3], double z, double* omega, int nblk,
                               hipStream_t st);
+hipError_t launch_tri_render(const float* energy, const int* faces, int n_faces, const float* xy,
+                             float gray, const double dom[4], int pm, int pn, float* image,
+                             hipStream_t st);
 hipError_t launch_scale(const BlockDesc* blocks, const double* weight, double scale, double* nu,
                         int nblk, hipStream_t st);
 hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const double* g, double beta,
@@ -1069,6 +1072,28 @@ int cot_cell_areas(cot_context* c, const double* w, double* area) {
   std::vector<EvalSum> sum;
   CHK(evaluate_device(c, c->d_w, nullptr, act, sum));
   CHK(download_sorted(c, c->d_area, area));
+  return COT_OK;
+}
+
+int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, const float* xy,
+                    int n_points, float total_gray, const double* proj_domain, int pm, int pn,
+                    float* image, void* stream) {
+  if (n_faces < 0 || n_points < 0 || pm <= 0 || pn <= 0 || !proj_domain || !image ||
+      (n_faces > 0 && (!tri_energy || !faces || !xy)))
+    return fail(COT_EINVAL, "bad argument");
+  if (!(proj_domain[1] > proj_domain[0] && proj_domain[3] > proj_domain[2]))
+    return fail(COT_EINVAL, "empty proj_domain");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (n_faces > 0) {
+    // every vertex index must name a point before a kernel dereferences it
+    std::vector<int> fh(3 * (size_t)n_faces);
+    HIPCHK(hipMemcpyAsync(fh.data(), faces, sizeof(int) * fh.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int v : fh)
+      if (v < 0 || v >= n_points) return fail(COT_EINVAL, "face index out of range");
+  }
+  HIPCHK(launch_tri_render(tri_energy, faces, n_faces, xy, total_gray, proj_domain, pm, pn, image,
+                           st));
   return COT_OK;
 }
 

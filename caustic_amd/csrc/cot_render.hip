@@ -109,6 +109,68 @@ __global__ __launch_bounds__(kCellBlock) void render_kernel(const ProbDesc* prob
   }
 }
 
+// Irradiance image of a triangle mesh (compute_render of the irra_render extension,
+// render_exe_program/render.cu:93-171): face f spreads tri_energy[f] * gray uniformly over its
+// triangle, pixel (ix, iy) of the pm x pn grid over [x0, x1] x [y0, y1] receiving
+// e_f |T_f n p| / |T_f|.  The image is written in the extension's output layout, row
+// pm - 1 - ix, column iy of a pm x pn array.  FP32 in and out; clipping and areas in FP64.
+// One face per lane.
+__global__ void tri_render_kernel(const float* energy, const int* faces, int n_faces,
+                                  const float* xy, float gray, double x0, double y0, double dx,
+                                  double dy, int pm, int pn, float* image) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  double px[3], py[3];
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int k = 0; k < 3; ++k) {
+    const int v = faces[3 * f + k];
+    px[k] = xy[2 * v];
+    py[k] = xy[2 * v + 1];
+    mnx = fmin(mnx, px[k]); mxx = fmax(mxx, px[k]);
+    mny = fmin(mny, py[k]); mxy = fmax(mxy, py[k]);
+  }
+  const double area = shoelace(px, py, 3);
+  if (!(area > 0.0)) return;
+  const int ix0 = max((int)floor((mnx - x0) / dx), 0);
+  const int iy0 = max((int)floor((mny - y0) / dy), 0);
+  const int ix1 = min((int)floor((mxx - x0) / dx) + 1, pm);
+  const int iy1 = min((int)floor((mxy - y0) / dy) + 1, pn);
+  const double scale = (double)energy[f] * gray / area;
+  double ax[kClipMax], ay[kClipMax], bx[kClipMax], by[kClipMax];
+  for (int ix = ix0; ix < ix1; ++ix) {
+    const double cx0 = x0 + ix * dx, cx1 = x0 + (ix + 1) * dx;
+    float* row = image + (size_t)(pm - 1 - ix) * pn;
+    for (int iy = iy0; iy < iy1; ++iy) {
+      const double cy0 = y0 + iy * dy, cy1 = y0 + (iy + 1) * dy;
+      int n = clip_side(px, py, 3, 0, cx0, cx1, cy0, cy1, ax, ay);
+      n = clip_side(ax, ay, n, 1, cx0, cx1, cy0, cy1, bx, by);
+      n = clip_side(bx, by, n, 2, cx0, cx1, cy0, cy1, ax, ay);
+      n = clip_side(ax, ay, n, 3, cx0, cx1, cy0, cy1, bx, by);
+      if (n < 3) continue;
+      const double a = shoelace(bx, by, n);
+      if (a > 0.0) atomicAdd(&row[iy], (float)(scale * a));
+    }
+  }
+}
+
+__global__ void fill_f32_kernel(float* p, size_t n) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0f;
+}
+
+hipError_t launch_tri_render(const float* energy, const int* faces, int n_faces, const float* xy,
+                             float gray, const double dom[4], int pm, int pn, float* image,
+                             hipStream_t st) {
+  const size_t px = (size_t)pm * pn;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, image,
+                     px);
+  if (n_faces > 0)
+    hipLaunchKernelGGL(tri_render_kernel, dim3((n_faces + 255) / 256), dim3(256), 0, st, energy,
+                       faces, n_faces, xy, gray, dom[0], dom[2], (dom[1] - dom[0]) / pm,
+                       (dom[3] - dom[2]) / pn, pm, pn, image);
+  return hipGetLastError();
+}
+
 // Solid angle under which a point light at (lx, ly, lz) sees each cell placed in the plane z:
 // the cell fanned from its vertex 0 into triangles, each a spherical triangle of the unit
 // vectors a, b, c towards its corners, Omega = 2 atan(a.(b x c) / (1 + a.b + b.c + c.a))

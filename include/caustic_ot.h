@@ -15,6 +15,8 @@
  *   cot_centroids              compute_center, weighted or geometric, with if_exist
  *                              (basic_function.cuh:1056-1191; ot_op.cuh:539-546)
  *   cot_cell_areas             compute_clip_area, the source-mass step (basic_function.cuh:610-633)
+ *   cot_irra_render            render() of the irra_render extension, the irradiance image of a
+ *                              triangle mesh (render_exe_program/render.cu:93-171, 553-591)
  *   cot_cell_solid_angles      solid_angle + compute_solid_angle, the point-light source-mass
  *                              step (basic_function.cuh:558-588, 635-651; ot_op.cuh:160-184)
  */
@@ -195,6 +197,19 @@ int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centr
 
 /* Area of each cell (inside proj_domain) at weights w. */
 int cot_cell_areas(cot_context* ctx, const double* w, double* area);
+
+/* Irradiance image of a triangle mesh: the forward pass of the irra_render extension's render()
+ * (render_exe_program/render.cu:553-591, with compute_render, :93-171).  Face f (vertex indices
+ * faces[3f..3f+2] into the n_points points xy, (x, y) pairs) spreads tri_energy[f] * total_gray
+ * uniformly over its triangle; pixel (ix, iy) of the pm x pn grid over proj_domain
+ * {x0, x1, y0, y1} receives e_f |T_f n pixel| / |T_f| (degenerate triangles add nothing).
+ * image gets pm x pn floats in render()'s output layout: row pm - 1 - ix, column iy.  All
+ * pointers are device memory; the work is ordered on stream (a hipStream_t; NULL: the default
+ * stream) and the call waits for it only to check that every face index names a point
+ * (COT_EINVAL otherwise).  No context is needed. */
+int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, const float* xy,
+                    int n_points, float total_gray, const double* proj_domain, int pm, int pn,
+                    float* image, void* stream);
 
 /* Solid angle under which a point light at light[0..2] sees each cell (inside proj_domain, at
  * weights w) placed in the plane z (solid_angle, basic_function.cuh:635-651, with

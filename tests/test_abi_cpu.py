@@ -155,3 +155,15 @@ def test_cli_rejects_unknown_light(tmp_path):
     cfgp.write_text(json.dumps(cfg))
     r = subprocess.run([CLI, "0", xi], cwd=build, capture_output=True, text=True)
     assert r.returncode == 2 and "point_light_coordinate" in r.stderr
+
+
+def test_irra_render_rejects_bad_arguments_without_touching_the_gpu():
+    lib = L.load()
+    dom = (ctypes.c_double * 4)(0.0, 1.0, 0.0, 1.0)
+    img = ctypes.c_void_p(0)
+    assert lib.cot_irra_render(None, None, -1, None, 0, 1.0, dom, 4, 4, img, None) == L.COT_EINVAL
+    assert lib.cot_irra_render(None, None, 0, None, 0, 1.0, dom, 0, 4, img, None) == L.COT_EINVAL
+    bad = (ctypes.c_double * 4)(1.0, 1.0, 0.0, 1.0)
+    assert lib.cot_irra_render(None, None, 0, None, 0, 1.0, bad, 4, 4, ctypes.c_void_p(8),
+                               None) == L.COT_EINVAL
+    assert b"proj_domain" in lib.cot_last_error()
