@@ -90,6 +90,10 @@ SIGNATURES = {
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, _dp,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "cot_irra_render_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                            ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
+                             + [ctypes.c_void_p] * 8),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }

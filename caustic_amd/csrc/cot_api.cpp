@@ -81,6 +81,11 @@ hipError_t launch_solid_angle(const BlockDesc* blocks, const double2* vtx, const
 hipError_t launch_tri_render(const float* energy, const int* faces, int n_faces, const float* xy,
                              float gray, const double dom[4], int pm, int pn, float* image,
                              hipStream_t st);
+hipError_t launch_tri_grad(const float* energy, const int* faces, int n_faces, const float* xy,
+                           int n_points, float gray, const float* real, const double dom[4],
+                           int pm, int pn, float* render, float* diff, float* xy_render_grad,
+                           float* tri_render_grad, float* inte_diff, float* xy_inte_grad,
+                           float* inte_result, hipStream_t st);
 hipError_t launch_scale(const BlockDesc* blocks, const double* weight, double scale, double* nu,
                         int nblk, hipStream_t st);
 hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const double* g, double beta,
@@ -1075,25 +1080,54 @@ int cot_cell_areas(cot_context* c, const double* w, double* area) {
   return COT_OK;
 }
 
-int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, const float* xy,
-                    int n_points, float total_gray, const double* proj_domain, int pm, int pn,
-                    float* image, void* stream) {
-  if (n_faces < 0 || n_points < 0 || pm <= 0 || pn <= 0 || !proj_domain || !image ||
+namespace {
+
+// Arguments common to the irradiance render entry points; every vertex index must name a point
+// before a kernel dereferences it (the faces are read back once to check).
+int check_mesh(const float* tri_energy, const int* faces, int n_faces, const float* xy,
+               int n_points, const double* proj_domain, int pm, int pn, hipStream_t st) {
+  if (n_faces < 0 || n_points < 0 || pm <= 0 || pn <= 0 || !proj_domain ||
       (n_faces > 0 && (!tri_energy || !faces || !xy)))
     return fail(COT_EINVAL, "bad argument");
   if (!(proj_domain[1] > proj_domain[0] && proj_domain[3] > proj_domain[2]))
     return fail(COT_EINVAL, "empty proj_domain");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (n_faces > 0) {
-    // every vertex index must name a point before a kernel dereferences it
     std::vector<int> fh(3 * (size_t)n_faces);
     HIPCHK(hipMemcpyAsync(fh.data(), faces, sizeof(int) * fh.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (int v : fh)
       if (v < 0 || v >= n_points) return fail(COT_EINVAL, "face index out of range");
   }
+  return COT_OK;
+}
+
+}  // namespace
+
+int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, const float* xy,
+                    int n_points, float total_gray, const double* proj_domain, int pm, int pn,
+                    float* image, void* stream) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!image) return fail(COT_EINVAL, "bad argument");
+  CHK(check_mesh(tri_energy, faces, n_faces, xy, n_points, proj_domain, pm, pn, st));
   HIPCHK(launch_tri_render(tri_energy, faces, n_faces, xy, total_gray, proj_domain, pm, pn, image,
                            st));
+  return COT_OK;
+}
+
+int cot_irra_render_grad(const float* tri_energy, const int* faces, int n_faces, const float* xy,
+                         int n_points, float total_gray, const float* real_picture,
+                         const double* proj_domain, int pm, int pn, float* render,
+                         float* render_diff, float* xy_render_grad, float* tri_render_grad,
+                         float* inte_diff, float* xy_inte_grad, float* inte_result, void* stream) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!real_picture || !render || !render_diff ||
+      (n_points > 0 && (!xy_render_grad || !xy_inte_grad)) ||
+      (n_faces > 0 && (!tri_render_grad || !inte_diff || !inte_result)))
+    return fail(COT_EINVAL, "bad argument");
+  CHK(check_mesh(tri_energy, faces, n_faces, xy, n_points, proj_domain, pm, pn, st));
+  HIPCHK(launch_tri_grad(tri_energy, faces, n_faces, xy, n_points, total_gray, real_picture,
+                         proj_domain, pm, pn, render, render_diff, xy_render_grad,
+                         tri_render_grad, inte_diff, xy_inte_grad, inte_result, st));
   return COT_OK;
 }
 

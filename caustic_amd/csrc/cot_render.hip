@@ -153,6 +153,149 @@ __global__ void tri_render_kernel(const float* energy, const int* faces, int n_f
   }
 }
 
+// diff = render - real, render and diff in the output layout (row pm - 1 - ix, column iy), real
+// in the grid layout iy * pm + ix that the extension's kernels read.
+__global__ void tri_diff_kernel(const float* render, const float* real, int pm, int pn,
+                                float* diff) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)pm * pn) return;
+  const int ix = pm - 1 - (int)(k / pn), iy = (int)(k % pn);
+  diff[k] = render[k] - real[(size_t)iy * pm + ix];
+}
+
+namespace {
+
+// Parameter range [t0, t1] of the segment a + t d (t in [0, 1]) inside the rectangle
+// [x0, x1] x [y0, y1] (Liang-Barsky); false when it misses.
+__device__ bool segment_in_rect(double ax, double ay, double dxs, double dys, double x0, double x1,
+                                double y0, double y1, double& t0, double& t1) {
+  t0 = 0.0;
+  t1 = 1.0;
+  const double p[4] = {-dxs, dxs, -dys, dys};
+  const double q[4] = {ax - x0, x1 - ax, ay - y0, y1 - ay};
+  for (int k = 0; k < 4; ++k) {
+    if (p[k] == 0.0) {
+      if (q[k] < 0.0) return false;
+      continue;
+    }
+    const double r = q[k] / p[k];
+    if (p[k] < 0.0) {
+      if (r > t1) return false;
+      t0 = fmax(t0, r);
+    } else {
+      if (r < t0) return false;
+      t1 = fmin(t1, r);
+    }
+  }
+  return t1 > t0;
+}
+
+}  // namespace
+
+// Gradients of the extension's render_grad (render_exe_program/render.cu:173-288, 422-542,
+// 593-728; sobel off) for the loss sum_p (render_p - real_p)^2 and the per-face terms
+// (inte_f - e_f)^2, inte_f = sum_p |T_f n p| real_p / |p|.  One face per lane.  The
+// derivative of a clipped area |T n p| with respect to vertex v_k is the transport of the
+// triangle's boundary: sum over the edges at v_k of N_e * integral over the part of the edge
+// inside p of the hat function of v_k (N_e the outward edge normal scaled by the edge length),
+// so each pixel needs the area of one clip and three segment clips; d|T|/dv_k follows with the
+// whole edges.  Vertex gradients are added with FP32 atomics (vertices are shared by faces).
+__global__ void tri_grad_kernel(const float* energy, const int* faces, int n_faces,
+                                const float* xy, float gray, const float* real, const float* diff,
+                                double x0, double y0, double dx, double dy, int pm, int pn,
+                                float* xy_render_grad, float* tri_render_grad, float* inte_diff,
+                                float* xy_inte_grad, float* inte_result) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  int vid[3];
+  double px[3], py[3];
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int k = 0; k < 3; ++k) {
+    vid[k] = faces[3 * f + k];
+    px[k] = xy[2 * vid[k]];
+    py[k] = xy[2 * vid[k] + 1];
+    mnx = fmin(mnx, px[k]); mxx = fmax(mxx, px[k]);
+    mny = fmin(mny, py[k]); mxy = fmax(mxy, py[k]);
+  }
+  const double e = (double)energy[f] * gray;
+  const double signed2 = (px[1] - px[0]) * (py[2] - py[0]) - (px[2] - px[0]) * (py[1] - py[0]);
+  const double A = 0.5 * fabs(signed2);
+  const double orient = signed2 >= 0.0 ? 1.0 : -1.0;
+  // outward edge normals times edge lengths, edge k from vertex k to vertex k + 1
+  double nx[3], ny[3];
+  for (int k = 0; k < 3; ++k) {
+    const int j = k == 2 ? 0 : k + 1;
+    nx[k] = orient * (py[j] - py[k]);
+    ny[k] = -orient * (px[j] - px[k]);
+  }
+  double dAx[3], dAy[3];
+  for (int k = 0; k < 3; ++k) {
+    const int km = k == 0 ? 2 : k - 1;  // the edge ending at vertex k
+    dAx[k] = 0.5 * (nx[k] + nx[km]);
+    dAy[k] = 0.5 * (ny[k] + ny[km]);
+  }
+  const int ix0 = max((int)floor((mnx - x0) / dx), 0);
+  const int iy0 = max((int)floor((mny - y0) / dy), 0);
+  const int ix1 = min((int)floor((mxx - x0) / dx) + 1, pm);
+  const int iy1 = min((int)floor((mxy - y0) / dy) + 1, pn);
+  const double cell = dx * dy;
+  double gr[6] = {0, 0, 0, 0, 0, 0}, gi[6] = {0, 0, 0, 0, 0, 0};
+  double tri_g = 0.0, inte = 0.0;
+  double ax[kClipMax], ay[kClipMax], bx[kClipMax], by[kClipMax];
+  for (int ix = ix0; ix < ix1; ++ix) {
+    const double cx0 = x0 + ix * dx, cx1 = x0 + (ix + 1) * dx;
+    for (int iy = iy0; iy < iy1; ++iy) {
+      const double cy0 = y0 + iy * dy, cy1 = y0 + (iy + 1) * dy;
+      int n = clip_side(px, py, 3, 0, cx0, cx1, cy0, cy1, ax, ay);
+      n = clip_side(ax, ay, n, 1, cx0, cx1, cy0, cy1, bx, by);
+      n = clip_side(bx, by, n, 2, cx0, cx1, cy0, cy1, ax, ay);
+      n = clip_side(ax, ay, n, 3, cx0, cx1, cy0, cy1, bx, by);
+      const double a = n >= 3 ? shoelace(bx, by, n) : 0.0;
+      // d a / d v_k from the triangle's edges inside the pixel
+      double dax[3] = {0, 0, 0}, day[3] = {0, 0, 0};
+      bool any = false;
+      for (int k = 0; k < 3; ++k) {
+        const int j = k == 2 ? 0 : k + 1;
+        double t0, t1;
+        if (!segment_in_rect(px[k], py[k], px[j] - px[k], py[j] - py[k], cx0, cx1, cy0, cy1, t0,
+                             t1))
+          continue;
+        const double i0 = t1 - t0, i1 = 0.5 * (t1 * t1 - t0 * t0);
+        dax[k] += (i0 - i1) * nx[k];
+        day[k] += (i0 - i1) * ny[k];
+        dax[j] += i1 * nx[k];
+        day[j] += i1 * ny[k];
+        any = true;
+      }
+      if (!(a > 0.0) && !any) continue;
+      const double rp = (double)real[(size_t)iy * pm + ix] / cell;
+      inte += a * rp;
+      for (int k = 0; k < 3; ++k) {
+        gi[2 * k] += 2.0 * rp * dax[k];
+        gi[2 * k + 1] += 2.0 * rp * day[k];
+      }
+      if (A > 0.0) {
+        const double d2 = 2.0 * (double)diff[(size_t)(pm - 1 - ix) * pn + iy];
+        tri_g += d2 * a / A;
+        const double s = d2 * e / (A * A);
+        for (int k = 0; k < 3; ++k) {
+          gr[2 * k] += s * (dax[k] * A - a * dAx[k]);
+          gr[2 * k + 1] += s * (day[k] * A - a * dAy[k]);
+        }
+      }
+    }
+  }
+  tri_render_grad[f] = (float)(tri_g * gray);
+  inte_result[f] = (float)inte;
+  inte_diff[f] = (float)(inte - e);
+  for (int k = 0; k < 3; ++k) {
+    atomicAdd(&xy_render_grad[2 * vid[k]], (float)gr[2 * k]);
+    atomicAdd(&xy_render_grad[2 * vid[k] + 1], (float)gr[2 * k + 1]);
+    atomicAdd(&xy_inte_grad[2 * vid[k]], (float)((inte - e) * gi[2 * k]));
+    atomicAdd(&xy_inte_grad[2 * vid[k] + 1], (float)((inte - e) * gi[2 * k + 1]));
+  }
+}
+
 __global__ void fill_f32_kernel(float* p, size_t n) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0.0f;
@@ -168,6 +311,30 @@ hipError_t launch_tri_render(const float* energy, const int* faces, int n_faces,
     hipLaunchKernelGGL(tri_render_kernel, dim3((n_faces + 255) / 256), dim3(256), 0, st, energy,
                        faces, n_faces, xy, gray, dom[0], dom[2], (dom[1] - dom[0]) / pm,
                        (dom[3] - dom[2]) / pn, pm, pn, image);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_grad(const float* energy, const int* faces, int n_faces, const float* xy,
+                           int n_points, float gray, const float* real, const double dom[4],
+                           int pm, int pn, float* render, float* diff, float* xy_render_grad,
+                           float* tri_render_grad, float* inte_diff, float* xy_inte_grad,
+                           float* inte_result, hipStream_t st) {
+  const hipError_t e0 = launch_tri_render(energy, faces, n_faces, xy, gray, dom, pm, pn, render, st);
+  if (e0 != hipSuccess) return e0;
+  const size_t px = (size_t)pm * pn, nv = 2 * (size_t)n_points;
+  hipLaunchKernelGGL(tri_diff_kernel, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, render,
+                     real, pm, pn, diff);
+  if (nv > 0) {
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st,
+                       xy_render_grad, nv);
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st,
+                       xy_inte_grad, nv);
+  }
+  if (n_faces > 0)
+    hipLaunchKernelGGL(tri_grad_kernel, dim3((n_faces + 255) / 256), dim3(256), 0, st, energy,
+                       faces, n_faces, xy, gray, real, diff, dom[0], dom[2],
+                       (dom[1] - dom[0]) / pm, (dom[3] - dom[2]) / pn, pm, pn, xy_render_grad,
+                       tri_render_grad, inte_diff, xy_inte_grad, inte_result);
   return hipGetLastError();
 }
 

@@ -19,6 +19,8 @@ Pieces and what they follow:
   * ``solid_angle`` / ``point_light_source`` -- the point-light source masses (compute_solid_angle,
     solid_angle, len_solid_angle: basic_function.cuh:558-588, 635-658; ot_op.cuh:160-206) and the
     light's directions at the points (main.cu:218-231).
+  * ``irra_face_areas`` -- the pixel pieces of a triangle mesh, for the irra_render extension's
+    image and gradients (render_exe_program/render.cu:93-288, 422-542).
   * ``read_pic_txt`` / ``format_weight`` -- text formats (basic_function.cuh:145-204,
     main.cu:295-299).
 """
@@ -201,6 +203,23 @@ def render(cells, nu, rho_shape, proj):
     lib().orc_render(start.shape[0], _d(_f64(nu)), _i(start), _i(length), _d(verts), W, H,
                      _d(_f64(proj)), _d(img))
     return img.reshape(H, W)
+
+
+def irra_face_areas(points, faces, pm, pn, proj):
+    """|T_f n p| for every face f of a triangle mesh lying inside proj and every pixel p of the
+    pm x pn grid: [F, pn, pm] (pixel (ix, iy) at [f, iy, ix]), the pixel pieces of the
+    irra_render extension's kernels (render_exe_program/render.cu:93-171, 422-542) by the same
+    Sutherland-Hodgman clipping as ``render``."""
+    pts = _f64(points)
+    out = np.zeros((len(faces), pn, pm))
+    one = np.zeros(1, dtype=np.int32)
+    three = np.full(1, 3, dtype=np.int32)
+    for f, tri in enumerate(faces):
+        v = np.ascontiguousarray(pts[np.asarray(tri)].reshape(-1))
+        a = 0.5 * abs((v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1]))
+        if a > 0.0:
+            out[f] = render((one, three, v), np.array([a]), (pn, pm), proj)
+    return out
 
 
 def source_masses(points, sample_domain):

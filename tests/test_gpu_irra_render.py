@@ -71,3 +71,72 @@ def test_render_triangle_partly_outside_and_bad_index(gpu_available):
     with pytest.raises(L.CausticLibError):
         irra_render.render(e, torch.tensor([[0, 1, 3]], dtype=torch.int32, device=dev), pts,
                            *args)
+
+
+def _losses(pts, tri, e, gray, real, pm, pn, proj):
+    """The oracle's render (grid layout [iy, ix]), sum_p (render - real)^2 and
+    sum_f (inte_f - e_f gray)^2 for the mesh."""
+    a = O.irra_face_areas(pts, tri, pm, pn, proj)
+    u, v = pts[tri[:, 1]] - pts[tri[:, 0]], pts[tri[:, 2]] - pts[tri[:, 0]]
+    A = 0.5 * np.abs(u[:, 0] * v[:, 1] - u[:, 1] * v[:, 0])
+    img = np.tensordot(e * gray / A, a, axes=1)
+    cell = (proj[1] - proj[0]) * (proj[3] - proj[2]) / (pm * pn)
+    inte = np.tensordot(a, real, axes=([1, 2], [0, 1])) / cell
+    return img, float(np.sum((img - real) ** 2)), inte, float(np.sum((inte - e * gray) ** 2))
+
+
+def test_render_grad_matches_finite_differences(gpu_available):
+    """render_grad (sobel off) on a small mesh inside the domain: image and image - target in the
+    output layout, the per-face integrals of the target, and the derivatives of both losses by
+    central differences of the FP64 oracle (the clipped areas are piecewise quadratic in the
+    vertices, so the differences are exact away from the kinks; FP32 outputs: 2e-3 of the
+    largest component)."""
+    import torch
+
+    from caustic_amd import irra_render
+
+    print("[case] irra_render render_grad, 24-point mesh 12x10", flush=True)
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0.05, 0.95, (24, 2)).astype(np.float32).astype(np.float64)
+    tri = Delaunay(pts).simplices.astype(np.int32)
+    e = rng.uniform(0.5, 1.5, tri.shape[0]).astype(np.float32).astype(np.float64)
+    gray, pm, pn = 0.75, 12, 10
+    proj = (0.0, 1.0, 0.0, 1.0)
+    real = rng.uniform(0.0, 3.0, (pn, pm)).astype(np.float32).astype(np.float64)  # [iy, ix]
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float32: torch.tensor(a, dtype=dt, device=dev)  # noqa: E731
+    out = irra_render.render_grad(t(e), t(tri, torch.int32), t(pts), t([gray]),
+                                  t(real.reshape(-1)), t([False], torch.bool), t(proj),
+                                  t([pm, pn]))
+    out = [o.cpu().numpy().astype(np.float64) for o in out]
+    img, L0, inte, J0 = _losses(pts, tri, e, gray, real, pm, pn, proj)
+    np.testing.assert_allclose(out[0], img.T[::-1], rtol=0, atol=2e-6 * img.max())
+    np.testing.assert_allclose(out[1], (img - real).T[::-1], rtol=0, atol=2e-6 * img.max())
+    np.testing.assert_allclose(out[6], inte, rtol=1e-5)
+    np.testing.assert_allclose(out[4], inte - e * gray, rtol=0, atol=1e-5 * np.abs(inte).max())
+    h = 1e-6
+    gL, gJ = np.zeros_like(pts), np.zeros_like(pts)
+    for i in range(pts.shape[0]):
+        for c in range(2):
+            p1, p2 = pts.copy(), pts.copy()
+            p1[i, c] += h
+            p2[i, c] -= h
+            _, l1, _, j1 = _losses(p1, tri, e, gray, real, pm, pn, proj)
+            _, l2, _, j2 = _losses(p2, tri, e, gray, real, pm, pn, proj)
+            gL[i, c], gJ[i, c] = (l1 - l2) / (2 * h), (j1 - j2) / (2 * h)
+    np.testing.assert_allclose(out[2], gL, rtol=0, atol=2e-3 * np.abs(gL).max())
+    np.testing.assert_allclose(out[5], gJ, rtol=0, atol=2e-3 * np.abs(gJ).max())
+    gE = np.zeros_like(e)
+    for f in range(e.shape[0]):  # the image loss is quadratic in the energies
+        e1, e2 = e.copy(), e.copy()
+        e1[f] += 1e-4
+        e2[f] -= 1e-4
+        gE[f] = (_losses(pts, tri, e1, gray, real, pm, pn, proj)[1]
+                 - _losses(pts, tri, e2, gray, real, pm, pn, proj)[1]) / 2e-4
+    np.testing.assert_allclose(out[3], gE, rtol=0, atol=2e-4 * np.abs(gE).max())
+    assert not out[7].any() and not out[8].any()
+    with pytest.raises(NotImplementedError):
+        irra_render.render_grad(t(e), t(tri, torch.int32), t(pts), t([gray]),
+                                t(real.reshape(-1)), t([True], torch.bool), t(proj), t([pm, pn]))
