@@ -93,7 +93,7 @@ SIGNATURES = {
     "cot_irra_render_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                                             ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
-                             + [ctypes.c_void_p] * 8),
+                             + [ctypes.c_void_p] * 7 + [ctypes.c_int] + [ctypes.c_void_p] * 3),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }

@@ -85,7 +85,8 @@ hipError_t launch_tri_grad(const float* energy, const int* faces, int n_faces, c
                            int n_points, float gray, const float* real, const double dom[4],
                            int pm, int pn, float* render, float* diff, float* xy_render_grad,
                            float* tri_render_grad, float* inte_diff, float* xy_inte_grad,
-                           float* inte_result, hipStream_t st);
+                           float* inte_result, float* sob, float* xy_sobel_grad,
+                           float* tri_sobel_grad, hipStream_t st);
 hipError_t launch_scale(const BlockDesc* blocks, const double* weight, double scale, double* nu,
                         int nblk, hipStream_t st);
 hipError_t launch_aa_store(const BlockDesc* blocks, const double* w, const double* g, double beta,
@@ -1118,16 +1119,23 @@ int cot_irra_render_grad(const float* tri_energy, const int* faces, int n_faces,
                          int n_points, float total_gray, const float* real_picture,
                          const double* proj_domain, int pm, int pn, float* render,
                          float* render_diff, float* xy_render_grad, float* tri_render_grad,
-                         float* inte_diff, float* xy_inte_grad, float* inte_result, void* stream) {
+                         float* inte_diff, float* xy_inte_grad, float* inte_result,
+                         int use_sobel, float* xy_sobel_grad, float* tri_sobel_grad,
+                         void* stream) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (!real_picture || !render || !render_diff ||
-      (n_points > 0 && (!xy_render_grad || !xy_inte_grad)) ||
-      (n_faces > 0 && (!tri_render_grad || !inte_diff || !inte_result)))
+      (n_points > 0 && (!xy_render_grad || !xy_inte_grad || (use_sobel && !xy_sobel_grad))) ||
+      (n_faces > 0 && (!tri_render_grad || !inte_diff || !inte_result ||
+                       (use_sobel && !tri_sobel_grad))))
     return fail(COT_EINVAL, "bad argument");
   CHK(check_mesh(tri_energy, faces, n_faces, xy, n_points, proj_domain, pm, pn, st));
+  float* sob = nullptr;  // the Sobel weight image, held for this call
+  CallBuffers buf(st);
+  if (use_sobel) CHK(buf.get(&sob, (size_t)pm * pn));
   HIPCHK(launch_tri_grad(tri_energy, faces, n_faces, xy, n_points, total_gray, real_picture,
                          proj_domain, pm, pn, render, render_diff, xy_render_grad,
-                         tri_render_grad, inte_diff, xy_inte_grad, inte_result, st));
+                         tri_render_grad, inte_diff, xy_inte_grad, inte_result, sob,
+                         xy_sobel_grad, tri_sobel_grad, st));
   return COT_OK;
 }
 

@@ -163,6 +163,23 @@ __global__ void tri_diff_kernel(const float* render, const float* real, int pm, 
   diff[k] = render[k] - real[(size_t)iy * pm + ix];
 }
 
+// Sobel weights of compute_sobel_weight (render_exe_program/render.cu:63-91), output layout:
+// w_p = sum over the 4-neighbours q inside the image of (diff_p - diff_q), the graph Laplacian
+// of the difference image (so that sum_p 2 w_p d render_p is the derivative of
+// sum over neighbouring pairs of (diff_p - diff_q)^2).
+__global__ void tri_sobel_kernel(const float* diff, int pm, int pn, float* w) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)pm * pn) return;
+  const int r = (int)(k / pn), c = (int)(k % pn);
+  const float d = diff[k];
+  float s = 0.0f;
+  if (r > 0) s += d - diff[k - pn];
+  if (r + 1 < pm) s += d - diff[k + pn];
+  if (c > 0) s += d - diff[k - 1];
+  if (c + 1 < pn) s += d - diff[k + 1];
+  w[k] = s;
+}
+
 namespace {
 
 // Parameter range [t0, t1] of the segment a + t d (t in [0, 1]) inside the rectangle
@@ -204,7 +221,8 @@ __global__ void tri_grad_kernel(const float* energy, const int* faces, int n_fac
                                 const float* xy, float gray, const float* real, const float* diff,
                                 double x0, double y0, double dx, double dy, int pm, int pn,
                                 float* xy_render_grad, float* tri_render_grad, float* inte_diff,
-                                float* xy_inte_grad, float* inte_result) {
+                                float* xy_inte_grad, float* inte_result, const float* sob,
+                                float* xy_sobel_grad, float* tri_sobel_grad) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n_faces) return;
   int vid[3];
@@ -239,8 +257,8 @@ __global__ void tri_grad_kernel(const float* energy, const int* faces, int n_fac
   const int ix1 = min((int)floor((mxx - x0) / dx) + 1, pm);
   const int iy1 = min((int)floor((mxy - y0) / dy) + 1, pn);
   const double cell = dx * dy;
-  double gr[6] = {0, 0, 0, 0, 0, 0}, gi[6] = {0, 0, 0, 0, 0, 0};
-  double tri_g = 0.0, inte = 0.0;
+  double gr[6] = {0, 0, 0, 0, 0, 0}, gi[6] = {0, 0, 0, 0, 0, 0}, gs[6] = {0, 0, 0, 0, 0, 0};
+  double tri_g = 0.0, tri_s = 0.0, inte = 0.0;
   double ax[kClipMax], ay[kClipMax], bx[kClipMax], by[kClipMax];
   for (int ix = ix0; ix < ix1; ++ix) {
     const double cx0 = x0 + ix * dx, cx1 = x0 + (ix + 1) * dx;
@@ -275,17 +293,34 @@ __global__ void tri_grad_kernel(const float* energy, const int* faces, int n_fac
         gi[2 * k + 1] += 2.0 * rp * day[k];
       }
       if (A > 0.0) {
-        const double d2 = 2.0 * (double)diff[(size_t)(pm - 1 - ix) * pn + iy];
+        const size_t q = (size_t)(pm - 1 - ix) * pn + iy;
+        const double d2 = 2.0 * (double)diff[q];
         tri_g += d2 * a / A;
         const double s = d2 * e / (A * A);
         for (int k = 0; k < 3; ++k) {
           gr[2 * k] += s * (dax[k] * A - a * dAx[k]);
           gr[2 * k + 1] += s * (day[k] * A - a * dAy[k]);
         }
+        if (sob) {
+          const double w2 = 2.0 * (double)sob[q];
+          tri_s += w2 * a / A;
+          const double sw = w2 * e / (A * A);
+          for (int k = 0; k < 3; ++k) {
+            gs[2 * k] += sw * (dax[k] * A - a * dAx[k]);
+            gs[2 * k + 1] += sw * (day[k] * A - a * dAy[k]);
+          }
+        }
       }
     }
   }
   tri_render_grad[f] = (float)(tri_g * gray);
+  if (sob) {
+    tri_sobel_grad[f] = (float)(tri_s * gray);
+    for (int k = 0; k < 3; ++k) {
+      atomicAdd(&xy_sobel_grad[2 * vid[k]], (float)gs[2 * k]);
+      atomicAdd(&xy_sobel_grad[2 * vid[k] + 1], (float)gs[2 * k + 1]);
+    }
+  }
   inte_result[f] = (float)inte;
   inte_diff[f] = (float)(inte - e);
   for (int k = 0; k < 3; ++k) {
@@ -318,23 +353,31 @@ hipError_t launch_tri_grad(const float* energy, const int* faces, int n_faces, c
                            int n_points, float gray, const float* real, const double dom[4],
                            int pm, int pn, float* render, float* diff, float* xy_render_grad,
                            float* tri_render_grad, float* inte_diff, float* xy_inte_grad,
-                           float* inte_result, hipStream_t st) {
+                           float* inte_result, float* sob, float* xy_sobel_grad,
+                           float* tri_sobel_grad, hipStream_t st) {
   const hipError_t e0 = launch_tri_render(energy, faces, n_faces, xy, gray, dom, pm, pn, render, st);
   if (e0 != hipSuccess) return e0;
   const size_t px = (size_t)pm * pn, nv = 2 * (size_t)n_points;
   hipLaunchKernelGGL(tri_diff_kernel, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, render,
                      real, pm, pn, diff);
+  if (sob)
+    hipLaunchKernelGGL(tri_sobel_kernel, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st,
+                       diff, pm, pn, sob);
   if (nv > 0) {
     hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st,
                        xy_render_grad, nv);
     hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st,
                        xy_inte_grad, nv);
+    if (sob)
+      hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st,
+                         xy_sobel_grad, nv);
   }
   if (n_faces > 0)
     hipLaunchKernelGGL(tri_grad_kernel, dim3((n_faces + 255) / 256), dim3(256), 0, st, energy,
                        faces, n_faces, xy, gray, real, diff, dom[0], dom[2],
                        (dom[1] - dom[0]) / pm, (dom[3] - dom[2]) / pn, pm, pn, xy_render_grad,
-                       tri_render_grad, inte_diff, xy_inte_grad, inte_result);
+                       tri_render_grad, inte_diff, xy_inte_grad, inte_result, sob, xy_sobel_grad,
+                       tri_sobel_grad);
   return hipGetLastError();
 }
 

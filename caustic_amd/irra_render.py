@@ -9,9 +9,8 @@ pieces are exact polygon clips (cot_irra_render in libcaustic_ot.so).
 
 ``render_grad(compute_tri_area, faces, proj_xy, total_gray_value, real_picture, if_use_sobel,
 proj_domain, pic_size)`` mirrors ``irra_render.render_grad`` (render.cu:593-728; caller
-utils/optimization.py:281) with its nine outputs (cot_irra_render_grad); the Sobel-weighted
-branch is not built and ``if_use_sobel`` true raises, so its two outputs are returned as zeros
-only for the false case.
+utils/optimization.py:281) with its nine outputs (cot_irra_render_grad); the two Sobel
+outputs are zeros when ``if_use_sobel`` is false, as in the reference.
 """
 from __future__ import annotations
 
@@ -51,8 +50,7 @@ def render_grad(compute_tri_area, faces, proj_xy, total_gray_value, real_picture
     reference returns them (render.cu:716-727)."""
     import torch
 
-    if bool(torch.as_tensor(if_use_sobel).reshape(-1)[0].item()):
-        raise NotImplementedError("the Sobel-weighted gradient is not built")
+    sobel = bool(torch.as_tensor(if_use_sobel).reshape(-1)[0].item())
     dev, tri, fc, xy, gray, dom, H, W = _mesh(compute_tri_area, faces, proj_xy,
                                               total_gray_value, proj_domain, pic_size)
     real = real_picture.detach().reshape(-1).to(device=dev, dtype=torch.float32).contiguous()
@@ -62,16 +60,17 @@ def render_grad(compute_tri_area, faces, proj_xy, total_gray_value, real_picture
     f32 = dict(dtype=torch.float32, device=dev)
     out = [torch.empty((H, W), **f32), torch.empty((H, W), **f32), torch.empty((P, 2), **f32),
            torch.empty(F, **f32), torch.empty(F, **f32), torch.empty((P, 2), **f32),
-           torch.empty(F, **f32)]
+           torch.empty(F, **f32), torch.zeros((P, 2), **f32), torch.zeros(F, **f32)]
     lib = L.load()
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream(dev).cuda_stream
         rc = lib.cot_irra_render_grad(
             _p(tri), _p(fc), int(F), _p(xy), int(P), gray, _p(real),
             dom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), H, W, _p(out[0]), _p(out[1]),
-            _p(out[2]), _p(out[3]), _p(out[4]), _p(out[5]), _p(out[6]), ctypes.c_void_p(stream))
+            _p(out[2]), _p(out[3]), _p(out[4]), _p(out[5]), _p(out[6]), int(sobel), _p(out[7]),
+            _p(out[8]), ctypes.c_void_p(stream))
     L.check(rc)
-    return out + [torch.zeros((P, 2), **f32), torch.zeros(F, **f32)]
+    return out
 
 
 def render(compute_tri_area, faces, proj_xy, total_gray_value, proj_domain, pic_size):

@@ -17,8 +17,8 @@
  *   cot_cell_areas             compute_clip_area, the source-mass step (basic_function.cuh:610-633)
  *   cot_irra_render            render() of the irra_render extension, the irradiance image of a
  *                              triangle mesh (render_exe_program/render.cu:93-171, 553-591)
- *   cot_irra_render_grad       render_grad() of the same extension without the Sobel branch
- *                              (render_exe_program/render.cu:173-288, 422-542, 593-728)
+ *   cot_irra_render_grad       render_grad() of the same extension
+ *                              (render_exe_program/render.cu:173-420, 422-542, 593-728)
  *   cot_cell_solid_angles      solid_angle + compute_solid_angle, the point-light source-mass
  *                              step (basic_function.cuh:558-588, 635-651; ot_op.cuh:160-184)
  */
@@ -213,9 +213,10 @@ int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, cons
                     int n_points, float total_gray, const double* proj_domain, int pm, int pn,
                     float* image, void* stream);
 
-/* The gradient pass of the irra_render extension, render_grad with if_use_sobel false
- * (render_exe_program/render.cu:593-728 with compute_render_diff, computer_grad_inte and
- * compute_grad_render, :53-61, 422-542, 173-288).  With the mesh as in cot_irra_render and
+/* The gradient pass of the irra_render extension, render_grad (render_exe_program/
+ * render.cu:593-728 with compute_render_diff, computer_grad_inte, compute_grad_render and,
+ * for use_sobel, compute_sobel_weight and compute_grad_render_sobel, :53-91, 422-542, 173-420).
+ * With the mesh as in cot_irra_render and
  * real_picture the target in the grid layout the extension's kernels read (pixel (ix, iy) at
  * iy * pm + ix):
  *   render, render_diff   pm x pn images in render()'s output layout: the mesh's image and
@@ -224,16 +225,23 @@ int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, cons
  *                         sum_p (render_p - real_p)^2;
  *   inte_result, inte_diff (n_faces): inte_f = sum_p |T_f n p| real_p / |p| and
  *                         inte_f - tri_energy_f * total_gray;
- *   xy_inte_grad (2 n_points): d/dxy of sum_f inte_diff_f^2.
+ *   xy_inte_grad (2 n_points): d/dxy of sum_f inte_diff_f^2;
+ *   with use_sobel, xy_sobel_grad (2 n_points) and tri_sobel_grad (n_faces): d/dxy and
+ *                         d/dtri_energy of the sum over 4-neighbouring pixel pairs of
+ *                         (diff_p - diff_q)^2, diff = render - real (unused otherwise).
  * The vertex derivatives are exact where the clipped areas are differentiable (each is the
  * boundary transport of the triangle's edges through the pixel).  Device pointers; work
- * ordered on stream; COT_EINVAL for a face index that names no point.  The Sobel-weighted
- * variant (compute_grad_render_sobel, :290-420) is not provided. */
+ * ordered on stream; COT_EINVAL for a face index that names no point.  A Sobel call holds a
+ * pm x pn float image of device memory for its duration.  The reference's Sobel weights read
+ * past the image when pm > pn (row bound pm - 1, :77); here a neighbour outside the image is
+ * absent, which is the reference's result whenever pm == pn. */
 int cot_irra_render_grad(const float* tri_energy, const int* faces, int n_faces, const float* xy,
                          int n_points, float total_gray, const float* real_picture,
                          const double* proj_domain, int pm, int pn, float* render,
                          float* render_diff, float* xy_render_grad, float* tri_render_grad,
-                         float* inte_diff, float* xy_inte_grad, float* inte_result, void* stream);
+                         float* inte_diff, float* xy_inte_grad, float* inte_result,
+                         int use_sobel, float* xy_sobel_grad, float* tri_sobel_grad,
+                         void* stream);
 
 /* Solid angle under which a point light at light[0..2] sees each cell (inside proj_domain, at
  * weights w) placed in the plane z (solid_angle, basic_function.cuh:635-651, with

@@ -86,7 +86,7 @@ def _losses(pts, tri, e, gray, real, pm, pn, proj):
 
 
 def test_render_grad_matches_finite_differences(gpu_available):
-    """render_grad (sobel off) on a small mesh inside the domain: image and image - target in the
+    """render_grad on a small mesh inside the domain, Sobel branch off and on: image and image - target in the
     output layout, the per-face integrals of the target, and the derivatives of both losses by
     central differences of the FP64 oracle (the clipped areas are piecewise quadratic in the
     vertices, so the differences are exact away from the kinks; FP32 outputs: 2e-3 of the
@@ -137,6 +137,31 @@ def test_render_grad_matches_finite_differences(gpu_available):
                  - _losses(pts, tri, e2, gray, real, pm, pn, proj)[1]) / 2e-4
     np.testing.assert_allclose(out[3], gE, rtol=0, atol=2e-4 * np.abs(gE).max())
     assert not out[7].any() and not out[8].any()
-    with pytest.raises(NotImplementedError):
-        irra_render.render_grad(t(e), t(tri, torch.int32), t(pts), t([gray]),
-                                t(real.reshape(-1)), t([True], torch.bool), t(proj), t([pm, pn]))
+
+    # the Sobel branch: the same outputs plus the derivatives of the smoothness loss
+    # sum over 4-neighbouring pixel pairs of (diff_p - diff_q)^2
+    def smooth(p_, e_):
+        d = _losses(p_, tri, e_, gray, real, pm, pn, proj)[0] - real
+        return float(np.sum((d[:, 1:] - d[:, :-1]) ** 2) + np.sum((d[1:, :] - d[:-1, :]) ** 2))
+
+    sob = irra_render.render_grad(t(e), t(tri, torch.int32), t(pts), t([gray]),
+                                  t(real.reshape(-1)), t([True], torch.bool), t(proj),
+                                  t([pm, pn]))
+    sob = [o.cpu().numpy().astype(np.float64) for o in sob]
+    for k in range(7):
+        np.testing.assert_allclose(sob[k], out[k], rtol=1e-6, atol=1e-6 * np.abs(out[k]).max())
+    gS = np.zeros_like(pts)
+    for i in range(pts.shape[0]):
+        for c in range(2):
+            p1, p2 = pts.copy(), pts.copy()
+            p1[i, c] += h
+            p2[i, c] -= h
+            gS[i, c] = (smooth(p1, e) - smooth(p2, e)) / (2 * h)
+    np.testing.assert_allclose(sob[7], gS, rtol=0, atol=2e-3 * np.abs(gS).max())
+    gSE = np.zeros_like(e)
+    for f in range(e.shape[0]):
+        e1, e2 = e.copy(), e.copy()
+        e1[f] += 1e-4
+        e2[f] -= 1e-4
+        gSE[f] = (smooth(pts, e1) - smooth(pts, e2)) / 2e-4
+    np.testing.assert_allclose(sob[8], gSE, rtol=0, atol=2e-4 * np.abs(gSE).max())
