@@ -90,6 +90,9 @@ SIGNATURES = {
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, _dp,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "cot_splat": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, _dp, ctypes.c_float, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
     "cot_irra_render_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                                             ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]

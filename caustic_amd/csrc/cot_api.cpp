@@ -78,6 +78,9 @@ hipError_t launch_fresnel_weight(const BlockDesc* blocks, const double2* points,
 hipError_t launch_solid_angle(const BlockDesc* blocks, const double2* vtx, const uint8_t* nedge,
                               const double light[3], double z, double* omega, int nblk,
                               hipStream_t st);
+hipError_t launch_splat(const float* pts, const float* energy, int n, int ry, int rx,
+                        const double dom[4], float sigma, int r, const float* G, float c,
+                        float* out, hipStream_t st);
 hipError_t launch_tri_render(const float* energy, const int* faces, int n_faces, const float* xy,
                              float gray, const double dom[4], int pm, int pn, float* image,
                              hipStream_t st);
@@ -1103,6 +1106,19 @@ int check_mesh(const float* tri_energy, const int* faces, int n_faces, const flo
 }
 
 }  // namespace
+
+int cot_splat(const float* points, const float* energy, int n, int res_y, int res_x,
+              const double* proj_domain, float sigma, int radius, const float* image_grad,
+              float scale, float* out, void* stream) {
+  if (n < 0 || res_y <= 0 || res_x <= 0 || radius < 0 || radius > 16 || !(sigma > 0.0f) ||
+      !proj_domain || !out || (n > 0 && (!points || !energy)))
+    return fail(COT_EINVAL, "bad argument");
+  if (!(proj_domain[1] > proj_domain[0] && proj_domain[3] > proj_domain[2]))
+    return fail(COT_EINVAL, "empty proj_domain");
+  HIPCHK(launch_splat(points, energy, n, res_y, res_x, proj_domain, sigma, radius, image_grad,
+                      scale, out, reinterpret_cast<hipStream_t>(stream)));
+  return COT_OK;
+}
 
 int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, const float* xy,
                     int n_points, float total_gray, const double* proj_domain, int pm, int pn,

@@ -19,6 +19,8 @@
  *                              triangle mesh (render_exe_program/render.cu:93-171, 553-591)
  *   cot_irra_render_grad       render_grad() of the same extension
  *                              (render_exe_program/render.cu:173-420, 422-542, 593-728)
+ *   cot_splat                  _gaussian_splat_raw_chunk and the point gradient of
+ *                              render_gaussian_splat_two_pass_backward (utils/basic_function.py:674-772)
  *   cot_cell_solid_angles      solid_angle + compute_solid_angle, the point-light source-mass
  *                              step (basic_function.cuh:558-588, 635-651; ot_op.cuh:160-184)
  */
@@ -242,6 +244,19 @@ int cot_irra_render_grad(const float* tri_energy, const int* faces, int n_faces,
                          float* inte_diff, float* xy_inte_grad, float* inte_result,
                          int use_sobel, float* xy_sobel_grad, float* tri_sobel_grad,
                          void* stream);
+
+/* Gaussian splatting of n ray hits (points, (x, y) pairs inside proj_domain, finite) onto a
+ * res_y x res_x image (pixel (x, y) at y * res_x + x): _gaussian_splat_raw_chunk,
+ * utils/basic_function.py:674-709.  Each point spreads energy over the (2 radius + 1)^2 pixels
+ * around the one it falls in, weights exp(-d^2 / (2 sigma^2)) (d in pixels from the pixel
+ * centre) normalised over the whole window, pixels outside the image dropped.
+ * image_grad NULL: out (the image) += the splat.  image_grad given: out (2n) = d/d(points) of
+ * sum_p scale * splat_p * image_grad_p with the normalisation held constant (the second pass of
+ * render_gaussian_splat_two_pass_backward, :711-772).  Device pointers; work ordered on stream;
+ * radius at most 16. */
+int cot_splat(const float* points, const float* energy, int n, int res_y, int res_x,
+              const double* proj_domain, float sigma, int radius, const float* image_grad,
+              float scale, float* out, void* stream);
 
 /* Solid angle under which a point light at light[0..2] sees each cell (inside proj_domain, at
  * weights w) placed in the plane z (solid_angle, basic_function.cuh:635-651, with

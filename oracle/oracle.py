@@ -21,6 +21,7 @@ Pieces and what they follow:
     light's directions at the points (main.cu:218-231).
   * ``irra_face_areas`` -- the pixel pieces of a triangle mesh, for the irra_render extension's
     image and gradients (render_exe_program/render.cu:93-288, 422-542).
+  * ``gaussian_splat`` -- the receiver splat of utils/basic_function.py:674-709.
   * ``read_pic_txt`` / ``format_weight`` -- text formats (basic_function.cuh:145-204,
     main.cu:295-299).
 """
@@ -220,6 +221,44 @@ def irra_face_areas(points, faces, pm, pn, proj):
         if a > 0.0:
             out[f] = render((one, three, v), np.array([a]), (pn, pm), proj)
     return out
+
+
+def gaussian_splat(points, energy, res_y, res_x, proj, sigma=0.5, radius=2, window_sums=None):
+    """_gaussian_splat_raw_chunk (utils/basic_function.py:674-709) in float64 numpy: each
+    point's (2r+1)^2 window around its pixel, Gaussian weights normalised over the window,
+    pixels outside the image dropped; image [res_y, res_x].  window_sums: the normalising sums
+    to use instead (the reference detaches them, :699, so its gradient holds them fixed)."""
+    x0, x1, y0, y1 = proj
+    sx, sy = (x1 - x0) / res_x, (y1 - y0) / res_y
+    img = np.zeros((res_y, res_x))
+    off = np.arange(-radius, radius + 1)
+    for k, ((x, y), e) in enumerate(zip(_f64(points), _f64(energy))):
+        px, py = (x - x0) / sx, (y - y0) / sy
+        cx, cy = int(np.floor(px)), int(np.floor(py))
+        xs, ys = cx + off, cy + off
+        w = np.exp(-((xs[None, :] + 0.5 - px) ** 2 + (ys[:, None] + 0.5 - py) ** 2)
+                   / (2 * sigma * sigma))
+        w *= e / max(w.sum() if window_sums is None else window_sums[k], 1e-12)
+        for a, yy in enumerate(ys):
+            if 0 <= yy < res_y:
+                for b, xx in enumerate(xs):
+                    if 0 <= xx < res_x:
+                        img[yy, xx] += w[a, b]
+    return img
+
+
+def splat_window_sums(points, res_y, res_x, proj, sigma=0.5, radius=2):
+    """The normalising sum of each point's window weights (gaussian_splat)."""
+    x0, x1, y0, y1 = proj
+    sx, sy = (x1 - x0) / res_x, (y1 - y0) / res_y
+    off = np.arange(-radius, radius + 1)
+    out = []
+    for x, y in _f64(points):
+        px, py = (x - x0) / sx, (y - y0) / sy
+        cx, cy = np.floor(px), np.floor(py)
+        out.append(np.exp(-((cx + off[None, :] + 0.5 - px) ** 2
+                            + (cy + off[:, None] + 0.5 - py) ** 2) / (2 * sigma * sigma)).sum())
+    return np.array(out)
 
 
 def source_masses(points, sample_domain):

@@ -335,3 +335,15 @@ def test_point_light_cells_add_up_and_tend_to_areas():
     assert np.abs(nu - par).max() > 0.05 * par.max()  # a near light reweights the cells
     far, _, _ = O.point_light_source(pts, sd, (1.0, 0.75, -1e5), 0.0)
     np.testing.assert_allclose(far, par, rtol=1e-8)
+
+
+def test_gaussian_splat_conserves_interior_energy():
+    """A point whose window lies inside the image deposits all its energy, symmetric about the
+    pixel centre it sits on; at a corner only the window's in-image weights remain."""
+    img = O.gaussian_splat(np.array([[0.55, 0.45]]), np.array([2.0]), 10, 10, (0, 1, 0, 1))
+    assert abs(img.sum() - 2.0) < 1e-14
+    np.testing.assert_allclose(img[4, 3:8], img[4, 3:8][::-1], rtol=1e-14)
+    np.testing.assert_allclose(img[2:7, 5], img[2:7, 5][::-1], rtol=1e-14)
+    corner = O.gaussian_splat(np.array([[0.05, 0.05]]), np.array([1.0]), 10, 10, (0, 1, 0, 1))
+    w = np.exp(-(np.arange(-2, 3)[:, None] ** 2 + np.arange(-2, 3)[None, :] ** 2) / 0.5)
+    assert abs(corner.sum() - w[2:, 2:].sum() / w.sum()) < 1e-14
