@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_PKG, "libcaustic_ot.so")
 COT_OK, COT_EINVAL, COT_ENOMEM, COT_EHIP, COT_ENOTCONV, COT_ECOMPLEX, COT_EUNVERIFIED = range(7)
 COT_PREC_FP64, COT_PREC_MIXED = 0, 1
 COT_STOP_REFERENCE, COT_STOP_MAXREL, COT_STOP_HANDOFF = 0, 1, 2
+COT_MAX_VERTS = 48  # vertex slots per cell in cot_cell_polygons (include/caustic_ot.h)
 
 STATUS_NAMES = {
     COT_OK: "ok", COT_EINVAL: "invalid argument", COT_ENOMEM: "out of device memory",

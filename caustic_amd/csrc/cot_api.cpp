@@ -23,6 +23,8 @@
 #include "cot_internal.h"
 #include "cot_layout.h"
 
+static_assert(COT_MAX_VERTS == cot::kMaxVerts, "cot_cell_polygons writes kMaxVerts slots per cell");
+
 namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st);

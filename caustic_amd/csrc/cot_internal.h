@@ -5,8 +5,16 @@
 
 namespace cot {
 
-// Largest polygon a cell may hold while it is clipped (vertices == edges).
-constexpr int kMaxVerts = 24;
+// Largest polygon a cell may hold (vertices == edges): the stride of the per-site edge and
+// vertex records and the polygon of the FP64 repair pass.  A cell that would exceed it rejects
+// the line-search trial, as the reference's 200-vertex rule does (ot_op.cuh:366-382).
+constexpr int kMaxVerts = 48;
+// Polygon of the full construction passes (LDS per block: kPassVerts x kCellBlock vertices, the
+// limit on their occupancy).  A cell whose clipping would exceed it is handed to the repair pass,
+// which rebuilds it with kMaxVerts; at the reference's own configuration (N = 160,978, 76.6 %
+// zero density) cells reach ~24 vertices along an L-BFGS trajectory and intermediate polygons
+// a few more.
+constexpr int kPassVerts = 24;
 // Threads per block of the cell kernel; one block covers kCellBlock consecutive sites of one
 // problem (sites are stored in bucket order, so a block is a compact patch of the plane).
 constexpr int kCellBlock = 128;

@@ -90,20 +90,21 @@ struct KSum {
   __host__ __device__ __forceinline__ double value() const { return s + c; }
 };
 
-template <typename T>
+template <typename T, int V>
 struct PolyLDS {
-  T x[kMaxVerts][kCellBlock];
-  T y[kMaxVerts][kCellBlock];
-  int lab[kMaxVerts][kCellBlock];
+  static constexpr int kVerts = V;
+  T x[V][kCellBlock];
+  T y[V][kCellBlock];
+  int lab[V][kCellBlock];
 };
 
 // Keep the part of the lane's convex polygon with u.d <= c; the new edge gets `label`.
 // Returns the new vertex count (0 when empty).  Clipping is in place: at step k the writes land
 // at positions <= k + 1, and vertex k + 1 has already been read into registers.
 // When the polygon is emptied, sdmin is the smallest u.d - c over its vertices.
-template <typename T>
-__host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy, T c, int label,
-                                       bool& overflow, bool& changed, T& sdmin) {
+template <typename T, int V>
+__host__ __device__ int clip_halfplane(PolyLDS<T, V>& P, int ln, int n, T dx, T dy, T c,
+                                       int label, bool& overflow, bool& changed, T& sdmin) {
   int nout = 0, trans = 0;
   {
     T sd_first = P.x[0][ln] * dx + P.y[0][ln] * dy - c;
@@ -128,7 +129,7 @@ __host__ __device__ int clip_halfplane(PolyLDS<T>& P, int ln, int n, T dx, T dy,
     return 0;
   }
   if (trans != 2) return n;  // inconsistent signs from rounding on a degenerate polygon: keep
-  if (n + 1 > kMaxVerts) {
+  if (n + 1 > V) {
     overflow = true;
     return n;
   }
@@ -408,10 +409,12 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
 // ---------------------------------------------------------------------------------------------
 
 // One site of the cell kernel (lane `site_ln` of block `blk`); returns -1 when there is no site
-// to build, 1 when the cell exceeded kMaxVerts, 0 otherwise.  Shared by the kernels and the
+// to build, 1 when the cell exceeded kMaxVerts, 0 otherwise.  A full pass clips in a polygon of
+// V = kPassVerts vertices: a cell that would exceed it is handed to the repair pass (like an
+// FP32 cell FP32 cannot resolve), whose polygon holds kMaxVerts.  Shared by the kernels and the
 // host build used to test this code on the CPU.
-template <typename T>
-__host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyLDS<T>& P,
+template <typename T, int V>
+__host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyLDS<T, V>& P,
                                   int ln) {
   // site_ln: the site's lane in its cell block; ln: the LDS column of the polygon
   const BlockDesc bd = a.blocks[blk];
@@ -494,7 +497,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     }
     bool changed;
     T sdmin;
-    nv = clip_halfplane<T>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed, sdmin);
+    nv = clip_halfplane<T, V>(P, ln, nv, (T)ddx, (T)ddy, (T)c, j, overflow, changed, sdmin);
     if (changed) {
       used = max(used, r);
       if (nv > 0) {
@@ -701,7 +704,11 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
   a.kbox[i] = kb;
-  if (kF32 && coarse && a.coarse) {
+  // a full pass whose polygon overflowed kPassVerts skipped a constraint: the repair pass
+  // rebuilds the cell in its larger polygon
+  const bool handoff = overflow && V < kMaxVerts && !a.redo_mode && a.coarse;
+  if (handoff) overflow = false;
+  if (((kF32 && coarse) || handoff) && a.coarse) {
     a.redo[i] = 2;  // the check that follows lists it
     fetch_add(&a.coarse[bd.prob], 1);
   }
@@ -838,9 +845,9 @@ __device__ void timer_close(unsigned long long* tm) {
 
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
-  __shared__ PolyLDS<T> P;
+  __shared__ PolyLDS<T, kPassVerts> P;
   const unsigned long long t0 = timer_begin();
-  const int rc = cell_site<T>(a, blockIdx.x, threadIdx.x, P, threadIdx.x);
+  const int rc = cell_site<T, kPassVerts>(a, blockIdx.x, threadIdx.x, P, threadIdx.x);
   if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
   timer_mark(a.timer, rc >= 0, t0);
 }
@@ -849,7 +856,7 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
 // profiles and timers keep full evaluations and repairs apart.
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
-  __shared__ PolyLDS<T> P;
+  __shared__ PolyLDS<T, kMaxVerts> P;
   const unsigned long long t0 = timer_begin();
   const int n = *a.list_count;
   bool worked = false;
@@ -857,7 +864,7 @@ __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   for (int e = threadIdx.x * gridDim.x + blockIdx.x; e < n; e += gridDim.x * kCellBlock) {
     const int slot = a.list[e];
     const int blk = slot / kCellBlock;
-    const int rc = cell_site<T>(a, blk, slot % kCellBlock, P, threadIdx.x);
+    const int rc = cell_site<T, kMaxVerts>(a, blk, slot % kCellBlock, P, threadIdx.x);
     if (rc == 1) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
     worked = worked || rc >= 0;
   }

@@ -131,8 +131,9 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
   for (int e = threadIdx.x; e < bd.count * kMaxVerts; e += blockDim.x) {
     const int i = bd.first + e / kMaxVerts, k = e % kMaxVerts;
     const int nv = nedge[i];
+    if (k >= nv) continue;  // readers stop at the cell's edge count
     double hv = 0.0;
-    const int j = k < nv ? elab[(size_t)i * kMaxVerts + k] : -1;
+    const int j = elab[(size_t)i * kMaxVerts + k];
     if (on && nv >= 3 && j >= 0) {
       const double2 a = vtx[(size_t)i * kMaxVerts + k];
       const double2 b = vtx[(size_t)i * kMaxVerts + (k + 1 < nv ? k + 1 : 0)];
@@ -152,7 +153,8 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
   if ((int)threadIdx.x < bd.count) {
     const int i = bd.first + threadIdx.x;
     double s = 0.0;
-    for (int k = 0; k < kMaxVerts; ++k) s += h[(size_t)i * kMaxVerts + k];
+    const int nv = nedge[i];
+    for (int k = 0; k < nv; ++k) s += h[(size_t)i * kMaxVerts + k];
     diag[i] = s;
     if (sagg) sagg[i] = s > 0.0 ? bagg[blockIdx.x] : -1;
   }

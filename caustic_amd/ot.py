@@ -149,10 +149,11 @@ class OTBatch:
         return out
 
     def cell_polygons(self, w=None):
-        """(counts (N,), vertices (N, 24, 2)) of the cells at w, CCW; count 0 for an empty cell."""
+        """(counts (N,), vertices (N, COT_MAX_VERTS, 2)) of the cells at w, CCW; count 0 for an
+        empty cell."""
         w = self._w(w)
         counts = np.zeros(self.n_total, dtype=np.int32)
-        verts = np.zeros((self.n_total, 24, 2))
+        verts = np.zeros((self.n_total, L.COT_MAX_VERTS, 2))
         L.check(self._lib.cot_cell_polygons(self._ctx, _ptr(w),
                                             counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                             _ptr(verts)))

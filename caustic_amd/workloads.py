@@ -226,3 +226,64 @@ def tilted_curves(seed: int = 5000, n_side: int = 512, res: int = 2048,
                      seed)
     p.site_box = (0.5 - 0.5 * c, 0.5 + 0.5 * c, 0.0, 1.0)
     return p
+
+
+# --------------------------------------------------------------------------------------------
+# The reference's own configuration: the seeds spline_irra.py writes for caustic_GPU
+# --------------------------------------------------------------------------------------------
+
+def triangular_lattice(point_num: int, sample_domain) -> np.ndarray:
+    """The "triangle" seed lattice of initial_xy_point (utils/data_utils.py:374-410).
+
+    nx = int(sqrt(sqrt(3)/2 * point_num * X / Y)) + 1 points along x, ny rows (made odd) with
+    ny = int(nx_real * 2/sqrt(3) * Y / X) + 1; even rows hold nx points at x = j * dx, odd rows
+    the two ends plus nx - 1 midpoints at x = (j + 1/2) dx.  Coordinates are offsets from 0
+    (the reference adds no sample_domain origin).  point_num 160,000 on 40 x 40 gives
+    nx = 373, ny = 431 and 160,978 points."""
+    x0, x1, y0, y1 = (float(v) for v in sample_domain)
+    xs, ys = x1 - x0, y1 - y0
+    nx_real = math.sqrt(math.sqrt(3.0) / 2.0 * float(point_num) * xs / ys)
+    ny_real = nx_real * 2.0 / math.sqrt(3.0) * ys / xs
+    nx, ny = int(nx_real) + 1, int(ny_real) + 1
+    if ny % 2 == 0:
+        ny += 1
+    dx, dy = xs / float(nx - 1), ys / float(ny - 1)
+    rows = []
+    for i in range(ny):
+        y = i * dy
+        if i % 2 == 0:
+            x = np.arange(nx) * dx
+        else:
+            x = np.concatenate([[0.0], np.arange(nx - 1) * dx + dx / 2.0, [xs]])
+        rows.append(np.column_stack([x, np.full(x.shape[0], y)]))
+    return np.concatenate(rows).astype(np.float64)
+
+
+def similarity_map(points: np.ndarray, sample_domain, proj_domain) -> np.ndarray:
+    """proj_xy of initial_proj_point (utils/data_utils.py:413-435): the sample domain's centre
+    to the projection domain's centre, scaled by min(proj_w / sample_w, proj_h / sample_h)."""
+    sx0, sx1, sy0, sy1 = (float(v) for v in sample_domain)
+    px0, px1, py0, py1 = (float(v) for v in proj_domain)
+    s = min((px1 - px0) / (sx1 - sx0), (py1 - py0) / (sy1 - sy0))
+    out = np.empty_like(points, dtype=np.float64)
+    out[:, 0] = (points[:, 0] - 0.5 * (sx0 + sx1)) * s + 0.5 * (px0 + px1)
+    out[:, 1] = (points[:, 1] - 0.5 * (sy0 + sy1)) * s + 0.5 * (py0 + py1)
+    return out
+
+
+def lattice_boundary_indices(points: np.ndarray, point_num: int, sample_domain):
+    """(left, right, bottom, top) index arrays of find_boundary_points (utils/data_utils.py:
+    437-491) for a convex lattice: every point within tol = 0.1 sqrt(area / point_num) of a side
+    of the points' bounding box lies on the Delaunay boundary, and no other point does."""
+    x0, x1, y0, y1 = (float(v) for v in sample_domain)
+    tol = 0.1 * math.sqrt((x1 - x0) * (y1 - y0) / float(point_num))
+    x, y = points[:, 0], points[:, 1]
+    return tuple(np.nonzero(np.abs(v - e) <= tol)[0]
+                 for v, e in ((x, x.min()), (x, x.max()), (y, y.min()), (y, y.max())))
+
+
+def picture_from_text_rows(T: np.ndarray) -> np.ndarray:
+    """A picture text matrix (m rows, n columns, as in data/pic_data/*.txt) in the layout
+    rho[iy, ix]: element (ix, iy) is text row m - 1 - ix, column iy
+    (read_pic_txt_to_array, basic_function.cuh:145-204)."""
+    return np.ascontiguousarray(np.asarray(T, dtype=np.float64)[::-1, :].T)

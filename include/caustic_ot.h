@@ -187,7 +187,7 @@ int cot_render(cot_context* ctx, const double* w, double* image);
 /* The cells at weights w (the polygons writeOBJ writes, basic_function.cuh:15-59): counts[i]
  * vertices (0 for an empty cell, at most COT_MAX_VERTS), CCW, in verts[(i * COT_MAX_VERTS + k) *
  * 2 + {0, 1}] (caller order).  The cells are the restricted cells inside proj_domain. */
-#define COT_MAX_VERTS 24
+#define COT_MAX_VERTS 48
 int cot_cell_polygons(cot_context* ctx, const double* w, int* counts, double* verts);
 
 /* y = H x, H the Hessian of the dual objective at w (d mass_i / d w_j, the matrix the Newton

@@ -36,15 +36,17 @@ struct Session {
   int ring_max = 32;  // the library's default
 };
 
-template <typename T>
+template <typename T, int V>
 struct Lanes {
-  PolyLDS<T>* P = new PolyLDS<T>();
+  PolyLDS<T, V>* P = new PolyLDS<T, V>();
   ~Lanes() { delete P; }
 };
 
 }  // namespace
 
 extern "C" {
+
+int emu_max_verts() { return kMaxVerts; }
 
 void* emu_create(int n, const double* sites, const double* nu, const double* rho, int pm, int pn,
                  const double* proj, int fp32, int ring_init, int max_repairs, double verify_tol) {
@@ -202,13 +204,16 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
     const bool f32 = S.fp32 && !a.redo_mode;
 #pragma omp parallel
     {
-      Lanes<float> lf;
-      Lanes<double> ld;
+      Lanes<float, kPassVerts> lf;
+      Lanes<double, kPassVerts> ld;
+      Lanes<double, kMaxVerts> lr;
 #pragma omp for schedule(dynamic, 1) reduction(+ : overflow, built)
       for (int b = 0; b < nb; ++b)
         for (int ln = 0; ln < kCellBlock; ++ln) {
-          const int rc = f32 ? cell_site<float>(a, b, ln, *lf.P, ln)
-                             : cell_site<double>(a, b, ln, *ld.P, ln);
+          // full passes clip in kPassVerts, repairs in kMaxVerts (as the kernels)
+          const int rc = a.redo_mode ? cell_site<double, kMaxVerts>(a, b, ln, *lr.P, ln)
+                         : f32       ? cell_site<float, kPassVerts>(a, b, ln, *lf.P, ln)
+                                     : cell_site<double, kPassVerts>(a, b, ln, *ld.P, ln);
           overflow += rc == 1 ? 1 : 0;
           built += rc >= 0 ? 1 : 0;
         }

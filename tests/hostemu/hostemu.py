@@ -71,7 +71,7 @@ class Session:
     def state(self):
         """Per-site state after the last evaluation: conformity flags, edge counts, edge labels
         (caller indices; sides below zero), spans, bucket rings."""
-        n, K = self.n, 24
+        n, K = self.n, int(lib().emu_max_verts())
         redo = np.zeros(n, dtype=np.uint8)
         ne = np.zeros(n, dtype=np.int32)
         lab = np.zeros((n, K), dtype=np.int32)

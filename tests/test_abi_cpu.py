@@ -50,6 +50,11 @@ def test_struct_layouts_match_the_header(tmp_path):
     assert got == want
 
 
+def test_vertex_slots_match_the_header():
+    m = re.search(r"#define COT_MAX_VERTS (\d+)", open(HEADER).read())
+    assert m and int(m.group(1)) == L.COT_MAX_VERTS
+
+
 def test_create_rejects_bad_arguments_without_touching_the_gpu():
     lib = L.load()
     ctx = ctypes.c_void_p()
