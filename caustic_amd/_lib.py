@@ -63,6 +63,17 @@ class cot_reflection(ctypes.Structure):
                 ("if_aa_op_initial", ctypes.c_int), ("incident_dir", _dp)]
 
 
+COT_LADDER_MAX = 8
+
+
+class cot_ladder_opts(ctypes.Structure):
+    _fields_ = [("n_levels", ctypes.c_int), ("sides", ctypes.c_int * COT_LADDER_MAX),
+                ("floors", ctypes.c_double * COT_LADDER_MAX), ("n_final_floors", ctypes.c_int),
+                ("final_floors", ctypes.c_double * 4), ("coarse_tol", ctypes.c_double),
+                ("handoff", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("frames", _dp),
+                ("level_options", ctypes.POINTER(cot_options))]
+
+
 # name -> (restype, argtypes); every symbol include/caustic_ot.h declares
 SIGNATURES = {
     "cot_default_options": (None, [ctypes.POINTER(cot_options)]),
@@ -79,6 +90,12 @@ SIGNATURES = {
                                             ctypes.POINTER(cot_reflection),
                                             ctypes.POINTER(cot_report)]),
     "cot_solve_newton": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(cot_solve_opts),
+                                        ctypes.POINTER(cot_report)]),
+    "cot_default_ladder_opts": (None, [ctypes.POINTER(cot_problem), ctypes.c_int,
+                                       ctypes.POINTER(cot_ladder_opts)]),
+    "cot_solve_ladder": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(cot_problem), _dp,
+                                        ctypes.POINTER(cot_solve_opts),
+                                        ctypes.POINTER(cot_ladder_opts),
                                         ctypes.POINTER(cot_report)]),
     "cot_hessian_product": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "cot_render": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),

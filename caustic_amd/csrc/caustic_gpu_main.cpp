@@ -12,6 +12,11 @@
 // and, when the settings hold "mi355x": {"full_precision_weights": true}, also
 //         ../../data/weights/<xiushi>/<t>/final_proj_point_weights_full.txt (%.17g).
 //
+// "mi355x": {"solver": "ladder"} solves t = 0 coarse to fine (cot_solve_ladder: coarse grids,
+// density-floor stages for targets with zero-density regions, then the exact target to the same
+// stopping rule, |mass - nu|_2 < convergence_tolerance with no empty cell) instead of L-BFGS from
+// w = 0; "lbfgs" (the default) is the reference's algorithm.
+//
 // and the diagnostics ../render_result/<xiushi>/<t>/{render_initial,render_<iter>}.txt and
 // ../mesh_output/<xiushi>/<t>/{mesh_initial,mesh_<iter>}.obj (t >= 1: pre_render_initial.txt,
 // pre_initial_mesh.obj, <t-1>_render_<iter>.txt, <t-1>_mesh_<iter>.obj), unless the settings
@@ -335,6 +340,7 @@ int main(int argc, char** argv) {
   std::vector<double> sample_domain, proj_domain, point_light;
   bool full_precision = false;
   bool diagnostics = true;
+  std::string solver = "lbfgs";
   int aa_history_size = 7;
   double aa_fixed_para = -2.0, z_of_len = 0.0, z_of_receiver = 0.0, refraction_index = 1.5;
   bool if_aa_op_initial = true;
@@ -364,11 +370,15 @@ int main(int argc, char** argv) {
       if (const JVal* fp = ext->find("full_precision_weights")) full_precision = fp->as_bool();
     if (const JVal* ext = J.find("mi355x"))
       if (const JVal* dg = ext->find("diagnostics")) diagnostics = dg->as_bool();
+    if (const JVal* ext = J.find("mi355x"))
+      if (const JVal* sv = ext->find("solver")) solver = sv->as_str();
   } catch (const std::exception& e) {
     return die(std::string("settings: ") + e.what(), 2);
   }
   if (sample_domain.size() != 4 || proj_domain.size() != 4)
     return die("settings: sample_domain and proj_domain need 4 numbers", 2);
+  if (solver != "lbfgs" && solver != "ladder")
+    return die("settings: mi355x.solver must be \"lbfgs\" or \"ladder\"", 2);
 
   // inputs
   const std::string seed = "../seed_point/" + xiushi + "/";
@@ -505,7 +515,11 @@ int main(int argc, char** argv) {
   if (diagnostics && !diagnose(pre + "render_initial.txt",
                                ot_time == "0" ? "mesh_initial.obj" : "pre_initial_mesh.obj"))
     return die(std::string("render: ") + cot_last_error(), 5);
-  if (ot_time == "0") {
+  if (ot_time == "0" && solver == "ladder") {
+    cot_ladder_opts lo;
+    cot_default_ladder_opts(&pr, 1, &lo);
+    rc = cot_solve_ladder(g.c, &pr, w.data(), &so, &lo, &rep);
+  } else if (ot_time == "0") {
     rc = cot_solve(g.c, w.data(), &so, &rep);
   } else {
     // reflection-aware path: Fresnel-weighted source masses, L-BFGS then Anderson

@@ -826,6 +826,13 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
 
 }  // namespace
 
+// Accessors for the host drivers built on the C ABI (cot_ladder.cpp).
+namespace cot {
+const cot_options* context_options(const cot_context* c) { return &c->opt; }
+int context_problems(const cot_context* c) { return c->np; }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace cot
+
 extern "C" {
 
 void cot_default_options(cot_options* o) {
@@ -1609,6 +1616,12 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   std::vector<uint8_t> act(np, 1), conv(np, 0), failed(np, 0);
   std::vector<EvalSum> cur, trial;
   std::vector<int> iters(np, 0), evals(np, 1);
+  // stopping rule: the reference's |g|_2 < tol (Newton steps keep every cell non-empty), or
+  // max-rel < tol
+  auto met = [&](const EvalSum& e) {
+    return o.stop_rule == COT_STOP_REFERENCE ? (std::sqrt(e.g2) < o.tol && e.zeros == 0)
+                                             : e.maxrel < o.tol;
+  };
   CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
   HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
   std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
@@ -1687,7 +1700,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     bool any = false;
     for (int p = 0; p < np; ++p) {
       if (!conv[p] && !failed[p]) {
-        if (cur[p].maxrel < o.tol) {
+        if (met(cur[p])) {
           conv[p] = 1;
         } else if (cur[p].zeros > 0 || cur[p].fails > 0) {
           failed[p] = 1;  // Newton needs every cell non-empty and verified
@@ -1895,7 +1908,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   bool all = true;
   for (int p = 0; p < np; ++p) {
-    if (!conv[p] && !failed[p] && cur[p].maxrel < o.tol) conv[p] = 1;
+    if (!conv[p] && !failed[p] && met(cur[p])) conv[p] = 1;
     all = all && conv[p];
     if (reports) {
       cot_report& r = reports[p];

@@ -25,6 +25,27 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _problem_array(problems):
+    """(cot_problem array, the numpy arrays it points into) for workloads.Problem-like objects."""
+    arr = (L.cot_problem * len(problems))()
+    keep = []
+    for k, p in enumerate(problems):
+        sites = np.ascontiguousarray(p.sites, dtype=np.float64).reshape(-1, 2)
+        nu = np.ascontiguousarray(p.nu, dtype=np.float64)
+        rho = np.ascontiguousarray(p.rho, dtype=np.float64)
+        if nu.shape[0] != sites.shape[0]:
+            raise ValueError("nu and sites differ in length")
+        keep += [sites, nu, rho]
+        arr[k].n_sites = sites.shape[0]
+        arr[k].sites = _ptr(sites)
+        arr[k].nu = _ptr(nu)
+        arr[k].rho = _ptr(rho)
+        arr[k].pn, arr[k].pm = rho.shape
+        for q in range(4):
+            arr[k].proj_domain[q] = float(p.proj_domain[q])
+    return arr, keep
+
+
 class OTBatch:
     """A context of one or more OT problems on the current GPU."""
 
@@ -41,24 +62,9 @@ class OTBatch:
             sites_per_bucket = float(tuning.get("sites_per_bucket", sites_per_bucket))
             full_pass_ring = int(tuning.get("full_pass_ring", full_pass_ring))
         lib = L.load()
-        self._keep = []
-        arr = (L.cot_problem * len(problems))()
-        self.sizes = []
-        for k, p in enumerate(problems):
-            sites = np.ascontiguousarray(p.sites, dtype=np.float64).reshape(-1, 2)
-            nu = np.ascontiguousarray(p.nu, dtype=np.float64)
-            rho = np.ascontiguousarray(p.rho, dtype=np.float64)
-            if nu.shape[0] != sites.shape[0]:
-                raise ValueError("nu and sites differ in length")
-            self._keep += [sites, nu, rho]
-            arr[k].n_sites = sites.shape[0]
-            arr[k].sites = _ptr(sites)
-            arr[k].nu = _ptr(nu)
-            arr[k].rho = _ptr(rho)
-            arr[k].pn, arr[k].pm = rho.shape
-            for q in range(4):
-                arr[k].proj_domain[q] = float(p.proj_domain[q])
-            self.sizes.append(sites.shape[0])
+        self.problems = list(problems)
+        arr, self._keep = _problem_array(self.problems)
+        self.sizes = [int(arr[k].n_sites) for k in range(len(self.problems))]
         opt = L.cot_options()
         lib.cot_default_options(ctypes.byref(opt))
         opt.precision = {"fp64": L.COT_PREC_FP64, "mixed": L.COT_PREC_MIXED}[precision]
@@ -73,6 +79,7 @@ class OTBatch:
         ctx = ctypes.c_void_p()
         L.check(lib.cot_create(arr, len(problems), ctypes.byref(opt), ctypes.byref(ctx)))
         self._keep = []  # the library copied everything
+        self._opt = opt
         self._ctx = ctx
         self._lib = lib
         self.precision = precision
@@ -263,6 +270,73 @@ class OTBatch:
             L.check(rc, allow=(L.COT_ENOTCONV,))
         out = [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
         return w, out
+
+    def ladder_options(self, frames=None, level_tunings=None, **overrides):
+        """cot_ladder_opts for this batch: the library's defaults (cot_default_ladder_opts),
+        then overrides (n_levels/sides/floors/final_floors as sequences, coarse_tol, handoff,
+        seed).  frames: per-problem (x0, x1, y0, y1) regions the sites sample (default: each
+        problem's frame).  level_tunings: launch shapes per coarse level (autotune dicts)."""
+        arr, keep = _problem_array(self.problems)
+        lo = L.cot_ladder_opts()
+        self._lib.cot_default_ladder_opts(arr, len(self.problems), ctypes.byref(lo))
+        if "sides" in overrides:
+            sides = list(overrides.pop("sides"))
+            lo.n_levels = len(sides)
+            floors = list(overrides.pop("floors", [0.0] * len(sides)))
+            for k, (sd, fl) in enumerate(zip(sides, floors)):
+                lo.sides[k], lo.floors[k] = int(sd), float(fl)
+        if "final_floors" in overrides:
+            ff = list(overrides.pop("final_floors"))
+            lo.n_final_floors = len(ff)
+            for k, v in enumerate(ff):
+                lo.final_floors[k] = float(v)
+        for k, v in overrides.items():
+            setattr(lo, k, v)
+        fr = np.ascontiguousarray([p.frame if frames is None else frames[k]
+                                   for k, p in enumerate(self.problems)], dtype=np.float64)
+        keep.append(fr)
+        lo.frames = _ptr(fr)
+        if level_tunings:
+            lvl = (L.cot_options * lo.n_levels)()
+            for k in range(lo.n_levels):
+                ctypes.memmove(ctypes.byref(lvl[k]), ctypes.byref(self._opt),
+                               ctypes.sizeof(L.cot_options))
+                t = level_tunings[k] if k < len(level_tunings) and level_tunings[k] else {}
+                for name in ("int_lanes", "repair_blocks", "sites_per_bucket", "full_pass_ring"):
+                    if name in t:
+                        setattr(lvl[k], name, type(getattr(lvl[k], name))(t[name]))
+            keep.append(lvl)
+            lo.level_options = ctypes.cast(lvl, ctypes.POINTER(L.cot_options))
+        return lo, keep
+
+    def solve_ladder(self, history: int = 10, max_iter: int = 20000, tol: float = 1e-5,
+                     stop: str = "maxrel", h0_threshold: float = 0.0, verbose: int = 0,
+                     device_loop=True, raise_on_fail: bool = True, ladder=None, **ladder_kw):
+        """Coarse-to-fine solve from nothing but the problems (cot_solve_ladder): coarse
+        jittered grids, density-floor stages for targets with zero-density regions, then the
+        exact target to `stop`/`tol` ("maxrel" or "reference").  ladder: (cot_ladder_opts,
+        keep-alive list) from ladder_options(); else ladder_options(**ladder_kw).  Returns
+        (w, reports)."""
+        lo, keep = ladder if ladder is not None else self.ladder_options(**ladder_kw)
+        arr, keep2 = _problem_array(self.problems)
+        o = L.cot_solve_opts()
+        self._lib.cot_default_solve_opts(ctypes.byref(o))
+        o.history = history
+        o.max_iter = max_iter
+        o.tol = tol
+        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL}[stop]
+        o.h0_threshold = h0_threshold
+        o.verbose = verbose
+        o.device_loop = int(device_loop)
+        o.accept_rule = 1 if self.precision == "mixed" else 0
+        w = np.zeros(self.n_total)
+        reps = (L.cot_report * self.n_problems)()
+        rc = self._lib.cot_solve_ladder(self._ctx, arr, _ptr(w), ctypes.byref(o), ctypes.byref(lo),
+                                        reps)
+        del keep, keep2
+        if raise_on_fail:
+            L.check(rc, allow=(L.COT_ENOTCONV,))
+        return w, [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
 
     def _newton(self, w0, tol, max_iter, verbose, repair_rounds, device_loop=True, sweeps=2,
                 coarse=256, small_cg=True):

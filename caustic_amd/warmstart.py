@@ -360,6 +360,7 @@ def coarse_problem(p: W.Problem, n_side: int, seed: int) -> W.Problem:
     x0, x1, y0, y1 = p.frame
     sites = W.jittered_grid(n_side, W.rng_for(seed)) * [x1 - x0, y1 - y0] + [x0, y0]
     cp = W.make_problem(sites, rho / rho.sum(), p.tol, p.name + f"_c{n_side}", seed)
+    cp.proj_domain = tuple(p.proj_domain)
     cp.site_box = p.site_box
     return cp
 

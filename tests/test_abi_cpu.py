@@ -32,11 +32,13 @@ def test_struct_layouts_match_the_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "caustic_ot.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(cot_problem), "
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", "
+        "sizeof(cot_problem), "
         "sizeof(cot_options), sizeof(cot_solve_opts), sizeof(cot_report), sizeof(cot_stats), "
         "offsetof(cot_solve_opts, newton_sweeps), sizeof(cot_reflection), "
         "offsetof(cot_reflection, if_aa_op_initial), offsetof(cot_options, full_pass_ring), "
-        "offsetof(cot_solve_opts, newton_coarse));"
+        "offsetof(cot_solve_opts, newton_coarse), sizeof(cot_ladder_opts), "
+        "offsetof(cot_ladder_opts, seed));"
         "return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
@@ -46,7 +48,8 @@ def test_struct_layouts_match_the_header(tmp_path):
             ctypes.sizeof(L.cot_solve_opts), ctypes.sizeof(L.cot_report),
             ctypes.sizeof(L.cot_stats), L.cot_solve_opts.newton_sweeps.offset,
             ctypes.sizeof(L.cot_reflection), L.cot_reflection.if_aa_op_initial.offset,
-            L.cot_options.full_pass_ring.offset, L.cot_solve_opts.newton_coarse.offset]
+            L.cot_options.full_pass_ring.offset, L.cot_solve_opts.newton_coarse.offset,
+            ctypes.sizeof(L.cot_ladder_opts), L.cot_ladder_opts.seed.offset]
     assert got == want
 
 
