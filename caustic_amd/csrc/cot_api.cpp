@@ -59,7 +59,8 @@ hipError_t launch_dev_direction(const ProbDesc* probs, const BlockDesc* blocks, 
                                 hipStream_t stream);
 hipError_t launch_dev_decide(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
                              const SolveParams& sp, const double* mass, const double* fcell,
-                             const double* nu, const double* g, const double* d, const int* fails,
+                             const double* area, const double* nu, const double* g,
+                             const double* d, const int* fails,
                              int* overflow, double* part, size_t pstride, int* counter,
                              uint8_t* active, int* fail_counters, int n_fail_counters, int np,
                              int nblk, hipStream_t stream);
@@ -256,7 +257,7 @@ struct cot_context {
   DevState* d_state = nullptr;
   int* d_cnt = nullptr;        // last-block counters: direction, decide, accept (np each)
   double* d_pdir = nullptr;    // direction partials  [1][nblk]
-  double* d_peval = nullptr;   // decide partials     [5][nblk]
+  double* d_peval = nullptr;   // decide partials     [6][nblk]
   double* d_pacc = nullptr;    // accept partials     [5K][nblk]
   int* d_rfails = nullptr;     // per-round conformity failures [(R+1)][np]
   int rfail_rounds = 0;
@@ -510,10 +511,19 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   }
   c->stats.repair_rounds += rounds;
   sum_partials(c, sum);
+  // the cover check: cells that passed the edge check but do not tile the target (a consistent
+  // multiple cover) leave the evaluation unverified, and the next evaluation searches bucket
+  // rings rather than their adjacency
+  bool covered = true;
+  for (int p = 0; p < c->np; ++p)
+    if (active[p] && sum[p].fails == 0 && !covers_domain(c->probs[p], sum[p].area)) {
+      sum[p].fails = 1;
+      covered = false;
+    }
   // the cells just built are the next evaluation's candidates
   HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
                                c->nblk, st));
-  c->have_adj = true;
+  c->have_adj = covered;
   // after a clean check the next bucket search starts from the outermost used ring plus one;
   // after an unresolved one the raised rings stay
   bool clean = true;
@@ -603,7 +613,7 @@ int ensure_device_solver(cot_context* c, int repair_rounds) {
     CHK(dalloc(c, &c->d_state, c->np));
     CHK(dalloc(c, &c->d_cnt, 3 * (size_t)c->np));
     CHK(dalloc(c, &c->d_pdir, c->nblk));
-    CHK(dalloc(c, &c->d_peval, 5 * (size_t)c->nblk));
+    CHK(dalloc(c, &c->d_peval, 6 * (size_t)c->nblk));
     CHK(dalloc(c, &c->d_pacc, (size_t)kAcceptPartials * c->nblk));
     CHK(dalloc(c, &c->d_ndone, 2));
     if (hipHostMalloc((void**)&c->h_ndone, 2 * sizeof(int)) != hipSuccess)
@@ -672,7 +682,8 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
                                c->opt.ring_max, c->nblk, st));
   HIPCHK(launch_dev_decide(c->d_probs, c->d_blocks, c->d_state, sp, c->d_mass, c->d_fcell,
-                           c->d_nu, c->d_g, c->d_dir, final_fails, c->d_overflow, c->d_peval, ps,
+                           c->d_area, c->d_nu, c->d_g, c->d_dir, final_fails, c->d_overflow,
+                           c->d_peval, ps,
                            c->d_cnt + np, c->d_active, c->d_rfails, R + 1, np, c->nblk, st));
   HIPCHK(launch_dev_accept(c->d_probs, c->d_blocks, c->d_state, sp, c->d_mass, c->d_nu,
                            c->d_dir, c->d_w, c->d_g, c->d_S, c->d_Y, ld, c->d_pacc, ps,

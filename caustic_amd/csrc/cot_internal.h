@@ -34,6 +34,13 @@ constexpr int kMaxHist = 16;
 // whole chunks.  Sums between two columns then need no subtraction of large prefixes.
 constexpr int kChunk = 32;
 
+// The last step of the conformity argument: when every cell edge is matched by its neighbour's,
+// the cells cover the target rectangle D a constant number of times, so sum |P_i| = |D| forces a
+// tiling.  An evaluation whose cell areas sum to more than kCoverTol |D| away from |D| counts as
+// unverified (FP64 vertices shared by neighbouring cells agree to rounding, so a tiling sums to
+// |D| within ~1e-13 |D|; a double cover misses by |D|).
+constexpr double kCoverTol = 1e-8;
+
 // Edge labels below zero: the four sides of the target rectangle.
 constexpr int kSideBottom = -1, kSideRight = -2, kSideTop = -3, kSideLeft = -4;
 // Candidate sources of the cell kernel.  kCandBucket: the bucket rings 0..ring[i] around the
@@ -75,6 +82,12 @@ struct ProbDesc {
   double binv;            // 1 / bucket size
   double bsize;           // bucket size
 };
+
+// The cover check (kCoverTol): do the cells' areas add up to the target rectangle's?
+__host__ __device__ inline bool covers_domain(const ProbDesc& pd, double area) {
+  const double dom = (pd.x1 - pd.x0) * (pd.y1 - pd.y0);
+  return fabs(area - dom) <= kCoverTol * dom;
+}
 
 // Per-block site range.
 struct BlockDesc {
@@ -223,7 +236,10 @@ struct SolveParams {
 
 struct DevState {
   int phase, loop_iter, restart, iters, evals, conv, status, steep, accept, hcount;
-  int unverified, pad;
+  int unverified;
+  int fresh;         // the last trial's cells passed the edge check but did not tile the
+                     // target (cover check): the next trial searches bucket rings instead of
+                     // that trial's adjacency
   double t, f_cur, f_prev, g2_cur, zeros_cur, maxrel_cur, snap_g, snap_zero, H0;
   double f_try, g2_try, zeros_try, maxrel_try, gd_try;
   double SY[kMaxHist][kMaxHist];

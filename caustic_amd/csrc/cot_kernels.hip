@@ -508,7 +508,10 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     }
   };
   const int R0 = a.ring[i];
-  const int nadj = a.cand != kCandBucket ? (int)a.adjn[i] : 0;
+  // the previous cells' adjacency, unless their cells failed the cover check (then fresh
+  // bucket rings)
+  const bool fresh = a.st && a.st[bd.prob].fresh;
+  const int nadj = a.cand != kCandBucket && !fresh ? (int)a.adjn[i] : 0;
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
   // candidates queued by the conformity check, taken in increasing site order (the check appends

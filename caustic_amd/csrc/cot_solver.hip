@@ -191,10 +191,12 @@ __global__ __launch_bounds__(kCellBlock) void dev_direction_kernel(
 }
 
 // Reductions of a trial evaluation and the line-search decision.
-// Partials (layout [v][block]): 0 f, 1 max rel err, 2 empty-mass count, 3 |g|^2, 4 g.d_trial
+// Partials (layout [v][block]): 0 f, 1 max rel err, 2 empty-mass count, 3 |g|^2, 4 g.d_trial,
+// 5 cell area (the cover check, kCoverTol)
 __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
     const ProbDesc* probs, const BlockDesc* blocks, DevState* st, SolveParams sp,
-    const double* mass, const double* fcell, const double* nu, const double* g, const double* d,
+    const double* mass, const double* fcell, const double* area, const double* nu,
+    const double* g, const double* d,
     const int* fails, int* overflow, double* part, size_t pstride, int* counter,
     uint8_t* active, int* fail_counters, int n_fail_counters, int np) {
   __shared__ double red[kCellBlock / 64];
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   if (s.phase != kPhaseTrial) return;
   const ProbDesc pd = probs[bd.prob];
   const int ln = threadIdx.x;
-  double f = 0, mr = 0, z = 0, g2 = 0, gd = 0;
+  double f = 0, mr = 0, z = 0, g2 = 0, gd = 0, ar = 0;
   if (ln < bd.count) {
     const int i = bd.first + ln;
     const double gi = mass[i] - nu[i];
@@ -213,6 +215,7 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
     z = mass[i] == 0.0 ? 1.0 : 0.0;
     g2 = gi * gi;
     gd = gi * di;
+    ar = area[i];
   }
   double v;
   v = block_sum1(f, red);  if (ln == 0) part[0 * pstride + blockIdx.x] = v;
@@ -220,10 +223,11 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   v = block_sum1(z, red);  if (ln == 0) part[2 * pstride + blockIdx.x] = v;
   v = block_sum1(g2, red); if (ln == 0) part[3 * pstride + blockIdx.x] = v;
   v = block_sum1(gd, red); if (ln == 0) part[4 * pstride + blockIdx.x] = v;
+  v = block_sum1(ar, red); if (ln == 0) part[5 * pstride + blockIdx.x] = v;
   if (!arrive_last(&counter[bd.prob], pd.nblk)) return;
   __shared__ double lw[4 * 16];
-  __shared__ double tot[5];
-  sum_partials(part, pstride, pd.blk_off, pd.nblk, 5, lw, tot);
+  __shared__ double tot[6];
+  sum_partials(part, pstride, pd.blk_off, pd.nblk, 6, lw, tot);
   // the max relative error is a max, not a sum
   double mloc = 0.0;
   for (int b = ln; b < pd.nblk; b += blockDim.x) mloc = fmax(mloc, part[1 * pstride + pd.blk_off + b]);
@@ -244,10 +248,15 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   s.g2_try = tot[3];
   s.gd_try = tot[4];
   bool ok = false;
-  if (ovf > 0 || unresolved > 0) {
-    // a cell over the vertex limit, or cells still unverified after the repair rounds: reject
-    // the trial like the reference's "polygon too complex" rule (ot_op.cuh:376-382); the raised
-    // rings stay for the next trial
+  // the cover check carries information when the edge check passed (cells still unverified
+  // miss the target's area anyway, and their adjacency is the right start for the next trial):
+  // a consistent multiple cover is rejected and the next trial searches bucket rings
+  const bool covered = covers_domain(pd, tot[5]);
+  s.fresh = (!covered && unresolved == 0 && ovf == 0) ? 1 : 0;
+  if (ovf > 0 || unresolved > 0 || !covered) {
+    // a cell over the vertex limit, cells still unverified after the repair rounds, or cells
+    // that do not tile the target: reject the trial like the reference's "polygon too complex"
+    // rule (ot_op.cuh:376-382); the raised rings stay for the next trial
     s.t *= 0.5;
   } else if (s.f_try < s.f_prev || (sp.mixed_ls && s.gd_try <= 0.0)) {
     ok = true;
@@ -468,12 +477,13 @@ hipError_t launch_dev_direction(const ProbDesc* probs, const BlockDesc* blocks, 
 
 hipError_t launch_dev_decide(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
                              const SolveParams& sp, const double* mass, const double* fcell,
-                             const double* nu, const double* g, const double* d, const int* fails,
+                             const double* area, const double* nu, const double* g,
+                             const double* d, const int* fails,
                              int* overflow, double* part, size_t pstride, int* counter,
                              uint8_t* active, int* fail_counters, int n_fail_counters, int np,
                              int nblk, hipStream_t stream) {
   hipLaunchKernelGGL(dev_decide_kernel, dim3(nblk), dim3(kCellBlock), 0, stream, probs, blocks,
-                     st, sp, mass, fcell, nu, g, d, fails, overflow, part, pstride, counter,
+                     st, sp, mass, fcell, area, nu, g, d, fails, overflow, part, pstride, counter,
                      active, fail_counters, n_fail_counters, np);
   return hipGetLastError();
 }

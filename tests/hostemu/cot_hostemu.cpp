@@ -120,6 +120,23 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
 
 void emu_destroy(void* h) { delete static_cast<Session*>(h); }
 
+// Candidates of the next evaluation as if a previous evaluation had left them: counts[i]
+// neighbour labels of site i at labels[i * kMaxVerts + k] (caller indices).
+void emu_set_adjacency(void* h, const int* counts, const int* labels) {
+  Session& S = *static_cast<Session*>(h);
+  std::vector<int> sorted_of(S.n);
+  for (int r = 0; r < S.n; ++r) sorted_of[S.order[r]] = r;
+  for (int r = 0; r < S.n; ++r) {
+    const int o = S.order[r];
+    S.adjn[r] = (uint8_t)std::min(counts[o], kMaxVerts);
+    for (int k = 0; k < S.adjn[r]; ++k) {
+      const int j = labels[(size_t)o * kMaxVerts + k];
+      S.adj[(size_t)r * kMaxVerts + k] = j >= 0 ? sorted_of[j] : j;
+    }
+  }
+  S.have_adj = true;
+}
+
 // Per-site state after the last evaluation (caller order; any may be NULL): the conformity
 // flags of the last check, edge counts, edge labels (caller indices, sides below zero) and spans
 // (kMaxVerts per site), and the bucket rings.
@@ -142,8 +159,9 @@ void emu_state(void* h, unsigned char* redo, int* nedge, int* elab, float* espan
 }
 
 // Evaluate the session's problem at w (caller order).  Outputs in caller order; any may be NULL.
-// stats[0] = conformity failures left, stats[1] = repair rounds, stats[2] = overflow cells,
-// stats[3] = largest bucket ring, stats[4] = cells rebuilt by repairs.
+// stats[0] = conformity failures left (plus one when the cells do not tile the target),
+// stats[1] = repair rounds, stats[2] = overflow cells, stats[3] = largest bucket ring,
+// stats[4] = cells rebuilt by repairs, stats[5] = 1 when the cell areas add up to the target's.
 int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen, double* geo,
              unsigned char* exist, int* stats) {
   Session& S = *static_cast<Session*>(h);
@@ -278,16 +296,23 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   // as the library: the cells just built are the next candidates; after a clean check the next
   // bucket ring is the outermost used ring plus one
   save_adjacency();
-  S.have_adj = true;
+  // as the library: cells that do not tile the target leave the evaluation unverified, and the
+  // next evaluation searches bucket rings
+  double area = 0.0;
+  for (int r = 0; r < n; ++r) area += S.area[r];
+  const bool covered = covers_domain(S.pd, area);
+  const bool multiple = fails == 0 && !covered;  // edges matched, target covered twice or more
+  S.have_adj = !multiple;
   if (fails == 0)
     for (int r = 0; r < n; ++r)
       S.ring[r] = (uint8_t)std::min(std::max((int)S.used[r] + 1, 1), S.ring_max);
   if (stats) {
-    stats[0] = fails;
+    stats[0] = fails + (multiple ? 1 : 0);
     stats[1] = rounds;
     stats[2] = overflow;
     stats[3] = rmax;
     stats[4] = rebuilt;
+    stats[5] = covered ? 1 : 0;
   }
   for (int r = 0; r < n; ++r) {
     const int o = S.order[r];

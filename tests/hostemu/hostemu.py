@@ -27,6 +27,7 @@ def lib():
         _lib.emu_destroy.argtypes = [ctypes.c_void_p]
         _lib.emu_state.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
         _lib.emu_coarse_inverse.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _lib.emu_set_adjacency.argtypes = [ctypes.c_void_p] * 3
         _lib.emu_eval.restype = ctypes.c_int
         _lib.emu_eval.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p,
                                   ctypes.c_void_p]
@@ -58,15 +59,27 @@ class Session:
         out["centroid"] = np.zeros((n, 2))
         out["geo"] = np.zeros((n, 2))
         ex = np.zeros(n, dtype=np.uint8)
-        stats = np.zeros(5, dtype=np.int32)
+        stats = np.zeros(6, dtype=np.int32)
         lib().emu_eval(self.h, _P(w), _P(out["mass"]), _P(out["fcell"]), _P(out["centroid"]),
                        _P(out["geo"]), ex.ctypes.data_as(ctypes.c_void_p),
                        stats.ctypes.data_as(ctypes.c_void_p))
         out["exist"] = ex.astype(bool)
         out["f"] = float(out["fcell"].sum())
         (out["fails"], out["repairs"], out["overflow"], out["ring_max"],
-         out["rebuilt"]) = (int(x) for x in stats)
+         out["rebuilt"]) = (int(x) for x in stats[:5])
+        out["covered"] = bool(stats[5])
         return out
+
+    def set_adjacency(self, neighbours):
+        """The next evaluation's candidates: neighbours[i] lists site i's (caller indices), as
+        if a previous evaluation's cells had had these edges."""
+        K = int(lib().emu_max_verts())
+        counts = np.array([len(v) for v in neighbours], dtype=np.int32)
+        labels = np.full((self.n, K), -1, dtype=np.int32)
+        for i, v in enumerate(neighbours):
+            labels[i, :len(v)] = v
+        lib().emu_set_adjacency(self.h, counts.ctypes.data_as(ctypes.c_void_p),
+                                labels.ctypes.data_as(ctypes.c_void_p))
 
     def state(self):
         """Per-site state after the last evaluation: conformity flags, edge counts, edge labels

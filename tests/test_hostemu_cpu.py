@@ -170,3 +170,31 @@ def test_coarse_inverse_wide_band_and_disconnected_banded():
     b[:50, 50:] = 0.0
     b[50:, :50] = 0.0
     assert not E.coarse_inverse(b).any()
+
+
+def test_cover_check_rejects_a_consistent_double_cover():
+    """The last step of the conformity argument (edges matched => the cells cover the target a
+    constant number of times; sum of areas = |D| => once; power_diagram.cpp:8-134 semantics:
+    the cells tile the domain).  Four sites on y = 1/2 whose candidates pair them (A-B, C-D):
+    each pair tiles the square with one matched bisector edge, so the edge check finds nothing,
+    but the square is covered twice.  The cover check flags the evaluation, and the next one
+    (bucket-ring candidates) tiles the square."""
+    from caustic_amd.workloads import Problem
+
+    xs = [0.125, 0.375, 0.625, 0.875]
+    p = Problem(sites=np.array([[x, 0.5] for x in xs]), nu=np.full(4, 0.25),
+                rho=np.full((8, 8), 1.0 / 64))
+    s = E.Session(p, False, 2, 12, 1e-4)
+    try:
+        s.set_adjacency([[1], [0], [3], [2]])
+        out = s.evaluate(np.zeros(4))
+        st = s.state()
+        # every bisector edge present on both of its cells: nothing for the edge check
+        assert not st["redo"].any() and out["repairs"] == 0
+        np.testing.assert_allclose(out["mass"], [0.25, 0.75, 0.75, 0.25], atol=1e-14)
+        assert not out["covered"] and out["fails"] >= 1
+        out2 = s.evaluate(np.zeros(4))
+        assert out2["covered"] and out2["fails"] == 0
+        np.testing.assert_allclose(out2["mass"], [0.25] * 4, atol=1e-14)
+    finally:
+        s.close()
