@@ -15,6 +15,7 @@ COT_OK, COT_EINVAL, COT_ENOMEM, COT_EHIP, COT_ENOTCONV, COT_ECOMPLEX, COT_EUNVER
 COT_PREC_FP64, COT_PREC_MIXED = 0, 1
 COT_STOP_REFERENCE, COT_STOP_MAXREL, COT_STOP_HANDOFF = 0, 1, 2
 COT_MAX_VERTS = 48  # vertex slots per cell in cot_cell_polygons (include/caustic_ot.h)
+COT_NOALLOC_MAX_PIXELS = 1 << 20  # Sobel image limit of cot_irra_render_grad_noalloc
 
 STATUS_NAMES = {
     COT_OK: "ok", COT_EINVAL: "invalid argument", COT_ENOMEM: "out of device memory",
@@ -115,6 +116,12 @@ SIGNATURES = {
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
                                             ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
                              + [ctypes.c_void_p] * 7 + [ctypes.c_int] + [ctypes.c_void_p] * 3),
+    "cot_irra_render_grad_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_float, ctypes.c_void_p, _dp,
+                                                    ctypes.c_int, ctypes.c_int]
+                                     + [ctypes.c_void_p] * 7 + [ctypes.c_int]
+                                     + [ctypes.c_void_p] * 3),
     "cot_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cot_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }

@@ -5,6 +5,10 @@
 // nu_i |C_i n p| / |C_i|, the pieces found by clipping the cell (FP64 vertices of the last
 // evaluation) against each pixel of its bounding box (Sutherland-Hodgman) and accumulated with
 // FP64 atomics.  One site per lane; a diagnostic, not a hot path.
+#include <mutex>
+#include <string>
+
+#include "../../include/caustic_ot.h"
 #include "cot_internal.h"
 
 namespace cot {
@@ -449,3 +453,102 @@ hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const d
 }
 
 }  // namespace cot
+
+// ---------------------------------------------------------------------------------------------
+// cot_irra_render_grad_noalloc: render_grad with no device allocation
+// ---------------------------------------------------------------------------------------------
+
+namespace cot {
+
+int set_error(int code, const std::string& msg);
+
+namespace {
+
+// The entry point's device scratch, a file-scope array that each GPU holds from the module's
+// load to the end of the process: bytes [0, kScratchHead) hold the count of face indices that
+// name no point, the rest the Sobel weight image (at most COT_NOALLOC_MAX_PIXELS floats).  In
+// host code `scratch` is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (scratch_on_current_device).
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char
+    scratch[kScratchHead + sizeof(float) * (size_t)COT_NOALLOC_MAX_PIXELS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// The scratch's address on the current device (looked up on the first call there).
+int scratch_on_current_device(unsigned char** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(COT_EHIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices) return set_error(COT_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return set_error(COT_EHIP, "hipGetSymbolAddress(scratch) failed");
+    g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return COT_OK;
+}
+
+__global__ void clear_count_kernel(int* count) { *count = 0; }
+
+// count += the face indices outside [0, n_points)
+__global__ void face_check_kernel(const int* faces, size_t n3, int n_points, int* count) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3) return;
+  const int v = faces[k];
+  if (v < 0 || v >= n_points) atomicAdd(count, 1);
+}
+
+}  // namespace
+}  // namespace cot
+
+extern "C" int cot_irra_render_grad_noalloc(
+    const float* tri_energy, const int* faces, int n_faces, const float* xy, int n_points,
+    float total_gray, const float* real_picture, const double* proj_domain, int pm, int pn,
+    float* render, float* render_diff, float* xy_render_grad, float* tri_render_grad,
+    float* inte_diff, float* xy_inte_grad, float* inte_result, int use_sobel,
+    float* xy_sobel_grad, float* tri_sobel_grad, void* stream) {
+  using namespace cot;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!real_picture || !render || !render_diff || n_faces < 0 || n_points < 0 || pm <= 0 ||
+      pn <= 0 || !proj_domain || (n_faces > 0 && (!tri_energy || !faces || !xy)) ||
+      (n_points > 0 && (!xy_render_grad || !xy_inte_grad || (use_sobel && !xy_sobel_grad))) ||
+      (n_faces > 0 && (!tri_render_grad || !inte_diff || !inte_result ||
+                       (use_sobel && !tri_sobel_grad))))
+    return set_error(COT_EINVAL, "bad argument");
+  if (!(proj_domain[1] > proj_domain[0] && proj_domain[3] > proj_domain[2]))
+    return set_error(COT_EINVAL, "empty proj_domain");
+  if (use_sobel && (size_t)pm * (size_t)pn > (size_t)COT_NOALLOC_MAX_PIXELS)
+    return set_error(COT_EINVAL, "image of " + std::to_string((size_t)pm * pn) +
+                                     " pixels exceeds the allocation-free limit of " +
+                                     std::to_string(COT_NOALLOC_MAX_PIXELS));
+  unsigned char* base = nullptr;
+  const int rc = scratch_on_current_device(&base);
+  if (rc != COT_OK) return rc;
+  int* count = reinterpret_cast<int*>(base);
+  float* sob = use_sobel ? reinterpret_cast<float*>(base + kScratchHead) : nullptr;
+  // every vertex index must name a point before a kernel dereferences it: counted on the
+  // device, the count read back into a host variable
+  if (n_faces > 0) {
+    const size_t n3 = 3 * (size_t)n_faces;
+    int bad = 0;
+    hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(1), 0, st, count);
+    hipLaunchKernelGGL(face_check_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                       faces, n3, n_points, count);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad, count, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(COT_EHIP, "face check failed");
+    if (bad > 0) return set_error(COT_EINVAL, "face index out of range");
+  }
+  if (launch_tri_grad(tri_energy, faces, n_faces, xy, n_points, total_gray, real_picture,
+                      proj_domain, pm, pn, render, render_diff, xy_render_grad, tri_render_grad,
+                      inte_diff, xy_inte_grad, inte_result, sob, xy_sobel_grad, tri_sobel_grad,
+                      st) != hipSuccess)
+    return set_error(COT_EHIP, "render_grad launch failed");
+  return COT_OK;
+}

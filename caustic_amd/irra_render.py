@@ -44,10 +44,12 @@ def _p(t):
 
 
 def render_grad(compute_tri_area, faces, proj_xy, total_gray_value, real_picture, if_use_sobel,
-                proj_domain, pic_size):
+                proj_domain, pic_size, no_alloc=False):
     """[render_result, render_diff, render_proj_xy_grad, render_tri_area_grad, inte_diff,
     proj_xy_inte_grad, inte_result, sobel_proj_xy_grad, sobel_tri_area_grad], as the
-    reference returns them (render.cu:716-727)."""
+    reference returns them (render.cu:716-727).  no_alloc: the library allocates no device
+    memory for the call (cot_irra_render_grad_noalloc; Sobel images up to
+    COT_NOALLOC_MAX_PIXELS pixels); the outputs are still allocated here."""
     import torch
 
     sobel = bool(torch.as_tensor(if_use_sobel).reshape(-1)[0].item())
@@ -64,7 +66,8 @@ def render_grad(compute_tri_area, faces, proj_xy, total_gray_value, real_picture
     lib = L.load()
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream(dev).cuda_stream
-        rc = lib.cot_irra_render_grad(
+        fn = lib.cot_irra_render_grad_noalloc if no_alloc else lib.cot_irra_render_grad
+        rc = fn(
             _p(tri), _p(fc), int(F), _p(xy), int(P), gray, _p(real),
             dom.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), H, W, _p(out[0]), _p(out[1]),
             _p(out[2]), _p(out[3]), _p(out[4]), _p(out[5]), _p(out[6]), int(sobel), _p(out[7]),

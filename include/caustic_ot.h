@@ -291,6 +291,24 @@ int cot_irra_render_grad(const float* tri_energy, const int* faces, int n_faces,
                          int use_sobel, float* xy_sobel_grad, float* tri_sobel_grad,
                          void* stream);
 
+/* cot_irra_render_grad with no device allocation: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, outputs and errors as cot_irra_render_grad;
+ * the Sobel weight image lives in a fixed device array of the library (each GPU holds
+ * COT_NOALLOC_MAX_PIXELS floats plus 256 bytes from the library's load to the end of the
+ * process), so with use_sobel pm * pn may not exceed COT_NOALLOC_MAX_PIXELS (COT_EINVAL
+ * otherwise).  The face indices are checked on the device and the count read back (the call
+ * waits for the stream once, as cot_irra_render_grad does).  Calls on one GPU share that
+ * array: calls that overlap on one GPU (other host threads, other streams) race. */
+#define COT_NOALLOC_MAX_PIXELS (1 << 20)
+int cot_irra_render_grad_noalloc(const float* tri_energy, const int* faces, int n_faces,
+                                 const float* xy, int n_points, float total_gray,
+                                 const float* real_picture, const double* proj_domain, int pm,
+                                 int pn, float* render, float* render_diff, float* xy_render_grad,
+                                 float* tri_render_grad, float* inte_diff, float* xy_inte_grad,
+                                 float* inte_result, int use_sobel, float* xy_sobel_grad,
+                                 float* tri_sobel_grad, void* stream);
+
 /* Gaussian splatting of n ray hits (points, (x, y) pairs inside proj_domain, finite) onto a
  * res_y x res_x image (pixel (x, y) at y * res_x + x): _gaussian_splat_raw_chunk,
  * utils/basic_function.py:674-709.  Each point spreads energy over the (2 radius + 1)^2 pixels
