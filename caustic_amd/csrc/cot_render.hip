@@ -186,17 +186,22 @@ __global__ void tri_sobel_kernel(const float* diff, int pm, int pn, float* w) {
 
 namespace {
 
-// Parameter range [t0, t1] of the segment a + t d (t in [0, 1]) inside the rectangle
-// [x0, x1] x [y0, y1] (Liang-Barsky); false when it misses.
+// Parameter range [t0, t1] of the segment a + t d (t in [0, 1]) inside the pixel
+// [x0, x1) x [y0, y1) (Liang-Barsky); false when it misses.  The pixel is half-open -- closed on
+// the image's last column (close_x) and row (close_y) -- so that a segment lying on a grid line
+// belongs to one pixel only: the boundary transport then sums over the pixels to the
+// triangle's (the derivative for moving the segment towards +x or +y).
 __device__ bool segment_in_rect(double ax, double ay, double dxs, double dys, double x0, double x1,
-                                double y0, double y1, double& t0, double& t1) {
+                                double y0, double y1, bool close_x, bool close_y, double& t0,
+                                double& t1) {
   t0 = 0.0;
   t1 = 1.0;
   const double p[4] = {-dxs, dxs, -dys, dys};
   const double q[4] = {ax - x0, x1 - ax, ay - y0, y1 - ay};
+  const bool closed[4] = {true, close_x, true, close_y};
   for (int k = 0; k < 4; ++k) {
     if (p[k] == 0.0) {
-      if (q[k] < 0.0) return false;
+      if (q[k] < 0.0 || (q[k] == 0.0 && !closed[k])) return false;
       continue;
     }
     const double r = q[k] / p[k];
@@ -279,8 +284,8 @@ __global__ void tri_grad_kernel(const float* energy, const int* faces, int n_fac
       for (int k = 0; k < 3; ++k) {
         const int j = k == 2 ? 0 : k + 1;
         double t0, t1;
-        if (!segment_in_rect(px[k], py[k], px[j] - px[k], py[j] - py[k], cx0, cx1, cy0, cy1, t0,
-                             t1))
+        if (!segment_in_rect(px[k], py[k], px[j] - px[k], py[j] - py[k], cx0, cx1, cy0, cy1,
+                             ix == pm - 1, iy == pn - 1, t0, t1))
           continue;
         const double i0 = t1 - t0, i1 = 0.5 * (t1 * t1 - t0 * t0);
         dax[k] += (i0 - i1) * nx[k];
@@ -290,11 +295,15 @@ __global__ void tri_grad_kernel(const float* energy, const int* faces, int n_fac
         any = true;
       }
       if (!(a > 0.0) && !any) continue;
+      // the per-face integral takes the face's orientation, as the reference's signed clipped
+      // areas do (compute_grad_area, render_exe_program/clip.h:277-292): negative for a
+      // clockwise face, and so are its derivatives; the render terms below use a / A, where
+      // the sign cancels
       const double rp = (double)real[(size_t)iy * pm + ix] / cell;
-      inte += a * rp;
+      inte += orient * a * rp;
       for (int k = 0; k < 3; ++k) {
-        gi[2 * k] += 2.0 * rp * dax[k];
-        gi[2 * k + 1] += 2.0 * rp * day[k];
+        gi[2 * k] += 2.0 * rp * orient * dax[k];
+        gi[2 * k + 1] += 2.0 * rp * orient * day[k];
       }
       if (A > 0.0) {
         const size_t q = (size_t)(pm - 1 - ix) * pn + iy;

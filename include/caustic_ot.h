@@ -271,14 +271,17 @@ int cot_irra_render(const float* tri_energy, const int* faces, int n_faces, cons
  *                         image - target;
  *   xy_render_grad (2 n_points), tri_render_grad (n_faces): d/dxy and d/dtri_energy of
  *                         sum_p (render_p - real_p)^2;
- *   inte_result, inte_diff (n_faces): inte_f = sum_p |T_f n p| real_p / |p| and
- *                         inte_f - tri_energy_f * total_gray;
+ *   inte_result, inte_diff (n_faces): inte_f = sum_p orient_f |T_f n p| real_p / |p| and
+ *                         inte_f - tri_energy_f * total_gray (orient_f = -1 for a clockwise
+ *                         face: the reference's signed clipped areas, compute_grad_area,
+ *                         render_exe_program/clip.h:277-292);
  *   xy_inte_grad (2 n_points): d/dxy of sum_f inte_diff_f^2;
  *   with use_sobel, xy_sobel_grad (2 n_points) and tri_sobel_grad (n_faces): d/dxy and
  *                         d/dtri_energy of the sum over 4-neighbouring pixel pairs of
  *                         (diff_p - diff_q)^2, diff = render - real (unused otherwise).
  * The vertex derivatives are exact where the clipped areas are differentiable (each is the
- * boundary transport of the triangle's edges through the pixel).  Device pointers; work
+ * boundary transport of the triangle's edges through the pixel; pixels are half-open, so an
+ * edge lying on a grid line counts once, with the derivative for moving it towards +x / +y).  Device pointers; work
  * ordered on stream; COT_EINVAL for a face index that names no point.  A Sobel call holds a
  * pm x pn float image of device memory for its duration.  The reference's Sobel weights read
  * past the image when pm > pn (row bound pm - 1, :77); here a neighbour outside the image is

@@ -206,20 +206,26 @@ def render(cells, nu, rho_shape, proj):
     return img.reshape(H, W)
 
 
-def irra_face_areas(points, faces, pm, pn, proj):
+def irra_face_areas(points, faces, pm, pn, proj, signed=False):
     """|T_f n p| for every face f of a triangle mesh lying inside proj and every pixel p of the
     pm x pn grid: [F, pn, pm] (pixel (ix, iy) at [f, iy, ix]), the pixel pieces of the
     irra_render extension's kernels (render_exe_program/render.cu:93-171, 422-542) by the same
-    Sutherland-Hodgman clipping as ``render``."""
+    Sutherland-Hodgman clipping as ``render``.  signed: each piece carries the face's
+    orientation (negative for a clockwise face), as computer_grad_inte sums them
+    (compute_grad_area, render_exe_program/clip.h:277-292: the fan area without abs, and
+    clipping keeps a polygon's orientation)."""
     pts = _f64(points)
     out = np.zeros((len(faces), pn, pm))
     one = np.zeros(1, dtype=np.int32)
     three = np.full(1, 3, dtype=np.int32)
     for f, tri in enumerate(faces):
         v = np.ascontiguousarray(pts[np.asarray(tri)].reshape(-1))
-        a = 0.5 * abs((v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1]))
+        s2 = (v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1])
+        a = 0.5 * abs(s2)
         if a > 0.0:
             out[f] = render((one, three, v), np.array([a]), (pn, pm), proj)
+            if signed and s2 < 0.0:
+                out[f] = -out[f]
     return out
 
 

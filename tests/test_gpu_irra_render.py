@@ -76,12 +76,14 @@ def test_render_triangle_partly_outside_and_bad_index(gpu_available):
 def _losses(pts, tri, e, gray, real, pm, pn, proj):
     """The oracle's render (grid layout [iy, ix]), sum_p (render - real)^2 and
     sum_f (inte_f - e_f gray)^2 for the mesh."""
-    a = O.irra_face_areas(pts, tri, pm, pn, proj)
+    a_s = O.irra_face_areas(pts, tri, pm, pn, proj, signed=True)  # a face's pieces share its sign
+    a = np.abs(a_s)
     u, v = pts[tri[:, 1]] - pts[tri[:, 0]], pts[tri[:, 2]] - pts[tri[:, 0]]
     A = 0.5 * np.abs(u[:, 0] * v[:, 1] - u[:, 1] * v[:, 0])
     img = np.tensordot(e * gray / A, a, axes=1)
     cell = (proj[1] - proj[0]) * (proj[3] - proj[2]) / (pm * pn)
-    inte = np.tensordot(a, real, axes=([1, 2], [0, 1])) / cell
+    # the per-face integrals with the faces' orientations (computer_grad_inte's signed areas)
+    inte = np.tensordot(a_s, real, axes=([1, 2], [0, 1])) / cell
     return img, float(np.sum((img - real) ** 2)), inte, float(np.sum((inte - e * gray) ** 2))
 
 
@@ -165,3 +167,96 @@ def test_render_grad_matches_finite_differences(gpu_available):
         e2[f] -= 1e-4
         gSE[f] = (smooth(pts, e1) - smooth(pts, e2)) / 2e-4
     np.testing.assert_allclose(sob[8], gSE, rtol=0, atol=2e-4 * np.abs(gSE).max())
+
+
+def test_render_grad_inverted_faces_use_signed_integrals(gpu_available):
+    """Faces wound clockwise (the pipeline produces them: its proj_area_loss penalises inverted
+    projected triangles): the image and its derivatives do not depend on a face's winding, the
+    per-face integral inte_f, inte_diff and the derivative of sum_f inte_diff_f^2 take the
+    reference's signed clipped areas (computer_grad_inte, render_exe_program/render.cu:422-542
+    with compute_grad_area, clip.h:277-292): inte_f of a reversed face is minus its
+    counter-clockwise value, and the gradient matches central differences of the oracle's
+    signed integrals."""
+    import torch
+
+    from caustic_amd import irra_render
+    from scipy.spatial import Delaunay
+
+    print("[case] irra_render render_grad, every third face reversed", flush=True)
+    rng = np.random.default_rng(13)
+    pts = rng.uniform(0.05, 0.95, (24, 2)).astype(np.float32).astype(np.float64)
+    tri0 = Delaunay(pts).simplices.astype(np.int32)
+    tri = tri0.copy()
+    tri[::3, 1], tri[::3, 2] = tri0[::3, 2], tri0[::3, 1]
+    e = rng.uniform(0.5, 1.5, tri.shape[0]).astype(np.float32).astype(np.float64)
+    gray, pm, pn = 0.75, 12, 10
+    proj = (0.0, 1.0, 0.0, 1.0)
+    real = rng.uniform(0.0, 3.0, (pn, pm)).astype(np.float32).astype(np.float64)
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float32: torch.tensor(a, dtype=dt, device=dev)  # noqa: E731
+
+    def run(faces):
+        out = irra_render.render_grad(t(e), t(faces, torch.int32), t(pts), t([gray]),
+                                      t(real.reshape(-1)), t([False], torch.bool), t(proj),
+                                      t([pm, pn]))
+        return [o.cpu().numpy().astype(np.float64) for o in out]
+
+    ccw, out = run(tri0), run(tri)
+    rev = np.zeros(tri.shape[0], dtype=bool)
+    rev[::3] = True
+    img, _, inte, _ = _losses(pts, tri, e, gray, real, pm, pn, proj)
+    np.testing.assert_allclose(out[0], ccw[0], rtol=0, atol=1e-6 * np.abs(ccw[0]).max())
+    np.testing.assert_allclose(out[2], ccw[2], rtol=0, atol=1e-5 * np.abs(ccw[2]).max())
+    np.testing.assert_allclose(out[6][rev], -ccw[6][rev], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(out[6][~rev], ccw[6][~rev], rtol=1e-6, atol=1e-9)
+    assert (out[6][rev] < 0).all() and (inte[rev] < 0).all()
+    np.testing.assert_allclose(out[6], inte, rtol=1e-5)
+    np.testing.assert_allclose(out[4], inte - e * gray, rtol=0, atol=1e-5 * np.abs(inte).max())
+    h = 1e-6
+    gJ = np.zeros_like(pts)
+    for i in range(pts.shape[0]):
+        for c in range(2):
+            p1, p2 = pts.copy(), pts.copy()
+            p1[i, c] += h
+            p2[i, c] -= h
+            gJ[i, c] = (_losses(p1, tri, e, gray, real, pm, pn, proj)[3]
+                        - _losses(p2, tri, e, gray, real, pm, pn, proj)[3]) / (2 * h)
+    np.testing.assert_allclose(out[5], gJ, rtol=0, atol=2e-3 * np.abs(gJ).max())
+
+
+def test_render_grad_edge_on_a_grid_line(gpu_available):
+    """Two triangles sharing an edge that lies on the interior grid line x = 1/2 (a mesh aligned
+    with the image): each point of the edge belongs to one pixel, so the boundary transport is
+    counted once and the x-derivatives of the edge's vertices are the one-sided derivatives for
+    moving them towards +x (forward differences of the FP64 oracle); the y-derivatives, where
+    the edge's normal has no component, match central differences."""
+    import torch
+
+    from caustic_amd import irra_render
+
+    print("[case] irra_render render_grad, edge on the grid line x = 1/2", flush=True)
+    pts = np.array([[0.5, 0.2], [0.5, 0.8], [0.2, 0.55], [0.85, 0.4]])
+    tri = np.array([[0, 1, 2], [0, 3, 1]], dtype=np.int32)
+    e = np.array([0.6, 1.1])
+    gray, pm, pn = 1.0, 4, 5
+    proj = (0.0, 1.0, 0.0, 1.0)
+    rng = np.random.default_rng(17)
+    real = rng.uniform(0.0, 3.0, (pn, pm)).astype(np.float32).astype(np.float64)
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float32: torch.tensor(a, dtype=dt, device=dev)  # noqa: E731
+    out = irra_render.render_grad(t(e), t(tri, torch.int32), t(pts), t([gray]),
+                                  t(real.reshape(-1)), t([False], torch.bool), t(proj),
+                                  t([pm, pn]))
+    g = out[2].cpu().numpy().astype(np.float64)
+    L = lambda p_: _losses(p_, tri, e, gray, real, pm, pn, proj)[1]  # noqa: E731
+    h = 1e-7
+    for i in (0, 1):
+        p1 = pts.copy()
+        p1[i, 0] += h
+        fwd = (L(p1) - L(pts)) / h
+        assert abs(g[i, 0] - fwd) <= 1e-3 * max(abs(fwd), 1e-3), (i, g[i, 0], fwd)
+        p1, p2 = pts.copy(), pts.copy()
+        p1[i, 1] += 1e-6
+        p2[i, 1] -= 1e-6
+        ctr = (L(p1) - L(p2)) / 2e-6
+        assert abs(g[i, 1] - ctr) <= 1e-3 * max(abs(ctr), 1e-3), (i, g[i, 1], ctr)
