@@ -269,9 +269,12 @@ struct Moments {
 
 // Exact contribution of one edge piece inside pixel (ix, iy) (local pixel units) to
 // oint F dV for F = int rho, int rho*U, int rho*U^2 along U, and V*F0, V^2*F0.
-// TP = double: everything in FP64.  TP = float: the polynomial terms of the piece are evaluated
-// in FP32 in coordinates local to the pixel (u, v in [0, 1]); the row prefixes A0..A2, the
-// integer offsets and every accumulation stay FP64.
+// TP = double: everything in FP64.  TP = float: the polynomial terms of the first and second
+// moments are evaluated in FP32 in coordinates local to the pixel (u, v in [0, 1]); the mass
+// term, the row prefixes A0..A2, the integer offsets and every accumulation stay FP64 (the
+// edges' contributions to a cell's mass cancel in part -- Green's theorem -- which would
+// amplify FP32 rounding of the pixel-local coordinates past 1e-6 of the cell's mass in long
+// cells; four FP64 operations per piece in a latency-bound walk cost nothing measurable).
 template <typename TP>
 __host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, double V1, int ix,
                                                int iy, Prefix& px, const PrefixTaps& taps,
@@ -295,7 +298,7 @@ __host__ __device__ __forceinline__ void piece(double U0, double V0, double U1, 
     const float Ivv = (v0 * v0 + v0 * v1 + v1 * v1) * (1.0f / 3.0f) * dvf;        // int v^2
     const float Ivu = (v0 * u0 + 4.0f * vm * um + v1 * u1) * (1.0f / 6.0f) * dvf;
     const float Ivvu = (v0 * v0 * u0 + 4.0f * vm * vm * um + v1 * v1 * u1) * (1.0f / 6.0f) * dvf;
-    const double pm = a0 * dv + (double)(r * Iu);  // int F0 dV over the piece
+    const double pm = a0 * dv + rv * (0.5 * ((U0 - fx) + (U1 - fx)) * dv);  // int F0 dV
     M.m0.add(pm);
     M.m1u += a1 * dv + fx * (double)(r * Iu) + 0.5 * (double)(r * Iuu);
     const double vF0 = a0 * (double)Iv + (double)(r * Ivu);  // int v F0 dV

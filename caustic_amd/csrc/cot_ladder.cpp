@@ -215,8 +215,9 @@ void cot_default_ladder_opts(const cot_problem* problems, int n_problems, cot_la
     }
     if (P && mn < 1e-3 * s / (double)P) sparse = true;
   }
-  const double half = 0.5 * std::sqrt((double)std::max(nmin, 1));
-  for (int side = 16; side <= half && lo->n_levels < COT_LADDER_MAX; side *= 2) {
+  // each coarse level holds at most half the smallest problem's sites
+  for (int side = 16; 2.0 * side * side <= (double)nmin && lo->n_levels < COT_LADDER_MAX;
+       side *= 2) {
     lo->sides[lo->n_levels] = side;
     lo->floors[lo->n_levels] = sparse ? 0.1 * std::pow(10.0, -0.5 * lo->n_levels) : 0.0;
     ++lo->n_levels;

@@ -46,6 +46,18 @@ def _problem_array(problems):
     return arr, keep
 
 
+def ladder_defaults(problems):
+    """The library's default coarse-to-fine schedule for these problems
+    (cot_default_ladder_opts): {"sides", "floors", "final_floors"}."""
+    arr, keep = _problem_array(list(problems))
+    lo = L.cot_ladder_opts()
+    L.load().cot_default_ladder_opts(arr, len(problems), ctypes.byref(lo))
+    del keep
+    return {"sides": [lo.sides[k] for k in range(lo.n_levels)],
+            "floors": [lo.floors[k] for k in range(lo.n_levels)],
+            "final_floors": [lo.final_floors[k] for k in range(lo.n_final_floors)]}
+
+
 class OTBatch:
     """A context of one or more OT problems on the current GPU."""
 

@@ -216,8 +216,8 @@ typedef struct {
                                        defaults at the context's precision */
 } cot_ladder_opts;
 
-/* Defaults for these problems: levels 16, 32, 64, ... while the side is at most half the
- * smallest problem's sqrt(n); floors only for a target whose smallest pixel is below 1e-3 of
+/* Defaults for these problems: levels 16, 32, 64, ... while a level holds at most half the
+ * smallest problem's sites (side^2 <= n_min / 2); floors only for a target whose smallest pixel is below 1e-3 of
  * its mean (zero-density regions): 0.1 at the first level, divided by sqrt(10) per level, and one
  * full-resolution stage at 1e-3; coarse_tol 1e-2, handoff 1. */
 void cot_default_ladder_opts(const cot_problem* problems, int n_problems, cot_ladder_opts* lo);

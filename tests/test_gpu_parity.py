@@ -21,10 +21,15 @@ def _exact(p, w):
 
 
 def _compare(ev_gpu, ev_orc, nu, mass_tol, cen_tol):
-    """Masses within mass_tol relative to max(mass, nu) -- the cell's own mass, floored at its
-    source mass so that slivers are judged against the mass scale of the problem; centroids of
-    cells holding at least 1e-3 nu within cen_tol."""
-    dm = np.abs(ev_gpu["mass"] - ev_orc.mass) / np.maximum(ev_orc.mass, nu)
+    """Masses within mass_tol relative to each cell's own mass for every cell holding at least
+    a tenth of its source mass, and relative to max(mass, nu) for all (slivers judged against
+    the mass scale of the problem); centroids of cells holding at least 1e-3 nu within
+    cen_tol."""
+    err = np.abs(ev_gpu["mass"] - ev_orc.mass)
+    held = ev_orc.mass >= 0.1 * nu
+    own = (err[held] / ev_orc.mass[held]).max() if held.any() else 0.0
+    assert own <= mass_tol, f"mass error {own:.3e} (relative to the cell's own mass)"
+    dm = err / np.maximum(ev_orc.mass, nu)
     assert dm.max() <= mass_tol, f"mass error {dm.max():.3e} (relative to max(mass, nu))"
     ex = ev_gpu["exist"]
     assert np.array_equal(ex, ev_orc.exist.astype(bool))
