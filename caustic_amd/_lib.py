@@ -104,6 +104,12 @@ SIGNATURES = {
                                          _dp]),
     "cot_centroids": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _up]),
     "cot_cell_areas": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cot_evaluate_dev": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_void_p]),
+    "cot_solve_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(cot_solve_opts), ctypes.POINTER(cot_report),
+                                     ctypes.c_void_p]),
+    "cot_centroids_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "cot_cell_solid_angles": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]),
     "cot_irra_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, _dp,

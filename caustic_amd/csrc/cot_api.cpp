@@ -51,6 +51,9 @@ hipError_t launch_accept(const BlockDesc* b, const uint8_t* active, const double
                          double* part, int nblocks, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
 hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t st);
+// kind 0: gather f64 (caller -> sorted); 1: scatter f64; 2: scatter f64 pairs; 3: scatter u8
+hipError_t launch_permute(int kind, const void* src, const int* perm, size_t n, void* dst,
+                          hipStream_t st);
 hipError_t launch_dev_count(const DevState* st, int np, int* out, hipStream_t stream);
 hipError_t launch_dev_direction(const ProbDesc* probs, const BlockDesc* blocks, DevState* st,
                                 const SolveParams& sp, const double* S, const double* Y,
@@ -191,6 +194,12 @@ struct cot_context {
   std::vector<ProbDesc> probs;
   std::vector<BlockDesc> blocks;
   std::vector<int> perm;  // sorted global index -> caller global index
+  // the entry points on device arrays (cot_*_dev): inside one of them the caller's arrays are
+  // device memory in caller order, moved by gather / scatter kernels over d_perm; the events
+  // order the context's stream after the caller's and the caller's after the context's
+  bool io_device = false;
+  int* d_perm = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::vector<void*> allocs;
   // device
   ProbDesc* d_probs = nullptr;
@@ -286,6 +295,8 @@ struct cot_context {
     if (h_ismall) hipHostFree(h_ismall);
     if (h_active) hipHostFree(h_active);
     if (h_ndone) hipHostFree(h_ndone);
+    if (ev_in) hipEventDestroy(ev_in);
+    if (ev_out) hipEventDestroy(ev_out);
     if (st) hipStreamDestroy(st);
   }
 };
@@ -535,12 +546,20 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
 
 // Scatter caller-order host values into sorted order and upload.
 int upload_sorted(cot_context* c, const double* src, double* dst) {
+  if (c->io_device) {
+    HIPCHK(launch_permute(0, src, c->d_perm, c->ntot, dst, c->st));
+    return COT_OK;
+  }
   for (int r = 0; r < c->ntot; ++r) c->h_vec[r] = src[c->perm[r]];
   HIPCHK(hipMemcpyAsync(dst, c->h_vec, sizeof(double) * c->ntot, hipMemcpyHostToDevice, c->st));
   return COT_OK;
 }
 
 int download_sorted(cot_context* c, const double* src, double* dst) {
+  if (c->io_device) {
+    HIPCHK(launch_permute(1, src, c->d_perm, c->ntot, dst, c->st));
+    return COT_OK;
+  }
   HIPCHK(hipMemcpyAsync(c->h_vec, src, sizeof(double) * c->ntot, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   for (int r = 0; r < c->ntot; ++r) dst[c->perm[r]] = c->h_vec[r];
@@ -548,6 +567,10 @@ int download_sorted(cot_context* c, const double* src, double* dst) {
 }
 
 int download_sorted2(cot_context* c, const double2* src, double* dst) {
+  if (c->io_device) {
+    HIPCHK(launch_permute(2, src, c->d_perm, c->ntot, dst, c->st));
+    return COT_OK;
+  }
   std::vector<double> tmp(2 * (size_t)c->ntot);
   HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(double2) * c->ntot, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -559,6 +582,10 @@ int download_sorted2(cot_context* c, const double2* src, double* dst) {
 }
 
 int download_sorted_u8(cot_context* c, const uint8_t* src, unsigned char* dst) {
+  if (c->io_device) {
+    HIPCHK(launch_permute(3, src, c->d_perm, c->ntot, dst, c->st));
+    return COT_OK;
+  }
   std::vector<uint8_t> tmp(c->ntot);
   HIPCHK(hipMemcpyAsync(tmp.data(), src, c->ntot, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1073,8 +1100,13 @@ int cot_evaluate(cot_context* c, const double* w, double* mass, double* f, doubl
   std::vector<EvalSum> sum;
   CHK(evaluate_device(c, c->d_w, nullptr, act, sum));
   if (mass) CHK(download_sorted(c, c->d_mass, mass));
-  if (f)
+  if (f && c->io_device) {
+    for (int p = 0; p < c->np; ++p) c->h_small[p] = sum[p].f;
+    HIPCHK(hipMemcpyAsync(f, c->h_small, sizeof(double) * c->np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));  // the pinned staging is reused
+  } else if (f) {
     for (int p = 0; p < c->np; ++p) f[p] = sum[p].f;
+  }
   if (centroid) CHK(download_sorted2(c, c->d_cen, centroid));
   if (exist) CHK(download_sorted_u8(c, c->d_exist, exist));
   for (int p = 0; p < c->np; ++p)
@@ -1935,6 +1967,70 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   }
   if (!all) return fail(COT_ENOTCONV, "Newton steps did not reach the tolerance");
   return COT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One call of an entry point on device arrays: the caller's arrays are device memory; the
+// context's stream starts after the work queued on the caller's stream and the caller's stream
+// continues after the context's.
+struct IoScope {
+  cot_context* c;
+  hipStream_t caller;
+  int rc = COT_OK;
+  IoScope(cot_context* ctx, void* stream) : c(ctx), caller(reinterpret_cast<hipStream_t>(stream)) {
+    if (!c->d_perm) {
+      if ((rc = dalloc(c, &c->d_perm, c->ntot)) != COT_OK) return;
+      if (hipMemcpy(c->d_perm, c->perm.data(), sizeof(int) * c->ntot, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        rc = fail(COT_EHIP, "permutation upload failed");
+        return;
+      }
+    }
+    if ((!c->ev_in && hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming) != hipSuccess) ||
+        (!c->ev_out && hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming) != hipSuccess) ||
+        hipEventRecord(c->ev_in, caller) != hipSuccess ||
+        hipStreamWaitEvent(c->st, c->ev_in, 0) != hipSuccess) {
+      rc = fail(COT_EHIP, "stream ordering failed");
+      return;
+    }
+    c->io_device = true;
+  }
+  ~IoScope() {
+    if (!c->io_device) return;
+    c->io_device = false;
+    if (hipEventRecord(c->ev_out, c->st) == hipSuccess) (void)hipStreamWaitEvent(caller, c->ev_out, 0);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int cot_evaluate_dev(cot_context* c, const double* w, double* mass, double* f, double* centroid,
+                     unsigned char* exist, void* stream) {
+  if (!c || !w) return fail(COT_EINVAL, "null argument");
+  IoScope io(c, stream);
+  if (io.rc != COT_OK) return io.rc;
+  return cot_evaluate(c, w, mass, f, centroid, exist);
+}
+
+int cot_solve_dev(cot_context* c, double* w, const cot_solve_opts* so, cot_report* reports,
+                  void* stream) {
+  if (!c || !w) return fail(COT_EINVAL, "null argument");
+  IoScope io(c, stream);
+  if (io.rc != COT_OK) return io.rc;
+  return cot_solve(c, w, so, reports);
+}
+
+int cot_centroids_dev(cot_context* c, const double* w, int weighted, double* centroid,
+                      unsigned char* exist, void* stream) {
+  if (!c || !w) return fail(COT_EINVAL, "null argument");
+  IoScope io(c, stream);
+  if (io.rc != COT_OK) return io.rc;
+  return cot_centroids(c, w, weighted, centroid, exist);
 }
 
 int cot_get_stats(cot_context* c, cot_stats* out) {

@@ -1185,6 +1185,28 @@ __global__ void fill_u8_kernel(uint8_t* p, size_t n, uint8_t v) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
 }
+// Caller order <-> the library's bucket order on the device (perm: sorted -> caller index),
+// for the entry points on device arrays.
+__global__ void gather_f64_kernel(const double* src, const int* perm, size_t n, double* dst) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[perm[r]];
+}
+__global__ void scatter_f64_kernel(const double* src, const int* perm, size_t n, double* dst) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[perm[r]] = src[r];
+}
+__global__ void scatter_f64x2_kernel(const double2* src, const int* perm, size_t n, double* dst) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const size_t o = (size_t)perm[r];
+  dst[2 * o] = src[r].x;
+  dst[2 * o + 1] = src[r].y;
+}
+__global__ void scatter_u8_kernel(const uint8_t* src, const int* perm, size_t n, uint8_t* dst) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[perm[r]] = src[r];
+}
+
 __global__ void fill_i32_kernel(int* p, size_t n, int v) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
@@ -1296,6 +1318,31 @@ hipError_t launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st) {
   hipLaunchKernelGGL(fill_u8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
+hipError_t launch_permute(int kind, const void* src, const int* perm, size_t n, void* dst,
+                          hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL(gather_f64_kernel, g, b, 0, st, static_cast<const double*>(src), perm, n,
+                         static_cast<double*>(dst));
+      break;
+    case 1:
+      hipLaunchKernelGGL(scatter_f64_kernel, g, b, 0, st, static_cast<const double*>(src), perm, n,
+                         static_cast<double*>(dst));
+      break;
+    case 2:
+      hipLaunchKernelGGL(scatter_f64x2_kernel, g, b, 0, st, static_cast<const double2*>(src), perm,
+                         n, static_cast<double*>(dst));
+      break;
+    default:
+      hipLaunchKernelGGL(scatter_u8_kernel, g, b, 0, st, static_cast<const uint8_t*>(src), perm, n,
+                         static_cast<uint8_t*>(dst));
+      break;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n,

@@ -145,6 +145,92 @@ class OTBatch:
         return {"mass": mass, "f": f, "centroid": cen, "exist": ex.astype(bool),
                 "verified": rc == L.COT_OK}
 
+    # ---- torch tensors on the GPU (cot_*_dev: device arrays, the caller's current stream) ----
+
+    def _tensor_w(self, w):
+        import torch
+
+        if w is None:
+            return torch.zeros(self.n_total, dtype=torch.float64, device="cuda")
+        if not (isinstance(w, torch.Tensor) and w.is_cuda):
+            raise TypeError("expected a CUDA (HIP) tensor of weights")
+        w = w.detach().reshape(-1).to(torch.float64).contiguous()
+        if w.numel() != self.n_total:
+            raise ValueError(f"expected {self.n_total} weights, got {w.numel()}")
+        return w
+
+    @staticmethod
+    def _stream(t):
+        import torch
+
+        return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+    def evaluate_tensor(self, w=None):
+        """evaluate() on GPU tensors: masses (N,), per-problem f (P,), rho-weighted centroids
+        (N, 2) float64 and existence (N,) uint8, all on w's device, ordered on the current
+        stream (cot_evaluate_dev)."""
+        import torch
+
+        w = self._tensor_w(w)
+        kw = dict(dtype=torch.float64, device=w.device)
+        mass, f = torch.empty(self.n_total, **kw), torch.empty(self.n_problems, **kw)
+        cen = torch.empty((self.n_total, 2), **kw)
+        ex = torch.empty(self.n_total, dtype=torch.uint8, device=w.device)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        L.check(self._lib.cot_evaluate_dev(self._ctx, p(w), p(mass), p(f), p(cen), p(ex),
+                                           self._stream(w)))
+        return {"mass": mass, "f": f, "centroid": cen, "exist": ex}
+
+    def centroids_tensor(self, w=None, weighted: bool = True):
+        """centroids() on GPU tensors (cot_centroids_dev): ((N, 2) float64, (N,) uint8)."""
+        import torch
+
+        w = self._tensor_w(w)
+        cen = torch.empty((self.n_total, 2), dtype=torch.float64, device=w.device)
+        ex = torch.empty(self.n_total, dtype=torch.uint8, device=w.device)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        L.check(self._lib.cot_centroids_dev(self._ctx, p(w), int(weighted), p(cen), p(ex),
+                                            self._stream(w)))
+        return cen, ex
+
+    def solve_tensor(self, w0=None, history: int = 10, max_iter: int = 50001, tol: float = 1e-5,
+                     initial_step: float = 1.0, stop: str = "maxrel", h0_threshold: float = 1e-5,
+                     accept_rule: str = "auto", device_loop=True, raise_on_fail: bool = True):
+        """L-BFGS from the tensor w0 (zeros: None) on the GPU (cot_solve_dev); returns (a new
+        float64 tensor of weights, reports)."""
+        import torch
+
+        w = self._tensor_w(w0).clone()
+        o = self._solve_opts(history, max_iter, tol, initial_step, stop, h0_threshold, 0,
+                             device_loop, accept_rule, 3)
+        reps = (L.cot_report * self.n_problems)()
+        rc = self._lib.cot_solve_dev(self._ctx, ctypes.c_void_p(w.data_ptr()), ctypes.byref(o),
+                                     reps, self._stream(w))
+        if raise_on_fail:
+            L.check(rc, allow=(L.COT_ENOTCONV,))
+        return w, [{k: getattr(r, k) for k, _ in L.cot_report._fields_} for r in reps]
+
+    def _solve_opts(self, history, max_iter, tol, initial_step, stop, h0_threshold, verbose,
+                    device_loop, accept_rule, repair_rounds):
+        if accept_rule == "auto":
+            accept_rule = "derivative" if self.precision == "mixed" else "decrease"
+        o = L.cot_solve_opts()
+        self._lib.cot_default_solve_opts(ctypes.byref(o))
+        o.history = history
+        o.max_iter = max_iter
+        o.tol = tol
+        o.initial_step = initial_step
+        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL,
+                       "handoff": L.COT_STOP_HANDOFF}[stop]
+        o.h0_threshold = h0_threshold
+        o.verbose = verbose
+        # True/1: graph-replayed device rounds; 2: the same rounds launched directly; False/0:
+        # host-driven rounds
+        o.device_loop = int(device_loop)
+        o.accept_rule = {"decrease": 0, "derivative": 1}[accept_rule]
+        o.repair_rounds = repair_rounds
+        return o
+
     def centroids(self, w=None, weighted: bool = True):
         w = self._w(w)
         cen = np.zeros((self.n_total, 2))
@@ -258,24 +344,9 @@ class OTBatch:
             if raise_on_fail and not all(r["converged"] for r in out):
                 raise RuntimeError("solve did not converge")
             return w3, out
-        if accept_rule == "auto":
-            accept_rule = "derivative" if self.precision == "mixed" else "decrease"
         w = self._w(w0).copy()
-        o = L.cot_solve_opts()
-        self._lib.cot_default_solve_opts(ctypes.byref(o))
-        o.history = history
-        o.max_iter = max_iter
-        o.tol = tol
-        o.initial_step = initial_step
-        o.stop_rule = {"reference": L.COT_STOP_REFERENCE, "maxrel": L.COT_STOP_MAXREL,
-                       "handoff": L.COT_STOP_HANDOFF}[stop]
-        o.h0_threshold = h0_threshold
-        o.verbose = verbose
-        # True/1: graph-replayed device rounds; 2: the same rounds launched directly; False/0:
-        # host-driven rounds
-        o.device_loop = int(device_loop) if not isinstance(device_loop, bool) else int(device_loop)
-        o.accept_rule = {"decrease": 0, "derivative": 1}[accept_rule]
-        o.repair_rounds = repair_rounds
+        o = self._solve_opts(history, max_iter, tol, initial_step, stop, h0_threshold, verbose,
+                             device_loop, accept_rule, repair_rounds)
         reps = (L.cot_report * self.n_problems)()
         rc = self._lib.cot_solve(self._ctx, _ptr(w), ctypes.byref(o), reps)
         if raise_on_fail:

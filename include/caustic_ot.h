@@ -2,8 +2,9 @@
  *
  * One context holds a batch of OT problems (one or many, uneven sizes) resident on one GPU:
  * sites, source masses nu, a piecewise-constant target image rho per problem, and the solver's
- * device state.  Every pointer argument is a caller-owned HOST buffer in the caller's site order
- * (the library keeps its own bucket-sorted order internally).  Calls return COT_OK or a status
+ * device state.  Pointer arguments are caller-owned HOST buffers in the caller's site order
+ * (the library keeps its own bucket-sorted order internally), except where an entry point says
+ * device memory (the *_dev variants and the renderers).  Calls return COT_OK or a status
  * code and never abort; cot_last_error() describes the last failure on the calling thread.
  *
  * Reference interfaces replaced (rmk523/caustic, Optimal_Transport_rl/):
@@ -244,6 +245,19 @@ int cot_hessian_product(cot_context* ctx, const double* w, const double* x, doub
  * weighted == 0 -> geometric centroid of the domain-clipped cell; exist per site. */
 int cot_centroids(cot_context* ctx, const double* w, int weighted, double* centroid,
                   unsigned char* exist);
+
+/* The same three operations on caller-owned DEVICE arrays (caller order; f: n_problems
+ * doubles), ordered on the caller's stream (a hipStream_t; NULL: the default stream): the work
+ * starts after what is queued there and what is queued there afterwards runs after it.  The
+ * library's own stream does the work; reports stay host structs.  Replace direct_ot's device
+ * buffers and copies (ot_op.cuh:66-70, 102-152, 539-588) for a caller whose data already lives
+ * on the GPU (a torch pipeline). */
+int cot_evaluate_dev(cot_context* ctx, const double* w, double* mass, double* f, double* centroid,
+                     unsigned char* exist, void* stream);
+int cot_solve_dev(cot_context* ctx, double* w, const cot_solve_opts* opt, cot_report* reports,
+                  void* stream);
+int cot_centroids_dev(cot_context* ctx, const double* w, int weighted, double* centroid,
+                      unsigned char* exist, void* stream);
 
 /* Area of each cell (inside proj_domain) at weights w. */
 int cot_cell_areas(cot_context* ctx, const double* w, double* area);
