@@ -205,6 +205,7 @@ struct CellArgs {
   // candidate source (kCand*) and the previous evaluation's adjacency
   int cand;
   int ring_max;
+  int repair_lane;          // repair construction: 0 one wave per listed site, 1 one lane per site
   const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)

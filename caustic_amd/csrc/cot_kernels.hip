@@ -877,6 +877,391 @@ __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   timer_mark(a.timer, worked, t0);
 }
 
+// ---------------------------------------------------------------------------------------------
+// wave-cooperative repair: one wave per listed site (FP64)
+// ---------------------------------------------------------------------------------------------
+//
+// The repair pass rebuilds few cells, so a lane per site leaves the chip idle while each lane
+// walks its candidates one after another, every candidate a chain of dependent loads.  Here a
+// wave builds one cell: vertex k of the polygon lives in lane k (at most kMaxVerts), candidates
+// are taken 64 at a time (one per lane, loaded together), a ballot keeps those whose half-plane
+// may cut the polygon's bounding circle as it stood when the batch was loaded -- the polygon only
+// shrinks, so a half-plane holding that circle holds the polygon -- and the kept ones are clipped
+// in lane order, each first tested against the current circle as the lane code tests it.  The
+// clip is cooperative: signed distances per lane, the inside/crossing pattern from two ballots,
+// output positions from their prefix counts, the new polygon redistributed through LDS.  The
+// candidate order (queued sites ascending, the neighbours, their neighbours, the bucket rings
+// cell by cell), the clipping arithmetic and the order of the area and centroid sums are those of
+// cell_site<double, kMaxVerts>, so both build the same cells.
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// v from lane l (l wave-uniform)
+__device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rdl(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// inclusive prefix sum over the wave
+__device__ __forceinline__ int wave_scan_i(int v) {
+  const int ln = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (ln >= off) v += u;
+  }
+  return v;
+}
+
+struct WaveLDS {
+  double x[kMaxVerts], y[kMaxVerts];  // clip output, staged
+  int lab[kMaxVerts];
+  int off[64];   // per-slot first candidate of a flattened candidate batch
+  int key[64];   // per-slot base (a neighbour's site, a bucket's first site)
+};
+
+// Largest slot s in [0, n) with W.off[s] <= f (W.off ascending, W.off[0] = 0).
+__device__ __forceinline__ int slot_of(const WaveLDS& W, int n, int f) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (W.off[mid] <= f) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Bucket (cx, cy) of entry idx of ring r around (bx, by), in the lane code's order: row by - r
+// left to right, the rows between (left cell, right cell), row by + r left to right.
+__device__ __forceinline__ void ring_cell(int r, int idx, int bx, int by, int& cx, int& cy) {
+  if (r == 0) {
+    cx = bx;
+    cy = by;
+    return;
+  }
+  const int row = 2 * r + 1;
+  if (idx < row) {
+    cx = bx - r + idx;
+    cy = by - r;
+    return;
+  }
+  const int m = idx - row;
+  if (m < 2 * (2 * r - 1)) {
+    cy = by - r + 1 + (m >> 1);
+    cx = (m & 1) ? bx + r : bx - r;
+    return;
+  }
+  cx = bx - r + (m - 2 * (2 * r - 1));
+  cy = by + r;
+}
+
+// Builds the cell of site slot (blk, site_ln) with the calling wave; returns as cell_site.
+__device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& W) {
+  const int ln = lane_id();
+  const BlockDesc bd = a.blocks[blk];
+  if (site_ln >= bd.count) return -1;
+  if (!a.active[bd.prob]) return -1;
+  if (a.gate && a.gate[bd.prob] == 0) return -1;
+  const int i = bd.first + site_ln;
+  if (!a.redo[i]) return -1;
+  const ProbDesc pd = a.probs[bd.prob];
+  double tstep = 0.0;
+  bool steep = false;
+  if (a.st) {
+    tstep = a.st[bd.prob].t;
+    steep = a.st[bd.prob].steep != 0;
+  } else if (a.dir) {
+    tstep = a.step[bd.prob];
+  }
+  const bool trial = a.dir != nullptr;
+  auto wtrial = [&](int j) {
+    if (!trial) return a.w[j];
+    return a.w[j] + tstep * (steep ? -a.g[j] : a.dir[j]);
+  };
+  const double2 s = a.xy[i];
+  const double wi = wtrial(i);
+  // FP64 frame: the site
+  const double2 o = s;
+  const double osx = o.x - s.x, osy = o.y - s.y;
+  // the target rectangle, lanes 0..3
+  double px = 0.0, py = 0.0;
+  int pl = 0;
+  if (ln == 0) { px = pd.x0 - o.x; py = pd.y0 - o.y; pl = kSideBottom; }
+  if (ln == 1) { px = pd.x1 - o.x; py = pd.y0 - o.y; pl = kSideRight; }
+  if (ln == 2) { px = pd.x1 - o.x; py = pd.y1 - o.y; pl = kSideTop; }
+  if (ln == 3) { px = pd.x0 - o.x; py = pd.y1 - o.y; pl = kSideLeft; }
+  int nv = 4;
+  bool overflow = false;
+
+  double bcx, bcy, br2;
+  auto bound = [&]() {
+    const bool on = ln < nv;
+    const double mnx = wave_min(on ? px : INFINITY), mxx = wave_max(on ? px : -INFINITY);
+    const double mny = wave_min(on ? py : INFINITY), mxy = wave_max(on ? py : -INFINITY);
+    bcx = 0.5 * (mnx + mxx);
+    bcy = 0.5 * (mny + mxy);
+    const double ex = px - bcx, ey = py - bcy;
+    br2 = wave_max(on ? ex * ex + ey * ey : 0.0);
+  };
+  bound();
+
+  // u.d <= c: keep the part of the polygon inside; the new edge gets `label`
+  auto clip = [&](double dx, double dy, double c, int label) -> bool {
+    const bool on = ln < nv;
+    const double sd = px * dx + py * dy - c;
+    const unsigned long long mo = __ballot(on && sd > 0.0);
+    const int nout = __popcll(mo);
+    if (nout == 0) return false;
+    if (nout == nv) {
+      nv = 0;
+      return true;
+    }
+    // cyclic transitions of the outside pattern
+    const unsigned long long rot = (mo >> 1) | ((mo & 1ull) << (nv - 1));
+    if (__popcll(mo ^ rot) != 2) return false;
+    if (nv + 1 > kMaxVerts) {
+      overflow = true;
+      return false;
+    }
+    const int nx = ln + 1 < nv ? ln + 1 : 0;
+    const double ex = __shfl(px, nx, 64), ey = __shfl(py, nx, 64), esd = __shfl(sd, nx, 64);
+    const bool sin_ = on && sd <= 0.0;
+    const bool cross = on && ((sd <= 0.0) != (esd <= 0.0));
+    const unsigned long long mi = __ballot(sin_), mc = __ballot(cross);
+    const unsigned long long below = (1ull << ln) - 1ull;
+    int pos = __popcll(mi & below) + __popcll(mc & below);
+    if (sin_) {
+      W.x[pos] = px;
+      W.y[pos] = py;
+      W.lab[pos] = pl;
+      ++pos;
+    }
+    if (cross) {
+      const double tt = sd / (sd - esd);
+      W.x[pos] = px + tt * (ex - px);
+      W.y[pos] = py + tt * (ey - py);
+      W.lab[pos] = sin_ ? label : pl;
+    }
+    const int out = __popcll(mi) + __popcll(mc);
+    __syncthreads();
+    if (ln < out) {
+      px = W.x[ln];
+      py = W.y[ln];
+      pl = W.lab[ln];
+    }
+    __syncthreads();
+    nv = out >= 3 ? out : 0;
+    return true;
+  };
+
+  int used = 0;
+  // one batch: lane l offers candidate j (-1: none) from ring r
+  auto batch = [&](int j, int r, bool dedupe) {
+    double ddx = 0.0, ddy = 0.0, c = 0.0;
+    bool keep = false;
+    if (j >= 0) {
+      const double2 sj = a.xy[j];
+      const double wj = wtrial(j);
+      ddx = sj.x - s.x;
+      ddy = sj.y - s.y;
+      c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj) - (osx * ddx + osy * ddy);
+      const double slack = c - (bcx * ddx + bcy * ddy);
+      keep = !(slack >= 0.0 && slack * slack >= br2 * (ddx * ddx + ddy * ddy));
+    }
+    unsigned long long m = __ballot(keep);
+    while (m != 0ull && nv > 0) {
+      const int t = __ffsll((long long)m) - 1;
+      m &= m - 1ull;
+      const double tdx = rdl(ddx, t), tdy = rdl(ddy, t), tc = rdl(c, t);
+      const int tj = rdl(j, t), tr = rdl(r, t);
+      const double slack = tc - (bcx * tdx + bcy * tdy);
+      if (slack >= 0.0 && slack * slack >= br2 * (tdx * tdx + tdy * tdy)) continue;
+      if (dedupe && __ballot(ln < nv && pl == tj) != 0ull) continue;
+      if (clip(tdx, tdy, tc, tj)) {
+        used = max(used, tr);
+        if (nv > 0) bound();
+      }
+    }
+  };
+
+  const int R0 = a.ring[i];
+  const bool fresh = a.st && a.st[bd.prob].fresh;
+  const int nadj = a.cand != kCandBucket && !fresh ? (int)a.adjn[i] : 0;
+  const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
+  const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
+  if (a.xcnt) {
+    // the queued candidates, ascending and without repeats
+    const int nx = min(a.xcnt[i], kMaxExtra);
+    const int v = ln < nx ? a.xcand[(size_t)i * kMaxExtra + ln] : INT32_MAX;
+    bool first = ln < nx;
+    for (int q = 0; q < nx; ++q) {
+      const int u = __shfl(v, q, 64);
+      if (q < ln && u == v) first = false;
+    }
+    int rank = 0;
+    for (int q = 0; q < nx; ++q) {
+      const int u = __shfl(v, q, 64);
+      const int fq = __shfl((int)first, q, 64);
+      if (fq && u < v) ++rank;
+    }
+    const int nu = __popcll(__ballot(first));
+    if (first) W.key[rank] = v;
+    __syncthreads();
+    const int j = ln < nu ? W.key[ln] : -1;
+    __syncthreads();
+    if (nu > 0) batch(j, 0, true);
+    if (ln == 0) a.xcnt[i] = 0;
+  }
+  if (nadj > 0) {
+    const int* row = a.adj + (size_t)i * kMaxVerts;
+    const bool queued = a.xcnt != nullptr;
+    const int jn = ln < nadj ? row[ln] : -1;
+    if (nv > 0) batch(jn >= 0 && jn != i ? jn : -1, 0, queued);
+    if (a.cand == kCandAdj2 && nv > 0) {
+      // the neighbours' neighbours, flattened neighbour by neighbour
+      const int cnt = jn >= 0 ? (int)a.adjn[jn] : 0;
+      const int inc = wave_scan_i(cnt);
+      const int total = __shfl(inc, 63, 64);
+      W.off[ln] = inc - cnt;
+      W.key[ln] = jn;
+      __syncthreads();
+      for (int base = 0; base < total && nv > 0; base += 64) {
+        const int f = base + ln;
+        int jj = -1;
+        if (f < total) {
+          const int k = slot_of(W, nadj, f);
+          jj = a.adj[(size_t)W.key[k] * kMaxVerts + (f - W.off[k])];
+          if (jj == i) jj = -1;
+        }
+        batch(jj, 0, true);
+      }
+      __syncthreads();
+    }
+    // the bucket ring that holds every neighbour of the cell
+    int u = 0;
+    if (ln < nv && pl >= 0) {
+      const double2 sj = a.xy[pl];
+      const int qx = min(max((int)floor((sj.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
+      const int qy = min(max((int)floor((sj.y - pd.by0) * pd.binv), 0), pd.gy - 1);
+      u = max(abs(qx - bx), abs(qy - by));
+    }
+    used = max(used, wave_max_i(u));
+  } else {
+    const int R = a.cand == kCandAdj2 ? min(2 * R0 + 1, a.ring_max) : R0;
+    const int2* bk = a.buckets + pd.bkt_off;
+    for (int r = 0; r <= R && nv > 0; ++r) {
+      const int nb = r == 0 ? 1 : 8 * r;
+      for (int c0 = 0; c0 < nb && nv > 0; c0 += 64) {
+        int cnt = 0, start = 0;
+        if (c0 + ln < nb) {
+          int cx, cy;
+          ring_cell(r, c0 + ln, bx, by, cx, cy);
+          if (cx >= 0 && cx < pd.gx && cy >= 0 && cy < pd.gy) {
+            const int2 b = bk[cy * pd.gx + cx];
+            start = b.x;
+            cnt = b.y;
+          }
+        }
+        const int inc = wave_scan_i(cnt);
+        const int total = __shfl(inc, 63, 64);
+        W.off[ln] = inc - cnt;
+        W.key[ln] = start;
+        __syncthreads();
+        for (int base = 0; base < total && nv > 0; base += 64) {
+          const int f = base + ln;
+          int j = -1;
+          if (f < total) {
+            const int k = slot_of(W, 64, f);
+            j = pd.site_off + W.key[k] + (f - W.off[k]);
+            if (j == i) j = -1;
+          }
+          batch(j, r, false);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (ln == 0) {
+    a.ring_used[i] = (uint8_t)min(used, 255);
+    a.nedge[i] = (uint8_t)nv;
+  }
+
+  double ar = 0.0;
+  double2 geo = make_double2(0.0, 0.0);
+  int2 kb = make_int2(0, 0);
+  if (nv >= 3) {
+    const bool on = ln < nv;
+    const int nx = ln + 1 < nv ? ln + 1 : 0;
+    const double qx = __shfl(px, nx, 64), qy = __shfl(py, nx, 64);
+    const double cr = px * qy - qx * py;
+    const double tx = (px + qx) * cr, ty = (py + qy) * cr;
+    // sums in edge order
+    double A2 = 0.0, gx = 0.0, gy = 0.0;
+    for (int k = 0; k < nv; ++k) {
+      A2 += rdl(cr, k);
+      gx += rdl(tx, k);
+      gy += rdl(ty, k);
+    }
+    const double mnu = wave_min(on ? px : INFINITY), mnv = wave_min(on ? py : INFINITY);
+    if (on) {
+      float2 span = make_float2(0.f, 0.f);
+      if (pl >= 0) {
+        const int lo = min(i, pl), hi = max(i, pl);
+        const double2 sl = a.xy[lo], sh = a.xy[hi];
+        const double Dx = sh.x - sl.x, Dy = sh.y - sl.y;
+        const double inv = 1.0 / sqrt(Dx * Dx + Dy * Dy);
+        const double ex = -Dy * inv, ey = Dx * inv;
+        const double mx = 0.5 * (sl.x + sh.x), my = 0.5 * (sl.y + sh.y);
+        const double t0 = (o.x + px - mx) * ex + (o.y + py - my) * ey;
+        const double t1 = (o.x + qx - mx) * ex + (o.y + qy - my) * ey;
+        span = make_float2((float)fmin(t0, t1), (float)fmax(t0, t1));
+      }
+      a.elab[(size_t)i * kMaxVerts + ln] = pl;
+      a.espan[(size_t)i * kMaxVerts + ln] = span;
+      a.vtx[(size_t)i * kMaxVerts + ln] = make_double2(o.x + px, o.y + py);
+    }
+    ar = 0.5 * A2;
+    geo = A2 != 0.0 ? make_double2(o.x + gx / (3.0 * A2), o.y + gy / (3.0 * A2)) : o;
+    const int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
+    const int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
+    kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
+  }
+  if (ln == 0) {
+    a.kbox[i] = kb;
+    a.area[i] = ar;
+    a.geo[i] = geo;
+    a.exist[i] = nv >= 3 ? 1 : 0;
+  }
+  return overflow ? 1 : 0;
+}
+
+// Repair construction with one wave (block) per listed site; any grid size.
+__global__ __launch_bounds__(64) void cell_repair_wave_kernel(CellArgs a) {
+  __shared__ WaveLDS W;
+  const unsigned long long t0 = timer_begin();
+  const int n = *a.list_count;
+  bool worked = false;
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int slot = a.list[e];
+    const int blk = slot / kCellBlock;
+    const int rc = cell_site_wave(a, blk, slot % kCellBlock, W);
+    if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
+    worked = worked || rc >= 0;
+  }
+  timer_mark(a.timer, worked, t0);
+}
+
 // Integration pass.  Grid (cell blocks, kCellBlock * L / kIntBlock), block kIntBlock: L
 // consecutive lanes per site; the lanes' moments are summed with shuffles within their aligned
 // group.
@@ -1237,9 +1622,13 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st) {
   if (a.redo_mode) {
     // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
-    // of the cells the FP32 pass could not resolve), on a small grid over the work list
-    hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(repair_blocks), dim3(kCellBlock), 0, st,
-                       a);
+    // of the cells the FP32 pass could not resolve), on a small grid over the work list: one
+    // wave per site (4 waves per repair block's worth of lanes), or one lane per site
+    if (a.repair_lane)
+      hipLaunchKernelGGL((cell_repair_kernel<double>), dim3(repair_blocks), dim3(kCellBlock), 0,
+                         st, a);
+    else
+      hipLaunchKernelGGL(cell_repair_wave_kernel, dim3(4 * repair_blocks), dim3(64), 0, st, a);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   } else {

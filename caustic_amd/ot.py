@@ -64,10 +64,11 @@ class OTBatch:
     def __init__(self, problems: Sequence, precision: str = "fp64", sites_per_bucket: float = 1.0,
                  ring_init: int = 2, ring_max: int = 32, verify_tol: float = 1e-4,
                  max_repairs: int = 12, int_lanes: int = 8, repair_blocks: int = 256,
-                 full_pass_ring: int = 0, tuning: dict = None):
+                 full_pass_ring: int = 0, tuning: dict = None, repair_mode: str = "wave"):
         """tuning: launch-shape options as chosen by caustic_amd.autotune (int_lanes,
         repair_blocks, sites_per_bucket, full_pass_ring); they override the keyword
-        arguments."""
+        arguments.  repair_mode: "wave" (one wave per repaired cell) or "lane" (one lane per
+        cell), the same cells either way."""
         if tuning:
             int_lanes = int(tuning.get("int_lanes", int_lanes))
             repair_blocks = int(tuning.get("repair_blocks", repair_blocks))
@@ -88,6 +89,7 @@ class OTBatch:
         opt.int_lanes = int_lanes
         opt.repair_blocks = repair_blocks
         opt.full_pass_ring = full_pass_ring
+        opt.repair_mode = {"wave": 0, "lane": 1}[repair_mode]
         ctx = ctypes.c_void_p()
         L.check(lib.cot_create(arr, len(problems), ctypes.byref(opt), ctypes.byref(ctx)))
         self._keep = []  # the library copied everything

@@ -71,6 +71,8 @@ typedef struct {
   int full_pass_ring;       /* candidates of a FP32-geometry full pass: the previous cells'
                                one-ring (0: default, or 1) or two-ring (2); FP64 always uses
                                the two-ring */
+  int repair_mode;          /* construction of the repair passes: 0 one wave per listed cell
+                               (default), 1 one lane per cell; both build the same cells */
 } cot_options;
 
 typedef struct {
