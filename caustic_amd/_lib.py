@@ -37,7 +37,8 @@ class cot_options(ctypes.Structure):
                 ("ring_init", ctypes.c_int), ("ring_max", ctypes.c_int),
                 ("verify_tol", ctypes.c_double), ("max_repairs", ctypes.c_int),
                 ("int_lanes", ctypes.c_int), ("repair_blocks", ctypes.c_int),
-                ("full_pass_ring", ctypes.c_int), ("repair_mode", ctypes.c_int)]
+                ("full_pass_ring", ctypes.c_int), ("repair_mode", ctypes.c_int),
+                ("pass_mode", ctypes.c_int)]
 
 
 class cot_solve_opts(ctypes.Structure):

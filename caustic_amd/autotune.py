@@ -18,6 +18,11 @@ Knobs (fields of cot_options):
                     two-ring (2).  The two-ring costs more clipping per cell and leaves fewer
                     unmatched edges to the repair passes; it pays where weights move far between
                     evaluations.
+  pass_mode         construction of the full passes: one lane per cell (0) or one wave per cell
+                    (1, candidates loaded 64 at a time, cooperative clipping).  The same cells
+                    either way; one wave per cell puts many more waves in flight and wins on
+                    small problems (a few thousand cells), where one lane per cell leaves most
+                    of the chip idle.
 
 Objective: device time per full evaluation -- construction, integration and repair kernels, read
 from the in-kernel timers (cot_get_stats) -- over a stretch of the real solve: `iters` L-BFGS
@@ -47,8 +52,10 @@ KNOBS = {
     "repair_blocks": (128, 256, 512, 1024),
     "sites_per_bucket": (1.0, 2.0),
     "full_pass_ring": (1, 2),
+    "pass_mode": (0, 1),
 }
-DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 1}
+DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 1,
+           "pass_mode": 0}
 # a change must win by 5%: shapes within a few percent of each other (8 or 16 integration lanes,
 # 256 or 512 repair blocks) trade places from run to run, and each flip sends the solve down a
 # different (equally fast on average) trajectory, so near-ties keep the incumbent

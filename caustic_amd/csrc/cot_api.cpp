@@ -389,6 +389,7 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.org = c->d_org;
   ca.ring_max = c->opt.ring_max;
   ca.repair_lane = c->opt.repair_mode;
+  ca.pass_wave = c->opt.pass_mode;
   ca.adj = c->d_adj;
   ca.adjn = c->d_adjn;
   ca.xcnt = c->d_xcnt;
@@ -885,6 +886,7 @@ void cot_default_options(cot_options* o) {
   o->repair_blocks = kRepairBlocks;
   o->full_pass_ring = 0;
   o->repair_mode = 0;
+  o->pass_mode = 0;
 }
 
 void cot_default_solve_opts(cot_solve_opts* o) {
@@ -941,6 +943,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     return bail(fail(COT_EINVAL, "full_pass_ring must be 0, 1 or 2"));
   if (c->opt.repair_mode != 0 && c->opt.repair_mode != 1)
     return bail(fail(COT_EINVAL, "repair_mode must be 0 or 1"));
+  if (c->opt.pass_mode != 0 && c->opt.pass_mode != 1)
+    return bail(fail(COT_EINVAL, "pass_mode must be 0 or 1"));
   c->fp32 = c->opt.precision == COT_PREC_MIXED;
   c->np = n_problems;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)

@@ -59,6 +59,8 @@ constexpr double kMarginRel = 1e-3;
 // in increasing site order, so the result does not depend on the order they were queued in
 // unless more than kMaxExtra arrive).
 constexpr int kMaxExtra = 24;
+// Edges the conformity check reads per batch (their loads issued together).
+constexpr int kVerifyBatch = 8;
 // Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
 // folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
 // the next kernel on the stream closes the launch: [2 * kTimerSlots] accumulates the ticks from
@@ -206,6 +208,7 @@ struct CellArgs {
   int cand;
   int ring_max;
   int repair_lane;          // repair construction: 0 one wave per listed site, 1 one lane per site
+  int pass_wave;            // full construction: 0 one lane per site, 1 one wave per site
   const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)

@@ -105,6 +105,7 @@ struct PolyLDS {
 template <typename T, int V>
 __host__ __device__ int clip_halfplane(PolyLDS<T, V>& P, int ln, int n, T dx, T dy, T c,
                                        int label, bool& overflow, bool& changed, T& sdmin) {
+#pragma clang fp contract(on)
   int nout = 0, trans = 0;
   {
     T sd_first = P.x[0][ln] * dx + P.y[0][ln] * dy - c;
@@ -419,6 +420,9 @@ __host__ __device__ void walk_edge(double ua, double va, double ub, double vb, i
 template <typename T, int V>
 __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyLDS<T, V>& P,
                                   int ln) {
+  // multiply-adds fused only within one expression, so that the arithmetic is fixed by the
+  // source and the wave construction below reproduces it bit for bit
+#pragma clang fp contract(on)
   // site_ln: the site's lane in its cell block; ln: the LDS column of the polygon
   const BlockDesc bd = a.blocks[blk];
   if (site_ln >= bd.count) return -1;
@@ -925,15 +929,17 @@ __device__ __forceinline__ int wave_scan_i(int v) {
   return v;
 }
 
+template <typename T>
 struct WaveLDS {
-  double x[kMaxVerts], y[kMaxVerts];  // clip output, staged
+  T x[kMaxVerts], y[kMaxVerts];  // clip output, staged
   int lab[kMaxVerts];
   int off[64];   // per-slot first candidate of a flattened candidate batch
   int key[64];   // per-slot base (a neighbour's site, a bucket's first site)
 };
 
 // Largest slot s in [0, n) with W.off[s] <= f (W.off ascending, W.off[0] = 0).
-__device__ __forceinline__ int slot_of(const WaveLDS& W, int n, int f) {
+template <typename W_>
+__device__ __forceinline__ int slot_of(const W_& W, int n, int f) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -966,15 +972,22 @@ __device__ __forceinline__ void ring_cell(int r, int idx, int bx, int by, int& c
   cy = by + r;
 }
 
-// Builds the cell of site slot (blk, site_ln) with the calling wave; returns as cell_site.
-__device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& W) {
+// One site built by the calling wave (lane k holds vertex k of the polygon, V at most); returns
+// as cell_site<T, V>, and builds the same cell: the same candidates in the same order, the same
+// clipping arithmetic in T, FP32 vertices refined in FP64 from their two constraint lines, and
+// the same hand-offs to the FP64 repair.
+template <typename T, int V>
+__device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T>& W) {
+#pragma clang fp contract(on)
+  static_assert(V <= 64, "one vertex per lane");
   const int ln = lane_id();
   const BlockDesc bd = a.blocks[blk];
   if (site_ln >= bd.count) return -1;
   if (!a.active[bd.prob]) return -1;
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + site_ln;
-  if (!a.redo[i]) return -1;
+  if (a.redo_mode && !a.redo[i]) return -1;
+  if (!a.redo_mode && a.redo && ln == 0) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   double tstep = 0.0;
   bool steep = false;
@@ -991,53 +1004,58 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
   };
   const double2 s = a.xy[i];
   const double wi = wtrial(i);
-  // FP64 frame: the site
-  const double2 o = s;
+  constexpr bool kF32 = sizeof(T) == 4;
+  // clipping frame: the site (FP64) or the cell's last centroid (FP32)
+  const double2 o = kF32 ? a.org[i] : s;
   const double osx = o.x - s.x, osy = o.y - s.y;
   // the target rectangle, lanes 0..3
-  double px = 0.0, py = 0.0;
+  T px = T(0), py = T(0);
   int pl = 0;
-  if (ln == 0) { px = pd.x0 - o.x; py = pd.y0 - o.y; pl = kSideBottom; }
-  if (ln == 1) { px = pd.x1 - o.x; py = pd.y0 - o.y; pl = kSideRight; }
-  if (ln == 2) { px = pd.x1 - o.x; py = pd.y1 - o.y; pl = kSideTop; }
-  if (ln == 3) { px = pd.x0 - o.x; py = pd.y1 - o.y; pl = kSideLeft; }
+  if (ln == 0) { px = (T)(pd.x0 - o.x); py = (T)(pd.y0 - o.y); pl = kSideBottom; }
+  if (ln == 1) { px = (T)(pd.x1 - o.x); py = (T)(pd.y0 - o.y); pl = kSideRight; }
+  if (ln == 2) { px = (T)(pd.x1 - o.x); py = (T)(pd.y1 - o.y); pl = kSideTop; }
+  if (ln == 3) { px = (T)(pd.x0 - o.x); py = (T)(pd.y1 - o.y); pl = kSideLeft; }
   int nv = 4;
   bool overflow = false;
+  bool coarse = false;
 
   double bcx, bcy, br2;
   auto bound = [&]() {
     const bool on = ln < nv;
-    const double mnx = wave_min(on ? px : INFINITY), mxx = wave_max(on ? px : -INFINITY);
-    const double mny = wave_min(on ? py : INFINITY), mxy = wave_max(on ? py : -INFINITY);
+    const double dx_ = (double)px, dy_ = (double)py;
+    const double mnx = wave_min(on ? dx_ : INFINITY), mxx = wave_max(on ? dx_ : -INFINITY);
+    const double mny = wave_min(on ? dy_ : INFINITY), mxy = wave_max(on ? dy_ : -INFINITY);
     bcx = 0.5 * (mnx + mxx);
     bcy = 0.5 * (mny + mxy);
-    const double ex = px - bcx, ey = py - bcy;
+    const double ex = dx_ - bcx, ey = dy_ - bcy;
     br2 = wave_max(on ? ex * ex + ey * ey : 0.0);
   };
   bound();
 
-  // u.d <= c: keep the part of the polygon inside; the new edge gets `label`
-  auto clip = [&](double dx, double dy, double c, int label) -> bool {
+  // u.d <= c: keep the part of the polygon inside; the new edge gets `label`.  Returns whether
+  // the polygon changed; sdmin: when it was emptied, the smallest u.d - c over its vertices.
+  auto clip = [&](T dx, T dy, T c, int label, double& sdmin) -> bool {
     const bool on = ln < nv;
-    const double sd = px * dx + py * dy - c;
-    const unsigned long long mo = __ballot(on && sd > 0.0);
+    const T sd = px * dx + py * dy - c;
+    const unsigned long long mo = __ballot(on && sd > T(0));
     const int nout = __popcll(mo);
     if (nout == 0) return false;
     if (nout == nv) {
+      if (kF32) sdmin = wave_min(on ? (double)sd : INFINITY);
       nv = 0;
       return true;
     }
     // cyclic transitions of the outside pattern
     const unsigned long long rot = (mo >> 1) | ((mo & 1ull) << (nv - 1));
     if (__popcll(mo ^ rot) != 2) return false;
-    if (nv + 1 > kMaxVerts) {
+    if (nv + 1 > V) {
       overflow = true;
       return false;
     }
     const int nx = ln + 1 < nv ? ln + 1 : 0;
-    const double ex = __shfl(px, nx, 64), ey = __shfl(py, nx, 64), esd = __shfl(sd, nx, 64);
-    const bool sin_ = on && sd <= 0.0;
-    const bool cross = on && ((sd <= 0.0) != (esd <= 0.0));
+    const T ex = __shfl(px, nx, 64), ey = __shfl(py, nx, 64), esd = __shfl(sd, nx, 64);
+    const bool sin_ = on && sd <= T(0);
+    const bool cross = on && ((sd <= T(0)) != (esd <= T(0)));
     const unsigned long long mi = __ballot(sin_), mc = __ballot(cross);
     const unsigned long long below = (1ull << ln) - 1ull;
     int pos = __popcll(mi & below) + __popcll(mc & below);
@@ -1048,12 +1066,13 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
       ++pos;
     }
     if (cross) {
-      const double tt = sd / (sd - esd);
+      const T tt = sd / (sd - esd);
       W.x[pos] = px + tt * (ex - px);
       W.y[pos] = py + tt * (ey - py);
       W.lab[pos] = sin_ ? label : pl;
     }
     const int out = __popcll(mi) + __popcll(mc);
+    if (kF32 && out < 3) sdmin = wave_min(on ? (double)sd : INFINITY);
     __syncthreads();
     if (ln < out) {
       px = W.x[ln];
@@ -1088,9 +1107,14 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
       const double slack = tc - (bcx * tdx + bcy * tdy);
       if (slack >= 0.0 && slack * slack >= br2 * (tdx * tdx + tdy * tdy)) continue;
       if (dedupe && __ballot(ln < nv && pl == tj) != 0ull) continue;
-      if (clip(tdx, tdy, tc, tj)) {
+      double sdmin = 0.0;
+      if (clip((T)tdx, (T)tdy, (T)tc, tj, sdmin)) {
         used = max(used, tr);
-        if (nv > 0) bound();
+        if (nv > 0) {
+          bound();
+        } else if (kF32 && sdmin < kMarginRel * pd.bsize * sqrt(tdx * tdx + tdy * tdy)) {
+          coarse = true;  // emptied by a margin FP32 cannot resolve: may be a sliver
+        }
       }
     }
   };
@@ -1202,18 +1226,62 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
   int2 kb = make_int2(0, 0);
   if (nv >= 3) {
     const bool on = ln < nv;
+    // vertex k in FP64: as clipped (FP64 geometry), or the intersection of the constraint lines
+    // of edges k - 1 and k (FP32 geometry; kept as clipped where that is ill-conditioned)
+    double vx = (double)px, vy = (double)py;
+    if (kF32) {
+      double lx, ly, lc;  // the line of edge k (label pl)
+      if (pl >= 0) {
+        const double2 sj = a.xy[on ? pl : i];
+        lx = sj.x - s.x;
+        ly = sj.y - s.y;
+        lc = 0.5 * (lx * lx + ly * ly + wi - wtrial(on ? pl : i)) - (osx * lx + osy * ly);
+      } else if (pl == kSideBottom) {
+        lx = 0.0; ly = -1.0; lc = o.y - pd.y0;
+      } else if (pl == kSideRight) {
+        lx = 1.0; ly = 0.0; lc = pd.x1 - o.x;
+      } else if (pl == kSideTop) {
+        lx = 0.0; ly = 1.0; lc = pd.y1 - o.y;
+      } else if (pl == kSideLeft) {
+        lx = -1.0; ly = 0.0; lc = o.x - pd.x0;
+      } else {
+        lx = ly = lc = 0.0;
+      }
+      const int pv = ln > 0 ? ln - 1 : nv - 1;
+      const double ax = __shfl(lx, pv, 64), ay = __shfl(ly, pv, 64), ac = __shfl(lc, pv, 64);
+      const double mnx = wave_min(on ? vx : INFINITY), mxx = wave_max(on ? vx : -INFINITY);
+      const double mny = wave_min(on ? vy : INFINITY), mxy = wave_max(on ? vy : -INFINITY);
+      const double ext2 = (mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny);
+      bool bad = false;
+      const double det = ax * ly - ay * lx;
+      const double sc = sqrt((ax * ax + ay * ay) * (lx * lx + ly * ly));
+      if (!(fabs(det) > 1e-7 * sc)) {
+        bad = true;
+      } else {
+        const double rx = (ac * ly - lc * ay) / det, ry = (ax * lc - lx * ac) / det;
+        const double ex = rx - vx, ey = ry - vy;
+        if (ex * ex + ey * ey > 1e-6 * ext2) {
+          bad = true;
+        } else {
+          vx = rx;
+          vy = ry;
+        }
+      }
+      if (__ballot(on && bad) != 0ull) coarse = true;
+    }
     const int nx = ln + 1 < nv ? ln + 1 : 0;
-    const double qx = __shfl(px, nx, 64), qy = __shfl(py, nx, 64);
-    const double cr = px * qy - qx * py;
-    const double tx = (px + qx) * cr, ty = (py + qy) * cr;
-    // sums in edge order
+    const double qx = __shfl(vx, nx, 64), qy = __shfl(vy, nx, 64);
+    const double cr = vx * qy - qx * vy;
+    const double sx = vx + qx, sy = vy + qy;
+    // sums in edge order, each term's product fused into the running sum as in cell_site
     double A2 = 0.0, gx = 0.0, gy = 0.0;
     for (int k = 0; k < nv; ++k) {
-      A2 += rdl(cr, k);
-      gx += rdl(tx, k);
-      gy += rdl(ty, k);
+      const double crk = rdl(cr, k);
+      A2 += crk;
+      gx += rdl(sx, k) * crk;
+      gy += rdl(sy, k) * crk;
     }
-    const double mnu = wave_min(on ? px : INFINITY), mnv = wave_min(on ? py : INFINITY);
+    const double mnu = wave_min(on ? vx : INFINITY), mnv = wave_min(on ? vy : INFINITY);
     if (on) {
       float2 span = make_float2(0.f, 0.f);
       if (pl >= 0) {
@@ -1223,22 +1291,32 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
         const double inv = 1.0 / sqrt(Dx * Dx + Dy * Dy);
         const double ex = -Dy * inv, ey = Dx * inv;
         const double mx = 0.5 * (sl.x + sh.x), my = 0.5 * (sl.y + sh.y);
-        const double t0 = (o.x + px - mx) * ex + (o.y + py - my) * ey;
+        const double t0 = (o.x + vx - mx) * ex + (o.y + vy - my) * ey;
         const double t1 = (o.x + qx - mx) * ex + (o.y + qy - my) * ey;
         span = make_float2((float)fmin(t0, t1), (float)fmax(t0, t1));
       }
       a.elab[(size_t)i * kMaxVerts + ln] = pl;
       a.espan[(size_t)i * kMaxVerts + ln] = span;
-      a.vtx[(size_t)i * kMaxVerts + ln] = make_double2(o.x + px, o.y + py);
+      a.vtx[(size_t)i * kMaxVerts + ln] = make_double2(o.x + vx, o.y + vy);
     }
     ar = 0.5 * A2;
     geo = A2 != 0.0 ? make_double2(o.x + gx / (3.0 * A2), o.y + gy / (3.0 * A2)) : o;
     const int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
     const int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
     kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
+    if (kF32 && ln == 0) a.org[i] = geo;  // next frame: this centroid
+    if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
+  // a full pass whose polygon overflowed V skipped a constraint: the repair pass rebuilds the
+  // cell in its larger polygon
+  const bool handoff = overflow && V < kMaxVerts && !a.redo_mode && a.coarse;
+  if (handoff) overflow = false;
   if (ln == 0) {
     a.kbox[i] = kb;
+    if (((kF32 && coarse) || handoff) && a.coarse) {
+      a.redo[i] = 2;  // the check that follows lists it
+      atomicAdd(&a.coarse[bd.prob], 1);
+    }
     a.area[i] = ar;
     a.geo[i] = geo;
     a.exist[i] = nv >= 3 ? 1 : 0;
@@ -1246,16 +1324,28 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS& 
   return overflow ? 1 : 0;
 }
 
-// Repair construction with one wave (block) per listed site; any grid size.
+// Full construction pass with one wave (block) per site: block b builds site b % kCellBlock of
+// cell block b / kCellBlock.
+template <typename T>
+__global__ __launch_bounds__(64) void cell_wave_kernel(CellArgs a) {
+  __shared__ WaveLDS<T> W;
+  const unsigned long long t0 = timer_begin();
+  const int blk = blockIdx.x / kCellBlock;
+  const int rc = cell_site_wave<T, kPassVerts>(a, blk, blockIdx.x % kCellBlock, W);
+  if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
+  timer_mark(a.timer, rc >= 0, t0);
+}
+
+// Repair construction (FP64) with one wave (block) per listed site; any grid size.
 __global__ __launch_bounds__(64) void cell_repair_wave_kernel(CellArgs a) {
-  __shared__ WaveLDS W;
+  __shared__ WaveLDS<double> W;
   const unsigned long long t0 = timer_begin();
   const int n = *a.list_count;
   bool worked = false;
   for (int e = blockIdx.x; e < n; e += gridDim.x) {
     const int slot = a.list[e];
     const int blk = slot / kCellBlock;
-    const int rc = cell_site_wave(a, blk, slot % kCellBlock, W);
+    const int rc = cell_site_wave<double, kMaxVerts>(a, blk, slot % kCellBlock, W);
     if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
     worked = worked || rc >= 0;
   }
@@ -1366,31 +1456,53 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
     a.list[fetch_add(a.list_count, 1)] = slot;
   };
   if (a.redo[i] == 2) mark(i);  // an FP32 cell handed to the FP64 repair
-  for (int k = 0; k < ne; ++k) {
-    const int j = a.elab[(size_t)i * kMaxVerts + k];
-    if (j < 0) continue;
-    const float2 sp = a.espan[(size_t)i * kMaxVerts + k];
-    const double2 sj = a.xy[j];
-    const double ddx = sj.x - si.x, ddy = sj.y - si.y;
-    const double tol = a.tol_rel * sqrt(ddx * ddx + ddy * ddy);
-    const double len = (double)sp.y - (double)sp.x;
-    const int nj = a.nedge[j];
-    bool found = false, match = false;
-    for (int kk = 0; kk < nj; ++kk) {
-      if (a.elab[(size_t)j * kMaxVerts + kk] == i) {
-        const float2 sq = a.espan[(size_t)j * kMaxVerts + kk];
-        found = true;
-        match = fabs((double)sp.x - (double)sq.x) + fabs((double)sp.y - (double)sq.y) <= tol;
-        break;
-      }
+  // edges kVerifyBatch at a time: their labels and spans, then the neighbours' sites and edge
+  // counts, each round's loads in flight together; then, edge by edge in order, the neighbour's
+  // labels kVerifyBatch at a time until the reciprocal edge turns up
+  constexpr int B = kVerifyBatch;
+  for (int k0 = 0; k0 < ne; k0 += B) {
+    int jj[B], nj[B];
+    float2 sp[B];
+    double2 sj[B];
+    for (int u = 0; u < B; ++u) {
+      const bool in = k0 + u < ne;
+      jj[u] = in ? a.elab[(size_t)i * kMaxVerts + k0 + u] : -1;
+      sp[u] = in ? a.espan[(size_t)i * kMaxVerts + k0 + u] : make_float2(0.f, 0.f);
     }
-    if (found ? !match : (len > tol)) {
-      bad = true;
-      mark(j);
-      if (!found && a.xcnt) {
-        // j's cell lacks the edge: i is a candidate j missed
-        const int q = fetch_add(&a.xcnt[j], 1);
-        if (q < kMaxExtra) a.xcand[(size_t)j * kMaxExtra + q] = i;
+    for (int u = 0; u < B; ++u) {
+      const int j = jj[u];
+      sj[u] = j >= 0 ? a.xy[j] : make_double2(0.0, 0.0);
+      nj[u] = j >= 0 ? (int)a.nedge[j] : 0;
+    }
+    for (int u = 0; u < B; ++u) {
+      const int j = jj[u];
+      if (j < 0) continue;
+      const double ddx = sj[u].x - si.x, ddy = sj[u].y - si.y;
+      const double tol = a.tol_rel * sqrt(ddx * ddx + ddy * ddy);
+      const double len = (double)sp[u].y - (double)sp[u].x;
+      bool found = false, match = false;
+      for (int q0 = 0; q0 < nj[u] && !found; q0 += B) {
+        int lab[B];
+        for (int v = 0; v < B; ++v)
+          lab[v] = q0 + v < nj[u] ? a.elab[(size_t)j * kMaxVerts + q0 + v] : -1;
+        int at = -1;
+        for (int v = B - 1; v >= 0; --v)
+          if (lab[v] == i) at = q0 + v;  // the first occurrence
+        if (at >= 0) {
+          const float2 sq = a.espan[(size_t)j * kMaxVerts + at];
+          found = true;
+          match = fabs((double)sp[u].x - (double)sq.x) + fabs((double)sp[u].y - (double)sq.y) <=
+                  tol;
+        }
+      }
+      if (found ? !match : (len > tol)) {
+        bad = true;
+        mark(j);
+        if (!found && a.xcnt) {
+          // j's cell lacks the edge: i is a candidate j missed
+          const int q = fetch_add(&a.xcnt[j], 1);
+          if (q < kMaxExtra) a.xcand[(size_t)j * kMaxExtra + q] = i;
+        }
       }
     }
   }
@@ -1629,6 +1741,13 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                          st, a);
     else
       hipLaunchKernelGGL(cell_repair_wave_kernel, dim3(4 * repair_blocks), dim3(64), 0, st, a);
+  } else if (a.pass_wave) {
+    // one wave per site: many more waves in flight than sites per lane, for small problems
+    const dim3 g((unsigned)nblocks * kCellBlock);
+    if (fp32)
+      hipLaunchKernelGGL((cell_wave_kernel<float>), g, dim3(64), 0, st, a);
+    else
+      hipLaunchKernelGGL((cell_wave_kernel<double>), g, dim3(64), 0, st, a);
   } else if (fp32) {
     hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   } else {

@@ -271,7 +271,12 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
   // coarse levels, all problems batched per level
   std::vector<std::vector<double>> field(np);
   std::vector<cot_report> rep;
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   for (int l = 0; l < lo.n_levels; ++l) {
+    const auto tl0 = now();
     const int side = lo.sides[l], n = side * side;
     std::vector<Level> lv(np);
     std::vector<cot_problem> cp(np);
@@ -284,12 +289,15 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
     }
     cot_options oo = base;
     if (lo.level_options) oo = lo.level_options[l];
+    const auto tl1 = now();
     cot_context* cc = nullptr;
     int rc = cot_create(cp.data(), np, &oo, &cc);
     if (rc != COT_OK) return rc;
+    const auto tl2 = now();
     rc = stage(cc, wl.data(), coarse, handoff, np, rep);
     std::vector<double> cen(2 * (size_t)np * n);
     if (rc == COT_OK) rc = cot_centroids(cc, wl.data(), 1, cen.data(), nullptr);
+    const auto tl3 = now();
     cot_destroy(cc);
     if (rc != COT_OK) return rc;
     for (int p = 0; p < np; ++p)
@@ -298,10 +306,13 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
     if (so.verbose > 0) {
       int it = 0;
       for (const cot_report& r : rep) it = std::max(it, r.iterations);
-      std::printf("[ladder] level %d: %d problems, iterations max %d\n", side, np, it);
+      std::printf("[ladder] level %d: %d problems, iterations max %d; ms: set-up %.2f create %.2f "
+                  "solve %.2f destroy+field %.2f\n",
+                  side, np, it, ms(tl0, tl1), ms(tl1, tl2), ms(tl2, tl3), ms(tl3, now()));
       std::fflush(stdout);
     }
   }
+  const auto tf0 = now();
   // the full-resolution start
   size_t off = 0;
   for (int p = 0; p < np; ++p) {
@@ -341,8 +352,14 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
     }
   }
   // the exact target, to the caller's stopping rule
+  const auto tf1 = now();
   int rc = stage(ctx, w, so, handoff, np, rep);
   if (rc != COT_OK) return rc;
+  if (so.verbose > 0) {
+    std::printf("[ladder] final: ms: start and floor stages %.2f, solve %.2f\n", ms(tf0, tf1),
+                ms(tf1, now()));
+    std::fflush(stdout);
+  }
   // the reference's gauge: its L-BFGS from w = 0 keeps sum(w) = 0 (the gradient sums to zero);
   // Newton steps, whose preconditioned directions need not, may shift the constant, which
   // changes no cell

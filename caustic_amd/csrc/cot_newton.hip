@@ -26,6 +26,9 @@ namespace cot {
 
 namespace {
 
+// edges of a cell read per batch by the Hessian products (their loads issued together)
+constexpr int kEdgeBatch = 8;
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -96,13 +99,26 @@ __device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
   const int ln = threadIdx.x;
   const bool own = ln < bd.count && dg > 0.0;
   double z = own ? ri / dg : 0.0;
-  const int nv = ln < bd.count ? nedge[i] : 0;
+  if (sweeps <= 0) return z;
+  const int nv = own ? nedge[i] : 0;
+  // the first kEdgeBatch edges' in-block neighbours and couplings, loaded together once for all
+  // sweeps (a cell has 6 edges on average; the rest are read in the sweep)
+  int jl[kEdgeBatch];
+  double hl[kEdgeBatch];
+#pragma unroll
+  for (int u = 0; u < kEdgeBatch; ++u) {
+    jl[u] = u < nv ? elab[(size_t)i * kMaxVerts + u] - bd.first : -1;
+    hl[u] = u < nv ? h[(size_t)i * kMaxVerts + u] : 0.0;
+  }
   for (int s = 0; s < sweeps; ++s) {
     zl[ln] = z;
     __syncthreads();
     if (own) {
       double acc = ri;
-      for (int k = 0; k < nv; ++k) {
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u)
+        if (jl[u] >= 0 && jl[u] < bd.count) acc += hl[u] * zl[jl[u]];
+      for (int k = kEdgeBatch; k < nv; ++k) {
         const int j = elab[(size_t)i * kMaxVerts + k] - bd.first;
         if (j >= 0 && j < bd.count) acc += h[(size_t)i * kMaxVerts + k] * zl[j];
       }
@@ -330,11 +346,38 @@ __global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
     p_new[i] = pi;
     double s = diag[i] * pi;
     const int nv = nedge[i];
-    for (int k = 0; k < nv; ++k) {
-      const int j = elab[(size_t)i * kMaxVerts + k];
-      if (j >= 0) {
-        const double pj = first ? zc(j) : zc(j) + beta * p_old[j];
-        s -= h[(size_t)i * kMaxVerts + k] * pj;
+    // kEdgeBatch edges at a time: their labels and couplings, then the neighbours' values, then
+    // their patch terms, each round's loads in flight together; the terms are added in edge
+    // order as one edge at a time would add them
+    for (int k0 = 0; k0 < nv; k0 += kEdgeBatch) {
+      int jj[kEdgeBatch];
+      double hh[kEdgeBatch];
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) {
+        const bool in = k0 + u < nv;
+        jj[u] = in ? elab[(size_t)i * kMaxVerts + k0 + u] : -1;
+        hh[u] = in ? h[(size_t)i * kMaxVerts + k0 + u] : 0.0;
+      }
+      double zz[kEdgeBatch], pp[kEdgeBatch];
+      int aa[kEdgeBatch];
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) {
+        const int j = jj[u];
+        zz[u] = j >= 0 ? z[j] : 0.0;
+        pp[u] = (j >= 0 && !first) ? p_old[j] : 0.0;
+        aa[u] = (j >= 0 && sagg) ? sagg[j] : -1;
+      }
+      double cc[kEdgeBatch];
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) cc[u] = aa[u] >= 0 ? cvec[aa[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) {
+        if (jj[u] >= 0) {
+          double v = zz[u];
+          if (aa[u] >= 0) v += cc[u];
+          const double pj = first ? v : v + beta * pp[u];
+          s -= hh[u] * pj;
+        }
       }
     }
     q[i] = s;
@@ -356,9 +399,20 @@ __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
     const double pi = p[i];
     double s = diag[i] * pi;
     const int nv = nedge[i];
-    for (int k = 0; k < nv; ++k) {
-      const int j = elab[(size_t)i * kMaxVerts + k];
-      if (j >= 0) s -= h[(size_t)i * kMaxVerts + k] * p[j];
+    for (int k0 = 0; k0 < nv; k0 += kEdgeBatch) {
+      int jj[kEdgeBatch];
+      double hh[kEdgeBatch], pp[kEdgeBatch];
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) {
+        const bool in = k0 + u < nv;
+        jj[u] = in ? elab[(size_t)i * kMaxVerts + k0 + u] : -1;
+        hh[u] = in ? h[(size_t)i * kMaxVerts + k0 + u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u) pp[u] = jj[u] >= 0 ? p[jj[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kEdgeBatch; ++u)
+        if (jj[u] >= 0) s -= hh[u] * pp[u];
     }
     q[i] = s;
     pq = pi * s;

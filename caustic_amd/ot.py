@@ -64,16 +64,19 @@ class OTBatch:
     def __init__(self, problems: Sequence, precision: str = "fp64", sites_per_bucket: float = 1.0,
                  ring_init: int = 2, ring_max: int = 32, verify_tol: float = 1e-4,
                  max_repairs: int = 12, int_lanes: int = 8, repair_blocks: int = 256,
-                 full_pass_ring: int = 0, tuning: dict = None, repair_mode: str = "wave"):
+                 full_pass_ring: int = 0, tuning: dict = None, repair_mode: str = "wave",
+                 pass_mode: int = 0):
         """tuning: launch-shape options as chosen by caustic_amd.autotune (int_lanes,
-        repair_blocks, sites_per_bucket, full_pass_ring); they override the keyword
+        repair_blocks, sites_per_bucket, full_pass_ring, pass_mode); they override the keyword
         arguments.  repair_mode: "wave" (one wave per repaired cell) or "lane" (one lane per
-        cell), the same cells either way."""
+        cell); pass_mode: the full passes' construction, 0 one lane per cell, 1 one wave per
+        cell.  Either choice builds the same cells."""
         if tuning:
             int_lanes = int(tuning.get("int_lanes", int_lanes))
             repair_blocks = int(tuning.get("repair_blocks", repair_blocks))
             sites_per_bucket = float(tuning.get("sites_per_bucket", sites_per_bucket))
             full_pass_ring = int(tuning.get("full_pass_ring", full_pass_ring))
+            pass_mode = int(tuning.get("pass_mode", pass_mode))
         lib = L.load()
         self.problems = list(problems)
         arr, self._keep = _problem_array(self.problems)
@@ -90,6 +93,7 @@ class OTBatch:
         opt.repair_blocks = repair_blocks
         opt.full_pass_ring = full_pass_ring
         opt.repair_mode = {"wave": 0, "lane": 1}[repair_mode]
+        opt.pass_mode = int(pass_mode)
         ctx = ctypes.c_void_p()
         L.check(lib.cot_create(arr, len(problems), ctypes.byref(opt), ctypes.byref(ctx)))
         self._keep = []  # the library copied everything
@@ -98,7 +102,8 @@ class OTBatch:
         self._lib = lib
         self.precision = precision
         self.tuning = {"int_lanes": int_lanes, "repair_blocks": repair_blocks,
-                       "sites_per_bucket": sites_per_bucket, "full_pass_ring": full_pass_ring}
+                       "sites_per_bucket": sites_per_bucket, "full_pass_ring": full_pass_ring,
+                       "pass_mode": int(pass_mode)}
         self._img_shapes = [tuple(np.shape(p.rho)) for p in problems]
         self.n_total = int(sum(self.sizes))
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
@@ -387,7 +392,8 @@ class OTBatch:
                 ctypes.memmove(ctypes.byref(lvl[k]), ctypes.byref(self._opt),
                                ctypes.sizeof(L.cot_options))
                 t = level_tunings[k] if k < len(level_tunings) and level_tunings[k] else {}
-                for name in ("int_lanes", "repair_blocks", "sites_per_bucket", "full_pass_ring"):
+                for name in ("int_lanes", "repair_blocks", "sites_per_bucket", "full_pass_ring",
+                             "pass_mode"):
                     if name in t:
                         setattr(lvl[k], name, type(getattr(lvl[k], name))(t[name]))
             keep.append(lvl)

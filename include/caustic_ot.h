@@ -73,6 +73,9 @@ typedef struct {
                                the two-ring */
   int repair_mode;          /* construction of the repair passes: 0 one wave per listed cell
                                (default), 1 one lane per cell; both build the same cells */
+  int pass_mode;            /* construction of the full passes: 0 one lane per cell (default),
+                               1 one wave per cell (more parallelism for small problems); both
+                               build the same cells */
 } cot_options;
 
 typedef struct {

@@ -13,6 +13,7 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
         t += {128: 0.5, 256: 0.3, 512: 0.1, 1024: 0.2}[cfg["repair_blocks"]]
         t += {1.0: 0.0, 2.0: 0.3}[cfg["sites_per_bucket"]]
         t += {1: 0.2, 2: 0.0}[cfg["full_pass_ring"]]
+        t += {0: 0.4, 1: 0.0}[cfg["pass_mode"]]
         return {"per_eval": t * 1e-4, "evaluations": 10, "wall": 0.0, "cell": 0.0,
                 "integrate": 0.0, "repair": 0.0}
 
@@ -20,8 +21,8 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
     probs = [W.plain_fp64(n_side=8, res=16)]
     cfg, rep = AT.tune(probs, "mixed")
     assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0,
-                   "full_pass_ring": 2}
-    assert abs(rep["gain_vs_default"] - 2.5 / 1.1) < 1e-9
+                   "full_pass_ring": 2, "pass_mode": 1}
+    assert abs(rep["gain_vs_default"] - 2.9 / 1.1) < 1e-9
     # every candidate measured exactly `repeats` (2) times
     distinct = {tuple(sorted(c.items())) for c in calls}
     assert len(calls) == 2 * len(distinct)

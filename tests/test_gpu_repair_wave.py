@@ -97,3 +97,70 @@ def test_wave_repair_device_solve(gpu_available, c1, precision):
     print(f"iterations {rw['iterations']} (wave) {rl['iterations']} (lane), "
           f"max |dw| {np.abs(ww - wl).max():.2e}", flush=True)
     assert abs(rw["iterations"] - rl["iterations"]) <= max(3, rl["iterations"] // 10)
+
+
+# ---- the full construction pass with one wave per cell (pass_mode 1) against one lane per cell
+
+
+def _eval_pass(problems, w, pass_mode, evals=1, **kw):
+    from caustic_amd.ot import OTBatch
+
+    with OTBatch(problems, pass_mode=pass_mode, **kw) as b:
+        for _ in range(evals):  # later evaluations take the previous cells' adjacency
+            ev = b.evaluate(w)
+        st = b.stats()
+        counts, verts = b.cell_polygons(w)
+    return ev, st, counts, verts
+
+
+def _agree_pass(a, b, tag):
+    ev_w, st_w, n_w, v_w = a
+    ev_l, st_l, n_l, v_l = b
+    assert st_w["cell_launches"] > 0 and st_l["cell_launches"] > 0, tag
+    assert np.array_equal(ev_w["exist"], ev_l["exist"]), tag
+    assert np.array_equal(ev_w["mass"], ev_l["mass"]), \
+        f"{tag}: masses differ ({np.mean(ev_w['mass'] != ev_l['mass']):.4f} of cells)"
+    assert np.array_equal(ev_w["f"], ev_l["f"]), tag
+    assert np.array_equal(ev_w["centroid"], ev_l["centroid"]), tag
+    assert np.array_equal(n_w, n_l), f"{tag}: vertex counts differ"
+    k = np.arange(v_w.shape[1])[None, :] < n_w[:, None]
+    assert np.array_equal(v_w[k], v_l[k]), f"{tag}: vertices differ"
+    print(f"{tag}: identical; repair launches {st_w['repair_launches']} (wave) "
+          f"{st_l['repair_launches']} (lane)", flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("evals", [1, 2])
+def test_wave_pass_hidden_cells(gpu_available, c1, precision, evals):
+    """Cold start (bucket rings) and, with two evaluations, the adjacency full pass; the mixed
+    case covers FP32 clipping, the FP64 vertex refinement and the hand-off of FP32 cells."""
+    print(f"[case] wave vs lane full pass, c1 random w, {precision}, {evals} evals", flush=True)
+    w = np.random.default_rng(7).normal(0.0, 2e-4, c1.n)
+    _agree_pass(_eval_pass([c1], w, 1, evals, precision=precision),
+                _eval_pass([c1], w, 0, evals, precision=precision), f"pass {precision} {evals}")
+
+
+def test_wave_pass_bucket_rounds(gpu_available, c1):
+    print("[case] wave vs lane full pass, ring_init 1, w ~ N(0, 1.6e-3), mixed", flush=True)
+    w = np.random.default_rng(100).normal(0.0, 1.6e-3, c1.n)
+    _agree_pass(_eval_pass([c1], w, 1, 2, ring_init=1, precision="mixed"),
+                _eval_pass([c1], w, 0, 2, ring_init=1, precision="mixed"), "pass rings")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wave_pass_device_solve(gpu_available, c1, precision):
+    """The device-driven L-BFGS takes the same path with either full-pass construction."""
+    from caustic_amd.ot import OTBatch
+
+    print(f"[case] wave vs lane full pass, device solve c1 {precision}", flush=True)
+    out = {}
+    for pm in (1, 0):
+        with OTBatch([c1], precision=precision, pass_mode=pm) as b:
+            w, rep = b.solve(history=10, tol=1e-4, stop="maxrel", h0_threshold=0.0,
+                             max_iter=5000, device_loop=True)
+        assert rep[0]["converged"] == 1, (pm, rep)
+        out[pm] = (w, rep[0])
+    print(f"iterations {out[1][1]['iterations']} (wave) {out[0][1]['iterations']} (lane)",
+          flush=True)
+    assert out[1][1]["iterations"] == out[0][1]["iterations"]
+    assert np.array_equal(out[1][0], out[0][0])
