@@ -10,10 +10,13 @@ Knobs (fields of cot_options):
   repair_blocks     grid of the repair passes that walk the work list of flagged sites
                     (128 ... 1024 blocks: short lists want few blocks, long ones one per CU or
                     more).
-  sites_per_bucket  bucket-grid density (1, 2).  Sites are stored in 8x8-bucket tiles, so this
-                    also sets how compact the patch of plane under one 128-site block is, and the
-                    size of the fallback candidate rings (half a site per bucket has cost 50x
-                    per evaluation where cells are far from their sites, and is not tried).
+  sites_per_bucket  not tuned (kept at 1).  Sites are stored in 8x8-bucket tiles, so this sets
+                    the shape of the patch of plane under one 64-site block: one site per bucket
+                    makes a block one square tile, two make it half a tile, 2:1.  The Newton
+                    steps' preconditioner works on those blocks (block-local Jacobi sweeps, patches
+                    of whole blocks), and the elongated blocks doubled the CG iterations of the
+                    N=65,536 shapes solve (583 -> 1,176, 195 -> 212 ms a solve) while the
+                    evaluations it measures here ran 8% faster.
   full_pass_ring    candidates of the FP32 full pass: the previous cells' one-ring (1) or
                     two-ring (2).  The two-ring costs more clipping per cell and leaves fewer
                     unmatched edges to the repair passes; it pays where weights move far between
@@ -50,7 +53,6 @@ import numpy as np
 KNOBS = {
     "int_lanes": (4, 8, 16),
     "repair_blocks": (128, 256, 512, 1024),
-    "sites_per_bucket": (1.0, 2.0),
     "full_pass_ring": (1, 2),
     "pass_mode": (0, 1),
 }

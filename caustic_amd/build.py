@@ -53,7 +53,7 @@ def _shared(sources, out):
 
 def build_library():
     _shared(("cot_kernels.hip", "cot_solver.hip", "cot_reflect.hip", "cot_newton.hip", "cot_render.hip",
-             "cot_splat.hip", "cot_api.cpp", "cot_ladder.cpp"), LIB)
+             "cot_splat.hip", "cot_api.cpp", "cot_ladder.cpp", "cot_pool.cpp"), LIB)
     _shared(("wsn_kernels.hip", "wsn_api.cpp"), WSN_LIB)
 
 

@@ -29,6 +29,8 @@ namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, double4* pre,
+                         double4* tot, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
                                  const int* elab, uint8_t* adjn, int* adj, int nblocks,
                                  hipStream_t st);
@@ -284,17 +286,11 @@ struct cot_context {
 
   ~cot_context() {
     if (st) hipStreamSynchronize(st);
-    for (void* p : allocs) hipFree(p);
-    if (h_part) hipHostFree(h_part);
-    if (h_counts) hipHostFree(h_counts);
-    if (h_vec) hipHostFree(h_vec);
-    if (h_ac) hipHostFree(h_ac);
-    if (h_crows) hipHostFree(h_crows);
-    if (h_small) hipHostFree(h_small);
-    if (h_coef) hipHostFree(h_coef);
-    if (h_ismall) hipHostFree(h_ismall);
-    if (h_active) hipHostFree(h_active);
-    if (h_ndone) hipHostFree(h_ndone);
+    for (void* p : allocs) pool_free_device(p);
+    for (void* p : {(void*)h_part, (void*)h_counts, (void*)h_vec, (void*)h_ac, (void*)h_crows,
+                    (void*)h_small, (void*)h_coef, (void*)h_ismall, (void*)h_active,
+                    (void*)h_ndone})
+      pool_free_pinned(p);
     if (ev_in) hipEventDestroy(ev_in);
     if (ev_out) hipEventDestroy(ev_out);
     if (st) hipStreamDestroy(st);
@@ -303,11 +299,16 @@ struct cot_context {
 
 namespace {
 
+// Pinned host memory from the cache (hipSuccess or hipErrorOutOfMemory, as hipHostMalloc).
+hipError_t pinned_alloc(void** p, size_t bytes) {
+  *p = pool_alloc_pinned(bytes);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
 template <typename T>
 int dalloc(cot_context* c, T** p, size_t count) {
-  void* q = nullptr;
-  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  if (hipMalloc(&q, bytes) != hipSuccess) return fail(COT_ENOMEM, "hipMalloc failed");
+  void* q = pool_alloc_device(std::max<size_t>(count, 1) * sizeof(T));
+  if (!q) return fail(COT_ENOMEM, "device allocation failed");
   c->allocs.push_back(q);
   *p = reinterpret_cast<T*>(q);
   return COT_OK;
@@ -320,16 +321,15 @@ struct CallBuffers {
   explicit CallBuffers(hipStream_t s) : st(s) {}
   template <typename T>
   int get(T** out, size_t count) {
-    void* q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
-      return fail(COT_ENOMEM, "hipMalloc failed");
+    void* q = pool_alloc_device(std::max<size_t>(count, 1) * sizeof(T));
+    if (!q) return fail(COT_ENOMEM, "device allocation failed");
     held.push_back(q);
     *out = reinterpret_cast<T*>(q);
     return COT_OK;
   }
   ~CallBuffers() {
     if (!held.empty()) hipStreamSynchronize(st);
-    for (void* q : held) hipFree(q);
+    for (void* q : held) pool_free_device(q);
   }
 };
 
@@ -645,8 +645,8 @@ int ensure_device_solver(cot_context* c, int repair_rounds) {
     CHK(dalloc(c, &c->d_peval, 6 * (size_t)c->nblk));
     CHK(dalloc(c, &c->d_pacc, (size_t)kAcceptPartials * c->nblk));
     CHK(dalloc(c, &c->d_ndone, 2));
-    if (hipHostMalloc((void**)&c->h_ndone, 2 * sizeof(int)) != hipSuccess)
-      return fail(COT_ENOMEM, "hipHostMalloc failed");
+    if (pinned_alloc((void**)&c->h_ndone, 2 * sizeof(int)) != hipSuccess)
+      return fail(COT_ENOMEM, "pinned allocation failed");
     HIPCHK(hipMemset(c->d_cnt, 0, 3 * sizeof(int) * c->np));
   }
   if (c->rfail_rounds < repair_rounds + 1) {
@@ -1046,15 +1046,15 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   c->img_total = img_total;
   if ((rc = dalloc(c, &c->d_pre, img_total)) || (rc = dalloc(c, &c->d_tot, tot_total)))
     return bail(rc);
-  if (hipHostMalloc((void**)&c->h_part,
+  if (pinned_alloc((void**)&c->h_part,
                     sizeof(double) * c->nblk * std::max(kEvalPartials, kAcceptPartials)) ||
-      hipHostMalloc((void**)&c->h_counts, sizeof(int) * 2 * c->np) ||
-      hipHostMalloc((void**)&c->h_vec, sizeof(double) * N) ||
-      hipHostMalloc((void**)&c->h_small, sizeof(double) * c->np) ||
-      hipHostMalloc((void**)&c->h_coef, sizeof(double) * c->np * kCoefWidth) ||
-      hipHostMalloc((void**)&c->h_ismall, sizeof(int) * 2 * c->np) ||
-      hipHostMalloc((void**)&c->h_active, c->np))
-    return bail(fail(COT_ENOMEM, "hipHostMalloc failed"));
+      pinned_alloc((void**)&c->h_counts, sizeof(int) * 2 * c->np) ||
+      pinned_alloc((void**)&c->h_vec, sizeof(double) * N) ||
+      pinned_alloc((void**)&c->h_small, sizeof(double) * c->np) ||
+      pinned_alloc((void**)&c->h_coef, sizeof(double) * c->np * kCoefWidth) ||
+      pinned_alloc((void**)&c->h_ismall, sizeof(int) * 2 * c->np) ||
+      pinned_alloc((void**)&c->h_active, c->np))
+    return bail(fail(COT_ENOMEM, "pinned allocation failed"));
 
   // uploads (synchronous: inputs are caller-owned)
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -1065,13 +1065,20 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
             up(c->d_xy, xy.data(), sizeof(double2) * N) &&
             up(c->d_nu, nu.data(), sizeof(double) * N) &&
             up(c->d_buckets, all_buckets.data(), sizeof(int2) * all_buckets.size());
-  for (int p = 0; p < n_problems && ok; ++p) {
-    const cot_problem& pr = problems[p];
-    const ProbDesc& pd = c->probs[p];
-    std::vector<double4> pre((size_t)pr.pm * pr.pn), tot((size_t)pd.nchunk * pr.pn);
-    build_prefix(pr.rho, pr.pm, pr.pn, c->fp32, pre.data(), tot.data());
-    ok = up(c->d_pre + pd.pre_off, pre.data(), sizeof(double4) * pre.size()) &&
-         up(c->d_tot + pd.tot_off, tot.data(), sizeof(double4) * tot.size());
+  // the images, each at its offset of one staging buffer, and their prefix tables built on the
+  // device; the staging buffer goes back once the stream has drained
+  if (ok) {
+    double* d_rho = static_cast<double*>(pool_alloc_device(sizeof(double) * img_total));
+    if (!d_rho) return bail(fail(COT_ENOMEM, "device allocation failed"));
+    for (int p = 0; p < n_problems && ok; ++p) {
+      const cot_problem& pr = problems[p];
+      const ProbDesc& pd = c->probs[p];
+      ok = up(d_rho + pd.pre_off, pr.rho, sizeof(double) * pr.pm * pr.pn) &&
+           launch_prefix(d_rho + pd.pre_off, pr.pm, pr.pn, c->fp32, c->d_pre + pd.pre_off,
+                         c->d_tot + pd.tot_off, c->st) == hipSuccess;
+    }
+    ok = (hipStreamSynchronize(c->st) == hipSuccess) && ok;
+    pool_free_device(d_rho);
   }
   {
     // FP32 clipping frames start at the sites
@@ -1559,14 +1566,14 @@ int build_coarse(cot_context* c, int maxp) {
   CHK(dalloc(c, &c->d_ainv, moff));
   CHK(dalloc(c, &c->d_cvec, total));
   CHK(dalloc(c, &c->d_prc, 2 * tasks.size()));
-  if (c->h_ac) (void)hipHostFree(c->h_ac);
+  pool_free_pinned(c->h_ac);
   c->h_ac = nullptr;
-  if (hipHostMalloc((void**)&c->h_ac, sizeof(double) * moff) != hipSuccess)
-    return fail(COT_ENOMEM, "hipHostMalloc failed");
+  if (pinned_alloc((void**)&c->h_ac, sizeof(double) * moff) != hipSuccess)
+    return fail(COT_ENOMEM, "pinned allocation failed");
   if (!c->h_crows &&
-      hipHostMalloc((void**)&c->h_crows, sizeof(CoarseEntry) * (size_t)nb * kBlockNbr) !=
+      pinned_alloc((void**)&c->h_crows, sizeof(CoarseEntry) * (size_t)nb * kBlockNbr) !=
           hipSuccess)
-    return fail(COT_ENOMEM, "hipHostMalloc failed");
+    return fail(COT_ENOMEM, "pinned allocation failed");
   HIPCHK(hipMemcpy(c->d_bagg, bagg.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_agg_start, start.data(), sizeof(int) * start.size(),
                    hipMemcpyHostToDevice));

@@ -5,6 +5,13 @@
 
 namespace cot {
 
+// Device and pinned host memory cached by size across owners in the process (cot_pool.cpp): a
+// block is released only after the stream that used it has drained.  nullptr on failure.
+void* pool_alloc_device(size_t bytes);
+void pool_free_device(void* p);
+void* pool_alloc_pinned(size_t bytes);
+void pool_free_pinned(void* p);
+
 // Largest polygon a cell may hold (vertices == edges): the stride of the per-site edge and
 // vertex records and the polygon of the FP64 repair pass.  A cell that would exceed it rejects
 // the line-search trial, as the reference's 200-vertex rule does (ot_op.cuh:366-382).

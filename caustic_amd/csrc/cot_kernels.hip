@@ -1704,6 +1704,29 @@ __global__ void scatter_u8_kernel(const uint8_t* src, const int* perm, size_t n,
   if (r < n) dst[perm[r]] = src[r];
 }
 
+// Row prefixes and chunk totals of one target image (the layout of cot_internal.h), one thread
+// per (row, chunk): the sums of build_prefix (cot_layout.h, the host build), in the same order.
+// FP32 geometry reads the image rounded to float, as its pieces use it.
+__global__ void prefix_kernel(const double* rho, int pm, int pn, int nchunk, int fp32,
+                              double4* pre, double4* tot) {
+#pragma clang fp contract(on)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)pn * nchunk) return;
+  const int iy = (int)(t / nchunk), c = (int)(t % nchunk);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int k0 = c * kChunk, k1 = min(k0 + kChunk, pm);
+  for (int k = k0; k < k1; ++k) {
+    double r = rho[(size_t)iy * pm + k];
+    if (fp32) r = (double)(float)r;
+    pre[(size_t)iy * pm + k] = make_double4(s0, s1, s2, r);
+    const double kl = (double)(k - k0);
+    s0 += r;
+    s1 += r * (kl + 0.5);
+    s2 += r * (kl * kl + kl + 1.0 / 3.0);
+  }
+  tot[(size_t)iy * nchunk + c] = make_double4(s0, s1, s2, 0.0);
+}
+
 __global__ void fill_i32_kernel(int* p, size_t n, int v) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
@@ -1758,6 +1781,15 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
     case 16: launch_integrate<16>(a, nblocks, fp32, repair_blocks, st); break;
     default: launch_integrate<8>(a, nblocks, fp32, repair_blocks, st); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, double4* pre,
+                         double4* tot, hipStream_t st) {
+  const int nchunk = (pm + kChunk - 1) / kChunk;
+  const long long n = (long long)pn * nchunk;
+  hipLaunchKernelGGL(prefix_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rho, pm,
+                     pn, nchunk, fp32 ? 1 : 0, pre, tot);
   return hipGetLastError();
 }
 

@@ -51,19 +51,78 @@ const double* frame_of(const cot_ladder_opts& lo, const cot_problem& pr, int p) 
   return lo.frames ? lo.frames + 4 * p : pr.proj_domain;
 }
 
-// The target box-downsampled by the largest factor f dividing both image sides with at least
-// `min_px` pixels left along the shorter side and at most `want` pixels per coarse grid step,
-// plus floor x its mean pixel, scaled to `total`.
-void coarse_target(const cot_problem& pr, int want, double floor_rel, double total, Level& L) {
+// The box-downsampling factor of a level: the largest f dividing both image sides with at least
+// 32 pixels left along the shorter side and at most `want` fine pixels per coarse grid step.
+int target_factor(const cot_problem& pr, int want) {
   const int shorter = std::min(pr.pm, pr.pn);
   int f = std::max(1, std::min(want, shorter / 32));
   while (f > 1 && (pr.pm % f || pr.pn % f)) --f;
+  return f;
+}
+
+int level_want(const cot_problem& pr, int side) {
+  const int fine_side = (int)std::lround(std::sqrt((double)pr.n_sites));
+  return std::max(1, fine_side / std::max(side, 1));
+}
+
+// Sums of an image (pm x pn) over g x g blocks (g divides both sides).
+void block_sums(const double* src, int pm, int pn, int g, std::vector<double>& out) {
+  const int qm = pm / g, qn = pn / g;
+  out.assign((size_t)qm * qn, 0.0);
+  for (int iy = 0; iy < pn; ++iy) {
+    const double* row = src + (size_t)iy * pm;
+    double* o = out.data() + (size_t)(iy / g) * qm;
+    for (int qx = 0; qx < qm; ++qx) {
+      double v = 0.0;
+      for (int k = 0; k < g; ++k) v += row[qx * g + k];
+      o[qx] += v;
+    }
+  }
+}
+
+// A problem's target summed over f x f blocks for the factors of all levels, each computed from
+// the largest factor already done that divides it (the full image for the finest), so the
+// full-resolution image is read once for all levels.
+struct Pyramid {
+  std::vector<int> f;                    // ascending
+  std::vector<std::vector<double>> img;  // img[k]: f[k] x f[k] block sums
+  const std::vector<double>& at(int fk) const {
+    for (size_t k = 0; k < f.size(); ++k)
+      if (f[k] == fk) return img[k];
+    return img[0];  // not reached: every level's factor is built
+  }
+};
+
+void build_pyramid(const cot_problem& pr, std::vector<int> factors, Pyramid& P) {
+  std::sort(factors.begin(), factors.end());
+  factors.erase(std::unique(factors.begin(), factors.end()), factors.end());
+  P.f.clear();
+  P.img.clear();
+  for (int fk : factors) {
+    int from = -1;  // index of the largest built factor dividing fk
+    for (int k = (int)P.f.size() - 1; k >= 0 && from < 0; --k)
+      if (fk % P.f[k] == 0) from = k;
+    std::vector<double> out;
+    if (fk == 1) {
+      out.assign(pr.rho, pr.rho + (size_t)pr.pm * pr.pn);
+    } else if (from < 0) {
+      block_sums(pr.rho, pr.pm, pr.pn, fk, out);
+    } else {
+      const int g = P.f[from];
+      block_sums(P.img[from].data(), pr.pm / g, pr.pn / g, fk / g, out);
+    }
+    P.f.push_back(fk);
+    P.img.push_back(std::move(out));
+  }
+}
+
+// The target of a level: the block sums for its factor plus floor x their mean pixel, scaled to
+// `total`.
+void coarse_target(const cot_problem& pr, const Pyramid& P, int f, double floor_rel,
+                   double total, Level& L) {
   L.pm = pr.pm / f;
   L.pn = pr.pn / f;
-  L.rho.assign((size_t)L.pm * L.pn, 0.0);
-  for (int iy = 0; iy < pr.pn; ++iy)
-    for (int ix = 0; ix < pr.pm; ++ix)
-      L.rho[(size_t)(iy / f) * L.pm + ix / f] += pr.rho[(size_t)iy * pr.pm + ix];
+  L.rho = P.at(f);
   double s = 0.0;
   for (double v : L.rho) s += v;
   const double add = floor_rel * s / (double)L.rho.size();
@@ -75,8 +134,8 @@ void coarse_target(const cot_problem& pr, int want, double floor_rel, double tot
 
 // Level `side` of problem pr: jittered grid of the frame, site k = j * side + i near node (i, j)
 // (the layout the field transfer assumes), masses of the fine sites binned by grid cell.
-void coarse_level(const cot_problem& pr, const double* fr, int side, double floor_rel,
-                  unsigned long long seed, Level& L) {
+void coarse_level(const cot_problem& pr, const Pyramid& P, const double* fr, int side,
+                  double floor_rel, unsigned long long seed, Level& L) {
   Rng rng{seed};
   const int n = side * side;
   L.side = side;
@@ -103,8 +162,7 @@ void coarse_level(const cot_problem& pr, const double* fr, int side, double floo
   double t2 = 0.0;
   for (double& v : L.nu) t2 += (v = std::max(v, least));
   for (double& v : L.nu) v *= total / t2;
-  const int fine_side = (int)std::lround(std::sqrt((double)pr.n_sites));
-  coarse_target(pr, std::max(1, fine_side / std::max(side, 1)), floor_rel, total, L);
+  coarse_target(pr, P, target_factor(pr, level_want(pr, side)), floor_rel, total, L);
   for (int q = 0; q < 4; ++q) L.dom[q] = pr.proj_domain[q];
 }
 
@@ -268,6 +326,14 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
   coarse.tol = lo.coarse_tol > 0 ? lo.coarse_tol : 1e-2;
   const double handoff = lo.handoff > 0 ? lo.handoff : 1.0;
 
+  // the targets' block sums for every level
+  std::vector<Pyramid> pyr(np);
+  for (int p = 0; p < np; ++p) {
+    std::vector<int> fs;
+    for (int l = 0; l < lo.n_levels; ++l)
+      fs.push_back(target_factor(problems[p], level_want(problems[p], lo.sides[l])));
+    build_pyramid(problems[p], fs, pyr[p]);
+  }
   // coarse levels, all problems batched per level
   std::vector<std::vector<double>> field(np);
   std::vector<cot_report> rep;
@@ -283,7 +349,8 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
     std::vector<double> wl((size_t)np * n, 0.0);
     for (int p = 0; p < np; ++p) {
       const double* fr = frame_of(lo, problems[p], p);
-      coarse_level(problems[p], fr, side, lo.floors[l], lo.seed + 7919ull * (l + 1) + p, lv[p]);
+      coarse_level(problems[p], pyr[p], fr, side, lo.floors[l], lo.seed + 7919ull * (l + 1) + p,
+                   lv[p]);
       cp[p] = as_problem(lv[p], n);
       if (l > 0) weights_from(field[p], lo.sides[l - 1], fr, lv[p].sites.data(), n, &wl[(size_t)p * n]);
     }

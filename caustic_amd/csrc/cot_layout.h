@@ -38,20 +38,25 @@ inline void bucket_sort(const cot_problem& pr, double spb, ProbDesc& pd, std::ve
   pd.by0 = ymin;
   pd.bsize = bs;
   pd.binv = 1.0 / bs;
-  const int tiles_x = (gx + 7) / 8;
-  std::vector<long long> key(n);
+  const int tiles_x = (gx + 7) / 8, tiles_y = (gy + 7) / 8;
+  std::vector<int> key(n);
   std::vector<int> cell(n);
   for (int i = 0; i < n; ++i) {
     int cx = std::min(std::max((int)std::floor((pr.sites[2 * i] - xmin) * pd.binv), 0), gx - 1);
     int cy =
         std::min(std::max((int)std::floor((pr.sites[2 * i + 1] - ymin) * pd.binv), 0), gy - 1);
     cell[i] = cy * gx + cx;
-    long long tile = (long long)(cy / 8) * tiles_x + (cx / 8);
-    key[i] = (tile * 64 + (cy % 8) * 8 + (cx % 8));
+    const int tile = (cy / 8) * tiles_x + (cx / 8);
+    key[i] = tile * 64 + (cy % 8) * 8 + (cx % 8);
   }
+  // sites in increasing key, ties in index order (a stable counting sort: keys are below
+  // 64 x tiles, a few times n)
+  const size_t nkey = (size_t)tiles_x * tiles_y * 64;
+  std::vector<int> start(nkey + 1, 0);
+  for (int i = 0; i < n; ++i) ++start[(size_t)key[i] + 1];
+  for (size_t k = 0; k < nkey; ++k) start[k + 1] += start[k];
   order.resize(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int i = 0; i < n; ++i) order[start[key[i]]++] = i;
   buckets.assign((size_t)gx * gy, make_int2(0, 0));
   for (int r = 0; r < n; ++r) {
     int b = cell[order[r]];

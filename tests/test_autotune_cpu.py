@@ -22,6 +22,8 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
     cfg, rep = AT.tune(probs, "mixed")
     assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0,
                    "full_pass_ring": 2, "pass_mode": 1}
+    # the bucket density is not a knob: no candidate leaves 1.0
+    assert all(c["sites_per_bucket"] == 1.0 for c in calls)
     assert abs(rep["gain_vs_default"] - 2.9 / 1.1) < 1e-9
     # every candidate measured exactly `repeats` (2) times
     distinct = {tuple(sorted(c.items())) for c in calls}
