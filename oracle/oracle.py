@@ -34,7 +34,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# COT_ORACLE_LIB names another build of ot_oracle.cpp (the sanitizer build of the CPU tests)
+_LIB_PATH = os.environ.get("COT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -6,7 +6,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_PATH = os.path.join(_HERE, "libhostemu.so")
+# COT_HOSTEMU_LIB names another build of cot_hostemu.cpp (the sanitizer build of the CPU tests)
+_PATH = os.environ.get("COT_HOSTEMU_LIB") or os.path.join(_HERE, "libhostemu.so")
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
