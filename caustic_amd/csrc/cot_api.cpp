@@ -29,7 +29,7 @@ namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st);
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
-hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, double4* pre,
+hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, DensityGroup* grp,
                          double4* tot, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
                                  const int* elab, uint8_t* adjn, int* adj, int nblocks,
@@ -108,7 +108,7 @@ hipError_t launch_aa_direction(const BlockDesc* blocks, const double* alpha, int
                                hipStream_t st);
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
-                          const int* elab, const double4* pre, const int* bagg, double* h,
+                          const int* elab, const DensityGroup* grp, const int* bagg, double* h,
                           double* diag, int* sagg, int nblk, hipStream_t st);
 hipError_t launch_coarse_rows(const BlockDesc* blocks, const int* bagg, const int* sagg,
                               const uint8_t* nedge, const int* elab, const double* h,
@@ -209,7 +209,9 @@ struct cot_context {
   double2* d_xy = nullptr;
   double* d_nu = nullptr;
   int2* d_buckets = nullptr;
-  double4* d_pre = nullptr;       // row prefixes of the targets
+  DensityGroup* d_grp = nullptr;  // density groups (compact row prefixes) of the targets
+  int* d_defer = nullptr;         // sites the lane full pass hands to its wave pass
+  int* d_dcount = nullptr;        //   and their count
   double4* d_tot = nullptr;       // chunk totals
   double *d_w = nullptr, *d_dir = nullptr, *d_step = nullptr, *d_mass = nullptr;
   double *d_fcell = nullptr, *d_area = nullptr, *d_g = nullptr;
@@ -369,7 +371,7 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.w = c->d_w;
   ca.nu = c->d_nu;
   ca.buckets = c->d_buckets;
-  ca.pre = c->d_pre;
+  ca.grp = c->d_grp;
   ca.tot = c->d_tot;
   ca.ring = c->d_ring;
   ca.redo = c->d_redo;
@@ -396,6 +398,8 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.xcand = c->d_xcand;
   ca.vtx = c->d_vtx;
   ca.kbox = c->d_kbox;
+  ca.defer_list = c->d_defer;
+  ca.defer_count = c->d_dcount;
   va = VerifyArgs{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
@@ -962,7 +966,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   std::vector<int2> all_buckets;
   std::vector<double2> xy;
   std::vector<double> nu;
-  size_t img_total = 0, tot_total = 0;
+  size_t img_total = 0, tot_total = 0, grp_total = 0;
+  const int gsize = c->fp32 ? kGroup32 : kGroup64;
   int site_off = 0;
   for (int p = 0; p < n_problems; ++p) {
     const cot_problem& pr = problems[p];
@@ -974,6 +979,9 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     pd.pre_off = (int)img_total;
     pd.nchunk = (pr.pm + kChunk - 1) / kChunk;
     pd.tot_off = (int)tot_total;
+    pd.gsize = gsize;
+    pd.ngrp = (pr.pm + gsize - 1) / gsize;
+    pd.grp_off = (int)grp_total;
     pd.x0 = pr.proj_domain[0];
     pd.x1 = pr.proj_domain[1];
     pd.y0 = pr.proj_domain[2];
@@ -1010,7 +1018,9 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     c->nu_sum.push_back(nus);
     img_total += (size_t)pr.pm * pr.pn;
     tot_total += (size_t)pd.nchunk * pr.pn;
-    if (img_total > (size_t)INT32_MAX) return bail(fail(COT_EINVAL, "images too large"));
+    grp_total += (size_t)pd.ngrp * pr.pn;
+    if (img_total > (size_t)INT32_MAX || grp_total > (size_t)INT32_MAX)
+      return bail(fail(COT_EINVAL, "images too large"));
     c->probs.push_back(pd);
     site_off += pr.n_sites;
   }
@@ -1039,12 +1049,13 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
       (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
+      (rc = dalloc(c, &c->d_defer, N)) || (rc = dalloc(c, &c->d_dcount, 1)) ||
       (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
     return bail(rc);
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
   c->img_total = img_total;
-  if ((rc = dalloc(c, &c->d_pre, img_total)) || (rc = dalloc(c, &c->d_tot, tot_total)))
+  if ((rc = dalloc(c, &c->d_grp, grp_total)) || (rc = dalloc(c, &c->d_tot, tot_total)))
     return bail(rc);
   if (pinned_alloc((void**)&c->h_part,
                     sizeof(double) * c->nblk * std::max(kEvalPartials, kAcceptPartials)) ||
@@ -1074,7 +1085,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       const cot_problem& pr = problems[p];
       const ProbDesc& pd = c->probs[p];
       ok = up(d_rho + pd.pre_off, pr.rho, sizeof(double) * pr.pm * pr.pn) &&
-           launch_prefix(d_rho + pd.pre_off, pr.pm, pr.pn, c->fp32, c->d_pre + pd.pre_off,
+           launch_prefix(d_rho + pd.pre_off, pr.pm, pr.pn, c->fp32, c->d_grp + pd.grp_off,
                          c->d_tot + pd.tot_off, c->st) == hipSuccess;
     }
     ok = (hipStreamSynchronize(c->st) == hipSuccess) && ok;
@@ -1095,6 +1106,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
+      launch_fill_i32(c->d_dcount, 1, 0, c->st) != hipSuccess ||
       hipMemset(c->d_fails, 0, sizeof(int) * 2 * c->np) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return bail(fail(COT_EHIP, "initialisation failed"));
@@ -1649,7 +1661,7 @@ int cot_hessian_product(cot_context* c, const double* w, const double* x, double
   std::memset(c->h_active, 1, c->np);
   HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, c->st));
   HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                        c->d_elab, c->d_pre, nullptr, c->d_hh, c->d_hdiag, nullptr, nb, c->st));
+                        c->d_elab, c->d_grp, nullptr, c->d_hh, c->d_hdiag, nullptr, nb, c->st));
   CHK(upload_sorted(c, x, c->d_np));
   HIPCHK(launch_hessian_apply(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_np,
                               c->d_nq, c->d_ppq, nb, c->st));
@@ -1776,7 +1788,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
                           st));
     // Hessian at the current cells, then H x = -(g - mean g)
     HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                          c->d_elab, c->d_pre, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hdiag,
+                          c->d_elab, c->d_grp, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hdiag,
                           coarse ? c->d_sagg : nullptr, nb, st));
     if (coarse) {
       // the blocks' couplings down; the patches' Laplacian summed over each patch's blocks in

@@ -34,12 +34,38 @@ constexpr int kIntBlock = 256;
 constexpr int kVecBlock = 256;
 // Largest L-BFGS history.
 constexpr int kMaxHist = 16;
-// Row prefixes of the target: columns in chunks of kChunk.  Entry (iy, ix) holds the sums over
-// the pixels k of the chunk of ix that lie left of ix, of rho_k * (1, k' + 1/2, k'^2 + k' + 1/3)
-// with k' = k - (chunk start) -- the integrals of 1, U, U^2 over whole pixels in chunk-local
-// coordinates -- and rho at (iy, ix) itself; the chunk totals hold the same three sums over
-// whole chunks.  Sums between two columns then need no subtraction of large prefixes.
+// Row prefixes of the target: columns in chunks of kChunk.  The prefix at (iy, ix) is the three
+// sums over the pixels k of the chunk of ix that lie left of ix, of rho_k * (1, k' + 1/2,
+// k'^2 + k' + 1/3) with k' = k - (chunk start) -- the integrals of 1, U, U^2 over whole pixels in
+// chunk-local coordinates; the chunk totals hold the same three sums over whole chunks.  Sums
+// between two columns then need no subtraction of large prefixes.
 constexpr int kChunk = 32;
+// The prefixes are stored compactly: one 32-byte DensityGroup per pair of pixels of a row (FP32
+// image) or per pixel (FP64 image) -- the group's densities and the prefix at its first pixel; a
+// reader of the pair's second pixel adds the first pixel's terms with the fused multiply-adds that
+// built the prefix, so it gets the prefix the row's sequential sum reaches there, bit for bit.
+// Rows are padded to whole groups (density 0).  16 bytes per pixel of an FP32 image where a full
+// prefix per pixel took 32.  (Groups of 8 FP32 pixels in 64 bytes took a quarter of the bytes but
+// doubled the integration pass's time: the in-group sums and the records' registers.)
+constexpr int kGroup32 = 2;
+constexpr int kGroup64 = 1;
+struct alignas(32) DensityGroup {
+  double s0, s1, s2;  // prefix at the group's first pixel (chunk-local sums)
+  union {
+    float f[kGroup32];  // FP32 image
+    double d;           // FP64 image
+  } r;
+};
+static_assert(sizeof(DensityGroup) == 32, "one 32-byte record per group");
+static_assert(kChunk % kGroup32 == 0, "groups never straddle chunks");
+static_assert(kGroup32 == 2 && kGroup64 == 1, "readers index groups by shifts");
+// The three prefix sums one pixel further: the terms of pixel k' = kl of density r.
+__host__ __device__ inline void prefix_step(double& s0, double& s1, double& s2, double r,
+                                            double kl) {
+  s0 += r;
+  s1 = fma(r, kl + 0.5, s1);
+  s2 = fma(r, fma(kl, kl, kl) + 1.0 / 3.0, s2);
+}
 
 // The last step of the conformity argument: when every cell edge is matched by its neighbour's,
 // the cells cover the target rectangle D a constant number of times, so sum |P_i| = |D| forces a
@@ -68,6 +94,14 @@ constexpr double kMarginRel = 1e-3;
 constexpr int kMaxExtra = 24;
 // Edges the conformity check reads per batch (their loads issued together).
 constexpr int kVerifyBatch = 8;
+// Adjacency candidates the lane construction loads per batch (their loads issued together).
+constexpr int kCandBatch = 8;
+// Waves of the pass that builds the sites the lane full pass deferred (a grid-stride loop over
+// the list: 4 waves per SIMD of MI355X's 1,024).
+constexpr int kDeferWaves = 4096;
+// A lane full pass over the two-ring defers a site whose neighbours' edge counts add up to more
+// than this (a typical cell: 6 neighbours of 6 edges).
+constexpr int kDeferCand = 64;
 // Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
 // folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
 // the next kernel on the stream closes the launch: [2 * kTimerSlots] accumulates the ticks from
@@ -85,12 +119,25 @@ struct ProbDesc {
   int bkt_off, gx, gy;    // bucket grid: gx * gy buckets starting at bkt_off
   int blk_off, nblk;      // cell/vector blocks of this problem
   int tot_off, nchunk;    // chunk totals at tot_off (pn * nchunk entries)
+  int grp_off, ngrp;      // density groups at grp_off (pn * ngrp records)
+  int gsize;              // pixels per group: kGroup32 (FP32 image) or kGroup64
   double x0, x1, y0, y1;  // target rectangle (proj_domain)
   double dx, dy;          // pixel size
   double bx0, by0;        // bucket grid origin
   double binv;            // 1 / bucket size
   double bsize;           // bucket size
 };
+
+// Workgroups of a launch are dealt to MI355X's eight XCDs round robin (workgroup w runs on XCD
+// w % 8), and each XCD has its own L2.  xcd_slot renumbers them so that XCD x takes the contiguous
+// slots [x q + min(x, r), x q + min(x, r) + q + (x < r)) of the G workgroups (q = G / 8,
+// r = G % 8): neighbouring cell blocks -- whose cells read each other's sites and weights and the
+// same pixel rows -- then share an L2 instead of being spread over all eight.  A bijection on
+// [0, G).
+__host__ __device__ inline int xcd_slot(int w, int G) {
+  const int x = w & 7, q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + (w >> 3);
+}
 
 // The cover check (kCoverTol): do the cells' areas add up to the target rectangle's?
 __host__ __device__ inline bool covers_domain(const ProbDesc& pd, double area) {
@@ -180,7 +227,7 @@ struct CellArgs {
   const double* step;       // per-problem step (used when dir != nullptr)
   const double* nu;         // source masses
   const int2* buckets;      // (start, count) per bucket
-  const double4* pre;       // row prefixes (x, y, z: the three sums; w: the pixel's rho)
+  const DensityGroup* grp;  // density groups (compact row prefixes)
   const double4* tot;       // chunk totals (x, y, z)
   uint8_t* ring;            // per-site candidate ring radius (read, raised on repair)
   int* redo;                // repair flags: 0 none, 1 listed for a repair, 2 an FP32 cell that
@@ -216,6 +263,12 @@ struct CellArgs {
   int ring_max;
   int repair_lane;          // repair construction: 0 one wave per listed site, 1 one lane per site
   int pass_wave;            // full construction: 0 one lane per site, 1 one wave per site
+  // lane full pass: sites with no previous neighbours (their previous cells were empty) would
+  // scan bucket rings one candidate at a time in one lane -- the pass's longest lanes; they are
+  // appended here instead (site slots) and built by a wave per site right after (nullptr: built
+  // in the lane).  The integration pass clears the count.
+  int* defer_list;
+  int* defer_count;
   const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)

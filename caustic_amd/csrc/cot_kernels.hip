@@ -181,19 +181,42 @@ __host__ __device__ int clip_halfplane(PolyLDS<T, V>& P, int ln, int n, T dx, T 
 // rho_k * (1, k + 1/2, k^2 + k + 1/3) (the integrals of 1, U, U^2 over those pixels, local
 // coordinates) and rho at (iy, ix) come from the chunked prefixes: the part of kox's chunk right
 // of kox, whole chunks between, and the part of ix's chunk left of ix, each moved to the local
-// origin by at most a chunk plus the cell's width.  A lookup is split into load() -- three
-// independent reads, issued one piece ahead by the walk -- and eval().
+// origin by at most a chunk plus the cell's width.  The chunk-local prefix at a pixel comes from
+// its density group (cot_internal.h): the group's prefix plus the group's pixels left of it.  A
+// lookup is split into load() -- independent 64-byte record reads, issued one piece ahead by the
+// walk -- and eval().
 struct PrefixTaps {
-  double4 t;    // entry (row, kox + ix)
-  double4 qk;   // entry (row, kox)            (fresh rows only)
-  double4 tk;   // total of kox's chunk in row  (fresh rows only)
-  bool fresh;   // the piece starts a new row
+  DensityGroup t;   // group of (row, kox + ix)
+  DensityGroup qk;  // group of (row, kox)            (fresh rows only)
+  double4 tk;       // total of kox's chunk in row    (fresh rows only)
+  bool fresh;       // the piece starts a new row
 };
 
+// Prefix (s0, s1, s2) and density r at global column g of the row, from g's group record
+// (gsize pixels per group).  The second pixel of an FP32 pair adds the first one's terms with
+// prefix_step, as the prefix kernel summed them.
+__host__ __device__ __forceinline__ void group_tap(const DensityGroup& rec, int g, int gsize,
+                                                   double& s0, double& s1, double& s2,
+                                                   double& r) {
+  s0 = rec.s0;
+  s1 = rec.s1;
+  s2 = rec.s2;
+  if (gsize == kGroup32) {
+    const double r0 = (double)rec.r.f[0];
+    r = r0;
+    if (g & 1) {
+      prefix_step(s0, s1, s2, r0, (double)((g - 1) % kChunk));
+      r = (double)rec.r.f[1];
+    }
+  } else {
+    r = rec.r.d;
+  }
+}
+
 struct Prefix {
-  const double4* pre;  // this problem's row prefixes
-  const double4* tot;  // this problem's chunk totals
-  int pm, nchunk, kox, koy;
+  const DensityGroup* grp;  // this problem's density groups
+  const double4* tot;       // this problem's chunk totals
+  int ngrp, gsize, gshift, nchunk, kox, koy;  // gsize = 1 << gshift
   int lrow = -1;       // row of the last load
   double q0 = 0, q1 = 0, q2 = 0;  // prefix of kox within its chunk (chunk-local), current row
   double f0 = 0, f1 = 0, f2 = 0;  // kox .. end of its chunk, local coordinates, current row
@@ -207,12 +230,12 @@ struct Prefix {
   // Reads for the piece at (iy, ix); the row's two references only when the row changes.
   __host__ __device__ __forceinline__ PrefixTaps load(int iy, int ix) {
     const int grow = koy + iy;
-    const double4* qr = pre + (size_t)grow * pm;
+    const DensityGroup* gr = grp + (size_t)grow * ngrp;
     PrefixTaps p;
-    p.t = qr[kox + ix];
+    p.t = gr[(kox + ix) >> gshift];
     p.fresh = iy != lrow;
     if (p.fresh) {
-      p.qk = qr[kox];
+      p.qk = gr[kox >> gshift];
       p.tk = tot[(size_t)grow * nchunk + kox / kChunk];
       lrow = iy;
     }
@@ -222,9 +245,8 @@ struct Prefix {
   // those that add nothing).
   __host__ __device__ __forceinline__ void refresh(const PrefixTaps& p) {
     if (!p.fresh) return;
-    q0 = p.qk.x;
-    q1 = p.qk.y;
-    q2 = p.qk.z;
+    double rk;
+    group_tap(p.qk, kox, gsize, q0, q1, q2, rk);
     f0 = p.tk.x - q0;
     f1 = p.tk.y - q1;
     f2 = p.tk.z - q2;
@@ -235,11 +257,12 @@ struct Prefix {
                                                 double& a1, double& a2, double& r) const {
     const int g = kox + ix;
     const int ck = kox / kChunk, cg = g / kChunk;
-    r = p.t.w;
+    double t0, t1, t2;
+    group_tap(p.t, g, gsize, t0, t1, t2, r);
     if (cg == ck) {
-      a0 = p.t.x - q0;
-      a1 = p.t.y - q1;
-      a2 = p.t.z - q2;
+      a0 = t0 - q0;
+      a1 = t1 - q1;
+      a2 = t2 - q2;
       shift(a0, a1, a2, (double)(ck * kChunk - kox));
       return;
     }
@@ -255,7 +278,7 @@ struct Prefix {
       a1 += b1;
       a2 += b2;
     }
-    double b0 = p.t.x, b1 = p.t.y, b2 = p.t.z;
+    double b0 = t0, b1 = t1, b2 = t2;
     shift(b0, b1, b2, (double)(cg * kChunk - kox));
     a0 += b0;
     a1 += b1;
@@ -486,11 +509,10 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   double bcx, bcy, br2;
   bound(bcx, bcy, br2);
   int used = 0;
-  // clip by the bisector of site j (candidate from bucket ring r); `dedupe`: j may already bound
-  // an edge (a candidate list with repeats), then clipping by the same line again is skipped
-  auto consider = [&](int j, int r, bool dedupe) {
-    const double2 sj = a.xy[j];
-    const double wj = wtrial(j);
+  // clip by the bisector of site j (candidate from bucket ring r) at position sj with trial
+  // weight wj; `dedupe`: j may already bound an edge (a candidate list with repeats), then
+  // clipping by the same line again is skipped
+  auto consider_at = [&](int j, double2 sj, double wj, int r, bool dedupe) {
     const double ddx = sj.x - s.x, ddy = sj.y - s.y;
     // half-plane u.d <= c in the clipping frame (u = x - o)
     const double c = 0.5 * (ddx * ddx + ddy * ddy + wi - wj) - (osx * ddx + osy * ddy);
@@ -514,11 +536,53 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
       }
     }
   };
+  auto consider = [&](int j, int r, bool dedupe) { consider_at(j, a.xy[j], wtrial(j), r, dedupe); };
+  // candidates js[0 .. kCandBatch) (-1: none) in order, as consider() one after another; their
+  // sites and weights are loaded together first, so the batch waits on memory once rather than
+  // once per candidate (the lane walks its candidates alone: nothing else hides the latency)
+  auto consider_batch = [&](const int (&js)[kCandBatch], bool dedupe) {
+    double2 sj[kCandBatch];
+    double wj[kCandBatch];
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u) {
+      sj[u] = make_double2(0.0, 0.0);
+      wj[u] = 0.0;
+      if (js[u] >= 0) {
+        sj[u] = a.xy[js[u]];
+        wj[u] = wtrial(js[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kCandBatch; ++u)
+      if (js[u] >= 0 && nv > 0) consider_at(js[u], sj[u], wj[u], 0, dedupe);
+  };
   const int R0 = a.ring[i];
   // the previous cells' adjacency, unless their cells failed the cover check (then fresh
   // bucket rings)
   const bool fresh = a.st && a.st[bd.prob].fresh;
   const int nadj = a.cand != kCandBucket && !fresh ? (int)a.adjn[i] : 0;
+  if (a.cand != kCandBucket && !fresh && !a.redo_mode && a.defer_list) {
+    // No previous neighbours means the bucket rings, and a large two-ring many candidates: a long
+    // serial scan for one lane, which the whole wave waits on.  The wave pass that follows builds
+    // such a cell instead (the same cell: cell_site_wave), its candidates 64 at a time.
+    bool defer = nadj == 0;
+    if (!defer && a.cand == kCandAdj2) {
+      const int* row = a.adj + (size_t)i * kMaxVerts;
+      int tot2 = 0;
+      for (int k0 = 0; k0 < nadj; k0 += kCandBatch) {
+        int js[kCandBatch];
+#pragma unroll
+        for (int u = 0; u < kCandBatch; ++u) js[u] = k0 + u < nadj ? row[k0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < kCandBatch; ++u) tot2 += js[u] >= 0 ? (int)a.adjn[js[u]] : 0;
+      }
+      defer = tot2 > kDeferCand;
+    }
+    if (defer) {
+      a.defer_list[fetch_add(a.defer_count, 1)] = blk * kCellBlock + site_ln;
+      return -1;
+    }
+  }
   const int bx = min(max((int)floor((s.x - pd.bx0) * pd.binv), 0), pd.gx - 1);
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
   // candidates queued by the conformity check, taken in increasing site order (the check appends
@@ -544,9 +608,14 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     // the previous cell's neighbours ...
     const int* row = a.adj + (size_t)i * kMaxVerts;
     const bool queued = a.xcnt != nullptr;
-    for (int k = 0; k < nadj && nv > 0; ++k) {
-      const int j = row[k];
-      if (j >= 0 && j != i) consider(j, 0, queued);
+    for (int k0 = 0; k0 < nadj && nv > 0; k0 += kCandBatch) {
+      int js[kCandBatch];
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        const int j = k0 + u < nadj ? row[k0 + u] : -1;
+        js[u] = j >= 0 && j != i ? j : -1;
+      }
+      consider_batch(js, queued);
     }
     // ... and theirs (an edge flip makes a neighbour's neighbour adjacent)
     if (a.cand == kCandAdj2) {
@@ -555,9 +624,14 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
         if (j < 0) continue;
         const int nj = a.adjn[j];
         const int* rj = a.adj + (size_t)j * kMaxVerts;
-        for (int q = 0; q < nj && nv > 0; ++q) {
-          const int jj = rj[q];
-          if (jj >= 0 && jj != i) consider(jj, 0, true);
+        for (int q0 = 0; q0 < nj && nv > 0; q0 += kCandBatch) {
+          int js[kCandBatch];
+#pragma unroll
+          for (int u = 0; u < kCandBatch; ++u) {
+            const int jj = q0 + u < nj ? rj[q0 + u] : -1;
+            js[u] = jj >= 0 && jj != i ? jj : -1;
+          }
+          consider_batch(js, true);
         }
       }
     }
@@ -755,9 +829,11 @@ __host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, i
   if (nv < 3) return;
   const int2 kb = a.kbox[i];
   Prefix px;
-  px.pre = a.pre + pd.pre_off;
+  px.grp = a.grp + pd.grp_off;
   px.tot = a.tot + pd.tot_off;
-  px.pm = pd.pm;
+  px.ngrp = pd.ngrp;
+  px.gsize = pd.gsize;
+  px.gshift = pd.gsize == kGroup32 ? 1 : 0;
   px.nchunk = pd.nchunk;
   px.kox = kb.x;
   px.koy = kb.y;
@@ -857,8 +933,9 @@ template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   __shared__ PolyLDS<T, kPassVerts> P;
   const unsigned long long t0 = timer_begin();
-  const int rc = cell_site<T, kPassVerts>(a, blockIdx.x, threadIdx.x, P, threadIdx.x);
-  if (rc == 1) atomicAdd(&a.overflow[a.blocks[blockIdx.x].prob], 1);
+  const int blk = xcd_slot(blockIdx.x, gridDim.x);
+  const int rc = cell_site<T, kPassVerts>(a, blk, threadIdx.x, P, threadIdx.x);
+  if (rc == 1) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
   timer_mark(a.timer, rc >= 0, t0);
 }
 
@@ -1330,10 +1407,29 @@ template <typename T>
 __global__ __launch_bounds__(64) void cell_wave_kernel(CellArgs a) {
   __shared__ WaveLDS<T> W;
   const unsigned long long t0 = timer_begin();
-  const int blk = blockIdx.x / kCellBlock;
-  const int rc = cell_site_wave<T, kPassVerts>(a, blk, blockIdx.x % kCellBlock, W);
+  const int slot = xcd_slot(blockIdx.x, gridDim.x);
+  const int blk = slot / kCellBlock;
+  const int rc = cell_site_wave<T, kPassVerts>(a, blk, slot % kCellBlock, W);
   if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
   timer_mark(a.timer, rc >= 0, t0);
+}
+
+// The sites the lane full pass deferred (CellArgs::defer_list), one wave (block) per listed
+// site; any grid size.  Folds into the construction pass's timer.
+template <typename T>
+__global__ __launch_bounds__(64) void cell_defer_kernel(CellArgs a) {
+  __shared__ WaveLDS<T> W;
+  const unsigned long long t0 = timer_begin();
+  const int n = *a.defer_count;
+  bool worked = false;
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int slot = a.defer_list[e];
+    const int blk = slot / kCellBlock;
+    const int rc = cell_site_wave<T, kPassVerts>(a, blk, slot % kCellBlock, W);
+    if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
+    worked = worked || rc >= 0;
+  }
+  timer_mark(a.timer, worked, t0);
 }
 
 // Repair construction (FP64) with one wave (block) per listed site; any grid size.
@@ -1356,10 +1452,10 @@ __global__ __launch_bounds__(64) void cell_repair_wave_kernel(CellArgs a) {
 // consecutive lanes per site; the lanes' moments are summed with shuffles within their aligned
 // group.
 template <typename T, int L>
-__device__ __forceinline__ void integrate_block(const CellArgs& a) {
+__device__ __forceinline__ void integrate_block(const CellArgs& a, int blk, int quarter) {
   const int t = threadIdx.x;
-  const int i = integ_site<L>(a, blockIdx.x, blockIdx.y, t);
-  const int prob = a.blocks[blockIdx.x].prob;
+  const int i = integ_site<L>(a, blk, quarter, t);
+  const int prob = a.blocks[blk].prob;
   Moments M;
   if (i >= 0) integrate_lane<T, L>(a, a.probs[prob], i, t % L, M);
 #pragma unroll
@@ -1379,7 +1475,9 @@ __device__ __forceinline__ void integrate_block(const CellArgs& a) {
 // Clears the length of the work list the next check fills (nothing reads or writes that list
 // while this kernel runs).
 __device__ __forceinline__ void reset_list(const CellArgs& a) {
-  if (a.reset_count && threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0) *a.reset_count = 0;
+  if (threadIdx.x != 0 || blockIdx.x != 0 || blockIdx.y != 0) return;
+  if (a.reset_count) *a.reset_count = 0;
+  if (a.defer_count && !a.redo_mode) *a.defer_count = 0;
 }
 
 // a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
@@ -1388,8 +1486,11 @@ __global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a) {
   timer_close(a.timer);
   reset_list(a);
   const unsigned long long t0 = timer_begin();
-  const bool worked = integ_site<L>(a, blockIdx.x, blockIdx.y, threadIdx.x) >= 0;
-  integrate_block<T, L>(a);
+  // workgroup (blk, quarter) in XCD order: the quarters of a cell block stay on one XCD
+  const int slot = xcd_slot(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int blk = slot / gridDim.y, quarter = slot % gridDim.y;
+  const bool worked = integ_site<L>(a, blk, quarter, threadIdx.x) >= 0;
+  integrate_block<T, L>(a, blk, quarter);
   timer_mark(a.timer_int, worked, t0);
 }
 
@@ -1512,8 +1613,9 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
 
 __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
   timer_close(a.timer_close);
-  const int rc = verify_site(a, blockIdx.x, threadIdx.x);
-  if (rc == 1) atomicAdd(&a.fails[a.blocks[blockIdx.x].prob], 1);
+  const int blk = xcd_slot(blockIdx.x, gridDim.x);
+  const int rc = verify_site(a, blk, threadIdx.x);
+  if (rc == 1) atomicAdd(&a.fails[a.blocks[blk].prob], 1);
 }
 
 // Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
@@ -1704,25 +1806,32 @@ __global__ void scatter_u8_kernel(const uint8_t* src, const int* perm, size_t n,
   if (r < n) dst[perm[r]] = src[r];
 }
 
-// Row prefixes and chunk totals of one target image (the layout of cot_internal.h), one thread
+// Density groups and chunk totals of one target image (the layout of cot_internal.h), one thread
 // per (row, chunk): the sums of build_prefix (cot_layout.h, the host build), in the same order.
 // FP32 geometry reads the image rounded to float, as its pieces use it.
-__global__ void prefix_kernel(const double* rho, int pm, int pn, int nchunk, int fp32,
-                              double4* pre, double4* tot) {
-#pragma clang fp contract(on)
+__global__ void prefix_kernel(const double* rho, int pm, int pn, int nchunk, int ngrp, int gsize,
+                              DensityGroup* grp, double4* tot) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)pn * nchunk) return;
   const int iy = (int)(t / nchunk), c = (int)(t % nchunk);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   const int k0 = c * kChunk, k1 = min(k0 + kChunk, pm);
-  for (int k = k0; k < k1; ++k) {
-    double r = rho[(size_t)iy * pm + k];
-    if (fp32) r = (double)(float)r;
-    pre[(size_t)iy * pm + k] = make_double4(s0, s1, s2, r);
-    const double kl = (double)(k - k0);
-    s0 += r;
-    s1 += r * (kl + 0.5);
-    s2 += r * (kl * kl + kl + 1.0 / 3.0);
+  for (int g0 = k0; g0 < k1; g0 += gsize) {
+    DensityGroup* rec = grp + (size_t)iy * ngrp + g0 / gsize;
+    rec->s0 = s0;
+    rec->s1 = s1;
+    rec->s2 = s2;
+    for (int u = 0; u < gsize; ++u) {
+      const int k = g0 + u;
+      double r = k < pm ? rho[(size_t)iy * pm + k] : 0.0;
+      if (gsize == kGroup32) {
+        r = (double)(float)r;
+        rec->r.f[u] = (float)r;
+      } else {
+        rec->r.d = r;
+      }
+      if (k < k1) prefix_step(s0, s1, s2, r, (double)(k - k0));
+    }
   }
   tot[(size_t)iy * nchunk + c] = make_double4(s0, s1, s2, 0.0);
 }
@@ -1771,10 +1880,20 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
       hipLaunchKernelGGL((cell_wave_kernel<float>), g, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((cell_wave_kernel<double>), g, dim3(64), 0, st, a);
-  } else if (fp32) {
-    hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
   } else {
-    hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    if (fp32)
+      hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+    if (a.defer_list) {
+      // the deferred sites: a grid of at most kDeferWaves waves walks the list
+      const int waves = nblocks * kCellBlock < kDeferWaves ? nblocks * kCellBlock : kDeferWaves;
+      const dim3 g((unsigned)waves);
+      if (fp32)
+        hipLaunchKernelGGL((cell_defer_kernel<float>), g, dim3(64), 0, st, a);
+      else
+        hipLaunchKernelGGL((cell_defer_kernel<double>), g, dim3(64), 0, st, a);
+    }
   }
   switch (int_lanes) {
     case 4: launch_integrate<4>(a, nblocks, fp32, repair_blocks, st); break;
@@ -1784,12 +1903,14 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
   return hipGetLastError();
 }
 
-hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, double4* pre,
+hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, DensityGroup* grp,
                          double4* tot, hipStream_t st) {
   const int nchunk = (pm + kChunk - 1) / kChunk;
+  const int gsize = fp32 ? kGroup32 : kGroup64;
+  const int ngrp = (pm + gsize - 1) / gsize;
   const long long n = (long long)pn * nchunk;
   hipLaunchKernelGGL(prefix_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rho, pm,
-                     pn, nchunk, fp32 ? 1 : 0, pre, tot);
+                     pn, nchunk, ngrp, gsize, grp, tot);
   return hipGetLastError();
 }
 

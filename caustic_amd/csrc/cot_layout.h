@@ -65,23 +65,33 @@ inline void bucket_sort(const cot_problem& pr, double spb, ProbDesc& pd, std::ve
   }
 }
 
-// Row prefixes and chunk totals of one target image (layout in cot_internal.h).  FP32 geometry
-// reads the image rounded to float, as its pieces use it.
-inline void build_prefix(const double* rho, int pm, int pn, bool fp32, double4* pre,
+// Density groups and chunk totals of one target image (layout in cot_internal.h).  FP32
+// geometry reads the image rounded to float, as its pieces use it.
+inline void build_prefix(const double* rho, int pm, int pn, bool fp32, DensityGroup* grp,
                          double4* tot) {
   const int nchunk = (pm + kChunk - 1) / kChunk;
+  const int gsize = fp32 ? kGroup32 : kGroup64;
+  const int ngrp = (pm + gsize - 1) / gsize;
   for (int iy = 0; iy < pn; ++iy) {
     for (int c = 0; c < nchunk; ++c) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
       const int k0 = c * kChunk, k1 = std::min(k0 + kChunk, pm);
-      for (int k = k0; k < k1; ++k) {
-        double r = rho[(size_t)iy * pm + k];
-        if (fp32) r = (double)(float)r;
-        pre[(size_t)iy * pm + k] = make_double4(s0, s1, s2, r);
-        const double kl = (double)(k - k0);
-        s0 += r;
-        s1 += r * (kl + 0.5);
-        s2 += r * (kl * kl + kl + 1.0 / 3.0);
+      for (int g0 = k0; g0 < k1; g0 += gsize) {
+        DensityGroup& rec = grp[(size_t)iy * ngrp + g0 / gsize];
+        rec.s0 = s0;
+        rec.s1 = s1;
+        rec.s2 = s2;
+        for (int u = 0; u < gsize; ++u) {
+          const int k = g0 + u;
+          double r = k < pm ? rho[(size_t)iy * pm + k] : 0.0;
+          if (fp32) {
+            r = (double)(float)r;
+            rec.r.f[u] = (float)r;
+          } else {
+            rec.r.d = r;
+          }
+          if (k < k1) prefix_step(s0, s1, s2, r, (double)(k - k0));
+        }
       }
       tot[(size_t)iy * nchunk + c] = make_double4(s0, s1, s2, 0.0);
     }

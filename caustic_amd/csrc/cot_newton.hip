@@ -60,8 +60,8 @@ __device__ __forceinline__ double sum_partials2(const double* a, int na, const d
 
 // Sum over pieces of (parameter length) * rho(pixel) along the segment a -> b given in pixel
 // units of the problem's lattice (pm x pn pixels).
-__device__ double edge_rho(double ua, double va, double ub, double vb, const double4* pre,
-                           int pm, int pn) {
+__device__ double edge_rho(double ua, double va, double ub, double vb, const DensityGroup* grp,
+                           int ngrp, int gsize, int pm, int pn) {
   const double du = ub - ua, dv = vb - va;
   const double iu0 = floor(ua), iv0 = floor(va);
   double tu = du > 0.0 ? (iu0 + 1.0 - ua) / du : (du < 0.0 ? (iu0 - ua) / du : 2.0);
@@ -74,7 +74,9 @@ __device__ double edge_rho(double ua, double va, double ub, double vb, const dou
     const double tm = 0.5 * (t + tn);
     const int ix = min(max((int)floor(ua + tm * du), 0), pm - 1);
     const int iy = min(max((int)floor(va + tm * dv), 0), pn - 1);
-    acc += (tn - t) * pre[(size_t)iy * pm + ix].w;
+    const DensityGroup& rec = grp[(size_t)iy * ngrp + (gsize == kGroup32 ? ix >> 1 : ix)];
+    const double r = gsize == kGroup32 ? (double)rec.r.f[ix & 1] : rec.r.d;
+    acc += (tn - t) * r;
     t = tn;
     if (tu <= tv) {
       tu += su;
@@ -137,10 +139,12 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
                                                       const BlockDesc* blocks,
                                                       const uint8_t* active, const double2* xy,
                                                       const double2* vtx, const uint8_t* nedge,
-                                                      const int* elab, const double4* pre,
+                                                      const int* elab,
+                                                      const DensityGroup* grp,
                                                       const int* bagg, double* h, double* diag,
                                                       int* sagg) {
-  const BlockDesc bd = blocks[blockIdx.x];
+  const int blk = xcd_slot(blockIdx.x, gridDim.x);
+  const BlockDesc bd = blocks[blk];
   const ProbDesc pd = probs[bd.prob];
   const bool on = active[bd.prob] != 0;
   const double inv_pix = 1.0 / (pd.dx * pd.dy);
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
       if (len > 0.0) {
         const double r = edge_rho((a.x - pd.x0) / pd.dx, (a.y - pd.y0) / pd.dy,
                                   (b.x - pd.x0) / pd.dx, (b.y - pd.y0) / pd.dy,
-                                  pre + pd.pre_off, pd.pm, pd.pn);
+                                  grp + pd.grp_off, pd.ngrp, pd.gsize, pd.pm, pd.pn);
         const double2 si = xy[i], sj = xy[j];
         const double d = sqrt((si.x - sj.x) * (si.x - sj.x) + (si.y - sj.y) * (si.y - sj.y));
         hv = r * len * inv_pix / (2.0 * d);
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
     const int nv = nedge[i];
     for (int k = 0; k < nv; ++k) s += h[(size_t)i * kMaxVerts + k];
     diag[i] = s;
-    if (sagg) sagg[i] = s > 0.0 ? bagg[blockIdx.x] : -1;
+    if (sagg) sagg[i] = s > 0.0 ? bagg[blk] : -1;
   }
 }
 
@@ -667,10 +671,10 @@ __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* 
 
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
-                          const int* elab, const double4* pre, const int* bagg, double* h,
+                          const int* elab, const DensityGroup* grp, const int* bagg, double* h,
                           double* diag, int* sagg, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(hessian_kernel, dim3(nblk), dim3(256), 0, st, probs, blocks, active, xy, vtx,
-                     nedge, elab, pre, bagg, h, diag, sagg);
+                     nedge, elab, grp, bagg, h, diag, sagg);
   return hipGetLastError();
 }
 

@@ -24,7 +24,8 @@ struct Session {
   std::vector<BlockDesc> blocks;
   std::vector<double2> xy, org;
   std::vector<double> nu, w, mass, fcell, area;
-  std::vector<double4> pre, tot;
+  std::vector<DensityGroup> grp;
+  std::vector<double4> tot;
   std::vector<double2> cen, geo;
   std::vector<uint8_t> exist, ne, ring, used, adjn;
   std::vector<int> redo;
@@ -47,6 +48,37 @@ struct Lanes {
 extern "C" {
 
 int emu_max_verts() { return kMaxVerts; }
+
+int emu_xcd_slot(int w, int G) { return xcd_slot(w, G); }
+
+// Pixels (of all rows) where the density groups' lookup (group_tap) differs in any bit from the
+// sequential chunk-local prefix sum, or where the chunk totals differ from the sum over the chunk.
+int emu_prefix_mismatches(const double* rho, int pm, int pn, int fp32) {
+  const int nchunk = (pm + kChunk - 1) / kChunk;
+  const int gsize = fp32 ? kGroup32 : kGroup64;
+  const int ngrp = (pm + gsize - 1) / gsize;
+  std::vector<DensityGroup> grp((size_t)ngrp * pn);
+  std::vector<double4> tot((size_t)nchunk * pn);
+  build_prefix(rho, pm, pn, fp32 != 0, grp.data(), tot.data());
+  int bad = 0;
+  for (int iy = 0; iy < pn; ++iy) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int g = 0; g < pm; ++g) {
+      if (g % kChunk == 0) s0 = s1 = s2 = 0.0;
+      double r = rho[(size_t)iy * pm + g];
+      if (fp32) r = (double)(float)r;
+      double t0, t1, t2, tr;
+      group_tap(grp[(size_t)iy * ngrp + g / gsize], g, gsize, t0, t1, t2, tr);
+      if (t0 != s0 || t1 != s1 || t2 != s2 || tr != r) ++bad;
+      prefix_step(s0, s1, s2, r, (double)(g % kChunk));
+      if (g % kChunk == kChunk - 1 || g == pm - 1) {
+        const double4 c = tot[(size_t)iy * nchunk + g / kChunk];
+        if (c.x != s0 || c.y != s1 || c.z != s2) ++bad;
+      }
+    }
+  }
+  return bad;
+}
 
 void* emu_create(int n, const double* sites, const double* nu, const double* rho, int pm, int pn,
                  const double* proj, int fp32, int ring_init, int max_repairs, double verify_tol) {
@@ -112,9 +144,12 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
   S->xcand.assign((size_t)n * kMaxExtra, 0);
   S->espan.assign((size_t)n * kMaxVerts, make_float2(0.f, 0.f));
   pd.nchunk = (pm + kChunk - 1) / kChunk;
-  S->pre.resize((size_t)pm * pn);
+  pd.gsize = fp32 ? kGroup32 : kGroup64;
+  pd.ngrp = (pm + pd.gsize - 1) / pd.gsize;
+  pd.grp_off = 0;
+  S->grp.resize((size_t)pd.ngrp * pn);
   S->tot.resize((size_t)pd.nchunk * pn);
-  build_prefix(rho, pm, pn, fp32 != 0, S->pre.data(), S->tot.data());
+  build_prefix(rho, pm, pn, fp32 != 0, S->grp.data(), S->tot.data());
   return S;
 }
 
@@ -176,7 +211,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   a.w = S.w.data();
   a.nu = S.nu.data();
   a.buckets = S.bk.data();
-  a.pre = S.pre.data();
+  a.grp = S.grp.data();
   a.tot = S.tot.data();
   a.ring = S.ring.data();
   a.redo = S.redo.data();

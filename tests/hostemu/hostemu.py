@@ -26,6 +26,8 @@ def lib():
         _lib.emu_create.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         _lib.emu_destroy.argtypes = [ctypes.c_void_p]
+        _lib.emu_xcd_slot.argtypes = [ctypes.c_int, ctypes.c_int]
+        _lib.emu_prefix_mismatches.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib.emu_state.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
         _lib.emu_coarse_inverse.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib.emu_set_adjacency.argtypes = [ctypes.c_void_p] * 3
@@ -111,6 +113,18 @@ def evaluate(p, w, fp32=False, ring_init=2, max_repairs=12, verify_tol=1e-4):
         return s.evaluate(w)
     finally:
         s.close()
+
+
+def xcd_slot(w, G):
+    """The kernels' workgroup renumbering (cot_internal.h xcd_slot)."""
+    return int(lib().emu_xcd_slot(int(w), int(G)))
+
+
+def prefix_mismatches(rho, fp32):
+    """Pixels where the density groups' prefix lookup differs in any bit from the sequential
+    chunk-local prefix sums (or chunk totals from the chunk sums)."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    return int(lib().emu_prefix_mismatches(_P(r), r.shape[1], r.shape[0], int(fp32)))
 
 
 def coarse_inverse(a):

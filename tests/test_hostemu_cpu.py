@@ -198,3 +198,25 @@ def test_cover_check_rejects_a_consistent_double_cover():
         np.testing.assert_allclose(out2["mass"], [0.25] * 4, atol=1e-14)
     finally:
         s.close()
+
+
+def test_xcd_slot_is_a_bijection_with_contiguous_ranges_per_xcd():
+    """Workgroup w runs on XCD w % 8; xcd_slot must renumber the G workgroups onto [0, G) with
+    each XCD's slots contiguous and in increasing order of w."""
+    for G in list(range(1, 70)) + [255, 256, 257, 512, 1023, 1024, 4096 + 3]:
+        slots = [E.xcd_slot(w, G) for w in range(G)]
+        assert sorted(slots) == list(range(G)), G
+        for x in range(8):
+            mine = [slots[w] for w in range(x, G, 8)]
+            assert mine == list(range(mine[0], mine[0] + len(mine))) if mine else True, (G, x)
+
+
+@pytest.mark.parametrize("fp32", [True, False])
+@pytest.mark.parametrize("shape", [(5, 77), (3, 32), (4, 8), (2, 1), (6, 100)])
+def test_density_groups_reproduce_the_sequential_prefix(fp32, shape):
+    """The compact density groups (a prefix per group of 8 or 4 pixels plus the group's
+    densities) give, at every pixel, the same bits as summing the row's chunk sequentially --
+    rows whose length is no multiple of the group or the chunk included."""
+    rho = np.random.default_rng(11).uniform(0.0, 3.0, shape) ** 3
+    rho[0, : shape[1] // 2] = 0.0
+    assert E.prefix_mismatches(rho, fp32) == 0
