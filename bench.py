@@ -161,6 +161,12 @@ def main():
     ap.add_argument("--levels", default=None,
                     help="comma-separated coarse grid sides of the warm start (default: the "
                          "library's, cot_default_ladder_opts)")
+    ap.add_argument("--coarse-tol", type=float, default=None,
+                    help="max-rel the coarse levels are solved to (default: the library's, "
+                         "cot_default_ladder_opts)")
+    ap.add_argument("--handoff", type=float, default=None,
+                    help="max-rel at which every stage hands L-BFGS over to Newton (default: the "
+                         "library's)")
     ap.add_argument("--newton-sweeps", type=int, default=2,
                     help="block-local Jacobi sweeps of the Newton steps' CG preconditioner")
     ap.add_argument("--newton-coarse", type=int, default=256,
@@ -242,9 +248,11 @@ def main():
         the full-resolution reports."""
         with OTBatch(probs, precision="mixed", tuning=tuning["final"]) as b:
             if args.start == "multiscale":
+                extra = {k: v for k, v in (("coarse_tol", args.coarse_tol),
+                                           ("handoff", args.handoff)) if v is not None}
                 lad = b.ladder_options(level_tunings=tuning["levels"], sides=levels,
                                        floors=schedule["floors"],
-                                       final_floors=schedule["final_floors"])
+                                       final_floors=schedule["final_floors"], **extra)
                 w, rep = b.solve_ladder(ladder=lad, **kw)
             else:
                 w0 = None

@@ -2216,6 +2216,9 @@ int cot_solve(cot_context* c, double* w, const cot_solve_opts* so, cot_report* r
       if (ok) {
         acc[p] = 1;
         any_acc = true;
+        // an accepted trial passed the conformity check: an unverified start no longer
+        // describes the iterate
+        if (status[p] == COT_EUNVERIFIED) status[p] = COT_OK;
         continue;
       }
       if (t[p] <= 1e-4 * o.initial_step) {

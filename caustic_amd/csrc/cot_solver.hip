@@ -16,6 +16,7 @@
 // The "last block" of a problem is found with a per-problem counter: each block publishes its
 // partial sums, fences, and increments the counter; the block that sees count == nblk - 1 reads
 // all partials (after an acquire fence) and resets the counter.
+#include "../../include/caustic_ot.h"
 #include "cot_internal.h"
 
 namespace cot {
@@ -265,6 +266,9 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
   }
   if (ok) {
     s.accept = 1;
+    // an accepted trial passed the conformity check: an unverified start no longer describes
+    // the solve's iterate
+    if (s.status == COT_EUNVERIFIED) s.status = COT_OK;
     return;
   }
   if (s.t <= 1e-4 * sp.t0) {
