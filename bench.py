@@ -15,9 +15,9 @@ One step = complete solves from nothing but the problems, the inputs resident in
 as the caller hands them over: the context is created inside the timed region (target prefix
 tables, site bucketing, uploads) and the coarse-to-fine solve runs there too (cot_solve_ladder:
 each problem on coarse jittered grids -- 16, 32, 64 and 128 sites a side for shapes65k, up to
-256 for curves262k -- with box-downsampled targets, solved to 1e-2, each solution carried up
-through the weight field; then the full-resolution problem; every stage L-BFGS until max-rel 1
-and damped Newton steps after).  --start learned starts the full-resolution problem from the
+256 for curves262k -- with box-downsampled targets, solved to max-rel 0.1, each solution carried
+up through the weight field; then the full-resolution problem; every stage L-BFGS until max-rel
+2 with no empty cell and damped Newton steps after: the library's defaults).  --start learned starts the full-resolution problem from the
 network's field instead; --start zero from w = 0.  Launch shapes per level come from the
 autotuner (caustic_amd.autotune), run before the timed region.
 
@@ -199,11 +199,13 @@ def main():
     # verbose: the solver prints a progress line every 2 s of a solve
     kw = dict(history=10, tol=tol, stop="maxrel", h0_threshold=0.0, max_iter=20000,
               device_loop=loop, verbose=1)
-    newton = dict(newton_from=1.0, newton_coarse=args.newton_coarse,
-                  newton_sweeps=args.newton_sweeps)
     from caustic_amd.ot import ladder_defaults
 
     schedule = ladder_defaults(probs)
+    coarse_tol = args.coarse_tol if args.coarse_tol is not None else schedule["coarse_tol"]
+    handoff = args.handoff if args.handoff is not None else schedule["handoff"]
+    newton = dict(newton_from=handoff, newton_coarse=args.newton_coarse,
+                  newton_sweeps=args.newton_sweeps)
     if args.levels:
         schedule["sides"] = [int(v) for v in args.levels.split(",")]
         schedule["floors"] = [0.0] * len(schedule["sides"])
@@ -218,7 +220,7 @@ def main():
         from caustic_amd import autotune as AT
 
         tune_report = {}
-        stretch = dict(iters=400, stop_tol=1.0)
+        stretch = dict(iters=400, stop_tol=handoff)
         coarse = WS.coarse_problems(probs, levels)
         fields = None
         for li, (n_side, cps) in enumerate(zip(levels, coarse)):
@@ -227,7 +229,7 @@ def main():
             tuning["levels"][li], r = AT.tune(cps, "mixed", w0=init, log=log, **stretch)
             tune_report[f"level{n_side}"] = r
             with OTBatch(cps, precision="mixed", tuning=tuning["levels"][li]) as cb:
-                wc, _ = cb.solve(init, **dict(kw, tol=1e-2, verbose=0), **newton)
+                wc, _ = cb.solve(init, **dict(kw, tol=coarse_tol, verbose=0), **newton)
                 cen, _ = cb.centroids(wc)
                 fields = [WS.field_from_solution(cp.sites, wk, ck, n_side, cp.frame)
                           for cp, wk, ck in zip(cps, cb.split(wc), cb.split(cen.reshape(-1, 2)))]
@@ -293,7 +295,8 @@ def main():
                 "iterations": max(r["iterations"] for r in rc),
                 "evaluations": sum(r["evaluations"] for r in rc),
                 "converged": all(bool(r["converged"]) for r in rc),
-                "algorithm": "mixed precision, L-BFGS from w=0 to max-rel 1 then damped Newton"}
+                "algorithm": f"mixed precision, L-BFGS from w=0 to max-rel {handoff:g} with no "
+                             "empty cell, then damped Newton"}
         log(f"[zero start, mixed + Newton] {cold}")
     for _ in range(args.warmup):
         r = one_solve()

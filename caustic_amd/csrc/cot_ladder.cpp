@@ -28,6 +28,9 @@ namespace {
 
 using cot::set_error;
 
+constexpr double kDefaultCoarseTol = 0.1;
+constexpr double kDefaultHandoff = 2.0;
+
 // splitmix64: the coarse grids' jitter (any fixed generator serves; the jitter only has to keep
 // sites off a lattice, where cells would tie)
 struct Rng {
@@ -255,8 +258,13 @@ extern "C" {
 void cot_default_ladder_opts(const cot_problem* problems, int n_problems, cot_ladder_opts* lo) {
   if (!lo) return;
   std::memset(lo, 0, sizeof(*lo));
-  lo->coarse_tol = 1e-2;
-  lo->handoff = 1.0;
+  // measured on MI355X (N = 65,536 shapes, 64 uneven Gaussian-mixture problems, N = 262,144
+  // tilted curves): coarse levels to 0.1 rather than 1e-2 and the Newton hand-off at max-rel 2
+  // rather than 1 took 9%, 44% and 7% off the solves (the next level's transfer error dwarfs a
+  // coarse level's last decade; the last L-BFGS iterations before max-rel 1 toggle empty cells
+  // while Newton steps converge); a hand-off at 3 made the batch 2.6x slower
+  lo->coarse_tol = kDefaultCoarseTol;
+  lo->handoff = kDefaultHandoff;
   lo->seed = 0x5eedull;
   if (!problems || n_problems <= 0) return;
   int nmin = problems[0].n_sites;
@@ -323,8 +331,8 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
   const cot_options base = *cot::context_options(ctx);
   cot_solve_opts coarse = so;
   coarse.stop_rule = COT_STOP_MAXREL;
-  coarse.tol = lo.coarse_tol > 0 ? lo.coarse_tol : 1e-2;
-  const double handoff = lo.handoff > 0 ? lo.handoff : 1.0;
+  coarse.tol = lo.coarse_tol > 0 ? lo.coarse_tol : kDefaultCoarseTol;
+  const double handoff = lo.handoff > 0 ? lo.handoff : kDefaultHandoff;
 
   // the targets' block sums for every level
   std::vector<Pyramid> pyr(np);

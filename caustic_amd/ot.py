@@ -48,14 +48,15 @@ def _problem_array(problems):
 
 def ladder_defaults(problems):
     """The library's default coarse-to-fine schedule for these problems
-    (cot_default_ladder_opts): {"sides", "floors", "final_floors"}."""
+    (cot_default_ladder_opts): {"sides", "floors", "final_floors", "coarse_tol", "handoff"}."""
     arr, keep = _problem_array(list(problems))
     lo = L.cot_ladder_opts()
     L.load().cot_default_ladder_opts(arr, len(problems), ctypes.byref(lo))
     del keep
     return {"sides": [lo.sides[k] for k in range(lo.n_levels)],
             "floors": [lo.floors[k] for k in range(lo.n_levels)],
-            "final_floors": [lo.final_floors[k] for k in range(lo.n_final_floors)]}
+            "final_floors": [lo.final_floors[k] for k in range(lo.n_final_floors)],
+            "coarse_tol": lo.coarse_tol, "handoff": lo.handoff}
 
 
 class OTBatch:

@@ -225,7 +225,7 @@ typedef struct {
 /* Defaults for these problems: levels 16, 32, 64, ... while a level holds at most half the
  * smallest problem's sites (side^2 <= n_min / 2); floors only for a target whose smallest pixel is below 1e-3 of
  * its mean (zero-density regions): 0.1 at the first level, divided by sqrt(10) per level, and one
- * full-resolution stage at 1e-3; coarse_tol 1e-2, handoff 1. */
+ * full-resolution stage at 1e-3; coarse_tol 0.1, handoff 2. */
 void cot_default_ladder_opts(const cot_problem* problems, int n_problems, cot_ladder_opts* lo);
 int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
                      const cot_solve_opts* opt, const cot_ladder_opts* lo, cot_report* reports);
