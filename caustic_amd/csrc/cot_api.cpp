@@ -1758,12 +1758,17 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       HIPCHK(launch_cg_iteration(a, pbuf[k & 1], pbuf[(k + 1) & 1], k & 1, 0, st));
     return COT_OK;
   };
-  auto sum_rr = [&](double& rr) -> int {
+  // |r|^2 of every problem: the sum of its blocks' partials
+  auto sum_rr = [&](std::vector<double>& rr) -> int {
     HIPCHK(hipMemcpyAsync(rr_part.data(), c->d_prr, sizeof(double) * nb, hipMemcpyDeviceToHost,
                           st));
     HIPCHK(hipStreamSynchronize(st));
-    rr = 0.0;
-    for (double v : rr_part) rr += v;
+    for (int p = 0; p < np; ++p) {
+      const ProbDesc& pd = c->probs[p];
+      double v = 0.0;
+      for (int b = pd.blk_off; b < pd.blk_off + pd.nblk; ++b) v += rr_part[b];
+      rr[p] = v;
+    }
     return COT_OK;
   };
   for (int it = 0; it < o.max_iter; ++it) {
@@ -1820,12 +1825,29 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       HIPCHK(hipMemcpyAsync(c->d_ainv, c->h_ac, sizeof(double) * c->coarse_mat,
                             hipMemcpyHostToDevice, st));
     }
-    double rr0 = 0.0, rr = 0.0, worst = 0.0;
-    for (int p = 0; p < np; ++p)
-      if (act[p]) worst = std::max(worst, cur[p].maxrel);
-    const double eta = std::min(0.1, std::max(1e-4, 0.1 * worst));
-    // CG to |r| <= eta |r0| with the arguments a (graph slot gs); returns the iterations run, or
-    // -1 when the residual grew past 1e4 |r0| (or is not a number)
+    // each problem's CG tolerance from its own state (a batch's problems converge at their own
+    // pace: one problem's loose tolerance, or the batch's summed residual, would stop the CG of
+    // another far from its solution -- its Newton step then fails and L-BFGS takes over)
+    std::vector<double> rr0(np, 0.0), rr(np, 0.0), eta(np, 0.0);
+    double eta_min = 0.1;
+    for (int p = 0; p < np; ++p) {
+      if (!act[p]) continue;
+      eta[p] = std::min(0.1, std::max(1e-4, 0.1 * cur[p].maxrel));
+      eta_min = std::min(eta_min, eta[p]);
+    }
+    // stop when every active problem has |r_p| <= eta_p |r0_p|; diverged when one grew past
+    // 1e4 |r0_p| (or is not a number)
+    auto cg_state = [&]() -> int {  // 1 done, 0 continue, -1 diverged
+      bool done = true;
+      for (int p = 0; p < np; ++p) {
+        if (!act[p]) continue;
+        if (!(rr[p] <= 1e8 * rr0[p])) return -1;
+        if (rr[p] > eta[p] * eta[p] * rr0[p]) done = false;
+      }
+      return done ? 1 : 0;
+    };
+    // CG to every problem's tolerance with the arguments a (graph slot gs); returns the
+    // iterations run, negative when it diverged
     auto run_cg = [&](const CgArgs& a, int gs, int& iters) -> int {
       const bool co = a.ntask > 0;
       HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag,
@@ -1853,11 +1875,12 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
         }
         iters += check;
         CHK(sum_rr(rr));
-        if (!(rr <= 1e8 * rr0)) {
+        const int state = cg_state();
+        if (state < 0) {
           iters = -iters;
           return COT_OK;
         }
-        if (!(rr > eta * eta * rr0)) break;
+        if (state > 0) break;
       }
       return COT_OK;
     };
@@ -1883,15 +1906,15 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       sa.cdesc = c->d_cdesc;
       sa.ainv = c->d_ainv;
       sa.x = c->d_nx;
-      sa.eta2 = eta * eta;
+      sa.eta2 = eta_min * eta_min;  // one residual for the workgroup's problems: the strictest
       sa.out = c->d_cgout;
       HIPCHK(launch_cg_small(sa, st));
       double res[4];
       HIPCHK(hipMemcpyAsync(res, c->d_cgout, sizeof(res), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       iters = (int)res[0];
-      rr0 = res[1];
-      rr = res[2];
+      std::fill(rr0.begin(), rr0.end(), res[1]);
+      std::fill(rr.begin(), rr.end(), res[2]);
       if (res[3] != 0.0) iters = -iters;
       return COT_OK;
     };
@@ -1968,7 +1991,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       for (int p = 0; p < np; ++p)
         if (act[p])
           std::printf("[cot] newton prob %d iter %d cg %d |r|/|r0| %.2e step %g maxrel %.3e\n", p,
-                      iters[p], cg, std::sqrt(rr / std::max(rr0, 1e-300)), t[p],
+                      iters[p], cg, std::sqrt(rr[p] / std::max(rr0[p], 1e-300)), t[p],
                       cur[p].maxrel);
       std::fflush(stdout);
     }

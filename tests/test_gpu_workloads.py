@@ -44,6 +44,12 @@ def test_batch64_solve_confirmed_by_oracle(gpu_available):
         w, rep = b.solve_ladder(tol=1e-5, verbose=1)
         ws = b.split(w)
     assert all(r["converged"] for r in rep), [r for r in rep if not r["converged"]][:3]
+    # every problem's Newton steps converge: when the batch's conjugate gradients stopped on the
+    # summed residual at the loosest problem's tolerance, small problems' steps failed and
+    # L-BFGS took them the rest of the way (up to 1,051 iterations at the final level, against
+    # at most 87 with per-problem stopping)
+    its = [r["iterations"] for r in rep]
+    assert max(its) < 300, sorted(its)[-5:]
     worst = 0.0
     for k, (p, wk) in enumerate(zip(probs, ws)):
         mr, zeros = _confirm(p, wk)
