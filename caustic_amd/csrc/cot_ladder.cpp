@@ -220,33 +220,54 @@ void weights_from(const std::vector<double>& U, int side, const double* fr, cons
 }
 
 // One stage: L-BFGS until max-rel < handoff with no empty cell, Newton steps (at most
-// kNewtonSteps) to the stopping rule of `so`, L-BFGS to it if the Newton steps stall.  w in-out.
+// kNewtonSteps) to the stopping rule of `so`.  When a problem's Newton steps stall (a direction
+// no damping makes acceptable, far from the solution), the batch goes back to L-BFGS until a
+// hand-off kHandoffCut times tighter -- and below the stalled problems' max-rel over kHandoffCut,
+// so each of them moves before Newton restarts from a new point -- up to kNewtonRetries times;
+// then L-BFGS to the stopping rule.  Problems already converged stop at once in every later
+// call.  w in-out.
 constexpr int kNewtonSteps = 100;
+constexpr int kNewtonRetries = 2;
+constexpr double kHandoffCut = 4.0;
 int stage(cot_context* ctx, double* w, const cot_solve_opts& so, double handoff, int np,
           std::vector<cot_report>& rep) {
   rep.assign(np, cot_report{});
-  std::vector<cot_report> r1(np), r2(np);
-  cot_solve_opts a = so;
-  a.stop_rule = COT_STOP_HANDOFF;
-  a.tol = handoff;
-  int rc = cot_solve(ctx, w, &a, r1.data());
-  if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
+  std::vector<cot_report> r(np);
+  auto add = [&](bool last) {
+    for (int p = 0; p < np; ++p) {
+      const int it = rep[p].iterations + r[p].iterations;
+      const int ev = rep[p].evaluations + r[p].evaluations;
+      if (last) rep[p] = r[p];
+      rep[p].iterations = it;
+      rep[p].evaluations = ev;
+    }
+  };
   cot_solve_opts nt = so;
   nt.max_iter = std::min(so.max_iter, kNewtonSteps);
-  rc = cot_solve_newton(ctx, w, &nt, r2.data());
-  if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
-  bool all = true;
-  for (int p = 0; p < np; ++p) all = all && r2[p].converged;
-  std::vector<cot_report> r3(np);
-  if (!all) {
-    rc = cot_solve(ctx, w, &so, r3.data());
+  bool all = false;
+  double tol = handoff;
+  for (int attempt = 0; attempt <= kNewtonRetries && !all && tol > so.tol; ++attempt) {
+    cot_solve_opts a = so;
+    a.stop_rule = COT_STOP_HANDOFF;
+    a.tol = tol;
+    int rc = cot_solve(ctx, w, &a, r.data());
     if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
+    add(false);
+    rc = cot_solve_newton(ctx, w, &nt, r.data());
+    if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
+    add(true);
+    all = true;
+    double stalled = INFINITY;
+    for (int p = 0; p < np; ++p) {
+      all = all && r[p].converged;
+      if (!r[p].converged) stalled = std::min(stalled, r[p].max_rel_err);
+    }
+    tol = std::min(tol, stalled) / kHandoffCut;
   }
-  for (int p = 0; p < np; ++p) {
-    const cot_report& last = all ? r2[p] : r3[p];
-    rep[p] = last;
-    rep[p].iterations = r1[p].iterations + r2[p].iterations + (all ? 0 : r3[p].iterations);
-    rep[p].evaluations = r1[p].evaluations + r2[p].evaluations + (all ? 0 : r3[p].evaluations);
+  if (!all) {
+    const int rc = cot_solve(ctx, w, &so, r.data());
+    if (rc != COT_OK && rc != COT_ENOTCONV) return rc;
+    add(true);
   }
   return COT_OK;
 }
