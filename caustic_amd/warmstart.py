@@ -504,8 +504,11 @@ def residual(problems, ws, precision="mixed", batch=None):
     finally:
         if own:
             b.close()
-    if not ev["verified"]:  # cells too wild to resolve: no better than anything
-        return [float("inf")] * len(problems)
+    if not ev["verified"]:
+        if len(problems) == 1:  # cells too wild to resolve: no better than anything
+            return [float("inf")]
+        # the batch's check reports the batch: find the problems whose cells verify, one by one
+        return [residual([p], [w], precision)[0] for p, w in zip(problems, ws)]
     return [float(np.linalg.norm(m - p.nu)) for m, p in zip(ms, problems)]
 
 
@@ -551,6 +554,51 @@ def evaluate_warm_start(net: WarmStartNet, seeds, S: int = S_DEFAULT, res: int =
         log(f"[eval] seed {sd}: cold {rows[-1]['cold_iters']} it {t_cold:.2f}s, learned "
             f"{rows[-1]['warm_iters']} it {t_warm:.2f}s (start residual {rp:.3g} vs {rz:.3g})")
     return rows
+
+
+def evaluate_warm_start_batched(net: WarmStartNet, seeds, batch: int = 64,
+                                S: int = S_DEFAULT, res: int = 512, precision="mixed",
+                                newton_from: float = 2.0, log=print):
+    """Held-out problems solved to max-rel 1e-5 in batches (one context per batch and start):
+    from zeros, from the network (learned_start, zeros per problem where its |mass - nu|_2 is
+    the larger: choose_start) and, for reference, by the library's coarse-to-fine ladder
+    (cot_solve_ladder, no learning).  Per problem: the final stage's iterations of each start;
+    per batch: wall-clock of each (the learned start's includes the network, the low-pass and
+    the residual comparison; every problem of a batch finishes with the batch)."""
+    from .ot import OTBatch
+
+    kw = dict(history=10, tol=1e-5, stop="maxrel", h0_threshold=0.0, max_iter=20000,
+              raise_on_fail=False, newton_from=newton_from)
+    rows, batches = [], []
+    seeds = [int(v) for v in seeds]
+    for b0 in range(0, len(seeds), batch):
+        sd = seeds[b0:b0 + batch]
+        probs = [W.warm_start_sample(k, n_side=S, res=res) for k in sd]
+        with OTBatch(probs, precision=precision) as b:
+            t0 = time.perf_counter()
+            _, rc = b.solve(None, **kw)
+            t_zero = time.perf_counter() - t0
+        with OTBatch(probs, precision=precision) as b:
+            t0 = time.perf_counter()
+            w0, rp, rz = choose_start(probs, learned_start(net, probs), precision, b)
+            _, rw = b.solve(np.concatenate(w0), **kw)
+            t_learn = time.perf_counter() - t0
+        with OTBatch(probs, precision=precision) as b:
+            t0 = time.perf_counter()
+            _, rl = b.solve_ladder(tol=1e-5, raise_on_fail=False)
+            t_ladder = time.perf_counter() - t0
+        for k, seed in enumerate(sd):
+            rows.append({"seed": seed, "zero_iters": rc[k]["iterations"],
+                         "learned_iters": rw[k]["iterations"],
+                         "ladder_final_iters": rl[k]["iterations"],
+                         "learned_used": bool(rp[k] < rz[k]),
+                         "converged": bool(rc[k]["converged"] and rw[k]["converged"]
+                                           and rl[k]["converged"])})
+        batches.append({"problems": len(sd), "zero_s": t_zero, "learned_s": t_learn,
+                        "ladder_s": t_ladder})
+        log(f"[eval batch] {len(sd)} problems: zero {t_zero:.2f}s, learned {t_learn:.2f}s, "
+            f"ladder {t_ladder:.2f}s")
+    return rows, batches
 
 
 def main(argv=None):
