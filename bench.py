@@ -43,13 +43,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes moved per evaluation (construction + integration launch) of the N=65,536 workload,
-# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each scaled by the same
-# counter's reading for a copy of known size (profiles/shapes65k_eval_traffic_pmc.md)
-TRAFFIC_BYTES = {"shapes65k": 88.98e6}
+# HBM bytes moved per evaluation (construction: lane pass + deferred wave pass; integration) of
+# the N=65,536 workload, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each
+# scaled by the same counter's reading for a copy of known size
+# (profiles/shapes65k_eval_traffic_pmc.md)
+TRAFFIC_BYTES = {"shapes65k": 59.41e6}
 # VALU issue utilisation of the evaluation kernels, rocprofv3 SQ counters
-# (profiles/shapes65k_eval_valu_pmc.md): construction and integration
-VALU_BUSY = {"shapes65k": {"cell_kernel": 0.060, "integrate_kernel": 0.168}}
+# (profiles/shapes65k_eval_valu_pmc.md)
+VALU_BUSY = {"shapes65k": {"cell_kernel": 0.109, "cell_defer_kernel": 0.051,
+                           "integrate_kernel": 0.231}}
 WORKLOADS = {
     "shapes65k": {
         "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
@@ -381,7 +383,8 @@ def main():
         "zero_start_mixed_newton": cold,
         "learned_start_mixed_newton": learned,
         "roofline": {
-            "kernel": "cell evaluation: cell_kernel (construction) + integrate_kernel (exact "
+            "kernel": "cell evaluation: construction (cell_kernel + its deferred wave pass "
+                      "cell_defer_kernel, one in-kernel timer span) + integrate_kernel (exact "
                       "mass integral), full-resolution passes of the timed solves",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(args.workload),
