@@ -347,3 +347,37 @@ def test_gaussian_splat_conserves_interior_energy():
     corner = O.gaussian_splat(np.array([[0.05, 0.05]]), np.array([1.0]), 10, 10, (0, 1, 0, 1))
     w = np.exp(-(np.arange(-2, 3)[:, None] ** 2 + np.arange(-2, 3)[None, :] ** 2) / 0.5)
     assert abs(corner.sum() - w[2:, 2:].sum() / w.sum()) < 1e-14
+
+
+def test_tilted_plane_cost_reduces_to_the_projected_sites():
+    """The N = 262,144 workload's tilt: sites on a plane tilted by theta about the y axis, at
+    space points p = (a cos theta, b, a sin theta), with the quadratic cost |q - p|^2 to target
+    points q = (q1, q2, Z).  Brute force over the 3-D distances assigns every sample point to
+    the same site as the power diagram of the projected sites p' = (a cos theta, b) with
+    weights w - (Z - a sin theta)^2 -- the reduction tilted_curves relies on (its sites are
+    p' + (1/2) (1 - cos theta, 0)); and on the oracle, the cells' masses agree."""
+    rng = np.random.default_rng(31)
+    theta, Z = np.radians(30.0), 0.7
+    n = 40
+    ab = rng.uniform(0.0, 1.0, (n, 2))
+    w = rng.normal(0.0, 3e-3, n)
+    P3 = np.stack([ab[:, 0] * np.cos(theta), ab[:, 1], ab[:, 0] * np.sin(theta)], axis=1)
+    Pp = P3[:, :2]
+    h = (Z - P3[:, 2]) ** 2
+    g = (np.arange(150) + 0.5) / 150
+    Q = np.stack(np.meshgrid(g, g, indexing="xy"), axis=-1).reshape(-1, 2)
+    d3 = ((Q[:, None, 0] - P3[None, :, 0]) ** 2 + (Q[:, None, 1] - P3[None, :, 1]) ** 2
+          + (Z - P3[None, :, 2]) ** 2) - w[None, :]
+    d2 = ((Q[:, None, :] - Pp[None, :, :]) ** 2).sum(-1) - (w - h)[None, :]
+    a3, a2 = d3.argmin(1), d2.argmin(1)
+    gap = np.sort(d3, axis=1)[:, 1] - np.sort(d3, axis=1)[:, 0]
+    clear = gap > 1e-12
+    assert np.array_equal(a3[clear], a2[clear])
+    # the same problem through the oracle: masses of the projected problem with shifted weights
+    rho = np.ones((64, 64)) / 64 ** 2
+    nu = np.full(n, 1.0 / n)
+    e2 = O.evaluate(Pp, w - h, nu, rho, (0.0, 1.0, 0.0, 1.0))
+    e2b = O.evaluate(Pp, w - h + 0.123, nu, rho, (0.0, 1.0, 0.0, 1.0))
+    assert np.allclose(e2.mass, e2b.mass, atol=1e-14)  # the gauge: a constant changes no cell
+    share = np.bincount(a3[clear], minlength=n) / clear.sum()
+    assert np.abs(share - e2.mass).max() < 0.01  # sample-point shares against exact masses
