@@ -109,16 +109,16 @@ hipError_t launch_aa_direction(const BlockDesc* blocks, const double* alpha, int
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
                           const int* elab, const DensityGroup* grp, const int* bagg, double* h,
-                          double* diag, int* sagg, int nblk, hipStream_t st);
+                          int* hj, double* diag, int* sagg, int nblk, hipStream_t st);
 hipError_t launch_coarse_rows(const BlockDesc* blocks, const int* bagg, const int* sagg,
-                              const uint8_t* nedge, const int* elab, const double* h,
+                              const uint8_t* nedge, const int* hj, const double* h,
                               CoarseEntry* out, int nblk, hipStream_t st);
 hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdesc,
                             const int* agg_start, const int* agg_blk, const double* rsum,
                             const double* ainv, double* cvec, double* part_rzc, hipStream_t st);
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
                           const double* shift, const double* diag, const uint8_t* nedge,
-                          const int* elab, const double* h, int sweeps, double* x, double* r,
+                          const int* hj, const double* h, int sweeps, double* x, double* r,
                           double* z, double* part_rz, double* part_rr, double* rsum, int nblk,
                           hipStream_t st);
 hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_new, int cur,
@@ -129,7 +129,7 @@ hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, cons
 hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
                          const uint8_t* nedge, const double* nu, double* image, size_t pixels,
                          int nblk, hipStream_t st);
-hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* hj,
                                 const double* h, const double* diag, const double* p, double* q,
                                 double* part_pq, int nblk, hipStream_t st);
 hipError_t launch_axpy(const BlockDesc* blocks, double t, const double* d, double* w, int nblk,
@@ -240,6 +240,7 @@ struct cot_context {
   double* d_coef = nullptr;
   int *d_slot = nullptr, *d_hcount = nullptr;
   // Newton steps: edge Hessian entries, diagonal, CG vectors and partials
+  int* d_hj = nullptr;  // Newton: neighbour of each Hessian edge slot (hslot order)
   double *d_hh = nullptr, *d_hdiag = nullptr, *d_nx = nullptr, *d_nr = nullptr, *d_nz = nullptr;
   double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
   double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
@@ -1484,7 +1485,8 @@ namespace {
 int ensure_newton(cot_context* c) {
   if (c->d_hh) return COT_OK;
   const size_t N = c->ntot, nb = c->nblk;
-  CHK(dalloc(c, &c->d_hh, N * kMaxVerts));
+  CHK(dalloc(c, &c->d_hh, nb * kMaxVerts * kCellBlock));  // hslot order
+  CHK(dalloc(c, &c->d_hj, nb * kMaxVerts * kCellBlock));
   CHK(dalloc(c, &c->d_hdiag, N));
   CHK(dalloc(c, &c->d_nx, N));
   CHK(dalloc(c, &c->d_nr, N));
@@ -1661,9 +1663,10 @@ int cot_hessian_product(cot_context* c, const double* w, const double* x, double
   std::memset(c->h_active, 1, c->np);
   HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, c->np, hipMemcpyHostToDevice, c->st));
   HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                        c->d_elab, c->d_grp, nullptr, c->d_hh, c->d_hdiag, nullptr, nb, c->st));
+                        c->d_elab, c->d_grp, nullptr, c->d_hh, c->d_hj, c->d_hdiag, nullptr, nb,
+                        c->st));
   CHK(upload_sorted(c, x, c->d_np));
-  HIPCHK(launch_hessian_apply(c->d_blocks, c->d_nedge, c->d_elab, c->d_hh, c->d_hdiag, c->d_np,
+  HIPCHK(launch_hessian_apply(c->d_blocks, c->d_nedge, c->d_hj, c->d_hh, c->d_hdiag, c->d_np,
                               c->d_nq, c->d_ppq, nb, c->st));
   CHK(download_sorted(c, c->d_nq, y));
   return COT_OK;
@@ -1712,7 +1715,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   ca.sweeps = sweeps;
   ca.ntask = coarse ? c->coarse_ntask : 0;
   ca.nedge = c->d_nedge;
-  ca.elab = c->d_elab;
+  ca.hj = c->d_hj;
   ca.h = c->d_hh;
   ca.diag = c->d_hdiag;
   ca.x = c->d_nx;
@@ -1793,12 +1796,13 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
                           st));
     // Hessian at the current cells, then H x = -(g - mean g)
     HIPCHK(launch_hessian(c->d_probs, c->d_blocks, c->d_active, c->d_xy, c->d_vtx, c->d_nedge,
-                          c->d_elab, c->d_grp, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hdiag,
+                          c->d_elab, c->d_grp, coarse ? c->d_bagg : nullptr, c->d_hh, c->d_hj,
+                          c->d_hdiag,
                           coarse ? c->d_sagg : nullptr, nb, st));
     if (coarse) {
       // the blocks' couplings down; the patches' Laplacian summed over each patch's blocks in
       // block order and inverted (host FP64); the inverses up
-      HIPCHK(launch_coarse_rows(c->d_blocks, c->d_bagg, c->d_sagg, c->d_nedge, c->d_elab,
+      HIPCHK(launch_coarse_rows(c->d_blocks, c->d_bagg, c->d_sagg, c->d_nedge, c->d_hj,
                                 c->d_hh, c->d_crows, nb, st));
       HIPCHK(hipMemcpyAsync(c->h_crows, c->d_crows, sizeof(CoarseEntry) * (size_t)nb * kBlockNbr,
                             hipMemcpyDeviceToHost, st));
@@ -1851,7 +1855,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     auto run_cg = [&](const CgArgs& a, int gs, int& iters) -> int {
       const bool co = a.ntask > 0;
       HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag,
-                            c->d_nedge, c->d_elab, c->d_hh, a.sweeps, c->d_nx, c->d_nr, c->d_nz,
+                            c->d_nedge, c->d_hj, c->d_hh, a.sweeps, c->d_nx, c->d_nr, c->d_nz,
                             c->d_prz, c->d_prr, co ? c->d_rsum : nullptr, nb, st));
       if (co)
         HIPCHK(launch_cg_coarse(c->d_ctask, c->coarse_ntask, c->d_cdesc, c->d_agg_start,
@@ -1897,7 +1901,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       sa.shift = c->d_nshift;
       sa.diag = c->d_hdiag;
       sa.nedge = c->d_nedge;
-      sa.elab = c->d_elab;
+      sa.hj = c->d_hj;
       sa.h = c->d_hh;
       sa.sagg = a.ntask > 0 ? c->d_sagg : nullptr;
       sa.agg_start = c->d_agg_start;

@@ -139,6 +139,15 @@ __host__ __device__ inline int xcd_slot(int w, int G) {
   return x * q + (x < r ? x : r) + (w >> 3);
 }
 
+// Newton edge arrays (the Hessian's couplings h_ij and their neighbours' labels) in
+// block-transposed order: slot k of lane ln of cell block b at (b kMaxVerts + k) kCellBlock + ln,
+// so the lanes of a wave read one slot of 64 consecutive sites as one contiguous segment (the
+// cells' own edge arrays keep a site's kMaxVerts slots together, which the conjugate gradients'
+// lanes read 64 rows apart).
+__host__ __device__ inline size_t hslot(int b, int k, int ln) {
+  return ((size_t)b * kMaxVerts + k) * kCellBlock + ln;
+}
+
 // The cover check (kCoverTol): do the cells' areas add up to the target rectangle's?
 __host__ __device__ inline bool covers_domain(const ProbDesc& pd, double area) {
   const double dom = (pd.x1 - pd.x0) * (pd.y1 - pd.y0);
@@ -181,8 +190,8 @@ struct SmallCgArgs {
   const double* shift;
   const double* diag;
   const uint8_t* nedge;
-  const int* elab;
-  const double* h;
+  const int* hj;          // neighbour of each edge slot (hslot order)
+  const double* h;        // h_ij of each edge slot (hslot order)
   const int* sagg;        // nullptr: no coarse term
   const int* agg_start;
   const int* agg_blk;
@@ -201,8 +210,8 @@ struct CgArgs {
   const BlockDesc* blocks;
   int nblk, sweeps, ntask, pad;
   const uint8_t* nedge;
-  const int* elab;
-  const double* h;
+  const int* hj;          // neighbour of each edge slot (hslot order)
+  const double* h;        // h_ij of each edge slot (hslot order)
   const double* diag;
   double *x, *r, *z, *q;
   double *part_rz, *part_pq, *part_rr, *part_rzc;

@@ -95,8 +95,8 @@ __device__ double edge_rho(double ua, double va, double ub, double vb, const Den
 // definite operator (D^-1/2 B D^-1/2 has spectral radius below 1: H is diagonally dominant and a
 // block has neighbours outside it).  sweeps = 0 is Jacobi.  Every thread of the block must call
 // this (it synchronises).
-__device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
-                               const uint8_t* nedge, const int* elab, const double* h,
+__device__ double precondition(int b, const BlockDesc& bd, int i, double ri, double dg,
+                               const uint8_t* nedge, const int* hj, const double* h,
                                int sweeps, double* zl) {
   const int ln = threadIdx.x;
   const bool own = ln < bd.count && dg > 0.0;
@@ -109,8 +109,8 @@ __device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
   double hl[kEdgeBatch];
 #pragma unroll
   for (int u = 0; u < kEdgeBatch; ++u) {
-    jl[u] = u < nv ? elab[(size_t)i * kMaxVerts + u] - bd.first : -1;
-    hl[u] = u < nv ? h[(size_t)i * kMaxVerts + u] : 0.0;
+    jl[u] = u < nv ? hj[hslot(b, u, ln)] - bd.first : -1;
+    hl[u] = u < nv ? h[hslot(b, u, ln)] : 0.0;
   }
   for (int s = 0; s < sweeps; ++s) {
     zl[ln] = z;
@@ -121,8 +121,8 @@ __device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
       for (int u = 0; u < kEdgeBatch; ++u)
         if (jl[u] >= 0 && jl[u] < bd.count) acc += hl[u] * zl[jl[u]];
       for (int k = kEdgeBatch; k < nv; ++k) {
-        const int j = elab[(size_t)i * kMaxVerts + k] - bd.first;
-        if (j >= 0 && j < bd.count) acc += h[(size_t)i * kMaxVerts + k] * zl[j];
+        const int j = hj[hslot(b, k, ln)] - bd.first;
+        if (j >= 0 && j < bd.count) acc += h[hslot(b, k, ln)] * zl[j];
       }
       z = acc / dg;
     }
@@ -133,23 +133,27 @@ __device__ double precondition(const BlockDesc& bd, int i, double ri, double dg,
 
 }  // namespace
 
-// h[site * kMaxVerts + k] = h_ij of edge k of the site's cell (0 for sides and missing edges),
-// diag[site] = sum_k h.  Sites of inactive problems and empty cells get zeros.
+// h[hslot(block, k, lane)] = h_ij of edge k of the site's cell (0 for sides), hj the same slot's
+// neighbour (the edge label), diag[site] = sum_k h.  Sites of inactive problems and empty cells
+// get zeros.  Slots beyond a cell's edge count are not written (readers stop at nedge).
 __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
                                                       const BlockDesc* blocks,
                                                       const uint8_t* active, const double2* xy,
                                                       const double2* vtx, const uint8_t* nedge,
                                                       const int* elab,
                                                       const DensityGroup* grp,
-                                                      const int* bagg, double* h, double* diag,
-                                                      int* sagg) {
+                                                      const int* bagg, double* h, int* hj,
+                                                      double* diag, int* sagg) {
   const int blk = xcd_slot(blockIdx.x, gridDim.x);
   const BlockDesc bd = blocks[blk];
   const ProbDesc pd = probs[bd.prob];
   const bool on = active[bd.prob] != 0;
   const double inv_pix = 1.0 / (pd.dx * pd.dy);
-  for (int e = threadIdx.x; e < bd.count * kMaxVerts; e += blockDim.x) {
-    const int i = bd.first + e / kMaxVerts, k = e % kMaxVerts;
+  // slot-major: consecutive threads take consecutive lanes of one slot (coalesced stores)
+  for (int e = threadIdx.x; e < kMaxVerts * kCellBlock; e += blockDim.x) {
+    const int k = e / kCellBlock, ln = e % kCellBlock;
+    if (ln >= bd.count) continue;
+    const int i = bd.first + ln;
     const int nv = nedge[i];
     if (k >= nv) continue;  // readers stop at the cell's edge count
     double hv = 0.0;
@@ -167,14 +171,15 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
         hv = r * len * inv_pix / (2.0 * d);
       }
     }
-    h[(size_t)i * kMaxVerts + k] = hv;
+    h[hslot(blk, k, ln)] = hv;
+    hj[hslot(blk, k, ln)] = j;
   }
   __syncthreads();
   if ((int)threadIdx.x < bd.count) {
-    const int i = bd.first + threadIdx.x;
+    const int ln = threadIdx.x, i = bd.first + ln;
     double s = 0.0;
     const int nv = nedge[i];
-    for (int k = 0; k < nv; ++k) s += h[(size_t)i * kMaxVerts + k];
+    for (int k = 0; k < nv; ++k) s += h[hslot(blk, k, ln)];
     diag[i] = s;
     if (sagg) sagg[i] = s > 0.0 ? bagg[blk] : -1;
   }
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(256) void hessian_kernel(const ProbDesc* probs,
 // weaker but still valid (Laplacian, so SPD after the rank-one term) coarse matrix.
 __global__ __launch_bounds__(kCellBlock) void coarse_rows_kernel(
     const BlockDesc* blocks, const int* bagg, const int* sagg, const uint8_t* nedge,
-    const int* elab, const double* h, CoarseEntry* out) {
+    const int* hj, const double* h, CoarseEntry* out) {
   __shared__ int ids[kBlockNbr];
   __shared__ int wmin[kCellBlock / 64];
   __shared__ double red[kCellBlock / 64];
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(kCellBlock) void coarse_rows_kernel(
     int m = INT32_MAX;
     if (live)
       for (int e = 0; e < nv; ++e) {
-        const int j = elab[(size_t)i * kMaxVerts + e];
+        const int j = hj[hslot(blockIdx.x, e, threadIdx.x)];
         if (j < 0) continue;
         const int b = sagg[j];
         if (b >= 0 && b != a && b > last && b < m) m = b;
@@ -234,8 +239,8 @@ __global__ __launch_bounds__(kCellBlock) void coarse_rows_kernel(
     double v = 0.0;
     if (live)
       for (int e = 0; e < nv; ++e) {
-        const int j = elab[(size_t)i * kMaxVerts + e];
-        if (j >= 0 && sagg[j] == b) v += h[(size_t)i * kMaxVerts + e];
+        const int j = hj[hslot(blockIdx.x, e, threadIdx.x)];
+        if (j >= 0 && sagg[j] == b) v += h[hslot(blockIdx.x, e, threadIdx.x)];
       }
     v = block_sum(v, red);
     if (threadIdx.x == 0) row[k] = CoarseEntry{b, 0, v};
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(256) void cg_coarse_kernel(const int4* tasks, const
 // H), z = r / diag, partial r.z and r.r.
 __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
     const BlockDesc* blocks, const uint8_t* active, const double* g, const double* shift,
-    const double* diag, const uint8_t* nedge, const int* elab, const double* h, int sweeps,
+    const double* diag, const uint8_t* nedge, const int* hj, const double* h, int sweeps,
     double* x, double* r, double* z, double* part_rz, double* part_rr, double* rsum) {
   __shared__ double red[kCellBlock / 64];
   __shared__ double zl[kCellBlock];
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
     dg = diag[i];
     ri = (active[bd.prob] && dg > 0.0) ? shift[bd.prob] - g[i] : 0.0;
   }
-  const double zi = precondition(bd, i, ri, dg, nedge, elab, h, sweeps, zl);
+  const double zi = precondition(blockIdx.x, bd, i, ri, dg, nedge, hj, h, sweeps, zl);
   double rz = 0.0, rr = 0.0;
   if (ln < bd.count) {
     x[i] = 0.0;
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
 // (sagg_i < 0: no patch) and r.z adds the coarse partials.
 __global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
     const BlockDesc* blocks, int nblk, const double* part_rz, const double* part_rzc, int ntask,
-    const double* rz_prev, double* rz_out, int first, const uint8_t* nedge, const int* elab,
+    const double* rz_prev, double* rz_out, int first, const uint8_t* nedge, const int* hj,
     const double* h, const double* diag, const int* sagg, const double* cvec, const double* z,
     const double* p_old, double* p_new, double* q, double* part_pq) {
   __shared__ double red[kCellBlock / 64];
@@ -359,8 +364,8 @@ __global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
 #pragma unroll
       for (int u = 0; u < kEdgeBatch; ++u) {
         const bool in = k0 + u < nv;
-        jj[u] = in ? elab[(size_t)i * kMaxVerts + k0 + u] : -1;
-        hh[u] = in ? h[(size_t)i * kMaxVerts + k0 + u] : 0.0;
+        jj[u] = in ? hj[hslot(blockIdx.x, k0 + u, threadIdx.x)] : -1;
+        hh[u] = in ? h[hslot(blockIdx.x, k0 + u, threadIdx.x)] : 0.0;
       }
       double zz[kEdgeBatch], pp[kEdgeBatch];
       int aa[kEdgeBatch];
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(kCellBlock) void cg_step_kernel(
 
 // q = H p (edge by edge), partial p.q.
 __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
-    const BlockDesc* blocks, const uint8_t* nedge, const int* elab, const double* h,
+    const BlockDesc* blocks, const uint8_t* nedge, const int* hj, const double* h,
     const double* diag, const double* p, double* q, double* part_pq) {
   __shared__ double red[kCellBlock / 64];
   const BlockDesc bd = blocks[blockIdx.x];
@@ -409,8 +414,8 @@ __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
 #pragma unroll
       for (int u = 0; u < kEdgeBatch; ++u) {
         const bool in = k0 + u < nv;
-        jj[u] = in ? elab[(size_t)i * kMaxVerts + k0 + u] : -1;
-        hh[u] = in ? h[(size_t)i * kMaxVerts + k0 + u] : 0.0;
+        jj[u] = in ? hj[hslot(blockIdx.x, k0 + u, threadIdx.x)] : -1;
+        hh[u] = in ? h[hslot(blockIdx.x, k0 + u, threadIdx.x)] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kEdgeBatch; ++u) pp[u] = jj[u] >= 0 ? p[jj[u]] : 0.0;
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(kCellBlock) void cg_apply_kernel(
 // z = r / diag; partial r.z (new), r.r; with the coarse term rsum receives the block sums of r.
 __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
     const BlockDesc* blocks, int nblk, const double* rz_cur, const double* part_pq,
-    const double* diag, const uint8_t* nedge, const int* elab, const double* h, int sweeps,
+    const double* diag, const uint8_t* nedge, const int* hj, const double* h, int sweeps,
     const double* p, const double* q, double* x, double* r, double* z, double* part_rz_new,
     double* part_rr, double* rsum) {
   __shared__ double red[kCellBlock / 64];
@@ -446,7 +451,7 @@ __global__ __launch_bounds__(kCellBlock) void cg_update_kernel(
     ri = r[i] - alpha * q[i];
     dg = diag[i];
   }
-  const double zi = precondition(bd, i, ri, dg, nedge, elab, h, sweeps, zl);
+  const double zi = precondition(blockIdx.x, bd, i, ri, dg, nedge, hj, h, sweeps, zl);
   double rzn = 0.0, rr = 0.0;
   if (ln < bd.count) {
     r[i] = ri;
@@ -534,9 +539,10 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
           double acc = r[m];
           const int nv = a.nedge[i];
           const int bf = i - l;  // the block's first site
+          const int b = b0 + m * (kSmallThreads / kCellBlock);
           for (int k = 0; k < nv; ++k) {
-            const int j = a.elab[(size_t)i * kMaxVerts + k] - bf;
-            if (j >= 0 && j < bcount[m]) acc += a.h[(size_t)i * kMaxVerts + k] * zs[bf + j];
+            const int j = a.hj[hslot(b, k, l)] - bf;
+            if (j >= 0 && j < bcount[m]) acc += a.h[hslot(b, k, l)] * zs[bf + j];
           }
           zn[m] = acc / dg[m];
         }
@@ -614,9 +620,10 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
       if (i < 0) continue;
       double sacc = dg[m] * pl[m];
       const int nv = a.nedge[i];
+      const int b = b0 + m * (kSmallThreads / kCellBlock);
       for (int k = 0; k < nv; ++k) {
-        const int j = a.elab[(size_t)i * kMaxVerts + k];
-        if (j >= 0) sacc -= a.h[(size_t)i * kMaxVerts + k] * ps[j];
+        const int j = a.hj[hslot(b, k, l)];
+        if (j >= 0) sacc -= a.h[hslot(b, k, l)] * ps[j];
       }
       q[m] = sacc;
       pq += pl[m] * sacc;
@@ -672,17 +679,17 @@ __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* 
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
                           const double2* xy, const double2* vtx, const uint8_t* nedge,
                           const int* elab, const DensityGroup* grp, const int* bagg, double* h,
-                          double* diag, int* sagg, int nblk, hipStream_t st) {
+                          int* hj, double* diag, int* sagg, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(hessian_kernel, dim3(nblk), dim3(256), 0, st, probs, blocks, active, xy, vtx,
-                     nedge, elab, grp, bagg, h, diag, sagg);
+                     nedge, elab, grp, bagg, h, hj, diag, sagg);
   return hipGetLastError();
 }
 
 hipError_t launch_coarse_rows(const BlockDesc* blocks, const int* bagg, const int* sagg,
-                              const uint8_t* nedge, const int* elab, const double* h,
+                              const uint8_t* nedge, const int* hj, const double* h,
                               CoarseEntry* out, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(coarse_rows_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, bagg, sagg,
-                     nedge, elab, h, out);
+                     nedge, hj, h, out);
   return hipGetLastError();
 }
 
@@ -696,11 +703,11 @@ hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdes
 
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
                           const double* shift, const double* diag, const uint8_t* nedge,
-                          const int* elab, const double* h, int sweeps, double* x, double* r,
+                          const int* hj, const double* h, int sweeps, double* x, double* r,
                           double* z, double* part_rz, double* part_rr, double* rsum, int nblk,
                           hipStream_t st) {
   hipLaunchKernelGGL(cg_init_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, g, shift,
-                     diag, nedge, elab, h, sweeps, x, r, z, part_rz, part_rr, rsum);
+                     diag, nedge, hj, h, sweeps, x, r, z, part_rz, part_rr, rsum);
   return hipGetLastError();
 }
 
@@ -717,10 +724,10 @@ hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_n
   const bool coarse = a.ntask > 0;
   hipLaunchKernelGGL(cg_step_kernel, dim3(a.nblk), dim3(kCellBlock), 0, st, a.blocks, a.nblk,
                      a.part_rz + (size_t)cur * a.nblk, rzc_cur, a.ntask, a.rz + oth, a.rz + cur,
-                     first, a.nedge, a.elab, a.h, a.diag, coarse ? a.sagg : nullptr, a.cvec, a.z,
+                     first, a.nedge, a.hj, a.h, a.diag, coarse ? a.sagg : nullptr, a.cvec, a.z,
                      p_old, p_new, a.q, a.part_pq);
   hipLaunchKernelGGL(cg_update_kernel, dim3(a.nblk), dim3(kCellBlock), 0, st, a.blocks, a.nblk,
-                     a.rz + cur, a.part_pq, a.diag, a.nedge, a.elab, a.h, a.sweeps, p_new, a.q,
+                     a.rz + cur, a.part_pq, a.diag, a.nedge, a.hj, a.h, a.sweeps, p_new, a.q,
                      a.x, a.r, a.z, a.part_rz + (size_t)oth * a.nblk, a.part_rr,
                      coarse ? a.rsum : nullptr);
   if (coarse)
@@ -734,10 +741,10 @@ hipError_t launch_cg_small(const SmallCgArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* elab,
+hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, const int* hj,
                                 const double* h, const double* diag, const double* p, double* q,
                                 double* part_pq, int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, elab, h,
+  hipLaunchKernelGGL(cg_apply_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, nedge, hj, h,
                      diag, p, q, part_pq);
   return hipGetLastError();
 }
