@@ -141,7 +141,12 @@ class cot_stats(ctypes.Structure):
                 ("evaluations", ctypes.c_longlong), ("repair_rounds", ctypes.c_longlong),
                 ("integ_seconds", ctypes.c_double), ("integ_launches", ctypes.c_longlong),
                 ("newton_steps", ctypes.c_longlong), ("cg_iterations", ctypes.c_longlong),
-                ("coarse_seconds", ctypes.c_double), ("cg_restarts", ctypes.c_longlong)]
+                ("coarse_seconds", ctypes.c_double), ("cg_restarts", ctypes.c_longlong),
+                ("handoff_degenerate", ctypes.c_longlong), ("handoff_overflow", ctypes.c_longlong),
+                ("check_failures", ctypes.c_longlong),
+                ("repair_check_failures", ctypes.c_longlong),
+                ("repaired_cells", ctypes.c_longlong),
+                ("reappeared_cells", ctypes.c_longlong)]
 
 _lib = None
 

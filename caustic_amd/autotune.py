@@ -27,8 +27,14 @@ Knobs (fields of cot_options):
                     small problems (a few thousand cells), where one lane per cell leaves most
                     of the chip idle.
 
-Objective: device time per full evaluation -- construction, integration and repair kernels, read
-from the in-kernel timers (cot_get_stats) -- over a stretch of the real solve: `iters` L-BFGS
+Objective: elapsed time per full evaluation over a stretch of the real solve -- every kernel of
+the evaluation (construction, integration, conformity checks, repair rounds, adjacency saves)
+and of the L-BFGS steps around it, with the gaps between them.  The in-kernel timers
+(cot_get_stats) see only the construction, integration and repair kernels, and only while they
+have work: ranked by them, the one-ring full pass won at N=16,384 (cheaper construction) while
+the conformity checks and repair rounds it caused -- 21% of the sites failing per evaluation,
+each round's check over every site -- made that level of the coarse-to-fine solve 18% slower
+than with the two-ring.  The stretch: `iters` L-BFGS
 iterations from the caller's weights, or with `stop_tol` the whole L-BFGS phase until the Newton
 hand-off (max-rel stop_tol, no empty cell; a 40-iteration stretch from the start ranked two
 bucket densities the other way round from the full phase).  Per evaluation, not per phase: the
@@ -112,7 +118,8 @@ def _store_cache(key: str, entry: dict) -> None:
 
 
 def measure(problems, precision: str, cfg: dict, w0, iters: int, solve_kw: dict) -> dict:
-    """Device seconds per full evaluation of `iters` L-BFGS iterations from w0 under cfg."""
+    """Seconds per full evaluation (elapsed) of `iters` L-BFGS iterations from w0 under cfg,
+    with the in-kernel timers' share of it."""
     from .ot import OTBatch
 
     kw = dict(solve_kw, raise_on_fail=False)
@@ -125,9 +132,9 @@ def measure(problems, precision: str, cfg: dict, w0, iters: int, solve_kw: dict)
         st = b.stats()
     evals = max(int(st["cell_launches"]), 1)
     dev = st["cell_seconds"] + st["integ_seconds"] + st["repair_seconds"]
-    return {"per_eval": dev / evals, "total": dev, "evaluations": evals, "wall": wall,
-            "cell": st["cell_seconds"] / evals, "integrate": st["integ_seconds"] / evals,
-            "repair": st["repair_seconds"] / evals}
+    return {"per_eval": wall / evals, "total": wall, "evaluations": evals, "wall": wall,
+            "timed_kernels": dev / evals, "cell": st["cell_seconds"] / evals,
+            "integrate": st["integ_seconds"] / evals, "repair": st["repair_seconds"] / evals}
 
 
 def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw: dict = None,
@@ -151,7 +158,7 @@ def tune(problems, precision: str = "mixed", w0=None, iters: int = 40, solve_kw:
     w0 = np.zeros(n) if w0 is None else np.asarray(w0, dtype=np.float64)
     seen = {}
 
-    # objective: device time per evaluation (a property of the shape; the phase's evaluation
+    # objective: elapsed time per evaluation (a property of the shape; the phase's evaluation
     # count is a property of the trajectory, which the shape changes chaotically through
     # rounding, and does not carry over to other inputs of the same size)
     obj = "per_eval"

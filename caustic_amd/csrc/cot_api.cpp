@@ -31,9 +31,9 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
 hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, DensityGroup* grp,
                          double4* tot, hipStream_t st);
-hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
-                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
-                                 hipStream_t st);
+hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* active,
+                                 const uint8_t* nedge, const int* elab, uint8_t* adjn, int* adj,
+                                 int* elist, int* ecount, int ecap, int nblocks, hipStream_t st);
 hipError_t launch_raise_ring(const BlockDesc* b, const int* list, const int* list_count,
                              uint8_t* ring, int rmax, int repair_blocks, hipStream_t st);
 hipError_t launch_adapt_ring(const BlockDesc* b, const uint8_t* used, uint8_t* ring, int rmin,
@@ -226,12 +226,15 @@ struct cot_context {
   float2* d_espan = nullptr;
   int* d_adj = nullptr;          // previous evaluation's edge labels (candidates)
   uint8_t* d_adjn = nullptr;     // previous evaluation's edge counts (0: none yet)
+  int* d_elist = nullptr;         // cells the last evaluation left empty (see CellArgs)
+  int* d_ecount = nullptr;
   bool have_adj = false;
   double2* d_vtx = nullptr;      // cell vertices (construction -> integration)
   int2* d_kbox = nullptr;
   int* d_xcnt = nullptr;         // candidates queued by the conformity check
   int* d_xcand = nullptr;
   unsigned long long* d_timer = nullptr;  // kTimers records of kTimerWords
+  unsigned long long* d_diag = nullptr;   // repair counters (kDiagWords), after the timers
   double clock_hz = 1e8;                  // wall-clock rate of the in-kernel timers
   int *d_overflow = nullptr, *d_fails = nullptr;
   double* d_part = nullptr;
@@ -401,6 +404,9 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.kbox = c->d_kbox;
   ca.defer_list = c->d_defer;
   ca.defer_count = c->d_dcount;
+  ca.empty_list = c->d_elist;
+  ca.empty_count = c->d_ecount;
+  ca.empty_cap = (int)c->ntot;
   va = VerifyArgs{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
@@ -415,6 +421,8 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   va.xcnt = c->d_xcnt;
   va.xcand = c->d_xcand;
   va.timer_close = c->d_timer + kTimerInteg * kTimerWords;
+  ca.diag = c->d_diag;
+  va.diag_fail = c->d_diag + kDiagCheckFull;
 }
 
 // Candidates: the FP64 full pass takes the previous evaluation's cells' two-ring (an edge flip
@@ -435,7 +443,7 @@ inline int full_pass_cand(const cot_context* c) {
 
 // Fold the in-kernel timers into the context's statistics and clear their accumulators.
 int pull_timers(cot_context* c) {
-  unsigned long long tm[kTimers * kTimerWords];
+  unsigned long long tm[kTimers * kTimerWords + kDiagWords];
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(tm, c->d_timer, sizeof(tm), hipMemcpyDeviceToHost));
   const double inv = 1.0 / c->clock_hz;
@@ -451,6 +459,14 @@ int pull_timers(cot_context* c) {
     tm[k * kTimerWords + kAcc] = 0;
     tm[k * kTimerWords + kCnt] = 0;
   }
+  const unsigned long long* dg = tm + kTimers * kTimerWords;
+  c->stats.handoff_degenerate += (long long)dg[kDiagDegenerate];
+  c->stats.handoff_overflow += (long long)dg[kDiagOverflow];
+  c->stats.check_failures += (long long)dg[kDiagCheckFull];
+  c->stats.repair_check_failures += (long long)dg[kDiagCheckRepair];
+  c->stats.repaired_cells += (long long)dg[kDiagRebuilt];
+  c->stats.reappeared_cells += (long long)dg[kDiagReappeared];
+  std::fill(tm + kTimers * kTimerWords, tm + kTimers * kTimerWords + kDiagWords, 0ull);
   HIPCHK(hipMemcpy(c->d_timer, tm, sizeof(tm), hipMemcpyHostToDevice));
   return COT_OK;
 }
@@ -497,6 +513,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
   va.timer_close = ca.timer_int;
+  va.diag_fail = c->d_diag + kDiagCheckRepair;
   int rounds = 0;
   for (;;) {
     int tot = 0;
@@ -514,8 +531,8 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_nedge, c->d_elab, c->d_adjn,
-                                   c->d_adj, c->nblk, st));
+      HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_active, c->d_nedge, c->d_elab,
+                                   c->d_adjn, c->d_adj, nullptr, nullptr, 0, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
     HIPCHK(launch_verify(va, c->nblk, st));
@@ -539,7 +556,8 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
       covered = false;
     }
   // the cells just built are the next evaluation's candidates
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_active, c->d_nedge, c->d_elab,
+                               c->d_adjn, c->d_adj, c->d_elist, c->d_ecount, (int)c->ntot,
                                c->nblk, st));
   c->have_adj = covered;
   // after a clean check the next bucket search starts from the outermost used ring plus one;
@@ -690,6 +708,7 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
   va.timer_close = ca.timer_int;
+  va.diag_fail = c->d_diag + kDiagCheckRepair;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
     // round r rebuilds the list the previous check filled and its check fills the other one
@@ -703,14 +722,15 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
       HIPCHK(launch_raise_ring(c->d_blocks, ca.list, ca.list_count, c->d_ring, c->opt.ring_max,
                                c->opt.repair_blocks, st));
     else
-      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
-                                   c->nblk, st));
+      HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_active, c->d_nedge, c->d_elab,
+                                   c->d_adjn, c->d_adj, nullptr, nullptr, 0, c->nblk, st));
     HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
     HIPCHK(launch_verify(va, c->nblk, st));
   }
-  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_nedge, c->d_elab, c->d_adjn, c->d_adj,
+  HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_active, c->d_nedge, c->d_elab,
+                               c->d_adjn, c->d_adj, c->d_elist, c->d_ecount, (int)c->ntot,
                                c->nblk, st));
   const int* final_fails = c->d_rfails + (size_t)R * np;
   HIPCHK(launch_dev_adapt_ring(c->d_blocks, c->d_state, final_fails, c->d_used, c->d_ring, 1,
@@ -861,8 +881,10 @@ int solve_device(cot_context* c, double* w, const cot_solve_opts& o, cot_report*
       r.seconds = secs;
     }
     if (o.verbose > 0)
-      std::printf("[cot] prob %d: %s after %d iterations (%d evaluations), maxrel %.3e\n", p,
-                  s.conv ? "converged" : "stopped", s.iters, s.evals, s.maxrel_cur);
+      std::printf("[cot] prob %d: %s after %d iterations (%d evaluations), maxrel %.3e, "
+                  "%.2f ms\n",
+                  p, s.conv ? "converged" : "stopped", s.iters, s.evals, s.maxrel_cur,
+                  1e3 * secs);
   }
   if (result == COT_ENOTCONV) g_err = "iteration limit reached";
   if (result == COT_EUNVERIFIED) g_err = "cells failed the conformity check at the largest ring";
@@ -1047,12 +1069,14 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_coef, (size_t)c->np * kCoefWidth)) ||
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
       (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
-      (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
+      (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_elist, N)) ||
+      (rc = dalloc(c, &c->d_ecount, 1)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
       (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
       (rc = dalloc(c, &c->d_defer, N)) || (rc = dalloc(c, &c->d_dcount, 1)) ||
-      (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords)))
+      (rc = dalloc(c, &c->d_timer, (size_t)kTimers * kTimerWords + kDiagWords)))
     return bail(rc);
+  c->d_diag = c->d_timer + (size_t)kTimers * kTimerWords;
   // overflow counters live right after the fail counters so one copy reads both
   c->d_overflow = c->d_fails + c->np;
   c->img_total = img_total;
@@ -1096,7 +1120,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
     // FP32 clipping frames start at the sites
     ok = ok && up(c->d_org, xy.data(), sizeof(double2) * N);
     // timers: every slot's start at the maximum, ends and accumulators zero
-    unsigned long long tm[kTimers * kTimerWords] = {};
+    unsigned long long tm[kTimers * kTimerWords + kDiagWords] = {};
     for (int k = 0; k < kTimers; ++k)
       for (int r = 0; r < kTimerSlots; ++r) tm[k * kTimerWords + 2 * r] = ~0ull;
     ok = ok && up(c->d_timer, tm, sizeof(tm));
@@ -1104,6 +1128,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (!ok) return bail(fail(COT_EHIP, "upload failed"));
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
       launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
+      launch_fill_i32(c->d_ecount, 1, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||
@@ -1789,6 +1814,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       shift[p] = act[p] ? (cur[p].mass - c->nu_sum[p]) / c->probs[p].n : 0.0;
     }
     if (!any) break;
+    const auto t_it = std::chrono::steady_clock::now();
     std::memcpy(c->h_active, act.data(), np);
     HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, np, hipMemcpyHostToDevice, st));
     std::memcpy(c->h_small, shift.data(), sizeof(double) * np);
@@ -1943,6 +1969,8 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       }
     }
     if (cg < 0) cg = -cg;  // a diverged Jacobi CG: the line search rejects its direction
+    const auto t_cg = std::chrono::steady_clock::now();
+    int trials = 0;
     c->stats.newton_steps += 1;
     c->stats.cg_iterations += cg;
     // damped step per problem: every cell keeps mass and |g| falls by the factor (1 - t / 2)
@@ -1958,6 +1986,7 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice,
                             st));
       CHK(evaluate_device(c, c->d_w, c->d_nx, tr, trial, rounds));
+      ++trials;
       for (int p = 0; p < np; ++p) {
         if (!tr[p]) continue;
         ++evals[p];
@@ -1992,11 +2021,17 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     HIPCHK(hipStreamSynchronize(st));
     for (int p = 0; p < np; ++p) iters[p] += acc[p];
     if (o.verbose > 0) {
+      const auto t_end = std::chrono::steady_clock::now();
+      const auto msec = [](std::chrono::steady_clock::time_point a,
+                           std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+      };
       for (int p = 0; p < np; ++p)
         if (act[p])
-          std::printf("[cot] newton prob %d iter %d cg %d |r|/|r0| %.2e step %g maxrel %.3e\n", p,
-                      iters[p], cg, std::sqrt(rr[p] / std::max(rr0[p], 1e-300)), t[p],
-                      cur[p].maxrel);
+          std::printf("[cot] newton prob %d iter %d cg %d |r|/|r0| %.2e step %g maxrel %.3e "
+                      "(ms: Hessian + CG %.2f, %d trial evaluations %.2f)\n",
+                      p, iters[p], cg, std::sqrt(rr[p] / std::max(rr0[p], 1e-300)), t[p],
+                      cur[p].maxrel, msec(t_it, t_cg), trials, msec(t_cg, t_end));
       std::fflush(stdout);
     }
   }

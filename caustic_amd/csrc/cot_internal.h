@@ -110,6 +110,20 @@ constexpr int kDeferCand = 64;
 constexpr int kTimerSlots = 16;
 constexpr int kTimerWords = 2 * kTimerSlots + 2;
 enum : int { kTimerCell = 0, kTimerRepair = 1, kTimerInteg = 2, kTimerIntegRepair = 3, kTimers = 4 };
+// Repair counters (one 64-bit word each, accumulated over every evaluation): cells a full FP32
+// pass handed to the FP64 repair for a degeneracy (sliver, marginal emptying, vertex the
+// refinement could not place) or for a polygon above kPassVerts vertices; sites that failed the
+// conformity check after a full pass and after a repair pass; cells rebuilt by repair passes;
+// cells the last evaluation left empty that came back.
+enum : int {
+  kDiagDegenerate = 0,
+  kDiagOverflow = 1,
+  kDiagCheckFull = 2,
+  kDiagCheckRepair = 3,
+  kDiagRebuilt = 4,
+  kDiagReappeared = 5,
+  kDiagWords = 8
+};
 
 // One OT problem of a batch.  All per-site arrays are indexed by global sorted site index.
 struct ProbDesc {
@@ -278,12 +292,20 @@ struct CellArgs {
   // in the lane).  The integration pass clears the count.
   int* defer_list;
   int* defer_count;
+  // cells the last evaluation left empty (site slots, at most empty_cap; the evaluation's last
+  // save_adjacency fills the list, the integration pass clears the count): built before the full
+  // pass to queue those that come back with their new neighbours (cell_reappear_kernel)
+  int* empty_list;
+  int* empty_count;
+  int empty_cap;
+  int probe;                // build without side effects on the next build (FP32 frame, queue)
   const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)
   const int* xcand;         // [site * kMaxExtra + k]
   unsigned long long* timer;  // the construction pass's timer (kTimerWords words) or nullptr
   unsigned long long* timer_int;  // the integration pass's timer, closed by the next kernel
+  unsigned long long* diag;  // repair counters (kDiagWords words) or nullptr
   // construction -> integration
   double2* vtx;             // [site * kMaxVerts + k]: vertex k of the cell (absolute, FP64)
   int2* kbox;               // pixel of the cell's lower-left corner (integration origin)
@@ -338,6 +360,7 @@ struct VerifyArgs {
   int* xcnt;                // candidate queues of the repair (see kMaxExtra)
   int* xcand;
   unsigned long long* timer_close;  // the integration timer this check closes, or nullptr
+  unsigned long long* diag_fail;    // counter of the sites this check fails, or nullptr
 };
 
 }  // namespace cot

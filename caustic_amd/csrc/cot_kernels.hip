@@ -71,6 +71,13 @@ __host__ __device__ __forceinline__ int fetch_add(int* p, int v) {
   return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 #endif
 }
+__host__ __device__ __forceinline__ void count_add(unsigned long long* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd(p, 1ull);
+#else
+  __atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED);
+#endif
+}
 __host__ __device__ __forceinline__ int exchange(int* p, int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return atomicExch(p, v);
@@ -453,6 +460,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
+  if (a.redo_mode && a.diag) count_add(&a.diag[kDiagRebuilt]);
   // the full pass clears the flag (an FP32 cell may set it again below); in a repair pass the
   // integration pass that follows clears it
   if (!a.redo_mode && a.redo) a.redo[i] = 0;
@@ -784,7 +792,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
     int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
     kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
-    if (kF32) a.org[i] = geo;  // next frame: this centroid
+    if (kF32 && !a.probe) a.org[i] = geo;  // next frame: this centroid
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
   a.kbox[i] = kb;
@@ -795,6 +803,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   if (((kF32 && coarse) || handoff) && a.coarse) {
     a.redo[i] = 2;  // the check that follows lists it
     fetch_add(&a.coarse[bd.prob], 1);
+    if (a.diag) count_add(&a.diag[kF32 && coarse ? kDiagDegenerate : kDiagOverflow]);
   }
   a.area[i] = ar;
   a.geo[i] = geo;
@@ -1064,6 +1073,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
+  if (a.redo_mode && a.diag && ln == 0) count_add(&a.diag[kDiagRebuilt]);
   if (!a.redo_mode && a.redo && ln == 0) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   double tstep = 0.0;
@@ -1381,7 +1391,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
     const int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
     const int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
     kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
-    if (kF32 && ln == 0) a.org[i] = geo;  // next frame: this centroid
+    if (kF32 && ln == 0 && !a.probe) a.org[i] = geo;  // next frame: this centroid
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
   // a full pass whose polygon overflowed V skipped a constraint: the repair pass rebuilds the
@@ -1393,6 +1403,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
     if (((kF32 && coarse) || handoff) && a.coarse) {
       a.redo[i] = 2;  // the check that follows lists it
       atomicAdd(&a.coarse[bd.prob], 1);
+      if (a.diag) count_add(&a.diag[kF32 && coarse ? kDiagDegenerate : kDiagOverflow]);
     }
     a.area[i] = ar;
     a.geo[i] = geo;
@@ -1430,6 +1441,52 @@ __global__ __launch_bounds__(64) void cell_defer_kernel(CellArgs a) {
     worked = worked || rc >= 0;
   }
   timer_mark(a.timer, worked, t0);
+}
+
+// The cells the last evaluation left empty (CellArgs::empty_list), built at the new weights
+// before the full pass, one wave (block) per listed site; any grid size.  A cell that empties
+// and comes back reappears between sites whose adjacency (the previous cells) does not hold it:
+// each would miss the new edge, fail the conformity check and need a repair.  A listed cell that
+// comes back is queued as a candidate of each of its new neighbours (as the conformity check
+// queues a missed candidate), which the full pass then takes with their adjacency.  The build
+// only probes: the site's own queue and FP32 frame are left as they were, and the full pass
+// builds the cell again from the same inputs (the bucket rings: it has no adjacency).  Folds
+// into the construction pass's timer.
+template <typename T>
+__global__ __launch_bounds__(64) void cell_reappear_kernel(CellArgs a, int* xcand) {
+  __shared__ WaveLDS<T> W;
+  const unsigned long long t0 = timer_begin();
+  unsigned long long* const timer = a.timer;
+  int* const xcnt = a.xcnt;
+  unsigned long long* const diag = a.diag;
+  a.xcnt = nullptr;
+  a.coarse = nullptr;
+  a.defer_list = nullptr;
+  a.diag = nullptr;
+  a.timer = nullptr;
+  a.probe = 1;
+  const int n = min(*a.empty_count, a.empty_cap);
+  bool worked = false;
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int slot = a.empty_list[e];
+    const int blk = slot / kCellBlock;
+    const int rc = cell_site_wave<T, kPassVerts>(a, blk, slot % kCellBlock, W);
+    __syncthreads();  // the cell's edge labels, written by their lanes
+    if (rc < 0) continue;
+    worked = true;
+    const int i = a.blocks[blk].first + slot % kCellBlock;
+    const int ne = a.nedge[i];
+    const int ln = lane_id();
+    if (ln < ne) {
+      const int j = a.elab[(size_t)i * kMaxVerts + ln];
+      if (j >= 0) {
+        const int q = atomicAdd(&xcnt[j], 1);
+        if (q < kMaxExtra) xcand[(size_t)j * kMaxExtra + q] = i;
+      }
+    }
+    if (ne > 0 && ln == 0 && diag) count_add(&diag[kDiagReappeared]);
+  }
+  timer_mark(timer, worked, t0);  // part of the construction pass
 }
 
 // Repair construction (FP64) with one wave (block) per listed site; any grid size.
@@ -1478,6 +1535,7 @@ __device__ __forceinline__ void reset_list(const CellArgs& a) {
   if (threadIdx.x != 0 || blockIdx.x != 0 || blockIdx.y != 0) return;
   if (a.reset_count) *a.reset_count = 0;
   if (a.defer_count && !a.redo_mode) *a.defer_count = 0;
+  if (a.empty_count && !a.redo_mode) *a.empty_count = 0;
 }
 
 // a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
@@ -1615,17 +1673,19 @@ __global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
   timer_close(a.timer_close);
   const int blk = xcd_slot(blockIdx.x, gridDim.x);
   const int rc = verify_site(a, blk, threadIdx.x);
-  if (rc == 1) atomicAdd(&a.fails[a.blocks[blk].prob], 1);
+  if (rc == 1) {
+    atomicAdd(&a.fails[a.blocks[blk].prob], 1);
+    if (a.diag_fail) count_add(a.diag_fail);
+  }
 }
 
 // Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
-// repair).  gate: skip problems whose counter is zero (nullptr: none).
-__global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
-                                      const uint8_t* nedge, const int* elab, uint8_t* adjn,
-                                      int* adj) {
-  const BlockDesc bd = blocks[blockIdx.x];
-  if (gate && gate[bd.prob] == 0) return;
-  const int ln = threadIdx.x;
+// repair).  gate: skip problems whose counter is zero (nullptr: none).  elist set (the
+// evaluation's last save; problems in `active` only): also list the empty cells (site slots,
+// at most ecap) for the next full pass's cell_reappear_kernel.
+__host__ __device__ void save_adjacency_site(const BlockDesc& bd, int blk, int ln,
+                                             const uint8_t* nedge, const int* elab, uint8_t* adjn,
+                                             int* adj, int* elist, int* ecount, int ecap) {
   if (ln >= bd.count) return;
   const int i = bd.first + ln;
   const int ne = nedge[i];
@@ -1633,6 +1693,20 @@ __global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
   const int* src = elab + (size_t)i * kMaxVerts;
   int* dst = adj + (size_t)i * kMaxVerts;
   for (int k = 0; k < ne; ++k) dst[k] = src[k];
+  if (ne == 0 && elist) {
+    const int q = fetch_add(ecount, 1);
+    if (q < ecap) elist[q] = blk * kCellBlock + ln;
+  }
+}
+
+__global__ void save_adjacency_kernel(const BlockDesc* blocks, const int* gate,
+                                      const uint8_t* active, const uint8_t* nedge,
+                                      const int* elab, uint8_t* adjn, int* adj, int* elist,
+                                      int* ecount, int ecap) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (gate && gate[bd.prob] == 0) return;
+  if (elist && !active[bd.prob]) return;
+  save_adjacency_site(bd, blockIdx.x, threadIdx.x, nedge, elab, adjn, adj, elist, ecount, ecap);
 }
 
 // Grow the ring of every cell marked for a rebuild (r -> 2r + 1).
@@ -1864,6 +1938,8 @@ void launch_integrate(const CellArgs& a, int nblocks, bool fp32, int repair_bloc
 // The integration pass closes a.timer and marks a.timer_int.
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                        int repair_blocks, hipStream_t st) {
+  // waves of the passes that walk a list of sites: at most kDeferWaves
+  const int list_waves = nblocks * kCellBlock < kDeferWaves ? nblocks * kCellBlock : kDeferWaves;
   if (a.redo_mode) {
     // repairs rebuild few cells: always in FP64 (with FP32 geometry, this is also the rebuild
     // of the cells the FP32 pass could not resolve), on a small grid over the work list: one
@@ -1873,26 +1949,36 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
                          st, a);
     else
       hipLaunchKernelGGL(cell_repair_wave_kernel, dim3(4 * repair_blocks), dim3(64), 0, st, a);
-  } else if (a.pass_wave) {
-    // one wave per site: many more waves in flight than sites per lane, for small problems
-    const dim3 g((unsigned)nblocks * kCellBlock);
-    if (fp32)
-      hipLaunchKernelGGL((cell_wave_kernel<float>), g, dim3(64), 0, st, a);
-    else
-      hipLaunchKernelGGL((cell_wave_kernel<double>), g, dim3(64), 0, st, a);
   } else {
-    if (fp32)
-      hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-    else
-      hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
-    if (a.defer_list) {
-      // the deferred sites: a grid of at most kDeferWaves waves walks the list
-      const int waves = nblocks * kCellBlock < kDeferWaves ? nblocks * kCellBlock : kDeferWaves;
-      const dim3 g((unsigned)waves);
+    if (a.empty_list && a.cand != kCandBucket) {
+      // the cells the last evaluation left empty, before the full pass builds their neighbours
+      int* xcand = const_cast<int*>(a.xcand);
       if (fp32)
-        hipLaunchKernelGGL((cell_defer_kernel<float>), g, dim3(64), 0, st, a);
+        hipLaunchKernelGGL((cell_reappear_kernel<float>), dim3(list_waves), dim3(64), 0, st, a,
+                           xcand);
       else
-        hipLaunchKernelGGL((cell_defer_kernel<double>), g, dim3(64), 0, st, a);
+        hipLaunchKernelGGL((cell_reappear_kernel<double>), dim3(list_waves), dim3(64), 0, st, a,
+                           xcand);
+    }
+    if (a.pass_wave) {
+      // one wave per site: many more waves in flight than sites per lane, for small problems
+      const dim3 g((unsigned)nblocks * kCellBlock);
+      if (fp32)
+        hipLaunchKernelGGL((cell_wave_kernel<float>), g, dim3(64), 0, st, a);
+      else
+        hipLaunchKernelGGL((cell_wave_kernel<double>), g, dim3(64), 0, st, a);
+    } else {
+      if (fp32)
+        hipLaunchKernelGGL((cell_kernel<float>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+      else
+        hipLaunchKernelGGL((cell_kernel<double>), dim3(nblocks), dim3(kCellBlock), 0, st, a);
+      if (a.defer_list) {
+        // the deferred sites: a grid of at most kDeferWaves waves walks the list
+        if (fp32)
+          hipLaunchKernelGGL((cell_defer_kernel<float>), dim3(list_waves), dim3(64), 0, st, a);
+        else
+          hipLaunchKernelGGL((cell_defer_kernel<double>), dim3(list_waves), dim3(64), 0, st, a);
+      }
     }
   }
   switch (int_lanes) {
@@ -1919,11 +2005,11 @@ hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* nedge,
-                                 const int* elab, uint8_t* adjn, int* adj, int nblocks,
-                                 hipStream_t st) {
+hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* active,
+                                 const uint8_t* nedge, const int* elab, uint8_t* adjn, int* adj,
+                                 int* elist, int* ecount, int ecap, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(save_adjacency_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, b, gate,
-                     nedge, elab, adjn, adj);
+                     active, nedge, elab, adjn, adj, elist, ecount, ecap);
   return hipGetLastError();
 }
 

@@ -229,6 +229,21 @@ void weights_from(const std::vector<double>& U, int side, const double* fr, cons
 constexpr int kNewtonSteps = 100;
 constexpr int kNewtonRetries = 2;
 constexpr double kHandoffCut = 4.0;
+// Verbose output: a stage's evaluation and repair counts (the difference of two snapshots).
+void print_counts(const cot_stats& a, const cot_stats& b) {
+  const double ev = std::max(1.0, (double)(b.evaluations - a.evaluations));
+  std::printf("[ladder]   %lld evaluations; per evaluation: hand-offs %.1f degenerate + %.1f "
+              "overflow, reappeared %.1f, check failures %.1f full + %.1f repair, cells "
+              "rebuilt %.1f; Newton steps %lld, CG iterations %lld\n",
+              b.evaluations - a.evaluations, (b.handoff_degenerate - a.handoff_degenerate) / ev,
+              (b.handoff_overflow - a.handoff_overflow) / ev,
+              (b.reappeared_cells - a.reappeared_cells) / ev,
+              (b.check_failures - a.check_failures) / ev,
+              (b.repair_check_failures - a.repair_check_failures) / ev,
+              (b.repaired_cells - a.repaired_cells) / ev, b.newton_steps - a.newton_steps,
+              b.cg_iterations - a.cg_iterations);
+}
+
 int stage(cot_context* ctx, double* w, const cot_solve_opts& so, double handoff, int np,
           std::vector<cot_report>& rep) {
   rep.assign(np, cot_report{});
@@ -394,6 +409,8 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
     std::vector<double> cen(2 * (size_t)np * n);
     if (rc == COT_OK) rc = cot_centroids(cc, wl.data(), 1, cen.data(), nullptr);
     const auto tl3 = now();
+    cot_stats cs{};
+    if (so.verbose > 0 && rc == COT_OK) rc = cot_get_stats(cc, &cs);
     cot_destroy(cc);
     if (rc != COT_OK) return rc;
     for (int p = 0; p < np; ++p)
@@ -405,6 +422,7 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
       std::printf("[ladder] level %d: %d problems, iterations max %d; ms: set-up %.2f create %.2f "
                   "solve %.2f destroy+field %.2f\n",
                   side, np, it, ms(tl0, tl1), ms(tl1, tl2), ms(tl2, tl3), ms(tl3, now()));
+      print_counts(cot_stats{}, cs);
       std::fflush(stdout);
     }
   }
@@ -449,11 +467,16 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
   }
   // the exact target, to the caller's stopping rule
   const auto tf1 = now();
-  int rc = stage(ctx, w, so, handoff, np, rep);
+  cot_stats s0{}, s1{};
+  int rc = so.verbose > 0 ? cot_get_stats(ctx, &s0) : COT_OK;
+  if (rc == COT_OK) rc = stage(ctx, w, so, handoff, np, rep);
   if (rc != COT_OK) return rc;
   if (so.verbose > 0) {
+    const auto tf2 = now();
+    if ((rc = cot_get_stats(ctx, &s1)) != COT_OK) return rc;
     std::printf("[ladder] final: ms: start and floor stages %.2f, solve %.2f\n", ms(tf0, tf1),
-                ms(tf1, now()));
+                ms(tf1, tf2));
+    print_counts(s0, s1);
     std::fflush(stdout);
   }
   // the reference's gauge: its L-BFGS from w = 0 keeps sum(w) = 0 (the gradient sums to zero);

@@ -375,6 +375,17 @@ typedef struct {
   double coarse_seconds;      /* cot_solve_newton: host time of the coarse-matrix inverses */
   long long cg_restarts;      /* cot_solve_newton: steps whose CG diverged with the full
                                  preconditioner and was redone with Jacobi alone */
+  long long handoff_degenerate;    /* FP32 full passes: cells handed to the FP64 repair for a
+                                      degeneracy FP32 cannot resolve (sliver, marginal
+                                      emptying, a vertex the refinement could not place) */
+  long long handoff_overflow;      /* full passes: cells handed to the repair for a polygon
+                                      above the full pass's vertex limit */
+  long long check_failures;        /* sites failing the conformity check after a full pass */
+  long long repair_check_failures; /* sites failing it after a repair pass */
+  long long repaired_cells;        /* cells rebuilt by repair passes */
+  long long reappeared_cells;      /* cells left empty by an evaluation that the next one's
+                                      pre-pass found non-empty again (queued as candidates of
+                                      their new neighbours before the full pass) */
 } cot_stats;
 
 int cot_get_stats(cot_context* ctx, cot_stats* out);

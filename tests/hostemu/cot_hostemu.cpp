@@ -34,6 +34,8 @@ struct Session {
   std::vector<int2> kbox;
   std::vector<float2> espan;
   bool have_adj = false;
+  std::vector<int> elist;  // cells the last evaluation left empty (site slots)
+  int ecount = 0;
   int ring_max = 32;  // the library's default
 };
 
@@ -136,6 +138,7 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
   S->used.assign(n, 0);
   S->redo.assign(n, 0);
   S->adjn.assign(n, 0);
+  S->elist.assign(n, 0);
   S->elab.assign((size_t)n * kMaxVerts, 0);
   S->adj.assign((size_t)n * kMaxVerts, 0);
   S->xcnt.assign(n, 0);
@@ -300,13 +303,39 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
       for (int ln = 0; ln < kCellBlock; ++ln) fails += verify_site(v, b, ln) == 1 ? 1 : 0;
     return fails;
   };
-  auto save_adjacency = [&]() {
-    for (int r = 0; r < n; ++r) {
-      S.adjn[r] = S.ne[r];
-      for (int k = 0; k < S.ne[r]; ++k)
-        S.adj[(size_t)r * kMaxVerts + k] = S.elab[(size_t)r * kMaxVerts + k];
-    }
+  // as the library: the evaluation's last save also lists the empty cells for the next one
+  auto save_adjacency = [&](bool last) {
+    if (last) S.ecount = 0;
+    for (int b = 0; b < nb; ++b)
+      for (int ln = 0; ln < kCellBlock; ++ln)
+        save_adjacency_site(S.blocks[b], b, ln, S.ne.data(), S.elab.data(), S.adjn.data(),
+                            S.adj.data(), last ? S.elist.data() : nullptr, &S.ecount, n);
   };
+  if (S.have_adj) {
+    // as the library (cell_reappear_kernel): the cells the last evaluation left empty, built
+    // before the full pass without side effects on their next build; those that come back are
+    // queued as candidates of their new neighbours
+    CellArgs b = a;
+    b.xcnt = nullptr;
+    b.coarse = nullptr;
+    b.probe = 1;
+    Lanes<float, kPassVerts> lf;
+    Lanes<double, kPassVerts> ld;
+    const int ne_list = std::min(S.ecount, n);
+    for (int e = 0; e < ne_list; ++e) {
+      const int slot = S.elist[e], blk = slot / kCellBlock, ln = slot % kCellBlock;
+      const int rc = S.fp32 ? cell_site<float, kPassVerts>(b, blk, ln, *lf.P, ln)
+                            : cell_site<double, kPassVerts>(b, blk, ln, *ld.P, ln);
+      if (rc < 0) continue;
+      const int i = S.blocks[blk].first + ln;
+      for (int k = 0; k < S.ne[i]; ++k) {
+        const int j = S.elab[(size_t)i * kMaxVerts + k];
+        if (j < 0) continue;
+        const int q = S.xcnt[j]++;
+        if (q < kMaxExtra) S.xcand[(size_t)j * kMaxExtra + q] = i;
+      }
+    }
+  }
   run_cells();
   int fails = run_verify() + coarse;
   int rounds = 0;
@@ -320,7 +349,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
       for (int r = 0; r < n; ++r)
         if (S.redo[r]) S.ring[r] = (uint8_t)std::min(2 * (int)S.ring[r] + 1, S.ring_max);
     } else {
-      save_adjacency();
+      save_adjacency(false);
     }
     rebuilt += run_cells();
     std::fill(S.redo.begin(), S.redo.end(), 0);
@@ -330,7 +359,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   for (int r = 0; r < n; ++r) rmax = std::max(rmax, (int)S.ring[r]);
   // as the library: the cells just built are the next candidates; after a clean check the next
   // bucket ring is the outermost used ring plus one
-  save_adjacency();
+  save_adjacency(true);
   // as the library: cells that do not tile the target leave the evaluation unverified, and the
   // next evaluation searches bucket rings
   double area = 0.0;
