@@ -228,6 +228,7 @@ struct cot_context {
   uint8_t* d_adjn = nullptr;     // previous evaluation's edge counts (0: none yet)
   int* d_elist = nullptr;         // cells the last evaluation left empty (see CellArgs)
   int* d_ecount = nullptr;
+  uint8_t* d_prebuilt = nullptr;
   bool have_adj = false;
   double2* d_vtx = nullptr;      // cell vertices (construction -> integration)
   int2* d_kbox = nullptr;
@@ -407,6 +408,7 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   ca.empty_list = c->d_elist;
   ca.empty_count = c->d_ecount;
   ca.empty_cap = (int)c->ntot;
+  ca.prebuilt = c->d_prebuilt;
   va = VerifyArgs{};
   va.probs = c->d_probs;
   va.blocks = c->d_blocks;
@@ -1070,7 +1072,8 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
       (rc = dalloc(c, &c->d_slot, c->np)) || (rc = dalloc(c, &c->d_hcount, c->np)) ||
       (rc = dalloc(c, &c->d_org, N)) || (rc = dalloc(c, &c->d_adj, N * kMaxVerts)) ||
       (rc = dalloc(c, &c->d_adjn, N)) || (rc = dalloc(c, &c->d_elist, N)) ||
-      (rc = dalloc(c, &c->d_ecount, 1)) || (rc = dalloc(c, &c->d_xcnt, N)) ||
+      (rc = dalloc(c, &c->d_ecount, 1)) || (rc = dalloc(c, &c->d_prebuilt, N)) ||
+      (rc = dalloc(c, &c->d_xcnt, N)) ||
       (rc = dalloc(c, &c->d_vtx, N * kMaxVerts)) || (rc = dalloc(c, &c->d_kbox, N)) ||
       (rc = dalloc(c, &c->d_xcand, N * kMaxExtra)) ||
       (rc = dalloc(c, &c->d_defer, N)) || (rc = dalloc(c, &c->d_dcount, 1)) ||
@@ -1129,6 +1132,7 @@ int cot_create(const cot_problem* problems, int n_problems, const cot_options* o
   if (launch_fill_u8(c->d_ring, N, (uint8_t)c->opt.ring_init, c->st) != hipSuccess ||
       launch_fill_u8(c->d_adjn, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_ecount, 1, 0, c->st) != hipSuccess ||
+      launch_fill_u8(c->d_prebuilt, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_redo, N, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_lcount, 2, 0, c->st) != hipSuccess ||
       launch_fill_i32(c->d_xcnt, N, 0, c->st) != hipSuccess ||

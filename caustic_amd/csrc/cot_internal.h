@@ -294,11 +294,13 @@ struct CellArgs {
   int* defer_count;
   // cells the last evaluation left empty (site slots, at most empty_cap; the evaluation's last
   // save_adjacency fills the list, the integration pass clears the count): built before the full
-  // pass to queue those that come back with their new neighbours (cell_reappear_kernel)
+  // pass, which then skips them, to queue those that come back with their new neighbours
+  // (cell_reappear_kernel)
   int* empty_list;
   int* empty_count;
   int empty_cap;
-  int probe;                // build without side effects on the next build (FP32 frame, queue)
+  uint8_t* prebuilt;        // per site: built by that pre-pass (the full pass clears it)
+  int prepass;              // this launch is the pre-pass
   const int* adj;           // [site * kMaxVerts + k]: edge labels of the previous cell
   const uint8_t* adjn;      // edge count of the previous cell (0: none)
   int* xcnt;                // per site: queued candidates (cleared when the site is built)

@@ -461,6 +461,13 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
   if (a.redo_mode && a.diag) count_add(&a.diag[kDiagRebuilt]);
+  if (!a.redo_mode && !a.prepass && a.prebuilt && a.prebuilt[i]) {
+    // built by the pre-pass (cell_reappear_kernel); the candidates queued for it are dropped
+    // (its cell came from the bucket rings)
+    a.prebuilt[i] = 0;
+    if (a.xcnt) a.xcnt[i] = 0;
+    return -1;
+  }
   // the full pass clears the flag (an FP32 cell may set it again below); in a repair pass the
   // integration pass that follows clears it
   if (!a.redo_mode && a.redo) a.redo[i] = 0;
@@ -792,7 +799,7 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
     int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
     int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
     kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
-    if (kF32 && !a.probe) a.org[i] = geo;  // next frame: this centroid
+    if (kF32) a.org[i] = geo;  // next frame: this centroid
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
   a.kbox[i] = kb;
@@ -829,8 +836,24 @@ __host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, i
   return i;
 }
 
-// Moments of lane q's share (edges q, q + L, ...) of site i's edges (local pixel units, origin
-// kbox[i]).
+// A point of the segment a -> b at parameter t, by one formula wherever a split point of the
+// boundary is computed (the two lanes that share it get the same point).
+__host__ __device__ __forceinline__ double2 split_point(double2 a, double2 b, double t) {
+  return make_double2(fma(t, b.x - a.x, a.x), fma(t, b.y - a.y, a.y));
+}
+
+// Walking work of the edge a -> b (local pixel units): its L1 length plus one, about the pieces
+// its walk crosses; nothing for an edge along U (no dV: the walk returns at once).
+__host__ __device__ __forceinline__ double walk_weight(double2 a, double2 b) {
+  return b.y == a.y ? 0.0 : fabs(b.x - a.x) + fabs(b.y - a.y) + 1.0;
+}
+
+// Moments of lane q's share of site i's boundary (local pixel units, origin kbox[i]).  The
+// boundary, edge after edge, is cut into L runs of equal walking work (walk_weight) and lane q
+// walks run q: a cell whose edges differ in length -- most of them, and the large cells of
+// low-density regions above all -- would otherwise keep its site's lanes waiting on the lane
+// with the longest edge.  The runs meet at split points inside edges (split_point); Green's
+// integral is additive over them.
 template <typename T, int L = kIntLanes>
 __host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, int i, int q,
                                         Moments& M) {
@@ -850,10 +873,69 @@ __host__ __device__ void integrate_lane(const CellArgs& a, const ProbDesc& pd, i
   const double fu0 = (double)kb.x, fv0 = (double)kb.y;
   const int lim_u = pd.pm - kb.x, lim_v = pd.pn - kb.y;
   const double2* v = a.vtx + (size_t)i * kMaxVerts;
-  for (int k = q; k < nv; k += L) {
-    const double2 va = v[k], vb = v[k + 1 < nv ? k + 1 : 0];
-    walk_edge<T>((va.x - pd.x0) * idx_ - fu0, (va.y - pd.y0) * idy_ - fv0,
-                 (vb.x - pd.x0) * idx_ - fu0, (vb.y - pd.y0) * idy_ - fv0, lim_u, lim_v, px, M);
+  auto local = [&](int k) {
+    const double2 p = v[k];
+    return make_double2((p.x - pd.x0) * idx_ - fu0, (p.y - pd.y0) * idy_ - fv0);
+  };
+  const double2 first = local(0);
+  double total = 0.0;
+  {
+    double2 pa = first;
+    for (int k = 0; k < nv; ++k) {
+      const double2 pb = k + 1 < nv ? local(k + 1) : first;
+      total += walk_weight(pa, pb);
+      pa = pb;
+    }
+  }
+  // run q: [s0, s1) of the cumulative weight (the last run ends at the total itself)
+  const double s0 = q == 0 ? 0.0 : total * (double)q / (double)L;
+  const double s1 = q + 1 == L ? total : total * (double)(q + 1) / (double)L;
+  if (!(s1 > s0)) return;
+  // the run's first edge k (cumulative weight c0 before it, from pa to pb, weight wk)
+  int k = 0;
+  double c0 = 0.0;
+  double2 pa = first;
+  double2 pb = nv > 1 ? local(1) : first;
+  double wk = walk_weight(pa, pb);
+  while (c0 + wk <= s0 && k + 1 < nv) {
+    c0 += wk;
+    pa = pb;
+    ++k;
+    pb = k + 1 < nv ? local(k + 1) : first;
+    wk = walk_weight(pa, pb);
+  }
+  // one flat loop over the run's pieces, edge after edge: the lanes of a wave step through their
+  // runs together (a loop per edge would make them wait for each other edge by edge)
+  EdgeWalker wlk;
+  bool walking = false;
+  for (;;) {
+    if (!walking) {
+      if (k >= nv || c0 >= s1) break;
+      const double c1 = c0 + wk;
+      const double lo = fmax(c0, s0), hi = fmin(c1, s1);
+      if (hi > lo) {
+        const double2 p0 = lo == c0 ? pa : split_point(pa, pb, (lo - c0) / wk);
+        const double2 p1 = hi == c1 ? pb : split_point(pa, pb, (hi - c0) / wk);
+        if (p1.y != p0.y) {  // a part too short to change V adds nothing
+          wlk.init(p0.x, p0.y, p1.x, p1.y, lim_u, lim_v);
+          walking = true;
+        }
+      }
+      c0 = c1;
+      pa = pb;
+      ++k;
+      if (k < nv) {
+        pb = k + 1 < nv ? local(k + 1) : first;
+        wk = walk_weight(pa, pb);
+      }
+      if (!walking) continue;
+    }
+    double a0u, a0v, a1u, a1v;
+    int ix, iy;
+    const bool last = wlk.next(a0u, a0v, a1u, a1v, ix, iy);
+    const PrefixTaps taps = px.load(iy, ix);
+    piece<T>(a0u, a0v, a1u, a1v, ix, iy, px, taps, M);
+    if (last) walking = false;
   }
 }
 
@@ -1074,6 +1156,14 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
   if (a.redo_mode && a.diag && ln == 0) count_add(&a.diag[kDiagRebuilt]);
+  if (!a.redo_mode && !a.prepass && a.prebuilt && a.prebuilt[i]) {
+    // built by the pre-pass (as cell_site)
+    if (ln == 0) {
+      a.prebuilt[i] = 0;
+      if (a.xcnt) a.xcnt[i] = 0;
+    }
+    return -1;
+  }
   if (!a.redo_mode && a.redo && ln == 0) a.redo[i] = 0;
   const ProbDesc pd = a.probs[bd.prob];
   double tstep = 0.0;
@@ -1391,7 +1481,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
     const int kox = (int)floor((o.x + mnu - pd.x0) / pd.dx);
     const int koy = (int)floor((o.y + mnv - pd.y0) / pd.dy);
     kb = make_int2(min(max(kox, 0), pd.pm - 1), min(max(koy, 0), pd.pn - 1));
-    if (kF32 && ln == 0 && !a.probe) a.org[i] = geo;  // next frame: this centroid
+    if (kF32 && ln == 0) a.org[i] = geo;  // next frame: this centroid
     if (kF32 && fabs(ar) < kSliverRel * pd.bsize * pd.bsize) coarse = true;
   }
   // a full pass whose polygon overflowed V skipped a constraint: the repair pass rebuilds the
@@ -1448,23 +1538,19 @@ __global__ __launch_bounds__(64) void cell_defer_kernel(CellArgs a) {
 // and comes back reappears between sites whose adjacency (the previous cells) does not hold it:
 // each would miss the new edge, fail the conformity check and need a repair.  A listed cell that
 // comes back is queued as a candidate of each of its new neighbours (as the conformity check
-// queues a missed candidate), which the full pass then takes with their adjacency.  The build
-// only probes: the site's own queue and FP32 frame are left as they were, and the full pass
-// builds the cell again from the same inputs (the bucket rings: it has no adjacency).  Folds
-// into the construction pass's timer.
+// queues a missed candidate), which the full pass then takes with their adjacency.  The listed
+// cells are built as the full pass would build them (no adjacency: the bucket rings), except
+// that the candidates queued for them are not taken (other listed cells queue themselves here
+// while they are built); the full pass skips them (CellArgs::prebuilt).  Folds into the
+// construction pass's timer.
 template <typename T>
 __global__ __launch_bounds__(64) void cell_reappear_kernel(CellArgs a, int* xcand) {
   __shared__ WaveLDS<T> W;
   const unsigned long long t0 = timer_begin();
-  unsigned long long* const timer = a.timer;
   int* const xcnt = a.xcnt;
-  unsigned long long* const diag = a.diag;
   a.xcnt = nullptr;
-  a.coarse = nullptr;
   a.defer_list = nullptr;
-  a.diag = nullptr;
-  a.timer = nullptr;
-  a.probe = 1;
+  a.prepass = 1;
   const int n = min(*a.empty_count, a.empty_cap);
   bool worked = false;
   for (int e = blockIdx.x; e < n; e += gridDim.x) {
@@ -1473,10 +1559,12 @@ __global__ __launch_bounds__(64) void cell_reappear_kernel(CellArgs a, int* xcan
     const int rc = cell_site_wave<T, kPassVerts>(a, blk, slot % kCellBlock, W);
     __syncthreads();  // the cell's edge labels, written by their lanes
     if (rc < 0) continue;
+    if (rc == 1 && threadIdx.x == 0) atomicAdd(&a.overflow[a.blocks[blk].prob], 1);
     worked = true;
     const int i = a.blocks[blk].first + slot % kCellBlock;
     const int ne = a.nedge[i];
     const int ln = lane_id();
+    if (ln == 0) a.prebuilt[i] = 1;
     if (ln < ne) {
       const int j = a.elab[(size_t)i * kMaxVerts + ln];
       if (j >= 0) {
@@ -1484,9 +1572,9 @@ __global__ __launch_bounds__(64) void cell_reappear_kernel(CellArgs a, int* xcan
         if (q < kMaxExtra) xcand[(size_t)j * kMaxExtra + q] = i;
       }
     }
-    if (ne > 0 && ln == 0 && diag) count_add(&diag[kDiagReappeared]);
+    if (ne > 0 && ln == 0 && a.diag) count_add(&a.diag[kDiagReappeared]);
   }
-  timer_mark(timer, worked, t0);  // part of the construction pass
+  timer_mark(a.timer, worked, t0);  // part of the construction pass
 }
 
 // Repair construction (FP64) with one wave (block) per listed site; any grid size.
@@ -1950,7 +2038,7 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
     else
       hipLaunchKernelGGL(cell_repair_wave_kernel, dim3(4 * repair_blocks), dim3(64), 0, st, a);
   } else {
-    if (a.empty_list && a.cand != kCandBucket) {
+    if (a.empty_list && a.prebuilt && a.cand != kCandBucket) {
       // the cells the last evaluation left empty, before the full pass builds their neighbours
       int* xcand = const_cast<int*>(a.xcand);
       if (fp32)

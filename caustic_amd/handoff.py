@@ -40,10 +40,13 @@ class OTHandoff:
     points, proj_xy: (N, 2) seed points and their projections (utils/data_utils.py:374-435);
     inside_light_direct: (N, 3) directions inside the lens.  precision: "fp64" (the executable's)
     or "mixed"; solver: "lbfgs" (the reference's algorithm at t = 0) or "ladder"
-    (cot_solve_ladder, the same stopping rule)."""
+    (cot_solve_ladder, the same stopping rule).  diagnostics: as the executable by default, the
+    render and the cell polygons at the start weights before each solve and at the final weights
+    after it (ot_op.cuh:214-219, 499-533; kept in self.diagnostics[t]); they evaluate the cells
+    like the executable does, so a run follows the executable's evaluations exactly."""
 
     def __init__(self, settings, picture, points, proj_xy, inside_light_direct,
-                 precision="fp64", solver="lbfgs", text_precision=True):
+                 precision="fp64", solver="lbfgs", text_precision=True, diagnostics=True):
         sp, lp, li = settings["sample_point_para"], settings["len_para"], settings["light_para"]
         self.total_iter = int(sp["total_iter"])
         self.tol = float(sp["convergence_tolerance"])
@@ -64,6 +67,8 @@ class OTHandoff:
         if solver not in ("lbfgs", "ladder"):
             raise ValueError("solver must be 'lbfgs' or 'ladder'")
         self.solver, self.precision, self.text = solver, precision, text_precision
+        self.diagnose = bool(diagnostics)
+        self.diagnostics = {}
         self.points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
         self.proj_xy = np.ascontiguousarray(proj_xy, dtype=np.float64).reshape(-1, 2)
         n = self.points.shape[0]
@@ -121,6 +126,11 @@ class OTHandoff:
         prob = W.Problem(sites=self.proj_xy, nu=self.nu, rho=self.rho,
                          proj_domain=self.proj_domain)
         with OTBatch([prob], precision=self.precision) as b:
+            diag = {}
+            if self.diagnose:
+                w0 = np.zeros(prob.n) if t == 0 else self.weights
+                diag["render_initial"] = b.render(w0)[0]
+                diag["mesh_initial"] = b.cell_polygons(w0)
             if t == 0 and self.solver == "ladder":
                 w, rep = b.solve_ladder(history=self.history, max_iter=self.total_iter,
                                         tol=self.tol, stop="reference", h0_threshold=1e-5,
@@ -145,6 +155,10 @@ class OTHandoff:
                                             aa_fixed_para=self.aa_fixed,
                                             if_aa_op_initial=self.aa_initial,
                                             raise_on_fail=False)
+            if self.diagnose:
+                diag["render_final"] = b.render(w)[0]
+                diag["mesh_final"] = b.cell_polygons(w)
+                self.diagnostics[t] = diag
             cen, ex = b.centroids(w, weighted=True)
         self.reports[t] = rep
         # the next run reads the weights file (main.cu:193-198, written at 6 digits, :295-299)

@@ -36,6 +36,7 @@ struct Session {
   bool have_adj = false;
   std::vector<int> elist;  // cells the last evaluation left empty (site slots)
   int ecount = 0;
+  std::vector<uint8_t> prebuilt;
   int ring_max = 32;  // the library's default
 };
 
@@ -139,6 +140,7 @@ void* emu_create(int n, const double* sites, const double* nu, const double* rho
   S->redo.assign(n, 0);
   S->adjn.assign(n, 0);
   S->elist.assign(n, 0);
+  S->prebuilt.assign(n, 0);
   S->elab.assign((size_t)n * kMaxVerts, 0);
   S->adj.assign((size_t)n * kMaxVerts, 0);
   S->xcnt.assign(n, 0);
@@ -237,6 +239,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   a.cand = S.have_adj ? (S.fp32 ? kCandAdj1 : kCandAdj2) : kCandBucket;  // as the library
   a.xcnt = S.xcnt.data();
   a.xcand = S.xcand.data();
+  a.prebuilt = S.prebuilt.data();
   a.vtx = S.vtx.data();
   a.kbox = S.kbox.data();
   VerifyArgs v{};
@@ -313,12 +316,11 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   };
   if (S.have_adj) {
     // as the library (cell_reappear_kernel): the cells the last evaluation left empty, built
-    // before the full pass without side effects on their next build; those that come back are
-    // queued as candidates of their new neighbours
+    // before the full pass (which skips them) without their queued candidates; those that come
+    // back are queued as candidates of their new neighbours
     CellArgs b = a;
     b.xcnt = nullptr;
-    b.coarse = nullptr;
-    b.probe = 1;
+    b.prepass = 1;
     Lanes<float, kPassVerts> lf;
     Lanes<double, kPassVerts> ld;
     const int ne_list = std::min(S.ecount, n);
@@ -327,7 +329,9 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
       const int rc = S.fp32 ? cell_site<float, kPassVerts>(b, blk, ln, *lf.P, ln)
                             : cell_site<double, kPassVerts>(b, blk, ln, *ld.P, ln);
       if (rc < 0) continue;
+      overflow += rc == 1 ? 1 : 0;
       const int i = S.blocks[blk].first + ln;
+      S.prebuilt[i] = 1;
       for (int k = 0; k < S.ne[i]; ++k) {
         const int j = S.elab[(size_t)i * kMaxVerts + k];
         if (j < 0) continue;

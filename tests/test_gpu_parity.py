@@ -86,18 +86,27 @@ def test_solve_c1_maxrel(gpu_available, c1, device_loop):
 
 def test_device_and_host_loops_follow_the_same_iterates(gpu_available, c1):
     """Both drivers apply the same rules to the same FP64 evaluations; only the summation order
-    of the dot products differs, so the iterates agree to well below the weights' scale."""
-    print("[case] device vs host loop, 40 iterations", flush=True)
-    out = []
-    for dl in (False, True):
-        with _batch([c1], precision="fp64") as b:
-            w, rep = b.solve(history=10, tol=1e-12, stop="maxrel", h0_threshold=0.0,
-                             max_iter=40, device_loop=dl, raise_on_fail=False)
-        out.append((w, rep[0]))
-    assert out[0][1]["iterations"] == out[1][1]["iterations"] == 40
-    assert abs(out[0][1]["evaluations"] - out[1][1]["evaluations"]) <= 2
-    scale = np.abs(out[0][0]).max()
-    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-6 * scale
+    of the dot products differs.  So the iterates agree to rounding -- until a line-search
+    decision falls within that rounding (a trial whose f is within an ulp or two of the current
+    one), after which the drivers take different, equally valid paths, which shows in their
+    evaluation counts.  Checked after 5, 10, ... 40 iterations: every stretch with equal
+    evaluation counts agrees to 1e-9 of the weights' scale, and the first 10 iterations do."""
+    print("[case] device vs host loop, 5..40 iterations", flush=True)
+    agreed = 0
+    for k in range(5, 41, 5):
+        out = []
+        for dl in (False, True):
+            with _batch([c1], precision="fp64") as b:
+                w, rep = b.solve(history=10, tol=1e-12, stop="maxrel", h0_threshold=0.0,
+                                 max_iter=k, device_loop=dl, raise_on_fail=False)
+            out.append((w, rep[0]))
+        assert out[0][1]["iterations"] == out[1][1]["iterations"] == k
+        if out[0][1]["evaluations"] != out[1][1]["evaluations"]:
+            break
+        scale = np.abs(out[0][0]).max()
+        assert np.abs(out[0][0] - out[1][0]).max() <= 1e-9 * scale, k
+        agreed = k
+    assert agreed >= 10, agreed
 
 
 @pytest.mark.parametrize("device_loop", [False, True])
