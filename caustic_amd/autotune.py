@@ -17,10 +17,14 @@ Knobs (fields of cot_options):
                     of whole blocks), and the elongated blocks doubled the CG iterations of the
                     N=65,536 shapes solve (583 -> 1,176, 195 -> 212 ms a solve) while the
                     evaluations it measures here ran 8% faster.
-  full_pass_ring    candidates of the FP32 full pass: the previous cells' one-ring (1) or
-                    two-ring (2).  The two-ring costs more clipping per cell and leaves fewer
-                    unmatched edges to the repair passes; it pays where weights move far between
-                    evaluations.
+  full_pass_ring    not tuned (kept at 2): candidates of the FP32 full pass, the previous cells'
+                    one-ring (1) or two-ring (2).  The two-ring costs more clipping per cell and
+                    leaves fewer unmatched edges to the repair passes.  On the tuner's stretches
+                    the one-ring ranked within a few percent of the two-ring at N=4,096 and
+                    N=16,384, but in the coarse-to-fine solve's own levels (the start a coarser
+                    level hands over moves the cells further) it failed the conformity check at
+                    1,000 of 4,096 and 3,500 of 16,384 sites per evaluation against 55 and 150
+                    with the two-ring, and made those levels 13-24% slower.
   pass_mode         construction of the full passes: one lane per cell (0) or one wave per cell
                     (1, candidates loaded 64 at a time, cooperative clipping).  The same cells
                     either way; one wave per cell puts many more waves in flight and wins on
@@ -59,10 +63,9 @@ import numpy as np
 KNOBS = {
     "int_lanes": (4, 8, 16),
     "repair_blocks": (128, 256, 512, 1024),
-    "full_pass_ring": (1, 2),
     "pass_mode": (0, 1),
 }
-DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 1,
+DEFAULT = {"int_lanes": 8, "repair_blocks": 256, "sites_per_bucket": 1.0, "full_pass_ring": 2,
            "pass_mode": 0}
 # a change must win by 5%: shapes within a few percent of each other (8 or 16 integration lanes,
 # 256 or 512 repair blocks) trade places from run to run, and each flip sends the solve down a

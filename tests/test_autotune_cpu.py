@@ -22,9 +22,9 @@ def test_coordinate_descent_finds_the_optimum_and_caches(tmp_path, monkeypatch):
     cfg, rep = AT.tune(probs, "mixed")
     assert cfg == {"int_lanes": 16, "repair_blocks": 512, "sites_per_bucket": 1.0,
                    "full_pass_ring": 2, "pass_mode": 1}
-    # the bucket density is not a knob: no candidate leaves 1.0
-    assert all(c["sites_per_bucket"] == 1.0 for c in calls)
-    assert abs(rep["gain_vs_default"] - 2.9 / 1.1) < 1e-9
+    # the bucket density and the full pass's ring are not knobs: no candidate leaves 1.0 and 2
+    assert all(c["sites_per_bucket"] == 1.0 and c["full_pass_ring"] == 2 for c in calls)
+    assert abs(rep["gain_vs_default"] - 2.7 / 1.1) < 1e-9
     # every candidate measured exactly `repeats` (2) times
     distinct = {tuple(sorted(c.items())) for c in calls}
     assert len(calls) == 2 * len(distinct)
@@ -55,7 +55,7 @@ def test_a_slow_outlier_does_not_reject_the_best_shape(tmp_path, monkeypatch):
     def noisy(problems, precision, cfg, w0, iters, solve_kw):
         k = tuple(sorted(cfg.items()))
         seen[k] = seen.get(k, 0) + 1
-        best = cfg["full_pass_ring"] == 2
+        best = cfg["int_lanes"] == 16
         t = 1.0 if best else 1.6
         if best and seen[k] == 1:
             t *= 1.8
@@ -64,7 +64,7 @@ def test_a_slow_outlier_does_not_reject_the_best_shape(tmp_path, monkeypatch):
 
     monkeypatch.setattr(AT, "measure", noisy)
     cfg, rep = AT.tune([W.plain_fp64(n_side=8, res=16)], "mixed", use_cache=False)
-    assert cfg["full_pass_ring"] == 2, cfg
+    assert cfg["int_lanes"] == 16, cfg
     assert abs(rep["per_eval_us"] - 1e6) < 1e-6
 
 
@@ -76,12 +76,12 @@ def test_phase_stretch_runs_to_the_handoff(tmp_path, monkeypatch):
 
     def phase(problems, precision, cfg, w0, iters, solve_kw):
         assert solve_kw["stop"] == "handoff" and solve_kw["tol"] == 1.0 and iters == 400
-        two = cfg["full_pass_ring"] == 2
-        per, evals = (0.8, 80) if two else (1.0, 50)  # cheaper per evaluation, more of them
+        wide = cfg["int_lanes"] == 16
+        per, evals = (0.8, 80) if wide else (1.0, 50)  # cheaper per evaluation, more of them
         return {"per_eval": per, "total": per * evals, "evaluations": evals, "wall": 0.0,
                 "cell": 0.0, "integrate": 0.0, "repair": 0.0}
 
     monkeypatch.setattr(AT, "measure", phase)
     cfg, rep = AT.tune([W.plain_fp64(n_side=8, res=16)], "mixed", use_cache=False,
                        stop_tol=1.0, iters=400)
-    assert cfg["full_pass_ring"] == 2 and rep["objective"] == "per_eval", (cfg, rep)
+    assert cfg["int_lanes"] == 16 and rep["objective"] == "per_eval", (cfg, rep)
