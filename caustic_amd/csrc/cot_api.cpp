@@ -118,14 +118,14 @@ hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdes
                             const double* ainv, double* cvec, double* part_rzc, hipStream_t st);
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
                           const double* shift, const double* diag, const uint8_t* nedge,
-                          const int* hj, const double* h, int sweeps, double* x, double* r,
-                          double* z, double* part_rz, double* part_rr, double* rsum, int nblk,
-                          hipStream_t st);
+                          const int* hj, const double* h, int sweeps, const double* x0, double* x,
+                          double* r, double* z, double* part_rz, double* part_rr, double* part_bb,
+                          double* rsum, int nblk, hipStream_t st);
 hipError_t launch_cg_iteration(const CgArgs& a, const double* p_old, double* p_new, int cur,
                                int first, hipStream_t st);
 hipError_t launch_cg_small(const SmallCgArgs& a, hipStream_t st);
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
-                            const double* d, double* w, int nblk, hipStream_t st);
+                            const double* d, double* w, double* carry, int nblk, hipStream_t st);
 hipError_t launch_render(const ProbDesc* probs, const BlockDesc* blocks, const double2* vtx,
                          const uint8_t* nedge, const double* nu, double* image, size_t pixels,
                          int nblk, hipStream_t st);
@@ -247,6 +247,8 @@ struct cot_context {
   int* d_hj = nullptr;  // Newton: neighbour of each Hessian edge slot (hslot order)
   double *d_hh = nullptr, *d_hdiag = nullptr, *d_nx = nullptr, *d_nr = nullptr, *d_nz = nullptr;
   double *d_np = nullptr, *d_nq = nullptr, *d_prz = nullptr, *d_ppq = nullptr, *d_prr = nullptr;
+  double* d_ncarry = nullptr;  // the part of the last Newton direction its step left
+  double* d_pbb = nullptr;     // partials of |rhs|^2
   double* d_np2 = nullptr;  // the other CG direction buffer (directions alternate)
   double* d_nrz = nullptr;  // r.z of the current and the previous CG iteration
   double* d_nshift = nullptr;
@@ -1527,6 +1529,8 @@ int ensure_newton(cot_context* c) {
   CHK(dalloc(c, &c->d_nrz, 2));
   CHK(dalloc(c, &c->d_ppq, nb));
   CHK(dalloc(c, &c->d_prr, nb));
+  CHK(dalloc(c, &c->d_pbb, nb));
+  CHK(dalloc(c, &c->d_ncarry, N));
   CHK(dalloc(c, &c->d_nshift, c->np));
   CHK(dalloc(c, &c->d_cgout, 4));
   return COT_OK;
@@ -1726,6 +1730,8 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
   };
   CHK(evaluate_device(c, c->d_w, nullptr, act, cur));
   HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
+  // the first conjugate-gradient solve starts from zero
+  HIPCHK(launch_fill_i32(reinterpret_cast<int*>(c->d_ncarry), 2 * c->ntot, 0, st));
   std::vector<double> shift(np, 0.0), t(np, 1.0), rr_part(nb);
   // CG stops at |r| <= eta |rhs| -- an inexact Newton step, eta = 0.1 max-rel clamped to
   // [1e-4, 0.1], loose while far from the solution -- or after kMaxCG iterations.  Iteration 0
@@ -1790,10 +1796,10 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
       HIPCHK(launch_cg_iteration(a, pbuf[k & 1], pbuf[(k + 1) & 1], k & 1, 0, st));
     return COT_OK;
   };
-  // |r|^2 of every problem: the sum of its blocks' partials
-  auto sum_rr = [&](std::vector<double>& rr) -> int {
-    HIPCHK(hipMemcpyAsync(rr_part.data(), c->d_prr, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                          st));
+  // per problem, the sum of its blocks' partials (|r|^2 from part_rr by default)
+  auto sum_rr = [&](std::vector<double>& rr, const double* part = nullptr) -> int {
+    HIPCHK(hipMemcpyAsync(rr_part.data(), part ? part : c->d_prr, sizeof(double) * nb,
+                          hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (int p = 0; p < np; ++p) {
       const ProbDesc& pd = c->probs[p];
@@ -1884,13 +1890,18 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     // iterations run, negative when it diverged
     auto run_cg = [&](const CgArgs& a, int gs, int& iters) -> int {
       const bool co = a.ntask > 0;
+      // from the part of the last direction its step left (d_ncarry, zero after a full step)
       HIPCHK(launch_cg_init(c->d_blocks, c->d_active, c->d_g, c->d_nshift, c->d_hdiag,
-                            c->d_nedge, c->d_hj, c->d_hh, a.sweeps, c->d_nx, c->d_nr, c->d_nz,
-                            c->d_prz, c->d_prr, co ? c->d_rsum : nullptr, nb, st));
+                            c->d_nedge, c->d_hj, c->d_hh, a.sweeps, c->d_ncarry, c->d_nx,
+                            c->d_nr, c->d_nz, c->d_prz, c->d_prr, c->d_pbb,
+                            co ? c->d_rsum : nullptr, nb, st));
       if (co)
         HIPCHK(launch_cg_coarse(c->d_ctask, c->coarse_ntask, c->d_cdesc, c->d_agg_start,
                                 c->d_agg_blk, c->d_rsum, c->d_ainv, c->d_cvec, c->d_prc, st));
-      CHK(sum_rr(rr0));
+      CHK(sum_rr(rr0, c->d_pbb));  // |rhs|^2: the stopping rule's reference
+      CHK(sum_rr(rr));
+      iters = 0;
+      if (cg_state() > 0) return COT_OK;  // the carried direction meets the tolerance
       HIPCHK(launch_cg_iteration(a, pbuf[0], pbuf[1], 0, 1, st));
       iters = 1;
       while (iters < kMaxCG) {
@@ -2018,7 +2029,8 @@ int cot_solve_newton(cot_context* c, double* w, const cot_solve_opts* so, cot_re
     for (int p = 0; p < np; ++p) c->h_small[p] = t[p];
     HIPCHK(hipMemcpyAsync(c->d_active, c->h_active, np, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->d_step, c->h_small, sizeof(double) * np, hipMemcpyHostToDevice, st));
-    HIPCHK(launch_axpy_step(c->d_blocks, c->d_active, c->d_step, c->d_nx, c->d_w, nb, st));
+    HIPCHK(launch_axpy_step(c->d_blocks, c->d_active, c->d_step, c->d_nx, c->d_w, c->d_ncarry, nb,
+                            st));
     HIPCHK(launch_gradient(c->d_blocks, c->d_mass, c->d_nu, c->d_g, nb, st));
     // the pinned staging (h_active, h_small) is rewritten at the top of the next iteration:
     // the copies above must have been made

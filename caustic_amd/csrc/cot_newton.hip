@@ -287,35 +287,65 @@ __global__ __launch_bounds__(256) void cg_coarse_kernel(const int4* tasks, const
 // CG start: x = 0, r = -(g - shift[prob]) on the active problems' non-empty cells (0 elsewhere;
 // shift = the problem's mean of g, which puts the right-hand side in the range of the singular
 // H), z = r / diag, partial r.z and r.r.
+// x0 (nullptr: zero) starts the iteration: x = x0, r = b - H x0 (x0 is read, x written: two
+// arrays).  part_bb: the partials of |b|^2, the reference of the stopping rule.
 __global__ __launch_bounds__(kCellBlock) void cg_init_kernel(
     const BlockDesc* blocks, const uint8_t* active, const double* g, const double* shift,
     const double* diag, const uint8_t* nedge, const int* hj, const double* h, int sweeps,
-    double* x, double* r, double* z, double* part_rz, double* part_rr, double* rsum) {
+    const double* x0, double* x, double* r, double* z, double* part_rz, double* part_rr,
+    double* part_bb, double* rsum) {
   __shared__ double red[kCellBlock / 64];
   __shared__ double zl[kCellBlock];
   const BlockDesc bd = blocks[blockIdx.x];
   const int ln = threadIdx.x;
   const int i = bd.first + ln;
-  double ri = 0.0, dg = 0.0;
+  double ri = 0.0, dg = 0.0, bi = 0.0, xi = 0.0;
   if (ln < bd.count) {
     dg = diag[i];
-    ri = (active[bd.prob] && dg > 0.0) ? shift[bd.prob] - g[i] : 0.0;
+    if (x0) xi = x0[i];
+    if (active[bd.prob] && dg > 0.0) {
+      bi = shift[bd.prob] - g[i];
+      ri = bi;
+      if (x0) {
+        double s = dg * xi;
+        const int nv = nedge[i];
+        for (int k0 = 0; k0 < nv; k0 += kEdgeBatch) {
+          int jj[kEdgeBatch];
+          double hh[kEdgeBatch], xx[kEdgeBatch];
+#pragma unroll
+          for (int u = 0; u < kEdgeBatch; ++u) {
+            const bool in = k0 + u < nv;
+            jj[u] = in ? hj[hslot(blockIdx.x, k0 + u, ln)] : -1;
+            hh[u] = in ? h[hslot(blockIdx.x, k0 + u, ln)] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kEdgeBatch; ++u) xx[u] = jj[u] >= 0 ? x0[jj[u]] : 0.0;
+#pragma unroll
+          for (int u = 0; u < kEdgeBatch; ++u)
+            if (jj[u] >= 0) s -= hh[u] * xx[u];
+        }
+        ri = bi - s;
+      }
+    }
   }
   const double zi = precondition(blockIdx.x, bd, i, ri, dg, nedge, hj, h, sweeps, zl);
-  double rz = 0.0, rr = 0.0;
+  double rz = 0.0, rr = 0.0, bb = 0.0;
   if (ln < bd.count) {
-    x[i] = 0.0;
+    x[i] = xi;
     r[i] = ri;
     z[i] = zi;
     rz = ri * zi;
     rr = ri * ri;
+    bb = bi * bi;
   }
   rz = block_sum(rz, red);
   rr = block_sum(rr, red);
+  bb = block_sum(bb, red);
   const double rs = rsum ? block_sum(ri, red) : 0.0;
   if (ln == 0) {
     part_rz[blockIdx.x] = rz;
     part_rr[blockIdx.x] = rr;
+    part_bb[blockIdx.x] = bb;
     if (rsum) rsum[blockIdx.x] = rs;
   }
 }
@@ -666,14 +696,25 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 }
 
 // w += step[prob] * d for the problems with active[prob] != 0.
+// w += t d for the problems that took a step (active); carry (nullptr: none) receives the part
+// of the direction the step left, (1 - t) d (zero where no step was taken): after a damped step
+// the gradient has fallen by about the factor 1 - t, and so has the next Newton direction, which
+// carry then approximates as the start of the next conjugate-gradient solve.
 __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* blocks,
                                                                const uint8_t* active,
                                                                const double* step,
-                                                               const double* d, double* w) {
+                                                               const double* d, double* w,
+                                                               double* carry) {
   const BlockDesc bd = blocks[blockIdx.x];
-  if (!active[bd.prob] || (int)threadIdx.x >= bd.count) return;
+  if ((int)threadIdx.x >= bd.count) return;
   const int i = bd.first + threadIdx.x;
-  w[i] += step[bd.prob] * d[i];
+  if (!active[bd.prob]) {
+    if (carry) carry[i] = 0.0;
+    return;
+  }
+  const double t = step[bd.prob];
+  w[i] += t * d[i];
+  if (carry) carry[i] = (1.0 - t) * d[i];
 }
 
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
@@ -703,11 +744,11 @@ hipError_t launch_cg_coarse(const int4* tasks, int ntask, const CoarseDesc* cdes
 
 hipError_t launch_cg_init(const BlockDesc* blocks, const uint8_t* active, const double* g,
                           const double* shift, const double* diag, const uint8_t* nedge,
-                          const int* hj, const double* h, int sweeps, double* x, double* r,
-                          double* z, double* part_rz, double* part_rr, double* rsum, int nblk,
-                          hipStream_t st) {
+                          const int* hj, const double* h, int sweeps, const double* x0, double* x,
+                          double* r, double* z, double* part_rz, double* part_rr, double* part_bb,
+                          double* rsum, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(cg_init_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, g, shift,
-                     diag, nedge, hj, h, sweeps, x, r, z, part_rz, part_rr, rsum);
+                     diag, nedge, hj, h, sweeps, x0, x, r, z, part_rz, part_rr, part_bb, rsum);
   return hipGetLastError();
 }
 
@@ -750,9 +791,9 @@ hipError_t launch_hessian_apply(const BlockDesc* blocks, const uint8_t* nedge, c
 }
 
 hipError_t launch_axpy_step(const BlockDesc* blocks, const uint8_t* active, const double* step,
-                            const double* d, double* w, int nblk, hipStream_t st) {
+                            const double* d, double* w, double* carry, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(axpy_step_kernel, dim3(nblk), dim3(kCellBlock), 0, st, blocks, active, step,
-                     d, w);
+                     d, w, carry);
   return hipGetLastError();
 }
 
