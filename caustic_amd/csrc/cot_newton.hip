@@ -28,6 +28,8 @@ namespace {
 
 // edges of a cell read per batch by the Hessian products (their loads issued together)
 constexpr int kEdgeBatch = 8;
+// Smallest damped step whose remaining direction starts the next conjugate-gradient solve.
+constexpr double kCarryMinStep = 0.25;
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
@@ -697,9 +699,10 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(SmallCgArgs a) 
 
 // w += step[prob] * d for the problems with active[prob] != 0.
 // w += t d for the problems that took a step (active); carry (nullptr: none) receives the part
-// of the direction the step left, (1 - t) d (zero where no step was taken): after a damped step
-// the gradient has fallen by about the factor 1 - t, and so has the next Newton direction, which
-// carry then approximates as the start of the next conjugate-gradient solve.
+// of the direction the step left, (1 - t) d: after a damped step the gradient has fallen by about
+// the factor 1 - t, and so has the next Newton direction, which carry then approximates as the
+// start of the next conjugate-gradient solve.  Zero where no step was taken, and after a step
+// below kCarryMinStep: a direction the damping cut that short was a poor model of the next one.
 __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* blocks,
                                                                const uint8_t* active,
                                                                const double* step,
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(kCellBlock) void axpy_step_kernel(const BlockDesc* 
   }
   const double t = step[bd.prob];
   w[i] += t * d[i];
-  if (carry) carry[i] = (1.0 - t) * d[i];
+  if (carry) carry[i] = t >= kCarryMinStep ? (1.0 - t) * d[i] : 0.0;
 }
 
 hipError_t launch_hessian(const ProbDesc* probs, const BlockDesc* blocks, const uint8_t* active,
