@@ -27,8 +27,7 @@ static_assert(COT_MAX_VERTS == cot::kMaxVerts, "cot_cell_polygons writes kMaxVer
 
 namespace cot {
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
-                       int repair_blocks, hipStream_t st);
-hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st);
+                       int repair_blocks, const VerifyArgs* check, hipStream_t st);
 hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, DensityGroup* grp,
                          double4* tot, hipStream_t st);
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* active,
@@ -424,7 +423,6 @@ void base_args(cot_context* c, CellArgs& ca, VerifyArgs& va) {
   va.tol_rel = c->opt.verify_tol;
   va.xcnt = c->d_xcnt;
   va.xcand = c->d_xcand;
-  va.timer_close = c->d_timer + kTimerInteg * kTimerWords;
   ca.diag = c->d_diag;
   va.diag_fail = c->d_diag + kDiagCheckFull;
 }
@@ -501,8 +499,7 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   va.list = list(c, 0);
   va.list_count = list_count(c, 0);
 
-  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
-  HIPCHK(launch_verify(va, c->nblk, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, &va, st));
   HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu, c->d_exist,
                             d_dir, c->d_part, c->nblk, st));
   HIPCHK(hipMemcpyAsync(c->h_part, c->d_part, sizeof(double) * kEvalPartials * c->nblk,
@@ -516,7 +513,6 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
   ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
-  va.timer_close = ca.timer_int;
   va.diag_fail = c->d_diag + kDiagCheckRepair;
   int rounds = 0;
   for (;;) {
@@ -537,9 +533,8 @@ int evaluate_device(cot_context* c, const double* d_w, const double* d_dir,
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, c->d_fails, c->d_active, c->d_nedge, c->d_elab,
                                    c->d_adjn, c->d_adj, nullptr, nullptr, 0, c->nblk, st));
-    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     HIPCHK(launch_fill_i32(c->d_fails, (size_t)c->np, 0, st));
-    HIPCHK(launch_verify(va, c->nblk, st));
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, &va, st));
     HIPCHK(launch_reduce_eval(c->d_blocks, c->d_mass, c->d_fcell, c->d_area, c->d_nu,
                               c->d_exist, d_dir, c->d_part, c->nblk, st));
     HIPCHK(hipMemcpyAsync(c->h_part, c->d_part, sizeof(double) * kEvalPartials * c->nblk,
@@ -705,13 +700,11 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
   va.fails = c->d_rfails;
   va.list = list(c, 0);
   va.list_count = list_count(c, 0);
-  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
-  HIPCHK(launch_verify(va, c->nblk, st));
+  HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, &va, st));
   ca.redo_mode = 1;
   ca.coarse = nullptr;
   ca.timer = c->d_timer + kTimerRepair * kTimerWords;
   ca.timer_int = c->d_timer + kTimerIntegRepair * kTimerWords;
-  va.timer_close = ca.timer_int;
   va.diag_fail = c->d_diag + kDiagCheckRepair;
   for (int r = 1; r <= R; ++r) {
     const int* gate = c->d_rfails + (size_t)(r - 1) * np;
@@ -728,10 +721,9 @@ int launch_round(cot_context* c, const SolveParams& sp, int R) {
     else
       HIPCHK(launch_save_adjacency(c->d_blocks, gate, c->d_active, c->d_nedge, c->d_elab,
                                    c->d_adjn, c->d_adj, nullptr, nullptr, 0, c->nblk, st));
-    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, st));
     va.gate = gate;
     va.fails = c->d_rfails + (size_t)r * np;
-    HIPCHK(launch_verify(va, c->nblk, st));
+    HIPCHK(launch_cell(ca, c->nblk, c->fp32, c->opt.int_lanes, c->opt.repair_blocks, &va, st));
   }
   HIPCHK(launch_save_adjacency(c->d_blocks, nullptr, c->d_active, c->d_nedge, c->d_elab,
                                c->d_adjn, c->d_adj, c->d_elist, c->d_ecount, (int)c->ntot,

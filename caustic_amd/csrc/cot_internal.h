@@ -104,11 +104,12 @@ constexpr int kDeferWaves = 4096;
 constexpr int kDeferCand = 64;
 // Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
 // folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
-// the next kernel on the stream closes the launch: [2 * kTimerSlots] accumulates the ticks from
+// the launch is then closed -- by the next kernel on the stream, or by the launch's own last
+// block (arrival count at [2 * kTimerSlots + 2]): [2 * kTimerSlots] accumulates the ticks from
 // the earliest start to the latest end, [2 * kTimerSlots + 1] counts such launches, and the
 // slots are reset.
 constexpr int kTimerSlots = 16;
-constexpr int kTimerWords = 2 * kTimerSlots + 2;
+constexpr int kTimerWords = 2 * kTimerSlots + 3;
 enum : int { kTimerCell = 0, kTimerRepair = 1, kTimerInteg = 2, kTimerIntegRepair = 3, kTimers = 4 };
 // Repair counters (one 64-bit word each, accumulated over every evaluation): cells a full FP32
 // pass handed to the FP64 repair for a degeneracy (sliver, marginal emptying, vertex the
@@ -255,12 +256,12 @@ struct CellArgs {
   uint8_t* ring;            // per-site candidate ring radius (read, raised on repair)
   int* redo;                // repair flags: 0 none, 1 listed for a repair, 2 an FP32 cell that
                             // the next check lists; a full pass clears its sites' flags, a
-                            // repair pass those of the sites it rebuilt
+                            // repair construction those of the sites it rebuilds
   int redo_mode;            // 0: all sites of active problems; 1: the sites of the work list
   const int* list;          // repair pass: work list of site slots (block * kCellBlock + lane)
   const int* list_count;    //   and its length
-  int* reset_count;         // integration pass: the length of the list the next check fills,
-                            // cleared here (nullptr: none)
+  int* reset_count;         // construction pass: the length of the list the check that follows
+                            // fills, cleared here (nullptr: none)
   const uint8_t* active;    // per-problem: evaluate only problems with active[p] != 0
   const struct DevState* st;  // device-driven solve: step, direction sign, activity per problem
   const double* g;          // gradient (steepest-descent trials)
@@ -361,7 +362,6 @@ struct VerifyArgs {
   double tol_rel;           // tolerance relative to the site spacing of the pair
   int* xcnt;                // candidate queues of the repair (see kMaxExtra)
   int* xcand;
-  unsigned long long* timer_close;  // the integration timer this check closes, or nullptr
   unsigned long long* diag_fail;    // counter of the sites this check fails, or nullptr
 };
 

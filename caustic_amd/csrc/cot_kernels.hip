@@ -11,9 +11,10 @@
 //               compensated sum), across the site's lanes with shuffles.
 //               Reference semantics: polygons_in_VD + compute_inte + compute_f + compute_center
 //               (power_diagram.cpp:8-134, basic_function.cuh:743-1191).
-// verify_kernel checks that every edge of every cell is matched by the reciprocal edge of the
-//               neighbouring cell; a cell built from too few candidates fails this and is rebuilt
-//               with a larger ring.
+// the conformity check (verify_site, run by workgroups of the integration launch) checks that
+//               every edge of every cell is matched by the reciprocal edge of the neighbouring
+//               cell; a cell built from too few candidates fails this and is rebuilt with a
+//               larger ring.
 // reduce/accept/direction kernels: FP64 per-problem reductions and the L-BFGS vector updates.
 #include "cot_internal.h"
 
@@ -460,7 +461,11 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
-  if (a.redo_mode && a.diag) count_add(&a.diag[kDiagRebuilt]);
+  if (a.redo_mode) {
+    // a repair clears the flag of the site it rebuilds (the check that follows may set it again)
+    a.redo[i] = 0;
+    if (a.diag) count_add(&a.diag[kDiagRebuilt]);
+  }
   if (!a.redo_mode && !a.prepass && a.prebuilt && a.prebuilt[i]) {
     // built by the pre-pass (cell_reappear_kernel); the candidates queued for it are dropped
     // (its cell came from the bucket rings)
@@ -822,7 +827,8 @@ __host__ __device__ int cell_site(const CellArgs& a, int blk, int site_ln, PolyL
 // integration pass: L lanes per site (4, 8 or 16); lane q walks edges q, q + L, ...
 // ---------------------------------------------------------------------------------------------
 
-// The site of integration lane `t` of block (blk, quarter) with L lanes per site; -1 when none.
+// The site of integration lane `t` of block (blk, quarter) with L lanes per site; -1 when none
+// (a full pass: the repair passes integrate the sites of their work list).
 template <int L = kIntLanes>
 __host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, int quarter,
                                                    int t) {
@@ -831,9 +837,7 @@ __host__ __device__ __forceinline__ int integ_site(const CellArgs& a, int blk, i
   if (ln >= bd.count) return -1;
   if (!a.active[bd.prob]) return -1;
   if (a.gate && a.gate[bd.prob] == 0) return -1;
-  const int i = bd.first + ln;
-  if (a.redo_mode && !a.redo[i]) return -1;
-  return i;
+  return bd.first + ln;
 }
 
 // A point of the segment a -> b at parameter t, by one formula wherever a split point of the
@@ -980,14 +984,14 @@ __host__ __device__ void finish_site(const CellArgs& a, const ProbDesc& pd, int 
   a.mass[i] = m0;
   a.fcell[i] = fi;
   a.cen[i] = cen;
-  if (a.redo_mode) a.redo[i] = 0;
 }
 
 
 // In-kernel launch timer (kTimerWords words, see cot_internal.h), read with the constant-rate
 // wall clock.  timer_begin: the block's start (thread 0); timer_mark: after the block's work, a
 // block in which some thread worked folds (start, now) into its slot; timer_close: thread 0 of
-// block 0 of the kernel that follows on the stream accumulates the closed launch.
+// block 0 of the kernel that follows on the stream accumulates the closed launch; timer_finish:
+// the launch's last block to end does the same.
 __device__ __forceinline__ unsigned long long timer_begin() {
   return threadIdx.x == 0 ? (unsigned long long)wall_clock64() : 0ull;
 }
@@ -1020,9 +1024,39 @@ __device__ void timer_close(unsigned long long* tm) {
   }
 }
 
+// The launch's last block to reach this (every block calls it once, after its timer_mark) closes
+// the launch's timer.  Loads and stores of the record go through device-scope atomics: the other
+// blocks' marks are not in this block's cache.
+__device__ void timer_finish(unsigned long long* tm) {
+  if (!tm) return;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  __threadfence();
+  const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
+  if (atomicAdd(&tm[2 * kTimerSlots + 2], 1ull) + 1 != total) return;
+  __threadfence();
+  atomicExch(&tm[2 * kTimerSlots + 2], 0ull);
+  unsigned long long t0 = ~0ull, t1 = 0ull;
+  for (int r = 0; r < kTimerSlots; ++r) {
+    t0 = min(t0, atomicExch(&tm[2 * r], ~0ull));
+    t1 = max(t1, atomicExch(&tm[2 * r + 1], 0ull));
+  }
+  if (t0 != ~0ull && t1 > t0) {
+    atomicAdd(&tm[2 * kTimerSlots], t1 - t0);
+    atomicAdd(&tm[2 * kTimerSlots + 1], 1ull);
+  }
+}
+
+// Construction pass: clears the length of the work list the conformity check that follows fills
+// (no kernel reads or writes that list until the check).
+__device__ __forceinline__ void reset_check_list(const CellArgs& a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && a.reset_count) *a.reset_count = 0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
   __shared__ PolyLDS<T, kPassVerts> P;
+  reset_check_list(a);
   const unsigned long long t0 = timer_begin();
   const int blk = xcd_slot(blockIdx.x, gridDim.x);
   const int rc = cell_site<T, kPassVerts>(a, blk, threadIdx.x, P, threadIdx.x);
@@ -1035,6 +1069,7 @@ __global__ __launch_bounds__(kCellBlock) void cell_kernel(CellArgs a) {
 template <typename T>
 __global__ __launch_bounds__(kCellBlock) void cell_repair_kernel(CellArgs a) {
   __shared__ PolyLDS<T, kMaxVerts> P;
+  reset_check_list(a);
   const unsigned long long t0 = timer_begin();
   const int n = *a.list_count;
   bool worked = false;
@@ -1155,7 +1190,12 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
   if (a.gate && a.gate[bd.prob] == 0) return -1;
   const int i = bd.first + site_ln;
   if (a.redo_mode && !a.redo[i]) return -1;
-  if (a.redo_mode && a.diag && ln == 0) count_add(&a.diag[kDiagRebuilt]);
+  // every lane has read the flag before lane 0 clears it (as cell_site)
+  __syncthreads();
+  if (a.redo_mode && ln == 0) {
+    a.redo[i] = 0;
+    if (a.diag) count_add(&a.diag[kDiagRebuilt]);
+  }
   if (!a.redo_mode && !a.prepass && a.prebuilt && a.prebuilt[i]) {
     // built by the pre-pass (as cell_site)
     if (ln == 0) {
@@ -1507,6 +1547,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
 template <typename T>
 __global__ __launch_bounds__(64) void cell_wave_kernel(CellArgs a) {
   __shared__ WaveLDS<T> W;
+  reset_check_list(a);
   const unsigned long long t0 = timer_begin();
   const int slot = xcd_slot(blockIdx.x, gridDim.x);
   const int blk = slot / kCellBlock;
@@ -1580,6 +1621,7 @@ __global__ __launch_bounds__(64) void cell_reappear_kernel(CellArgs a, int* xcan
 // Repair construction (FP64) with one wave (block) per listed site; any grid size.
 __global__ __launch_bounds__(64) void cell_repair_wave_kernel(CellArgs a) {
   __shared__ WaveLDS<double> W;
+  reset_check_list(a);
   const unsigned long long t0 = timer_begin();
   const int n = *a.list_count;
   bool worked = false;
@@ -1617,42 +1659,78 @@ __device__ __forceinline__ void integrate_block(const CellArgs& a, int blk, int 
   if (i >= 0 && t % L == 0) finish_site(a, a.probs[prob], prob, i, M);
 }
 
-// Clears the length of the work list the next check fills (nothing reads or writes that list
-// while this kernel runs).
+// Integration pass: clears the lengths of the lists the construction consumed (the deferred
+// sites; the cells the last evaluation left empty).
 __device__ __forceinline__ void reset_list(const CellArgs& a) {
   if (threadIdx.x != 0 || blockIdx.x != 0 || blockIdx.y != 0) return;
-  if (a.reset_count) *a.reset_count = 0;
   if (a.defer_count && !a.redo_mode) *a.defer_count = 0;
   if (a.empty_count && !a.redo_mode) *a.empty_count = 0;
 }
 
-// a.timer: the construction pass's timer, closed here; a.timer_int: this pass's timer.
-template <typename T, int L>
-__global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a) {
-  timer_close(a.timer);
-  reset_list(a);
-  const unsigned long long t0 = timer_begin();
-  // workgroup (blk, quarter) in XCD order: the quarters of a cell block stay on one XCD
-  const int slot = xcd_slot(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-  const int blk = slot / gridDim.y, quarter = slot % gridDim.y;
-  const bool worked = integ_site<L>(a, blk, quarter, threadIdx.x) >= 0;
-  integrate_block<T, L>(a, blk, quarter);
-  timer_mark(a.timer_int, worked, t0);
+__host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln);
+
+// The conformity check of cell blocks g * (kIntBlock / kCellBlock) ... in a kIntBlock-thread
+// workgroup of an integration launch (one thread per site).
+__device__ void check_group(const VerifyArgs& v, int g, int nblk) {
+  const int cb = g * (kIntBlock / kCellBlock) + (int)threadIdx.x / kCellBlock;
+  if (cb >= nblk) return;
+  const int rc = verify_site(v, cb, (int)threadIdx.x % kCellBlock);
+  if (rc == 1) {
+    atomicAdd(&v.fails[v.blocks[cb].prob], 1);
+    if (v.diag_fail) count_add(v.diag_fail);
+  }
 }
 
-// Repair integration over the work list: L lanes per listed site, any grid size.
+// Integration pass (1-D grid).  The last `vgroups` workgroups run the conformity check `v` of
+// the cells just built (check_group; nblk cell blocks): it reads only the construction's output,
+// as the integration does, so it runs in the same launch, in the integration's tail (the
+// workgroups are dispatched in order: earlier, they would hold compute units the integration's
+// workgroups wait for).  The others integrate: workgroup (blk, quarter) in XCD order, the
+// quarters of a cell block on one XCD.  a.timer: the construction pass's timer, closed here;
+// a.timer_int: this pass's, closed by its last block.
 template <typename T, int L>
-__global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a) {
+__global__ __launch_bounds__(kIntBlock) void integrate_kernel(CellArgs a, VerifyArgs v,
+                                                              int vgroups, int nblk) {
   timer_close(a.timer);
   reset_list(a);
   const unsigned long long t0 = timer_begin();
+  bool worked = false;
+  const int ni = (int)gridDim.x - vgroups;
+  if ((int)blockIdx.x >= ni) {
+    check_group(v, (int)blockIdx.x - ni, nblk);
+  } else {
+    constexpr int kQuarters = kCellBlock * L / kIntBlock;
+    const int slot = xcd_slot((int)blockIdx.x, ni);
+    const int blk = slot / kQuarters, quarter = slot % kQuarters;
+    worked = integ_site<L>(a, blk, quarter, threadIdx.x) >= 0;
+    integrate_block<T, L>(a, blk, quarter);
+  }
+  timer_mark(a.timer_int, worked, t0);
+  timer_finish(a.timer_int);
+}
+
+// Repair integration over the work list: L lanes per listed site, any grid size; the last
+// `vgroups` workgroups run the check of the round (as integrate_kernel).
+template <typename T, int L>
+__global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a, VerifyArgs v,
+                                                                     int vgroups, int nblk) {
+  timer_close(a.timer);
+  reset_list(a);
+  const unsigned long long t0 = timer_begin();
+  bool worked = false;
+  const int nb = (int)gridDim.x - vgroups, b = (int)blockIdx.x;
+  if (b >= nb) {
+    check_group(v, b - nb, nblk);
+    timer_mark(a.timer_int, false, t0);
+    timer_finish(a.timer_int);
+    return;
+  }
   const int n = *a.list_count;
   constexpr int kSites = kIntBlock / L;
   const int t = threadIdx.x;
-  bool worked = false;
   // entries dealt round-robin over the blocks (as in cell_repair_kernel)
-  for (int base = 0; base < n; base += gridDim.x * kSites) {
-    const int e = base + (t / L) * gridDim.x + blockIdx.x;
+  for (int base = 0; base < n; base += nb * kSites) {
+    const int e = base + (t / L) * nb + b;
     int i = -1, prob = 0;
     if (e < n) {
       const int slot = a.list[e];
@@ -1677,6 +1755,7 @@ __global__ __launch_bounds__(kIntBlock) void integrate_repair_kernel(CellArgs a)
     worked = worked || i >= 0;
   }
   timer_mark(a.timer_int, worked, t0);
+  timer_finish(a.timer_int);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1755,16 +1834,6 @@ __host__ __device__ int verify_site(const VerifyArgs& a, int blk, int ln) {
   }
   if (bad) mark(i);
   return bad ? 1 : 0;
-}
-
-__global__ __launch_bounds__(kCellBlock) void verify_kernel(VerifyArgs a) {
-  timer_close(a.timer_close);
-  const int blk = xcd_slot(blockIdx.x, gridDim.x);
-  const int rc = verify_site(a, blk, threadIdx.x);
-  if (rc == 1) {
-    atomicAdd(&a.fails[a.blocks[blk].prob], 1);
-    if (a.diag_fail) count_add(a.diag_fail);
-  }
 }
 
 // Keep the cells' adjacency as the candidates of the next pass (the next evaluation, or a
@@ -2010,22 +2079,29 @@ __global__ void fill_i32_kernel(int* p, size_t n, int v) {
 // Integration pass with L lanes per site: the full grid, or the repair grid over the work list.
 template <int L>
 void launch_integrate(const CellArgs& a, int nblocks, bool fp32, int repair_blocks,
-                      hipStream_t st) {
-  const dim3 ig(nblocks, kCellBlock * L / kIntBlock);
+                      const VerifyArgs* check, hipStream_t st) {
+  // check workgroups: kIntBlock / kCellBlock cell blocks each
+  constexpr int kPer = kIntBlock / kCellBlock;
+  const int vgroups = check ? (nblocks + kPer - 1) / kPer : 0;
+  const VerifyArgs v = check ? *check : VerifyArgs{};
+  const int ig = vgroups + nblocks * (kCellBlock * L / kIntBlock);
   if (a.redo_mode) {
-    hipLaunchKernelGGL((integrate_repair_kernel<double, L>), dim3(repair_blocks), dim3(kIntBlock),
-                       0, st, a);
+    hipLaunchKernelGGL((integrate_repair_kernel<double, L>), dim3(vgroups + repair_blocks),
+                       dim3(kIntBlock), 0, st, a, v, vgroups, nblocks);
   } else if (fp32) {
-    hipLaunchKernelGGL((integrate_kernel<float, L>), ig, dim3(kIntBlock), 0, st, a);
+    hipLaunchKernelGGL((integrate_kernel<float, L>), dim3(ig), dim3(kIntBlock), 0, st, a, v,
+                       vgroups, nblocks);
   } else {
-    hipLaunchKernelGGL((integrate_kernel<double, L>), ig, dim3(kIntBlock), 0, st, a);
+    hipLaunchKernelGGL((integrate_kernel<double, L>), dim3(ig), dim3(kIntBlock), 0, st, a, v,
+                       vgroups, nblocks);
   }
 }
 
-// Construction (one lane per site) then integration (int_lanes lanes per site: 4, 8 or 16).
-// The integration pass closes a.timer and marks a.timer_int.
+// Construction (one lane per site) then integration (int_lanes lanes per site: 4, 8 or 16) with
+// the conformity check `check` (nullptr: none) of the cells built.  The integration pass closes
+// a.timer and a.timer_int.
 hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
-                       int repair_blocks, hipStream_t st) {
+                       int repair_blocks, const VerifyArgs* check, hipStream_t st) {
   // waves of the passes that walk a list of sites: at most kDeferWaves
   const int list_waves = nblocks * kCellBlock < kDeferWaves ? nblocks * kCellBlock : kDeferWaves;
   if (a.redo_mode) {
@@ -2070,9 +2146,9 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
     }
   }
   switch (int_lanes) {
-    case 4: launch_integrate<4>(a, nblocks, fp32, repair_blocks, st); break;
-    case 16: launch_integrate<16>(a, nblocks, fp32, repair_blocks, st); break;
-    default: launch_integrate<8>(a, nblocks, fp32, repair_blocks, st); break;
+    case 4: launch_integrate<4>(a, nblocks, fp32, repair_blocks, check, st); break;
+    case 16: launch_integrate<16>(a, nblocks, fp32, repair_blocks, check, st); break;
+    default: launch_integrate<8>(a, nblocks, fp32, repair_blocks, check, st); break;
   }
   return hipGetLastError();
 }
@@ -2088,10 +2164,6 @@ hipError_t launch_prefix(const double* rho, int pm, int pn, bool fp32, DensityGr
   return hipGetLastError();
 }
 
-hipError_t launch_verify(const VerifyArgs& a, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(verify_kernel, dim3(nblocks), dim3(kCellBlock), 0, st, a);
-  return hipGetLastError();
-}
 
 hipError_t launch_save_adjacency(const BlockDesc* b, const int* gate, const uint8_t* active,
                                  const uint8_t* nedge, const int* elab, uint8_t* adjn, int* adj,

@@ -3,7 +3,7 @@
 // A round gives every unfinished problem one line-search trial:
 //   dev_direction_kernel   problems that start an iteration: d from the two-loop coefficients;
 //                          last block: descent check (ot_op.cuh:339-351), trial set-up
-//   cell_kernel / verify_kernel / repair rounds (cot_kernels.hip)
+//   cell_kernel / integrate_kernel with the conformity check / repair rounds (cot_kernels.hip)
 //   dev_decide_kernel      FP64 reductions of the trial; last block: accept / halve / restart
 //                          (ot_op.cuh:360-431)
 //   dev_accept_kernel      w += t d, g = mass - nu, history pair and its dot products; last block:

@@ -258,9 +258,11 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
   int rebuilt = 0, coarse = 0;
   a.coarse = &coarse;
   // the full pass in the context's precision, repairs in FP64 (as the library)
+  std::vector<uint8_t> rebuilt_site(n, 0);
   auto run_cells = [&]() {
     int built = 0;
     const bool f32 = S.fp32 && !a.redo_mode;
+    std::fill(rebuilt_site.begin(), rebuilt_site.end(), 0);
 #pragma omp parallel
     {
       Lanes<float, kPassVerts> lf;
@@ -275,6 +277,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
                                      : cell_site<double, kPassVerts>(a, b, ln, *ld.P, ln);
           overflow += rc == 1 ? 1 : 0;
           built += rc >= 0 ? 1 : 0;
+          if (rc >= 0) rebuilt_site[S.blocks[b].first + ln] = 1;
         }
     }
     // integration pass: kIntLanes lanes per site, summed in the order of the GPU's butterfly
@@ -283,7 +286,8 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
       for (int quarter = 0; quarter < kCellBlock * kIntLanes / kIntBlock; ++quarter)
         for (int t0 = 0; t0 < kIntBlock; t0 += kIntLanes) {
           const int i = integ_site(a, b, quarter, t0);
-          if (i < 0) continue;
+          // a repair pass integrates the sites it rebuilt (the library: its work list)
+          if (i < 0 || (a.redo_mode && !rebuilt_site[i])) continue;
           Moments M[kIntLanes];
           for (int q = 0; q < kIntLanes; ++q)
             f32 ? integrate_lane<float>(a, S.pd, i, q, M[q])
