@@ -43,15 +43,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes moved per evaluation (construction: lane pass + deferred wave pass; integration) of
-# the N=65,536 workload, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each
-# scaled by the same counter's reading for a copy of known size
-# (profiles/shapes65k_eval_traffic_pmc.md)
-TRAFFIC_BYTES = {"shapes65k": 59.41e6}
+# HBM bytes moved per evaluation (construction: pre-pass, lane pass, deferred wave pass;
+# integration with the conformity check in the same launch) of the N=65,536 workload, measured
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each scaled by the same counter's
+# reading for a copy of known size (profiles/shapes65k_eval_traffic_pmc.md)
+TRAFFIC_BYTES = {"shapes65k": 85.12e6}
 # VALU issue utilisation of the evaluation kernels, rocprofv3 SQ counters
 # (profiles/shapes65k_eval_valu_pmc.md)
-VALU_BUSY = {"shapes65k": {"cell_kernel": 0.109, "cell_defer_kernel": 0.051,
-                           "integrate_kernel": 0.231}}
+VALU_BUSY = {"shapes65k": {"cell_reappear_kernel": 0.047, "cell_kernel": 0.108,
+                           "cell_defer_kernel": 0.014, "integrate_kernel": 0.246}}
 WORKLOADS = {
     "shapes65k": {
         "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
@@ -383,9 +383,11 @@ def main():
         "zero_start_mixed_newton": cold,
         "learned_start_mixed_newton": learned,
         "roofline": {
-            "kernel": "cell evaluation: construction (cell_kernel + its deferred wave pass "
-                      "cell_defer_kernel, one in-kernel timer span) + integrate_kernel (exact "
-                      "mass integral), full-resolution passes of the timed solves",
+            "kernel": "cell evaluation: construction (cell_reappear_kernel, the pre-pass of the "
+                      "cells the last evaluation left empty + cell_kernel + its deferred wave "
+                      "pass cell_defer_kernel, one in-kernel timer span) + integrate_kernel "
+                      "(exact mass integral; the conformity check runs in its last "
+                      "workgroups), full-resolution passes of the timed solves",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(args.workload),
             "limiter": ("latency: data-dependent walks, one site per lane; VALU issue "
