@@ -102,6 +102,9 @@ constexpr int kDeferWaves = 4096;
 // A lane full pass over the two-ring defers a site whose neighbours' edge counts add up to more
 // than this (a typical cell: 6 neighbours of 6 edges).
 constexpr int kDeferCand = 64;
+// Integration lanes per site of the repair passes: few cells, often the large ones of
+// low-density regions, so the latency of one cell's walk is what counts.
+constexpr int kRepairLanes = 64;
 // Kernel timer (in-kernel wall clock, one record per kernel kind).  Each block that did work
 // folds its start and end into one of kTimerSlots (start, end) pairs picked by its block index;
 // the launch is then closed -- by the next kernel on the stream, or by the launch's own last

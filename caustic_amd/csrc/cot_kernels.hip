@@ -2086,8 +2086,9 @@ void launch_integrate(const CellArgs& a, int nblocks, bool fp32, int repair_bloc
   const VerifyArgs v = check ? *check : VerifyArgs{};
   const int ig = vgroups + nblocks * (kCellBlock * L / kIntBlock);
   if (a.redo_mode) {
-    hipLaunchKernelGGL((integrate_repair_kernel<double, L>), dim3(vgroups + repair_blocks),
-                       dim3(kIntBlock), 0, st, a, v, vgroups, nblocks);
+    hipLaunchKernelGGL((integrate_repair_kernel<double, kRepairLanes>),
+                       dim3(vgroups + repair_blocks), dim3(kIntBlock), 0, st, a, v, vgroups,
+                       nblocks);
   } else if (fp32) {
     hipLaunchKernelGGL((integrate_kernel<float, L>), dim3(ig), dim3(kIntBlock), 0, st, a, v,
                        vgroups, nblocks);

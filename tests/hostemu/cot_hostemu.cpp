@@ -6,6 +6,7 @@
 // (the current cells' two-ring and the check's queued candidates, then bucket rings).  A session keeps rings, adjacency and FP32
 // frames between evaluations like a library context.  Built by tests/hostemu/build.sh.
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../caustic_amd/csrc/cot_kernels.hip"
@@ -280,7 +281,24 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
           if (rc >= 0) rebuilt_site[S.blocks[b].first + ln] = 1;
         }
     }
-    // integration pass: kIntLanes lanes per site, summed in the order of the GPU's butterfly
+    // integration pass: L lanes per site (kIntLanes in a full pass, kRepairLanes in a repair,
+    // as the library), summed in the order of the GPU's butterfly
+    auto integrate_site = [&](auto lanes, int i) {
+      constexpr int L = decltype(lanes)::value;
+      Moments M[L];
+      for (int q = 0; q < L; ++q)
+        f32 ? integrate_lane<float, L>(a, S.pd, i, q, M[q])
+            : integrate_lane<double, L>(a, S.pd, i, q, M[q]);
+      for (int off = L / 2; off > 0; off >>= 1) {
+        Moments nxt[L];
+        for (int q = 0; q < L; ++q) {
+          nxt[q] = M[q];
+          combine(nxt[q], M[q ^ off]);
+        }
+        for (int q = 0; q < L; ++q) M[q] = nxt[q];
+      }
+      finish_site(a, S.pd, 0, i, M[0]);
+    };
 #pragma omp parallel for schedule(dynamic, 1)
     for (int b = 0; b < nb; ++b)
       for (int quarter = 0; quarter < kCellBlock * kIntLanes / kIntBlock; ++quarter)
@@ -288,19 +306,10 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
           const int i = integ_site(a, b, quarter, t0);
           // a repair pass integrates the sites it rebuilt (the library: its work list)
           if (i < 0 || (a.redo_mode && !rebuilt_site[i])) continue;
-          Moments M[kIntLanes];
-          for (int q = 0; q < kIntLanes; ++q)
-            f32 ? integrate_lane<float>(a, S.pd, i, q, M[q])
-                : integrate_lane<double>(a, S.pd, i, q, M[q]);
-          for (int off = kIntLanes / 2; off > 0; off >>= 1) {
-            Moments nxt[kIntLanes];
-            for (int q = 0; q < kIntLanes; ++q) {
-              nxt[q] = M[q];
-              combine(nxt[q], M[q ^ off]);
-            }
-            for (int q = 0; q < kIntLanes; ++q) M[q] = nxt[q];
-          }
-          finish_site(a, S.pd, 0, i, M[0]);
+          if (a.redo_mode)
+            integrate_site(std::integral_constant<int, kRepairLanes>{}, i);
+          else
+            integrate_site(std::integral_constant<int, kIntLanes>{}, i);
         }
     return built;
   };
