@@ -29,10 +29,68 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// Wave-wide min / max (exact operations, so any pairing gives the same result) without the LDS
+// crossbar: four DPP steps within each row of 16 lanes (lane i with i ^ 1, then i ^ 2, then the
+// half-row mirror 7 - i, then the row mirror 15 - i), then the four rows' values by readlane.
+// Called with every lane of the wave active; the result is the same in every lane.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
+constexpr int kDppMirror = 0x140;      // row_mirror
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
+
+template <int C>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, C, 0xF, 0xF, true);
+}
+template <int C>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i<C>((int)(unsigned)b);
+  const unsigned hi = (unsigned)dpp_i<C>((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double rdl_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
+  v = fmax(v, dpp_d<kDppXor1>(v));
+  v = fmax(v, dpp_d<kDppXor2>(v));
+  v = fmax(v, dpp_d<kDppHalfMirror>(v));
+  v = fmax(v, dpp_d<kDppMirror>(v));
+  return fmax(fmax(rdl_d(v, 0), rdl_d(v, 16)), fmax(rdl_d(v, 32), rdl_d(v, 48)));
+}
+__device__ __forceinline__ double wave_min(double v) {
+  v = fmin(v, dpp_d<kDppXor1>(v));
+  v = fmin(v, dpp_d<kDppXor2>(v));
+  v = fmin(v, dpp_d<kDppHalfMirror>(v));
+  v = fmin(v, dpp_d<kDppMirror>(v));
+  return fmin(fmin(rdl_d(v, 0), rdl_d(v, 16)), fmin(rdl_d(v, 32), rdl_d(v, 48)));
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, dpp_i<kDppXor1>(v));
+  v = max(v, dpp_i<kDppXor2>(v));
+  v = max(v, dpp_i<kDppHalfMirror>(v));
+  v = max(v, dpp_i<kDppMirror>(v));
+  return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+// Inclusive prefix sum over the wave (every lane active): a shift-and-add scan within each row of
+// 16 lanes (DPP row shifts by 1, 2, 4, 8; lanes shifted in from outside the row add 0), then the
+// totals of the rows below.
+__device__ __forceinline__ int wave_scan_i(int v) {
+  const int ln = (int)__lane_id();
+  v += dpp_i<kDppShr1>(v);
+  v += dpp_i<kDppShr2>(v);
+  v += dpp_i<kDppShr4>(v);
+  v += dpp_i<kDppShr8>(v);
+  const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
+  const int r2 = __builtin_amdgcn_readlane(v, 47);
+  return v + (ln >= 16 ? r0 : 0) + (ln >= 32 ? r1 : 0) + (ln >= 48 ? r2 : 0);
 }
 
 // Sum of v over the block (blockDim.x a multiple of 64, at most 1024); result valid in thread 0.
@@ -1105,32 +1163,7 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 // v from lane l (l wave-uniform)
 __device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ double rdl(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
-// inclusive prefix sum over the wave
-__device__ __forceinline__ int wave_scan_i(int v) {
-  const int ln = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(v, off, 64);
-    if (ln >= off) v += u;
-  }
-  return v;
-}
+__device__ __forceinline__ double rdl(double v, int l) { return rdl_d(v, l); }
 
 template <typename T>
 struct WaveLDS {
@@ -1343,17 +1376,17 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
   const int by = min(max((int)floor((s.y - pd.by0) * pd.binv), 0), pd.gy - 1);
   if (a.xcnt) {
     // the queued candidates, ascending and without repeats
-    const int nx = min(a.xcnt[i], kMaxExtra);
+    const int nx = __builtin_amdgcn_readfirstlane(min(a.xcnt[i], kMaxExtra));
     const int v = ln < nx ? a.xcand[(size_t)i * kMaxExtra + ln] : INT32_MAX;
     bool first = ln < nx;
     for (int q = 0; q < nx; ++q) {
-      const int u = __shfl(v, q, 64);
+      const int u = rdl(v, q);
       if (q < ln && u == v) first = false;
     }
     int rank = 0;
     for (int q = 0; q < nx; ++q) {
-      const int u = __shfl(v, q, 64);
-      const int fq = __shfl((int)first, q, 64);
+      const int u = rdl(v, q);
+      const int fq = rdl((int)first, q);
       if (fq && u < v) ++rank;
     }
     const int nu = __popcll(__ballot(first));
@@ -1373,7 +1406,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
       // the neighbours' neighbours, flattened neighbour by neighbour
       const int cnt = jn >= 0 ? (int)a.adjn[jn] : 0;
       const int inc = wave_scan_i(cnt);
-      const int total = __shfl(inc, 63, 64);
+      const int total = rdl(inc, 63);
       W.off[ln] = inc - cnt;
       W.key[ln] = jn;
       __syncthreads();
@@ -1415,7 +1448,7 @@ __device__ int cell_site_wave(const CellArgs& a, int blk, int site_ln, WaveLDS<T
           }
         }
         const int inc = wave_scan_i(cnt);
-        const int total = __shfl(inc, 63, 64);
+        const int total = rdl(inc, 63);
         W.off[ln] = inc - cnt;
         W.key[ln] = start;
         __syncthreads();
