@@ -1083,16 +1083,18 @@ __device__ void timer_close(unsigned long long* tm) {
 }
 
 // The launch's last block to reach this (every block calls it once, after its timer_mark) closes
-// the launch's timer.  Loads and stores of the record go through device-scope atomics: the other
-// blocks' marks are not in this block's cache.
+// the launch's timer.  Every access to the record is a device-scope atomic, performed where all
+// XCDs see it, so ordering needs no cache maintenance: the block waits for its mark's atomics to
+// complete before it counts its arrival (an agent-scope fence here would write back and
+// invalidate the XCD's L2 once per workgroup), and the last block's reads are atomics issued
+// after its arrival returned.
 __device__ void timer_finish(unsigned long long* tm) {
   if (!tm) return;
   __syncthreads();
   if (threadIdx.x != 0) return;
-  __threadfence();
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
   if (atomicAdd(&tm[2 * kTimerSlots + 2], 1ull) + 1 != total) return;
-  __threadfence();
   atomicExch(&tm[2 * kTimerSlots + 2], 0ull);
   unsigned long long t0 = ~0ull, t1 = 0ull;
   for (int r = 0; r < kTimerSlots; ++r) {

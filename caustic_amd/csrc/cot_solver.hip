@@ -29,6 +29,16 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
+// Partials that one block publishes for the problem's last block: device-scope atomic stores and
+// loads, which reach the level every XCD shares, so the hand-over needs no L2 write-back or
+// invalidation (see arrive_last).
+__device__ __forceinline__ void put_partial(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double get_partial(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum `nv` per-block partials (layout part[v * stride + b], blocks [b0, b0 + nb)) into out[v]
 // using the whole block; result valid in out[] for all threads after return.
 __device__ void sum_partials(const double* part, size_t stride, int b0, int nb, int nv,
@@ -42,7 +52,7 @@ __device__ void sum_partials(const double* part, size_t stride, int b0, int nb, 
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
 #pragma unroll
       for (int k = 0; k < kChunk; ++k)
-        if (c0 + k < nv) acc[k] += part[(size_t)(c0 + k) * stride + b0 + b];
+        if (c0 + k < nv) acc[k] += get_partial(&part[(size_t)(c0 + k) * stride + b0 + b]);
     }
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
@@ -82,20 +92,20 @@ __device__ double block_max1(double v, double* red) {
   return s;
 }
 
-// Publish this block's partials and learn whether it is the problem's last block.
+// Count this block's arrival and learn whether it is the problem's last block.  Thread 0 stored
+// the block's partials (put_partial) and waits for those stores to complete before it counts the
+// arrival; the last block reads every partial with get_partial after its count returned.  (An
+// agent-scope fence in their place writes back and invalidates the XCD's L2 in every block.)
 __device__ bool arrive_last(int* counter, int nblk) {
   __shared__ int last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     const int prev = atomicAdd(counter, 1);
     last = (prev == nblk - 1);
   }
   __syncthreads();
-  if (last) {
-    __threadfence();  // acquire: see every other block's partials
-    if (threadIdx.x == 0) *counter = 0;
-  }
+  if (last && threadIdx.x == 0) atomicExch(counter, 0);
   return last != 0;
 }
 
@@ -166,7 +176,7 @@ __global__ __launch_bounds__(kCellBlock) void dev_direction_kernel(
     gd = g[i] * di;
   }
   const double v = block_sum1(gd, red);
-  if (ln == 0) part[blockIdx.x] = v;
+  if (ln == 0) put_partial(&part[blockIdx.x], v);
   if (!arrive_last(&counter[bd.prob], pd.nblk)) return;
   __shared__ double tot[1];
   __shared__ double lw[4 * 16];
@@ -219,19 +229,20 @@ __global__ __launch_bounds__(kCellBlock) void dev_decide_kernel(
     ar = area[i];
   }
   double v;
-  v = block_sum1(f, red);  if (ln == 0) part[0 * pstride + blockIdx.x] = v;
-  v = block_max1(mr, red); if (ln == 0) part[1 * pstride + blockIdx.x] = v;
-  v = block_sum1(z, red);  if (ln == 0) part[2 * pstride + blockIdx.x] = v;
-  v = block_sum1(g2, red); if (ln == 0) part[3 * pstride + blockIdx.x] = v;
-  v = block_sum1(gd, red); if (ln == 0) part[4 * pstride + blockIdx.x] = v;
-  v = block_sum1(ar, red); if (ln == 0) part[5 * pstride + blockIdx.x] = v;
+  v = block_sum1(f, red);  if (ln == 0) put_partial(&part[0 * pstride + blockIdx.x], v);
+  v = block_max1(mr, red); if (ln == 0) put_partial(&part[1 * pstride + blockIdx.x], v);
+  v = block_sum1(z, red);  if (ln == 0) put_partial(&part[2 * pstride + blockIdx.x], v);
+  v = block_sum1(g2, red); if (ln == 0) put_partial(&part[3 * pstride + blockIdx.x], v);
+  v = block_sum1(gd, red); if (ln == 0) put_partial(&part[4 * pstride + blockIdx.x], v);
+  v = block_sum1(ar, red); if (ln == 0) put_partial(&part[5 * pstride + blockIdx.x], v);
   if (!arrive_last(&counter[bd.prob], pd.nblk)) return;
   __shared__ double lw[4 * 16];
   __shared__ double tot[6];
   sum_partials(part, pstride, pd.blk_off, pd.nblk, 6, lw, tot);
   // the max relative error is a max, not a sum
   double mloc = 0.0;
-  for (int b = ln; b < pd.nblk; b += blockDim.x) mloc = fmax(mloc, part[1 * pstride + pd.blk_off + b]);
+  for (int b = ln; b < pd.nblk; b += blockDim.x)
+    mloc = fmax(mloc, get_partial(&part[1 * pstride + pd.blk_off + b]));
   mloc = block_max1(mloc, red);
   if (ln == 0) tot[1] = mloc;
   __syncthreads();
