@@ -47,11 +47,11 @@ HBM_PEAK_GBS = 8000.0
 # integration with the conformity check in the same launch) of the N=65,536 workload, measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on MI355X, each scaled by the same counter's
 # reading for a copy of known size (profiles/shapes65k_eval_traffic_pmc.md)
-TRAFFIC_BYTES = {"shapes65k": 85.12e6}
+TRAFFIC_BYTES = {"shapes65k": 85.17e6}
 # VALU issue utilisation of the evaluation kernels, rocprofv3 SQ counters
 # (profiles/shapes65k_eval_valu_pmc.md)
-VALU_BUSY = {"shapes65k": {"cell_reappear_kernel": 0.047, "cell_kernel": 0.108,
-                           "cell_defer_kernel": 0.014, "integrate_kernel": 0.246}}
+VALU_BUSY = {"shapes65k": {"cell_reappear_kernel": 0.102, "cell_kernel": 0.109,
+                           "cell_defer_kernel": 0.016, "integrate_kernel": 0.286}}
 WORKLOADS = {
     "shapes65k": {
         "desc": "single large solve: N=65536 jittered sites, 1024x1024 disks/rects target "
