@@ -2150,7 +2150,7 @@ hipError_t launch_cell(const CellArgs& a, int nblocks, bool fp32, int int_lanes,
     else
       hipLaunchKernelGGL(cell_repair_wave_kernel, dim3(4 * repair_blocks), dim3(64), 0, st, a);
   } else {
-    if (a.empty_list && a.prebuilt && a.cand != kCandBucket) {
+    if (a.empty_list && a.prebuilt && a.cand != kCandBucket && use_prepass(nblocks)) {
       // the cells the last evaluation left empty, before the full pass builds their neighbours
       int* xcand = const_cast<int*>(a.xcand);
       if (fp32)

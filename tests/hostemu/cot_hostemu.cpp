@@ -327,7 +327,7 @@ int emu_eval(void* h, const double* w, double* mass, double* fcell, double* cen,
         save_adjacency_site(S.blocks[b], b, ln, S.ne.data(), S.elab.data(), S.adjn.data(),
                             S.adj.data(), last ? S.elist.data() : nullptr, &S.ecount, n);
   };
-  if (S.have_adj) {
+  if (S.have_adj && use_prepass(nb)) {
     // as the library (cell_reappear_kernel): the cells the last evaluation left empty, built
     // before the full pass (which skips them) without their queued candidates; those that come
     // back are queued as candidates of their new neighbours
