@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -385,16 +386,25 @@ int cot_solve_ladder(cot_context* ctx, const cot_problem* problems, double* w,
   auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };
+  // every level's sites and target depend only on the problems: each level's are built on a
+  // worker thread from the start, while the levels below it solve
+  std::vector<std::vector<Level>> lvs(lo.n_levels, std::vector<Level>(np));
+  std::vector<std::future<void>> built;
+  for (int l = 0; l < lo.n_levels; ++l)
+    built.push_back(std::async(std::launch::async, [&, l] {
+      for (int p = 0; p < np; ++p)
+        coarse_level(problems[p], pyr[p], frame_of(lo, problems[p], p), lo.sides[l],
+                     lo.floors[l], lo.seed + 7919ull * (l + 1) + p, lvs[l][p]);
+    }));
   for (int l = 0; l < lo.n_levels; ++l) {
     const auto tl0 = now();
     const int side = lo.sides[l], n = side * side;
-    std::vector<Level> lv(np);
+    built[l].get();
+    std::vector<Level>& lv = lvs[l];
     std::vector<cot_problem> cp(np);
     std::vector<double> wl((size_t)np * n, 0.0);
     for (int p = 0; p < np; ++p) {
       const double* fr = frame_of(lo, problems[p], p);
-      coarse_level(problems[p], pyr[p], fr, side, lo.floors[l], lo.seed + 7919ull * (l + 1) + p,
-                   lv[p]);
       cp[p] = as_problem(lv[p], n);
       if (l > 0) weights_from(field[p], lo.sides[l - 1], fr, lv[p].sites.data(), n, &wl[(size_t)p * n]);
     }
